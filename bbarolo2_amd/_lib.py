@@ -1,0 +1,75 @@
+"""ctypes binding of libbbgpu.so (the C-ABI declared in include/bbgpu.h).
+
+The product path has no CPU fallback: if the library is missing, or no GPU is visible when a kernel is
+called, these functions raise.
+"""
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libbbgpu.so")
+MAX_LINES = 8
+
+
+class BBError(RuntimeError):
+    """An error reported by libbbgpu (bb_last_error)."""
+
+
+class bb_grid(C.Structure):
+    _fields_ = [("nz", C.c_int32), ("ltype", C.c_int32), ("axtyp", C.c_int32), ("velsys", C.c_int32),
+                ("nlines", C.c_int32), ("nzq", C.c_int32), ("cmode", C.c_int32), ("pad0", C.c_int32),
+                ("radsep", C.c_double), ("cdens", C.c_double), ("pixarea", C.c_double), ("cdelt0", C.c_double),
+                ("cdelt1", C.c_double), ("crota2", C.c_double), ("sig_instr", C.c_double), ("crval3", C.c_double),
+                ("cdelt3", C.c_double), ("crpix3", C.c_double), ("chmin", C.c_double), ("zstep", C.c_double),
+                ("relint", C.c_double * MAX_LINES), ("lfreq0", C.c_double * MAX_LINES)]
+
+
+class bb_box(C.Structure):
+    _fields_ = [("blo0", C.c_int32), ("blo1", C.c_int32), ("bhi0", C.c_int32), ("bhi1", C.c_int32)]
+
+
+_P = C.c_void_p
+_SIGS = {
+    "bb_last_error": (C.c_char_p, []),
+    "bb_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "bb_model_create": (C.c_int, [C.POINTER(bb_grid), _P, _P, _P, C.c_int, _P, C.c_int, C.c_int, C.c_double,
+                                  C.POINTER(_P)]),
+    "bb_model_destroy": (C.c_int, [_P]),
+    "bb_galmod_analytic": (C.c_int, [_P, bb_box, _P, C.c_int, C.c_int, C.c_int, _P, _P]),
+    "bb_conv2d_same": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, _P]),
+    "bb_residual_work_size": (C.c_int64, [C.c_int, C.c_int64]),
+    "bb_residual_gauss": (C.c_int, [_P, _P, C.c_int, C.c_int64, C.c_double, C.c_double, _P, C.c_int64, _P, _P,
+                                    _P]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library (loads it on first use); raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.isfile(LIBPATH):
+            raise BBError(f"{LIBPATH} is not built: run python -m bbarolo2_amd.build")
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise BBError(lib().bb_last_error().decode() or f"libbbgpu error {rc}")
+
+
+def declared_symbols():
+    """The entry points declared in include/bbgpu.h."""
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "bbgpu.h")
+    with open(hdr) as f:
+        txt = f.read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(bb_\w+)\s*\(", txt, re.M)))

@@ -1,0 +1,163 @@
+"""DiskModel on the GPU: the reference's model engine API (bbarolo/galmod.py:1147-1377) over libbbgpu.
+
+DiskModel(header, par, ringmode, densflux, galmodtype, smooth) computes the header-derived setup and the
+beam kernel once, copies them to the device (bb_model_create), and builds models of rings:
+  array(rings, blo, bhi, final, layout)   float64 numpy array (nz, ny, nx) or (ny, nx, nz), as the reference;
+  array_batch(tables, box)                a (B, by, bx, nz) float32 device tensor (yxz) of B sub-ring tables.
+Device memory and streams come from torch; the arithmetic runs in the HIP kernels. There is no CPU path:
+without a GPU or without libbbgpu.so these methods raise.
+"""
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from .galmod import (GalmodSetup, GalmodPar, model_rings, model_box, ringmode_name, zstep_for, nzq_for,
+                     PHI_TAB, ZSAMPLES_MAX)
+from .smooth import beam_kernel
+
+AN_THREADS = 256
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise _lib.BBError("no GPU visible: the model-cube likelihood runs only on the GPU")
+    return torch
+
+
+def _stream_ptr(torch, stream=None):
+    s = torch.cuda.current_stream() if stream is None else stream
+    return C.c_void_p(s.cuda_stream)
+
+
+def naz_of(R0, pixarea):
+    """Azimuthal samples of an ANALYTIC sub-ring (galmod.py:341-343)."""
+    circ = 2 * math.pi * np.asarray(R0, float) / math.sqrt(pixarea)
+    return np.maximum((3 * circ + 0.5).astype(np.int64), 16)
+
+
+class DiskModel:
+    """Models of a rotating disk on the grid of a cube, built on the GPU (see the module docstring)."""
+
+    def __init__(self, header, par=None, ringmode=None, densflux=None, galmodtype=None, smooth=None, device=None):
+        self.header = header
+        self.par = GalmodPar() if par is None else par
+        self.setup = GalmodSetup(header, self.par, galmodtype=galmodtype,
+                                 densflux=self.par.DENSFLUX if densflux is None else densflux)
+        self.ringmode = ringmode_name(self.par.RINGMODE if ringmode is None else ringmode)
+        self.smooth = bool(self.par.SM if smooth is None else smooth)
+        self.cfield, self.scalefactor = None, 1.
+        if self.smooth:
+            kb = beam_kernel(header) if header.Bmaj() > 0 else None
+            if kb is None:
+                self.smooth = False
+            else:
+                self.cfield, sc = kb
+                self.scalefactor = 1.0 if self.setup.densIsFlux else sc
+        self.setup.zstep = zstep_for(header, self.setup, self.cfield is not None)
+        self._handle = None
+        self._device = device
+
+    # ------------------------------------------------------------------ device objects
+    def grid(self, nzq):
+        st = self.setup
+        g = _lib.bb_grid()
+        g.nz, g.ltype, g.axtyp, g.velsys = st.nsubs, st.ltype, st.axtyp, st.velsys
+        g.nlines, g.nzq, g.cmode = len(st.lfreq0), int(nzq), st.cmode
+        if g.nlines > _lib.MAX_LINES:
+            raise _lib.BBError(f"at most {_lib.MAX_LINES} lines")
+        g.radsep, g.cdens, g.pixarea = st.radsep, st.cdens, st.pixarea
+        g.cdelt0, g.cdelt1, g.crota2 = st.cdelt[0], st.cdelt[1], st.crota2
+        g.sig_instr, g.crval3, g.cdelt3, g.crpix3 = st.sig_instr, st.crval3, st.cdelt3, st.crpix3
+        g.chmin, g.zstep = st.chmin, st.zstep
+        for i in range(g.nlines):
+            g.relint[i] = st.relint[i]
+            g.lfreq0[i] = st.lfreq0[i]
+        return g
+
+    def handle(self, nzq=ZSAMPLES_MAX):
+        """The bb_model of this DiskModel on the current device (created on first use; nzq fixes the
+        vertical-sample capacity: the reference uses 1 for thin disks and 1024 otherwise)."""
+        torch = _torch()
+        key = (torch.cuda.current_device(), int(nzq))
+        if self._handle is None:
+            self._handle = {}
+        h = self._handle.get(key)
+        if h is None:
+            L = _lib.lib()
+            st = self.setup
+            g = self.grid(nzq)
+            cd = np.ascontiguousarray(st.cd2i, np.float64)
+            ve = np.ascontiguousarray(st.vedges_lines, np.float64)
+            pt = np.ascontiguousarray(PHI_TAB, np.float64)
+            if self.cfield is not None:
+                k = np.ascontiguousarray(self.cfield, np.float64)
+                kp, ky, kx = k.ctypes.data, k.shape[0], k.shape[1]
+            else:
+                k, kp, ky, kx = None, None, 0, 0
+            out = C.c_void_p()
+            _lib.check(L.bb_model_create(C.byref(g), cd.ctypes.data, ve.ctypes.data, pt.ctypes.data, len(pt), kp,
+                                         ky, kx, float(self.scalefactor), C.byref(out)))
+            h = out
+            self._handle[key] = h
+        return h
+
+    def __del__(self):
+        try:
+            if self._handle:
+                L = _lib.lib()
+                for h in self._handle.values():
+                    L.bb_model_destroy(h)
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ reference API
+    def box(self, rings, final=False):
+        return model_box(self.header, rings, self.ringmode, final)
+
+    def subrings(self, rings, final=True, ringmode=None):
+        if final:
+            rings = model_rings(rings, self.ringmode if ringmode is None else ringmode, self.setup.radsep)
+        return self.setup.subrings(rings)
+
+    def array(self, rings, blo=None, bhi=None, final=True, ringmode=None, galmodtype=None, layout="zyx"):
+        """The model of the rings (convolved with the beam if smoothing is on) in the box [blo, bhi), as a
+        float64 array (nz, ny, nx), or (ny, nx, nz) with layout='yxz' (DiskModel.array, galmod.py:1242-1271).
+        final=False (models of single rings in the 3DFIT ring fits) uses the rings as nodes."""
+        gtype = (galmodtype or self.setup.galmodtype).upper()
+        if gtype != "ANALYTIC":
+            raise NotImplementedError("GALMODTYPE MC models are not built by this engine yet")
+        if blo is None or bhi is None:
+            blo, bhi = (0, 0), (self.setup.dimx, self.setup.dimy)
+        sr, _ = self.subrings(rings, final, ringmode)
+        m = self.array_batch(sr[None], blo, bhi, convolve=self.cfield is not None)
+        a = m[0].double().cpu().numpy()
+        return np.ascontiguousarray(a.transpose(2, 0, 1)) if layout == "zyx" else a
+
+    def array_batch(self, sr, blo, bhi, convolve=True, out=None, work=None, stream=None):
+        """ANALYTIC models of B sub-ring tables sr (B, 16, nsub) sharing their radii, in the box [blo, bhi):
+        a (B, by, bx, nz) float32 device tensor (yxz), convolved with the beam if convolve."""
+        torch = _torch()
+        sr = np.ascontiguousarray(sr, np.float64)
+        B, nrow, nsub = sr.shape
+        if nrow != 16:
+            raise ValueError("sub-ring tables must have 16 rows")
+        by, bx, nz = bhi[1] - blo[1], bhi[0] - blo[0], self.setup.nsubs
+        dev = torch.device("cuda", torch.cuda.current_device())
+        nzq = nzq_for(sr.reshape(16, -1))
+        h = self.handle(nzq)
+        st = _stream_ptr(torch, stream)
+        dep = torch.zeros((B, by, bx, nz), dtype=torch.float32, device=dev) if work is None else work.zero_()
+        if nsub:
+            d_sr = torch.from_numpy(sr).to(dev, non_blocking=False)
+            chunks = int(max(1, math.ceil(int(naz_of(sr[0, 0], self.setup.pixarea).max()) / AN_THREADS)))
+            box = _lib.bb_box(blo[0], blo[1], bhi[0], bhi[1])
+            _lib.check(_lib.lib().bb_galmod_analytic(h, box, d_sr.data_ptr(), nsub, B, chunks, dep.data_ptr(), st))
+        if not convolve or self.cfield is None:
+            return dep
+        res = torch.empty_like(dep) if out is None else out
+        _lib.check(_lib.lib().bb_conv2d_same(h, dep.data_ptr(), res.data_ptr(), B, by, bx, st))
+        return res

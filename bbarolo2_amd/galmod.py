@@ -1,0 +1,535 @@
+"""Host side of the GALMOD model engine: header-derived constants, ring conventions and sub-rings.
+
+These run once per fit (``GalmodSetup``) or once per evaluation on a few hundred numbers
+(``model_rings``, ``subrings``); the per-voxel work runs in the HIP kernels of ``csrc/``.
+
+Reference behaviour followed (bbarolo/galmod.py):
+  * GalmodSetup.__init__ / _NHItoRAD / _velocity_edges   galmod.py:613-758   (row A1)
+  * model_rings / _half_widths / edge_weights / outer_edge galmod.py:1055-1141 (row A2)
+  * GalmodSetup.subrings / side_edges                      galmod.py:761-863   (row A3)
+  * DiskModel.box / zstep                                  galmod.py:1186-1240 (row A7)
+  * the Phi table of the ANALYTIC line profiles            galmod.py:174-180
+The sub-ring table keeps the reference's layout, sr (16, nsub) float64 with rows
+  0 radius  1 vrot  2 vrad  3 vvert  4 vdisp  5 dens  6 z0  7 dvdz  8 zcyl  9 inc  10 phi
+  11 xpos  12 ypos  13 vsys  14 weight of the receding half  15 weight of the approaching half
+in internal units (arcmin, m/s, radians, pixels), because the kernels and the oracle read it.
+"""
+
+import math
+from dataclasses import dataclass, field
+from typing import List
+
+import numpy as np
+from scipy.special import erfc as _erfc
+
+from .header import C_MS, VEL_UNITS, WAVE_UNITS, HI_FREQ, delta_vel
+
+ZSAMPLES_MAX = 1024
+ZSTEP_FINE = 0.25
+ZSTEP_COARSE = 0.75
+ZSTEP_BEAM = 3.0
+
+# Tabulated standard normal CDF Phi(t) on a 1/512 grid over [-6, 6] (galmod.py:174-180)
+PHI_TMAX = 6.0
+PHI_ISTEP = 512.0
+PHI_TAB = np.array([0.5 * math.erfc(-(-PHI_TMAX + k / PHI_ISTEP) / math.sqrt(2.0))
+                    for k in range(int(2 * PHI_TMAX * PHI_ISTEP) + 2)])
+
+
+class GalmodError(ValueError):
+    """Invalid model input (the reference raises BBaroloError / ValueError before its kernels run)."""
+
+
+@dataclass
+class GalmodPar:
+    """The GALMOD keywords of the model engine, with the reference's defaults (bbarolo/params.py:137-174,
+    315-367)."""
+    LTYPE: int = 1
+    CMODE: int = 1
+    CDENS: float = 10.
+    ISEED: int = -1
+    NV: int = -1
+    EMPTY: bool = True
+    SM: bool = True
+    DENSFLUX: bool = True
+    RESTWAVE: List[float] = field(default_factory=lambda: [-1.0])
+    RESTFREQ: List[float] = field(default_factory=lambda: [-1.0])
+    RELINT: List[float] = field(default_factory=lambda: [1.0])
+    RADMAXA: float = -1.
+    RADMAXR: float = -1.
+    GALMODTYPE: str = "MC"
+    RINGMODE: str = "CENTRED"
+    SPECRES: float = -1.
+    LINEAR: float = -1.
+    REDSHIFT: float = 0.
+
+
+def seed32(iseed):
+    """ISEED -> the 32-bit seed of the MC models (galmod.py:593-594; -1 -> 0xFFFFFFFF)."""
+    return int(iseed) & 0xFFFFFFFF
+
+
+class GalmodSetup:
+    """Header-derived quantities of the models of a cube (row A1). Units: arcmin for the sky, m/s
+    for velocities, radians for angles."""
+
+    def __init__(self, header, par=None, galmodtype=None, densflux=None):
+        h = header
+        gm = GalmodPar() if par is None else par
+        self.header = h
+        self.par = gm
+        self.nsubs = h.DimAx(2)
+        self.dimx, self.dimy = h.DimAx(0), h.DimAx(1)
+        cdelt = [0., 0.]
+        for i in range(2):
+            cu = h.Cunit(i).lower()
+            if "deg" in cu: conv = 60.
+            elif "arcs" in cu: conv = 1 / 60.
+            elif "arcm" in cu: conv = 1.
+            else:
+                raise GalmodError(f"GALMOD error (unknown CUNIT for RA-DEC): cannot convert to ARCMIN.\n{cu}")
+            cdelt[i] = h.Cdelt(i) * conv
+        self.cdelt = cdelt
+        self.pixsize = [abs(cdelt[0]), abs(cdelt[1])]
+        self.pixarea = self.pixsize[0] * self.pixsize[1]
+        self.crota2 = h.Crota() * math.pi / 180.
+        self.zstep = ZSTEP_FINE
+        reds = float(gm.REDSHIFT)
+        cunit3 = h.Cunit(2).lower()
+        self.cdelt3, self.crval3, self.crpix3 = h.Cdelt(2), h.Crval(2), h.Crpix(2)
+        self.velsys = {"radio": 1, "optical": 2, "relativistic": 3}.get(h.VelDef(), 1)
+        sptype = h.SpectralType()
+        lfreq0 = None
+        self.relint = np.array([1.])
+        self.freq0 = HI_FREQ
+        if sptype == "wave":
+            self.axtyp = 2
+            if cunit3 not in WAVE_UNITS:
+                raise GalmodError(f"GALMOD ERROR: unknown CUNIT3={cunit3}. Cannot convert to meters.")
+            mconv = 1. / WAVE_UNITS[cunit3]
+            self.crval3 *= mconv
+            self.cdelt3 *= mconv
+            restw = gm.RESTWAVE[0]
+            self.freq0 = C_MS / (restw * (1 + reds) * mconv) if restw != -1 else C_MS / self.crval3
+            if len(gm.RESTWAVE) > 1:
+                lfreq0 = C_MS / (np.asarray(gm.RESTWAVE, float) * (1 + reds) * mconv)
+                self.relint = self._relint(gm.RELINT, len(gm.RESTWAVE))
+        elif sptype == "freq":
+            self.axtyp = 3
+            hz = {"hz": 1, "khz": 1E03, "mhz": 1E06, "ghz": 1E09}.get(cunit3)
+            if hz is None:
+                raise GalmodError(f"GALMOD ERROR: unknown CUNIT3={cunit3}. Cannot convert to Hz.")
+            self.crval3 *= hz
+            self.cdelt3 *= hz
+            self.freq0 = h.Freq0() / (1 + reds) or HI_FREQ
+            if len(gm.RESTFREQ) > 1:
+                lfreq0 = np.asarray(gm.RESTFREQ, float) / (1 + reds)
+                self.relint = self._relint(gm.RELINT, len(gm.RESTFREQ))
+        elif "velo" in sptype:
+            self.axtyp = 4
+            if cunit3 not in VEL_UNITS:
+                raise GalmodError(f"GALMOD ERROR: unknown CUNIT3={cunit3}. Cannot convert to m/s.")
+            msconv = 1.0E03 / VEL_UNITS[cunit3]
+            self.freq0 = h.Freq0() / (1 + reds) or HI_FREQ
+            self.crval3 *= msconv
+            self.cdelt3 *= msconv
+        else:
+            raise GalmodError("GALMOD ERROR: unknown Spectral axis type. Check CTYPE3 and CUNIT3 keywords.")
+        nch = gm.SPECRES if gm.SPECRES != -1 else gm.LINEAR
+        if nch == -1:
+            nch = 1. / (2 * math.sqrt(2 * math.log(2)))
+        self.chwidth = abs(delta_vel(h)) * 1000
+        self.sig_instr = nch * self.chwidth
+        self.ltype = int(gm.LTYPE)
+        self.cmode = int(gm.CMODE) if 0 <= gm.CMODE <= 2 else 1
+        self.cdens = float(gm.CDENS) if gm.CDENS >= 0 else 1.
+        self.iseed = gm.ISEED
+        self.empty = gm.EMPTY
+        self.radmaxa = float(gm.RADMAXA)
+        self.radmaxr = float(gm.RADMAXR)
+        self.densIsFlux = gm.DENSFLUX if densflux is None else densflux
+        self.nvpar = gm.NV
+        self.galmodtype = (galmodtype or gm.GALMODTYPE).upper()
+        self.radsep = 0.75 * min(self.pixsize)
+        if self.densIsFlux:
+            self.cd2i = np.full(self.nsubs, np.float32(3600. / self.chwidth * 1000.), dtype=np.float64)
+        else:
+            self.cd2i = self._nhi_to_rad()
+        self.lfreq0 = np.ascontiguousarray([self.freq0] if lfreq0 is None else lfreq0, dtype=np.float64)
+        self.relint = np.ascontiguousarray(self.relint, dtype=np.float64)
+        self.vedges = self._velocity_edges()
+        self.vedges_lines = np.ascontiguousarray([self._velocity_edges(f) for f in self.lfreq0])
+        self.chmin = float(np.min(np.abs(np.diff(self.vedges)))) if self.nsubs > 0 else self.chwidth
+
+    @staticmethod
+    def _relint(relint, nlines):
+        if len(relint) != nlines:
+            raise GalmodError(f"GALMOD ERROR: {nlines} lines, but RELINT has {len(relint)} values.")
+        return np.array(relint, float)
+
+    def _nhi_to_rad(self):
+        """The channel factor of column-density models (galmod.py:727-741)."""
+        jy, K = 1E-26, 1.3806488E-23
+        isubs = np.arange(self.nsubs)
+        w = self.crval3 + self.cdelt3 * (isubs + 1 - self.crpix3)
+        if self.axtyp == 2:
+            lab, fac = w, 2.588E04
+        elif self.axtyp == 3:
+            lab, fac = C_MS / w, 3.8475E-06
+        else:
+            f = self.freq0 * (1 - w / C_MS)
+            if self.velsys == 2: f = self.freq0 * (C_MS / (C_MS + w))
+            if self.velsys == 3: f = self.freq0 * np.sqrt((C_MS - w) / (C_MS + w))
+            lab, fac = C_MS / f, 1.823E-05
+        ddum = 2 * (K / jy) / (fac * lab * lab) / abs(self.cdelt3)
+        return (ddum * (math.pi / (180.0 * 60.0)) ** 2).astype(np.float32).astype(np.float64)
+
+    def _velocity_edges(self, freq0=None):
+        """Velocity (m/s) of the channel boundaries, the inverse of the channel grid at k - 1/2
+        (galmod.py:743-758)."""
+        g = np.arange(self.nsubs + 1) - 0.5
+        w = (g + 1 - self.crpix3) * self.cdelt3 + self.crval3
+        if self.axtyp == 4:
+            return np.ascontiguousarray(w, dtype=np.float64)
+        f = C_MS / w if self.axtyp == 2 else w
+        r = f / (self.freq0 if freq0 is None else freq0)
+        if self.velsys == 1:
+            v = C_MS * (1 - r)
+        elif self.velsys == 2:
+            v = C_MS * (1 / r - 1)
+        else:
+            v = C_MS * (1 - r * r) / (1 + r * r)
+        return np.ascontiguousarray(v, dtype=np.float64)
+
+    # ------------------------------------------------------------------
+    def side_edges(self, rings):
+        """(edgeapp, edgerec) in arcsec, None for a half without an edge (galmod.py:853-863)."""
+        out = []
+        for attr, default in (("edgeapp", self.radmaxa), ("edgerec", self.radmaxr)):
+            e = getattr(rings, attr, None)
+            e = default if e is None else float(e)
+            out.append(e if e >= 0 else None)
+        return tuple(out)
+
+    def subrings(self, rings):
+        """Sub-rings 0.75 pixel wide interpolated from the rings (row A3, galmod.py:761-851).
+        Returns (sr (16, nsub) float64, nv (nsub,) int64)."""
+        nr = rings.nr
+        radsep = self.radsep
+        rad = np.asarray(rings.radii, float) / 60.
+        for i in range(1, nr):
+            if rad[i - 1] + radsep >= rad[i]:
+                if rad[i - 1] > rad[i]:
+                    raise GalmodError("GALMOD ERROR: Radii not in increasing order.")
+                raise GalmodError("GALMOD ERROR: Radius separation too small.")
+        if np.any(np.asarray(rings.vdisp) < 0):
+            raise GalmodError("GALMOD ERROR: Negative velocity dispersion not allowed.")
+        if np.any(np.asarray(rings.dens) < 0):
+            raise GalmodError("GALMOD ERROR: Negative column-density or flux are not allowed.")
+        if np.any(np.asarray(rings.z0) < 0):
+            raise GalmodError("GALMOD ERROR: Negative scale height not allowed.")
+        thick = rings.thick()
+        if thick and np.any((np.asarray(rings.fthick) < 0) | (np.asarray(rings.fthick) > 1)):
+            raise GalmodError("GALMOD ERROR: The fraction of gas in the thick layer (FTHICK) must be 0 to 1.")
+        V = ring_values(rings, thick)
+        empty = self.empty if rings.empty is None else rings.empty
+        idx, dr, const, rads = subring_radii(rad, radsep, empty)
+        nsub = len(rads)
+        sr = np.empty((V.shape[0] + 1, nsub))
+        sr[0] = rads
+        if nsub:
+            sr[1:] = interp_values(V, rad, idx, dr, const)
+        return self._finish_subrings(sr, rings, thick, nsub)
+
+    def _finish_subrings(self, sr, rings, thick, nsub):
+        radsep = self.radsep
+        edge = getattr(rings, "edge", None)
+        if edge is not None and nsub:
+            sr[5] *= edge_weights(sr[0] * 60., radsep * 60., float(edge), float(rings.edgewidth or 0.))
+        if thick:
+            f = sr[14]
+            thin, thk = sr[:14].copy(), sr[:14].copy()
+            thin[5] *= 1. - f
+            thin[7] = 0.
+            thk[5] *= f
+            thk[6], thk[4] = sr[15], sr[16]
+            sr = np.concatenate([thin, thk], axis=1)
+        side = np.ones((2, sr.shape[1]))
+        for k, e in enumerate(self.side_edges(rings)[::-1]):        # (receding, approaching)
+            if e is not None and sr.shape[1]:
+                side[k] = edge_weights(sr[0] * 60., radsep * 60., e)
+        sr = np.ascontiguousarray(np.concatenate([sr[:14], side]))
+        nvtmp = self.nvpar if self.nvpar >= 1 else self.nsubs
+        nv = np.where(sr[4] == 0, 1, nvtmp).astype(np.int64)
+        return sr, nv
+
+    def subcloud_flux(self, sr, nv):
+        """Mean flux (before cd2i) of one MC sub-cloud over the sub-rings (galmod.py:879-897)."""
+        R0, dens = sr[0], sr[5]
+        area_pix = 2 * np.pi * R0 * self.radsep / self.pixarea
+        with np.errstate(divide="ignore", invalid="ignore"):
+            nc = np.floor(self.cdens * dens ** self.cmode * area_pix + 0.5)
+            low = np.floor(nc / area_pix + 0.5) < 10.
+            nc = np.where(low, np.floor(10 * area_pix + 0.5), nc)
+            w = dens * area_pix
+            ok = (w > 0) & (nc > 0)
+            if not np.any(ok):
+                return 0.
+            fsc = w[ok] * self.pixarea / (nc[ok] * nv[ok])
+            unit = w[ok].sum() / np.sum(w[ok] / fsc)
+        ri = self.relint
+        return float(unit * np.sum(ri * ri) / np.sum(ri))
+
+
+def ring_values(rings, thick):
+    """The ring values in internal units, rows 1..13 of the sub-ring table (+3 rows for a thick layer:
+    fthick, z0thick, vdispthick), galmod.py:793-804."""
+    V = np.vstack([np.asarray(rings.vrot) * 1000., np.asarray(rings.vrad) * 1000.,
+                   np.asarray(rings.vvert) * 1000., np.asarray(rings.vdisp) * 1000.,
+                   np.asarray(rings.dens, float), np.asarray(rings.z0) / 60.,
+                   np.asarray(rings.dvdz) * 1000 * 60, np.asarray(rings.zcyl) / 60.,
+                   np.radians(np.asarray(rings.inc, float)), np.radians(np.asarray(rings.phi, float)),
+                   np.asarray(rings.xpos, float), np.asarray(rings.ypos, float),
+                   np.asarray(rings.vsys) * 1000.]).astype(float)
+    if thick:
+        vdt = np.asarray(rings.vdispthick, float)
+        vdt = np.where(vdt >= 0, vdt, np.asarray(rings.vdisp, float))
+        V = np.vstack([V, np.asarray(rings.fthick, float), rings.thick_z0() / 60., vdt * 1000.])
+    return V
+
+
+def subring_radii(rad, radsep, empty):
+    """Sub-ring radii (arcmin) accumulated in float32 as the C++ GALMOD (galmod.py:805-819), with the
+    interpolation segment of each: idx (ring index i of the segment [i-1, i]), dr (distance from the
+    float32 radius of ring i-1), const (True: inner disk with the values of ring 0). Depends on the radii
+    only, so it is cached per fit when the radii are fixed."""
+    f32 = np.float32
+    rads, idx, drs, const = [], [], [], []
+    inner = not (rad[0] != 0 and empty)
+    r = f32(radsep / 2.0) if inner else f32(rad[0] + radsep / 2.0)
+    if inner:
+        r0 = f32(rad[0])
+        while r < r0:
+            rads.append(float(r)); idx.append(0); drs.append(0.); const.append(True)
+            r = f32(float(r) + radsep)
+    for i in range(1, len(rad)):
+        ri = f32(rad[i])
+        base = float(f32(rad[i - 1]))
+        while r < ri:
+            rads.append(float(r)); idx.append(i); drs.append(float(r) - base); const.append(False)
+            r = f32(float(r) + radsep)
+    return (np.asarray(idx, np.int64), np.asarray(drs, float), np.asarray(const, bool),
+            np.asarray(rads, float))
+
+
+def interp_values(V, rad, idx, dr, const):
+    """Linear interpolation of the ring values V (nrow, nr) at the sub-rings (galmod.py:822-832);
+    V may carry a leading batch axis (B, nrow, nr)."""
+    nr = len(rad)
+    nsub = len(idx)
+    i1 = np.maximum(idx, 1)
+    if nr > 1:
+        dur = rad[i1] - rad[i1 - 1]
+        lo = V[..., i1 - 1]
+        slope = (V[..., i1] - lo) / dur
+        vals = lo + slope * dr
+    else:
+        vals = V[..., np.zeros(nsub, np.int64)] * 1.0
+    if np.any(const):
+        vals[..., const] = V[..., [0]]
+    return vals
+
+
+# ----------------------------------------------------------------------------------------------
+# Ring conventions (row A2)
+# ----------------------------------------------------------------------------------------------
+RINGMODES = ("CENTRED", "NODES", "SHIFTED")
+
+
+def ringmode_name(mode):
+    m = str(mode).strip().upper()
+    m = "CENTRED" if m == "CENTERED" else m
+    if m not in RINGMODES:
+        raise ValueError(f"Unknown ring convention {mode!r}: RINGMODE can be CENTRED, NODES or SHIFTED.")
+    return m
+
+
+def half_widths(rings):
+    """Half widths of the first and last ring of CENTRED (galmod.py:1055-1063)."""
+    R = rings.radii
+    if rings.nr == 1:
+        w = rings.radsep / 2. if rings.radsep > 0 else R[0] / 2.
+        return w, w
+    return (R[1] - R[0]) / 2., (R[-1] - R[-2]) / 2.
+
+
+def model_rings(rings, mode="CENTRED", subring=0.):
+    """The rings that build the model of a set of rings with a ring convention (galmod.py:1066-1119):
+    CENTRED adds the inner and outer half rings with the values of the first and last ring; NODES keeps
+    the rings; SHIFTED moves the first and last radius by half of RADSEP."""
+    mode = ringmode_name(mode)
+    if mode == "NODES" and rings.nr == 1:
+        raise GalmodError("RINGMODE NODES needs two radii at least.")
+    if mode == "SHIFTED" and rings.nr == 1 and not rings.radsep > 0:
+        raise GalmodError("RINGMODE SHIFTED with one ring needs RADSEP > 0.")
+    if mode == "NODES" or rings.nr == 0:
+        return rings
+    r = rings.copy()
+    if mode == "SHIFTED":
+        if r.nr == 1:
+            vals = {k: getattr(r, k)[0] for k in r.as_dict()}
+            vals["radii"] = r.radii[0] + r.radsep / 2.
+            r.addRing(**vals)
+        else:
+            r.radii[-1] += r.radsep / 2.
+        r.radii[0] = max(r.radii[0] - r.radsep / 2., 0.)
+        return r
+    R = r.radii
+    w0, wn = half_widths(r)
+    rin, rout = max(R[0] - w0, 0.), R[-1] + wn
+    idx, rad = list(range(r.nr)), list(R)
+    if rin / 60. + subring < R[0] / 60.:
+        idx.insert(0, 0)
+        rad.insert(0, rin)
+    if R[-1] / 60. + subring < rout / 60.:
+        idx.append(r.nr - 1)
+        rad.append(rout)
+    out = r.subset(np.array(idx))
+    out.radii = np.array(rad, dtype=float)
+    return out
+
+
+def edge_weights(R, dr, edge, width=0.):
+    """Density weights of sub-rings at radii R (arcsec, dr wide) for a disk edge at `edge` arcsec
+    (galmod.py:1122-1131)."""
+    R = np.asarray(R, float)
+    if width > 0:
+        return 0.5 * _erfc((R - edge) / (math.sqrt(2.) * width))
+    return np.clip((edge - (R - dr / 2.)) / dr, 0., 1.) if dr > 0 else (R <= edge).astype(float)
+
+
+def outer_edge(rings, mode="CENTRED"):
+    """Outer radius (arcsec) of the disk of the rings (galmod.py:1134-1141)."""
+    mode = ringmode_name(mode)
+    if mode == "NODES":
+        return float(rings.radii[-1])
+    if mode == "SHIFTED":
+        return float(rings.radii[-1] + rings.radsep / 2.)
+    return float(rings.radii[-1] + half_widths(rings)[1])
+
+
+def model_box(h, rings, ringmode="CENTRED", final=False):
+    """Box [blo, bhi) of the cube (pixels) that holds the model of the rings (galmod.py:1224-1240)."""
+    from .header import arcsconv
+    conv = arcsconv(h.Cunit(0))
+    ext = rings.radii[-1] + rings.radsep / 2.
+    if final:
+        ext = max(ext, outer_edge(rings, ringmode))
+    ext = ext + 3 * float(rings.layer_height()[-1])
+    xdis = int(math.ceil(ext / (abs(h.Cdelt(0)) * conv)))
+    ydis = int(math.ceil(ext / (abs(h.Cdelt(1)) * conv)))
+    xpos = int(math.ceil(rings.xpos[-1]))
+    ypos = int(math.ceil(rings.ypos[-1]))
+    blo = [max(xpos - xdis, 0), max(ypos - ydis, 0)]
+    bhi = [min(xpos + xdis + 1, h.DimAx(0)), min(ypos + ydis + 1, h.DimAx(1))]
+    return blo, bhi
+
+
+def zstep_for(h, setup, smoothed):
+    """The projected spacing of the ANALYTIC vertical samples (galmod.py:1186-1198)."""
+    if not smoothed:
+        return ZSTEP_FINE
+    pix = max(setup.pixsize) / 60.
+    bmin = min(abs(h.Bmaj()), abs(h.Bmin()))
+    return ZSTEP_COARSE if pix > 0 and bmin / pix >= ZSTEP_BEAM else ZSTEP_FINE
+
+
+def nzq_for(sr):
+    """Vertical-sample capacity of the ANALYTIC kernels: 1 for thin disks, else ZSAMPLES_MAX
+    (galmod.py:909)."""
+    return 1 if not np.any(sr[6] > 0) else ZSAMPLES_MAX
+
+
+class SubringPlan:
+    """The part of GalmodSetup.subrings that depends only on the radii, for batches of rings that share
+    them (the radii are not free in most fits): the model rings of the ring convention (model_rings), the
+    float32-accumulated sub-ring radii and their interpolation segments. tables(V) then gives the sub-ring
+    tables of a batch of ring values with numpy only (no Python loop per parameter vector).
+
+    rings: a Rings giving radii, radsep, empty, edges and the layer switch of the batch."""
+
+    def __init__(self, setup, rings, ringmode="CENTRED", final=True):
+        self.setup = setup
+        self.thick = rings.thick()
+        probe = rings.copy()
+        probe.dens = np.arange(rings.nr, dtype=float)       # (to find which input ring each model ring copies)
+        mr = model_rings(probe, ringmode, setup.radsep) if final else probe
+        self.ring_index = np.asarray(mr.dens, np.int64)
+        mr = model_rings(rings, ringmode, setup.radsep) if final else rings
+        self.nr_in = rings.nr
+        rad = np.asarray(mr.radii, float) / 60.
+        for i in range(1, mr.nr):
+            if rad[i - 1] + setup.radsep >= rad[i]:
+                raise GalmodError("GALMOD ERROR: Radii not in increasing order." if rad[i - 1] > rad[i]
+                                  else "GALMOD ERROR: Radius separation too small.")
+        self.rad = rad
+        empty = setup.empty if mr.empty is None else mr.empty
+        self.idx, self.dr, self.const, self.rads = subring_radii(rad, setup.radsep, empty)
+        self.nsub = len(self.rads)
+        self.proto = mr
+
+    def tables(self, values):
+        """values: dict of ring arrays (B, nr) (or (nr,)) in the reference's units for the keys of
+        ring_values (missing keys: those of the rings the plan was built with). Returns sr (B, 16, nsub') and
+        nv (nsub',), nsub' = 2 nsub with a thick layer."""
+        B = None
+        for v in values.values():
+            a = np.asarray(v)
+            if a.ndim == 2:
+                B = a.shape[0]
+        B = 1 if B is None else B
+        p = self.proto
+        ri = self.ring_index
+        nm = len(ri)
+
+        def get(k):
+            """(B, nr_model) values of k: given per input ring (mapped to the model rings), or the plan's."""
+            if k in values:
+                a = np.asarray(values[k], float)
+                if a.ndim == 0:
+                    a = np.full(self.nr_in, float(a))
+                if a.ndim == 1:
+                    a = a[None]
+                return np.broadcast_to(a[:, ri], (B, nm))
+            return np.broadcast_to(np.asarray(getattr(p, k), float)[None], (B, nm))
+        rows = [get("vrot") * 1000., get("vrad") * 1000., get("vvert") * 1000., get("vdisp") * 1000.,
+                get("dens"), get("z0") / 60., get("dvdz") * 1000 * 60, get("zcyl") / 60.,
+                np.radians(get("inc")), np.radians(get("phi")), get("xpos"), get("ypos"), get("vsys") * 1000.]
+        if self.thick:
+            z0t = get("z0thick")
+            z0t = np.where(z0t >= 0, z0t, get("z0"))
+            vdt = get("vdispthick")
+            vdt = np.where(vdt >= 0, vdt, get("vdisp"))
+            rows += [get("fthick"), z0t / 60., vdt * 1000.]
+        V = np.stack(rows, axis=1)                                  # (B, nrow, nr_model)
+        sr = np.empty((B, V.shape[1] + 1, self.nsub))
+        sr[:, 0] = self.rads
+        if self.nsub:
+            sr[:, 1:] = interp_values(V, self.rad, self.idx, self.dr, self.const)
+        out = []
+        nv = None
+        for b in range(B) if (self.thick or self._edges()) else ():
+            s, nv = self.setup._finish_subrings(sr[b], p, self.thick, self.nsub)
+            out.append(s)
+        if out:
+            return np.ascontiguousarray(np.stack(out)), nv
+        full = np.empty((B, 16, self.nsub))
+        full[:, :14] = sr[:, :14]
+        full[:, 14:] = 1.
+        nvtmp = self.setup.nvpar if self.setup.nvpar >= 1 else self.setup.nsubs
+        nv = np.where(full[0, 4] == 0, 1, nvtmp).astype(np.int64)
+        return full, nv
+
+    def _edges(self):
+        p = self.proto
+        return getattr(p, "edge", None) is not None or any(e is not None for e in self.setup.side_edges(p))

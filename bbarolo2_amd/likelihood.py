@@ -1,0 +1,193 @@
+"""The Gaussian log-likelihood of Bayesian3DFIT (GLOBAL / BAYESIAN), evaluated for batches of parameter
+vectors on the GPU.
+
+Mirrors LikelihoodMixin._log_likelihood (bbarolo/bayesian/likelihood.py:567-619) for the default Gaussian
+residuals (_calculate_residuals / _residuals_fracerr, likelihood.py:457-531):
+
+    -logL = likefactor * [ 0.5 (S_box + ss_out) / sigma^2 + 0.5 nvox ln(2 pi sigma^2) ]      (+ fracerr terms)
+
+with S_box the masked residual sum inside the model box, ss_out the squared data of the likelihood voxels of
+the region outside the box (a summed-area table, likelihood.py:437-440) and nvox the number of likelihood
+voxels of the region. theta -> rings follows _update_rings (bayesian/fitter.py:1109-1136): the free values
+are stored as float32 in the ring table (bayesian/rings.py:54-61), then used as float64.
+
+The setup that the reference does once per fit (likelihood mask, region, noise) is taken as given: the
+caller passes the likelihood mask, the region and sigma (see synth.likelihood_inputs for the conventions
+of the synthetic configurations).
+"""
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from .engine import _torch, _stream_ptr
+from .galmod import SubringPlan
+from .rings import Rings
+
+# ring parameters that make the likelihood zero when negative (likelihood.py:532-535)
+POSITIVE = ('vrot', 'vdisp', 'inc', 'xpos', 'ypos', 'z0', 'dens', 'zcyl', 'z0thick', 'fthick', 'vdispthick')
+
+
+class FreePars:
+    """The free parameters of a fit: an ordered list of (ring key, 'ring' | 'single'); 'ring' takes nr
+    values of theta (one per ring), 'single' one value for all the rings (the FREE keys VROT and
+    INC_SINGLE of the reference, bayesian/fitter.py:434-473)."""
+
+    def __init__(self, spec, nr):
+        self.spec = [(k.lower(), m) for k, m in spec]
+        self.nr = nr
+        self.index = {}
+        i = 0
+        for k, m in self.spec:
+            n = nr if m == "ring" else 1
+            self.index[k] = np.arange(i, i + n)
+            i += n
+        self.ndim = i
+
+    def values(self, theta):
+        """dict key -> (B, nr) float64 ring values of the free keys, rounded to float32 as the ring table
+        of the reference stores them."""
+        theta = np.atleast_2d(np.asarray(theta, float))
+        B = theta.shape[0]
+        out = {}
+        for k, m in self.spec:
+            v = theta[:, self.index[k]]
+            if m != "ring":
+                v = np.repeat(v, self.nr, axis=1)
+            out[k] = v.astype(np.float32).astype(np.float64)
+        return out
+
+    def negative(self, theta):
+        """True where a free positive parameter is negative (likelihood.py:537-559)."""
+        theta = np.atleast_2d(np.asarray(theta, float))
+        bad = np.zeros(theta.shape[0], bool)
+        for k, _ in self.spec:
+            if k in POSITIVE:
+                bad |= np.any(theta[:, self.index[k]].astype(np.float32) < 0, axis=1)
+        return bad
+
+
+def float32_rings(rings):
+    """The rings as the ring table of Bayesian3DFIT holds them: every value rounded to float32, radsep
+    float32(R[-1] - R[-2]) (bayesian/rings.py:54-76)."""
+    r = rings.copy()
+    for k, v in r.as_dict().items():
+        setattr(r, k, np.asarray(v, np.float32).astype(np.float64))
+    if r.nr > 1:
+        r.radsep = float(np.float32(r.radii[-1] - r.radii[-2]))
+    return r
+
+
+class GpuLikelihood:
+    """Batched log-likelihood of a cube for the free parameters `free` around the rings `rings0`.
+
+    dm: engine.DiskModel of the cube (ANALYTIC). data (nz, ny, nx) float32. likemask (nz, ny, nx) bool
+    (maskmode 2: only those voxels; None: all). region (ny, nx) bool (None: all). likebox ((blo), (bhi)):
+    the likelihood region's box. sigma: noise per voxel. likefactor: 1 / correlation volume.
+    fracerr: fixed fractional model error (None: 0)."""
+
+    def __init__(self, dm, data, rings0, free, likemask=None, region=None, likebox=None, sigma=1.0,
+                 likefactor=1.0, fracerr=None, maskmode=2, chunk=256):
+        torch = _torch()
+        self.torch = torch
+        self.dm = dm
+        self.free = free
+        self.rings0 = float32_rings(rings0)
+        self.sigma = float(sigma)
+        self.likefactor = float(likefactor)
+        self.eps = float(fracerr) if fracerr else 0.
+        self.chunk = int(chunk)
+        nz, ny, nx = data.shape
+        if likebox is None:
+            likebox = ((0, 0), (nx, ny))
+        Lb, Hb = likebox
+        if region is None:
+            region = np.zeros((ny, nx), bool)
+            region[Lb[1]:Hb[1], Lb[0]:Hb[0]] = True
+        blank = ~np.isfinite(data) | (data == 0)
+        lm = np.ones(data.shape, bool) if likemask is None else np.asarray(likemask, bool)
+        if maskmode == 2:
+            use = lm & ~blank & region[None]
+            dused = np.where(use, data, np.nan)
+        elif maskmode == 1:
+            use = ~blank & region[None]
+            dused = np.where(use, np.where(lm, data, 0.), np.nan)
+        else:
+            use = ~blank & region[None]
+            dused = np.where(use, data, np.nan)
+        self.nvox = int(use.sum())
+        d2 = np.where(use, np.nan_to_num(dused), 0.).astype(np.float64) ** 2
+        dm2 = d2.sum(axis=0)
+        self._sat = np.zeros((ny + 1, nx + 1))
+        self._sat[1:, 1:] = np.cumsum(np.cumsum(dm2, axis=0), axis=1)
+        # the model box (fixed: the centre, heights and radii of the batch are those of rings0)
+        for k in ("xpos", "ypos", "z0", "z0thick", "radii"):
+            if k in free.index:
+                raise NotImplementedError(f"free {k.upper()} moves the model box; not supported by this engine yet")
+        blo, bhi = dm.box(self.rings0, final=True)
+        blo = [max(blo[0], Lb[0]), max(blo[1], Lb[1])]
+        bhi = [max(min(bhi[0], Hb[0]), blo[0]), max(min(bhi[1], Hb[1]), blo[1])]
+        self.blo, self.bhi = blo, bhi
+        S = self._sat
+        inbox = S[bhi[1], bhi[0]] - S[blo[1], bhi[0]] - S[bhi[1], blo[0]] + S[blo[1], blo[0]]
+        self.ss_out = max(float(S[-1, -1] - inbox), 0.)
+        # the data of the box in the layout of the models (yxz), NaN where not summed
+        dbox = dused[:, blo[1]:bhi[1], blo[0]:bhi[0]].transpose(1, 2, 0)
+        self.dev = torch.device("cuda", torch.cuda.current_device())
+        self.dvalid = torch.from_numpy(np.ascontiguousarray(dbox, np.float32)).to(self.dev)
+        self.nvox_box = int(np.prod(dbox.shape))
+        self.plan = SubringPlan(dm.setup, self.rings0, dm.ringmode, final=True)
+        self._bufs = {}
+
+    def _buffers(self, B):
+        b = self._bufs.get(B)
+        if b is None:
+            torch = self.torch
+            by, bx, nz = self.bhi[1] - self.blo[1], self.bhi[0] - self.blo[0], self.dm.setup.nsubs
+            L = _lib.lib()
+            wl = int(L.bb_residual_work_size(B, self.nvox_box))
+            b = dict(dep=torch.empty((B, by, bx, nz), dtype=torch.float32, device=self.dev),
+                     mod=torch.empty((B, by, bx, nz), dtype=torch.float32, device=self.dev),
+                     work=torch.empty(wl, dtype=torch.float64, device=self.dev),
+                     ss=torch.empty(B, dtype=torch.float64, device=self.dev),
+                     lg=torch.empty(B, dtype=torch.float64, device=self.dev), wl=wl)
+            self._bufs = {B: b}
+        return b
+
+    def tables(self, theta):
+        vals = self.free.values(theta)
+        sr, _ = self.plan.tables(vals)
+        return sr
+
+    def residuals(self, sr):
+        """(S_box, lg) per table: device launches for one chunk of sub-ring tables (B, 16, nsub)."""
+        torch = self.torch
+        B = sr.shape[0]
+        bufs = self._buffers(B)
+        mod = self.dm.array_batch(sr, self.blo, self.bhi, convolve=True, out=bufs["mod"], work=bufs["dep"])
+        st = _stream_ptr(torch)
+        s2 = self.sigma ** 2
+        _lib.check(_lib.lib().bb_residual_gauss(mod.data_ptr(), self.dvalid.data_ptr(), B, self.nvox_box, s2,
+                                                self.eps ** 2, bufs["work"].data_ptr(), bufs["wl"],
+                                                bufs["ss"].data_ptr(), bufs["lg"].data_ptr(), st))
+        return bufs["ss"].cpu().numpy().copy(), bufs["lg"].cpu().numpy().copy()
+
+    def loglike(self, theta):
+        """log-likelihood of each row of theta (B, ndim) -> (B,) float64 (-inf where a positive parameter
+        is negative)."""
+        theta = np.atleast_2d(np.asarray(theta, float))
+        out = np.full(theta.shape[0], -np.inf)
+        ok = ~self.free.negative(theta)
+        idx = np.flatnonzero(ok)
+        s2 = self.sigma ** 2
+        for c in range(0, len(idx), self.chunk):
+            ii = idx[c:c + self.chunk]
+            ss, lg = self.residuals(self.tables(theta[ii]))
+            if self.eps > 0:
+                nll = 0.5 * (ss + self.ss_out / s2) + 0.5 * self.nvox * math.log(2 * math.pi * s2) + 0.5 * lg
+            else:
+                nll = 0.5 * (ss + self.ss_out) / s2 + 0.5 * self.nvox * math.log(2 * math.pi * s2)
+            out[ii] = -self.likefactor * nll
+        return out

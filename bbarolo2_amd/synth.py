@@ -1,0 +1,132 @@
+"""Synthetic cubes of the benchmark configurations, and the likelihood inputs of a fit on them.
+
+Headers follow emptyfits (bbarolo/utils.py:36-102); truth cubes are GALMOD models with NORM NONE (as
+SimulatedGalaxyCube.run, utils.py:519-535); noise follows DiskModel.addnoise (galmod.py:1365-1377): numpy
+default_rng(seed) normal noise, convolved with the beam and rescaled to the rms.
+
+Likelihood inputs (the once-per-fit setup of LikelihoodMixin, bbarolo/bayesian/likelihood.py:83-440), with
+these simplifications, stated once here:
+  * the emission mask is the truth model above 2 noise rms (the reference's default is MASK SEARCH);
+  * the likelihood mask dilates it by one beam FWHM in each channel and +-3 channels (dilate_mask,
+    bayesian/noise.py:18-30, the default LIKEOPTS dilation (1, 3));
+  * sigma is 1.4826 x the MAD of the data outside the emission mask (likelihood.py:107-114);
+  * likefactor is 1 / beam area in pixels (LIKEOPTS noisecorr='beam');
+  * the region is the box of the model of the initial rings (DiskModel.box, final=True).
+"""
+
+import math
+
+import numpy as np
+
+from .header import emptyheader, beam_area_pix
+from .rings import Rings
+from .galmod import GalmodPar
+
+
+def config(n):
+    """(header, truth Rings, GalmodPar, noise fraction of the peak, noise seed, free spec, description) of
+    configuration n (1, 2, 3, 5)."""
+    if n == 1:
+        h = emptyheader([128, 128, 64], [-10 / 3600., 10 / 3600., 5.], beam=30 / 3600., obj="CFG1")
+        R = np.arange(15., 346., 30.)
+        r = Rings().setRings(radii=R, xpos=64., ypos=64., vsys=0., vrot=150 * (2 / np.pi) * np.arctan(R / 60.),
+                             vdisp=10., z0=10., dens=1., inc=60., phi=120.)
+        par = GalmodPar(LTYPE=2, GALMODTYPE="MC")
+        return h, r, par, 0.05, 42, [("vrot", "ring"), ("vdisp", "ring")], "128x128x64 CLASSIC"
+    if n == 2:
+        h = emptyheader([160, 160, 96], [-8 / 3600., 8 / 3600., 5.2], beam=30 / 3600., obj="CFG2")
+        R = np.arange(15., 1186., 30.)
+        r = Rings().setRings(radii=R, xpos=80., ypos=80., vsys=0., vrot=130 * (1 - np.exp(-R / 200.)), vdisp=8.,
+                             z0=10., dens=np.exp(-R / 400.), inc=60., phi=124.)
+        par = GalmodPar(GALMODTYPE="ANALYTIC")
+        return (h, r, par, 0.03, 7, [("vrot", "ring"), ("vdisp", "ring"), ("inc", "single"), ("phi", "single")],
+                "160x160x96 GLOBAL")
+    if n == 3:
+        h = emptyheader([128, 128, 64], [-10 / 3600., 10 / 3600., 10.], beam=40 / 3600., obj="CFG3")
+        R = np.arange(20., 301., 40.)
+        r = Rings().setRings(radii=R, xpos=64., ypos=64., vsys=0., vrot=200 * (2 / np.pi) * np.arctan(R / 60.),
+                             vdisp=10., z0=10., dens=1., inc=60., phi=45.)
+        par = GalmodPar(GALMODTYPE="ANALYTIC")
+        return h, r, par, 0.05, 3, [("vrot", "ring"), ("vdisp", "ring")], "128x128x64 BAYESIAN 16-D"
+    if n == 5:
+        h = emptyheader([64, 64, 40], [-10 / 3600., 10 / 3600., 20.], beam=(40 / 3600., 30 / 3600., 30.), obj="CFG5")
+        R = np.arange(10., 111., 20.)
+        r = Rings().setRings(radii=R, xpos=32., ypos=32., vsys=0., vrot=200 * (2 / np.pi) * np.arctan(R / 30.),
+                             vdisp=40., z0=0., dens=1., inc=50., phi=60.)
+        par = GalmodPar(GALMODTYPE="ANALYTIC")
+        return (h, r, par, 0.05, 5, [("vrot", "ring"), ("vdisp", "ring"), ("inc", "single")],
+                "64x64x40 BAYESIAN 13-D surrogate")
+    raise ValueError(f"no configuration {n}")
+
+
+def theta_of(rings, free):
+    """The parameter vector of the rings for the free parameters (FreePars)."""
+    th = np.zeros(free.ndim)
+    for k, m in free.spec:
+        v = np.asarray(getattr(rings, k), float)
+        th[free.index[k]] = v if m == "ring" else v[0]
+    return th
+
+
+def default_priors(rings, free):
+    """(lo, hi) of the default uniform priors (bayesian/fitter.py:355-372): VROT 0 - 2 max VROT, VDISP 0 -
+    max(2 max VDISP, 50), INC +-10 deg within 0-90, PA +-15 deg."""
+    lo, hi = np.zeros(free.ndim), np.zeros(free.ndim)
+    for k, m in free.spec:
+        v = np.asarray(getattr(rings, k), float)
+        if k == "vrot":
+            a, b = 0., 2. * v.max() if v.max() > 0 else 350.
+        elif k == "vdisp":
+            a, b = 0., max(2. * v.max(), 50.)
+        elif k == "inc":
+            a, b = max(v.min() - 10., 0.), min(v.max() + 10., 90.)
+        elif k == "phi":
+            a, b = v.min() - 15., v.max() + 15.
+        else:
+            raise ValueError(k)
+        lo[free.index[k]], hi[free.index[k]] = a, b
+    return lo, hi
+
+
+def beam_smooth(a, kern):
+    """'Same' convolution of every plane of a (nz, ny, nx) with the kernel (host, setup only)."""
+    from scipy.signal import fftconvolve
+    return fftconvolve(a, kern[None], mode="same", axes=(1, 2))
+
+
+def add_noise(model_zyx, rms, kern, seed):
+    """model + Gaussian noise of the given rms, smoothed to the beam and rescaled (galmod.py:1365-1377)."""
+    rng = np.random.default_rng(seed)
+    noise = rng.normal(0., rms, model_zyx.shape)
+    if kern is not None:
+        noise = beam_smooth(noise, kern)
+        noise *= rms / np.std(noise)
+    return (model_zyx + noise).astype(np.float32)
+
+
+def dilate_mask(mask, radius, nchan):
+    """A disk of the given radius (pixels) in each channel, then +-nchan channels (bayesian/noise.py:18-30)."""
+    from scipy.ndimage import distance_transform_edt, maximum_filter1d
+    m = np.asarray(mask, bool)
+    out = m.copy()
+    if radius > 0:
+        for k in np.flatnonzero(m.any(axis=(1, 2))):
+            out[k] = distance_transform_edt(~m[k]) <= radius
+    if nchan > 0:
+        out = maximum_filter1d(out.view(np.uint8), size=2 * int(nchan) + 1, axis=0, mode="constant",
+                               cval=0).astype(bool)
+    return out
+
+
+def likelihood_inputs(h, truth_zyx, data, noise_rms, box):
+    """(likemask, region, sigma, likefactor) for a synthetic cube (see the module docstring)."""
+    mask = truth_zyx > 2. * noise_rms
+    bpix = abs(h.Bmaj() / h.Cdelt(0)) if h.Bmaj() else 1.
+    likemask = dilate_mask(mask, bpix, 3)
+    out = data[~mask & (data != 0)]
+    sigma = float(1.4826 * np.median(np.abs(out - np.median(out)))) if out.size else float(noise_rms)
+    nz, ny, nx = data.shape
+    region = np.zeros((ny, nx), bool)
+    (blo, bhi) = box
+    region[blo[1]:bhi[1], blo[0]:bhi[0]] = True
+    return likemask, region, sigma, 1. / max(beam_area_pix(h), 1.)

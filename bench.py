@@ -1,0 +1,204 @@
+"""Benchmark: likelihood evaluations per second of the 160x160x96 GLOBAL configuration on one GPU.
+
+One step = one batch of B parameter vectors (free VROT, VDISP per ring, INC, PA: 82-D) evaluated by the
+model-cube likelihood (ANALYTIC GALMOD -> beam convolution -> masked Gaussian residual), inputs resident on
+the device. Prints one JSON line (see --help).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM = 8.0e12          # MI355X HBM3E, bytes/s
+
+
+def workload(n_theta, seed=1):
+    """The configuration-2 cube (truth built with the oracle on the host, noise 3% of the peak, seed 7), its
+    likelihood inputs and n_theta parameter vectors drawn from the default priors (default_rng(seed))."""
+    import oracle
+    from bbarolo2_amd.synth import config, default_priors, add_noise, likelihood_inputs, theta_of
+    from bbarolo2_amd.galmod import GalmodSetup, model_rings, nzq_for, PHI_TAB, model_box, zstep_for
+    from bbarolo2_amd.smooth import beam_kernel
+    from bbarolo2_amd.likelihood import FreePars, float32_rings
+    h, truth, par, noise_frac, noise_seed, spec, desc = config(2)
+    st = GalmodSetup(h, par)
+    kern, _ = beam_kernel(h)
+    st.zstep = zstep_for(h, st, True)
+    r32 = float32_rings(truth)
+    sr, _ = st.subrings(model_rings(r32, "CENTRED", st.radsep))
+    nx, ny = h.DimAx(0), h.DimAx(1)
+    mod = oracle.conv_same(oracle.galmod_analytic(st, sr, (0, 0), (nx, ny), nzq_for(sr), PHI_TAB), kern)
+    model_zyx = mod.transpose(2, 0, 1)
+    rms = noise_frac * model_zyx.max()
+    data = add_noise(model_zyx, rms, kern, noise_seed)
+    box = model_box(h, r32, "CENTRED", final=True)
+    likemask, region, sigma, likefactor = likelihood_inputs(h, model_zyx, data, rms, box)
+    free = FreePars(spec, truth.nr)
+    lo, hi = default_priors(truth, free)
+    u = np.random.default_rng(seed).random((n_theta, free.ndim))
+    theta = lo + u * (hi - lo)
+    return dict(h=h, truth=truth, par=par, data=data, likemask=likemask, region=region, sigma=sigma,
+                likefactor=likefactor, free=free, theta=theta, box=box, kern=kern, desc=desc,
+                theta_true=theta_of(truth, free))
+
+
+_W = {}
+
+
+def _cpu_worker_init(wl):
+    _W.update(wl)
+
+
+def _cpu_eval(i):
+    """One likelihood evaluation with the FP64 oracle (model, beam, residual); returns its wall time."""
+    import oracle
+    from bbarolo2_amd.galmod import GalmodSetup, SubringPlan, nzq_for, PHI_TAB, zstep_for
+    from bbarolo2_amd.likelihood import float32_rings
+    w = _W
+    if "st" not in w:
+        st = GalmodSetup(w["h"], w["par"])
+        st.zstep = zstep_for(w["h"], st, True)
+        w["st"] = st
+        w["plan"] = SubringPlan(st, float32_rings(w["truth"]), "CENTRED")
+    st, plan = w["st"], w["plan"]
+    t0 = time.perf_counter()
+    sr, _ = plan.tables(w["free"].values(w["theta"][i % len(w["theta"])][None]))
+    blo, bhi = w["box"]
+    m = oracle.galmod_analytic(st, sr[0], blo, bhi, nzq_for(sr[0]), PHI_TAB)
+    m = oracle.conv_fft(m, w["kern"])
+    d = w["data"][:, blo[1]:bhi[1], blo[0]:bhi[0]]
+    mk = w["likemask"][:, blo[1]:bhi[1], blo[0]:bhi[0]]
+    rg = w["region"][blo[1]:bhi[1], blo[0]:bhi[0]]
+    oracle.sumsq_residuals(np.ascontiguousarray(m.transpose(2, 0, 1)), d, mk, 2, rg)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(wl, per_worker=2):
+    """The oracle timed on the host cores: P workers (fork pool, one evaluation per core as the reference's
+    sampler pools), each doing 1 untimed + per_worker timed evaluations; evals/s = P / median time."""
+    import multiprocessing as mp
+    P = max(1, min(16, os.cpu_count() or 1))
+    ctx = mp.get_context("fork")
+    small = {k: wl[k] for k in ("h", "truth", "par", "data", "likemask", "region", "free", "theta", "box", "kern")}
+    with ctx.Pool(P, initializer=_cpu_worker_init, initargs=(small,)) as pool:
+        pool.map(_cpu_eval, range(P), chunksize=1)
+        t0 = time.perf_counter()
+        times = pool.map(_cpu_eval, range(P * per_worker), chunksize=per_worker)
+        wall = time.perf_counter() - t0
+    med = float(np.median(times))
+    return dict(value=P / med, unit="evals/s", cores=P, kind="port",
+                sample=f"{P * per_worker} FP64 oracle likelihood evaluations (ANALYTIC model, beam, residual) "
+                       f"of the 160x160x96 workload on {P} worker processes, 1 untimed each; median "
+                       f"{med * 1e3:.0f} ms/eval/core, wall {wall:.1f} s; single core {1 / med:.2f} evals/s",
+                single_core=1 / med)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (the likelihood batches "
+              "shard across GPUs by parameter vectors; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    B = a.batch
+    wl = workload(max(200, B))
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(wl)
+        print(f"[bench] cpu baseline {cpu['value']:.2f} evals/s on {cpu['cores']} cores", file=sys.stderr, flush=True)
+
+    import torch
+    from bbarolo2_amd import _lib
+    from bbarolo2_amd.engine import DiskModel, _stream_ptr
+    from bbarolo2_amd.likelihood import GpuLikelihood
+    torch.cuda.set_device(0)
+    dm = DiskModel(wl["h"], wl["par"], galmodtype="ANALYTIC")
+    like = GpuLikelihood(dm, wl["data"], wl["truth"], wl["free"], likemask=wl["likemask"], region=wl["region"],
+                         likebox=wl["box"], sigma=wl["sigma"], likefactor=wl["likefactor"], chunk=B)
+    theta = np.resize(wl["theta"], (B, wl["free"].ndim))
+    sr = like.tables(theta)
+    d_sr = torch.from_numpy(np.ascontiguousarray(sr)).cuda()
+    bufs = like._buffers(B)
+    blo, bhi = like.blo, like.bhi
+    by, bx, nz = bhi[1] - blo[1], bhi[0] - blo[0], dm.setup.nsubs
+    nvox = by * bx * nz
+    L = _lib.lib()
+    h = dm.handle(1 if not np.any(sr[:, 6] > 0) else 1024)
+    from bbarolo2_amd.engine import naz_of, AN_THREADS
+    chunks = int(max(1, math.ceil(int(naz_of(sr[0, 0], dm.setup.pixarea).max()) / AN_THREADS)))
+    box = _lib.bb_box(blo[0], blo[1], bhi[0], bhi[1])
+    s2 = like.sigma ** 2
+    stream = torch.cuda.current_stream()
+    sp = _stream_ptr(torch)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    stage = np.zeros(3)
+
+    def step(timed):
+        bufs["dep"].zero_()
+        if timed:
+            ev[0].record(stream)
+        _lib.check(L.bb_galmod_analytic(h, box, d_sr.data_ptr(), sr.shape[2], B, chunks, bufs["dep"].data_ptr(), sp))
+        if timed:
+            ev[1].record(stream)
+        _lib.check(L.bb_conv2d_same(h, bufs["dep"].data_ptr(), bufs["mod"].data_ptr(), B, by, bx, sp))
+        if timed:
+            ev[2].record(stream)
+        _lib.check(L.bb_residual_gauss(bufs["mod"].data_ptr(), like.dvalid.data_ptr(), B, nvox, s2, 0.0,
+                                       bufs["work"].data_ptr(), bufs["wl"], bufs["ss"].data_ptr(),
+                                       bufs["lg"].data_ptr(), sp))
+        if timed:
+            ev[3].record(stream)
+
+    for _ in range(a.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step(True)
+        ev[3].synchronize()
+        stage += [ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), ev[2].elapsed_time(ev[3])]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stage /= a.steps
+    ms = dt * 1e3 / a.steps
+    value = B * a.steps / dt
+    # parity of this batch's log-likelihoods with the host path (the same launches through GpuLikelihood)
+    ll = like.loglike(theta[:4])
+    names = ["galmod_analytic", "conv2d_same", "residual"]
+    bytes_per_eval = {"galmod_analytic": 4 * nvox, "conv2d_same": 8 * nvox, "residual": 8 * nvox}
+    dom = names[int(np.argmax(stage))]
+    ach = bytes_per_eval[dom] * B / (stage[names.index(dom)] * 1e-3)
+    out = {
+        "metric": "likelihood evals/s (160x160x96 cube)", "value": value, "unit": "evals/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (f64 geometry and sums)", "data": "synthetic",
+        "config": {"workload": "GLOBAL likelihood, 160x160x96, 8\" pixels, 5.2 km/s, beam 30\", 40 rings, "
+                               "ANALYTIC, free VROT VDISP INC PA (82-D), DENS fixed", "batch": B,
+                   "model_box": [int(blo[0]), int(blo[1]), int(bhi[0]), int(bhi[1])]},
+        "stages_ms_per_step": dict(zip(names, [round(float(x), 4) for x in stage])),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": ach / PEAK_HBM, "traffic": None},
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "loglike_sample": [float(x) for x in ll],
+    }
+    if cpu is not None:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,85 @@
+/*
+ * bbgpu -- the C-ABI of the MI355X model-cube likelihood (GALMOD -> beam convolution -> residual).
+ *
+ * The reference (BBarolo 2, pure Python + numba) has no FFI: its hot path sits behind numba kernels
+ * called with positional numpy arrays. Each entry point below replaces one of those kernels (cited
+ * file:line, paths relative to the reference repository) or fuses a chain of them, with the same
+ * argument meaning:
+ *   - device pointers (allocated by the caller, e.g. torch tensors), sizes and a HIP stream (void*,
+ *     NULL = the default stream); no torch types cross this boundary;
+ *   - model outputs are accumulated with += into caller-zeroed buffers, as the numba kernels do;
+ *   - every entry point returns 0 on success and a non-zero code on error; bb_last_error() gives the
+ *     message of the last error of the calling thread. Invalid arguments are rejected before any launch.
+ * Launches are asynchronous on the given stream; the caller synchronises.
+ */
+#ifndef BBGPU_H
+#define BBGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BB_MAX_LINES 8
+
+/* Header-derived constants of the models of one cube (GalmodSetup, bbarolo/galmod.py:613-726), in the
+ * reference's internal units: arcmin, m/s, radians. chmin is the narrowest channel (galmod.py:726), which
+ * the ANALYTIC kernels use as the channel width (galmod.py:909-914). */
+typedef struct bb_grid {
+    int32_t nz;                  /* channels */
+    int32_t ltype;               /* vertical layer: 1 gauss, 2 sech2, 3 exp, 4 lorentz, other box */
+    int32_t axtyp;               /* 2 wavelength, 3 frequency, 4 velocity */
+    int32_t velsys;              /* 1 radio, 2 optical, 3 relativistic */
+    int32_t nlines;              /* emission lines (<= BB_MAX_LINES) */
+    int32_t nzq;                 /* vertical-sample capacity of the ANALYTIC kernels (1 or 1024) */
+    int32_t cmode;               /* MC cloud-number exponent */
+    int32_t pad0;
+    double radsep, cdens, pixarea, cdelt0, cdelt1, crota2;
+    double sig_instr, crval3, cdelt3, crpix3, chmin, zstep;
+    double relint[BB_MAX_LINES];
+    double lfreq0[BB_MAX_LINES];
+} bb_grid;
+
+/* A box [blo, bhi) of the cube in pixels (DiskModel.box, galmod.py:1224-1240). */
+typedef struct bb_box {
+    int32_t blo0, blo1, bhi0, bhi1;
+} bb_box;
+
+const char* bb_last_error(void);
+int bb_device_count(int* n);
+
+/* The per-fit model object: the header-derived setup of GalmodSetup (bbarolo/galmod.py:609-758) and the beam
+ * kernel of DiskModel (galmod.py:1199-1214, FFTConvolver(kernel) of smooth.py:107-143), copied to the current
+ * device. cd2i (nz), vedges (nlines, nz+1), ptab (ntab) and kern (ky, kx) are HOST float64 arrays; kern may be
+ * NULL (no smoothing). scale: the flux scaling after the convolution (1 for DENSFLUX models). */
+typedef struct bb_model bb_model;
+int bb_model_create(const bb_grid* g, const double* cd2i, const double* vedges, const double* ptab, int ntab,
+                    const double* kern, int ky, int kx, double scale, bb_model** out);
+int bb_model_destroy(bb_model* m);
+
+/* ANALYTIC GALMOD (_galmod_analytic, bbarolo/galmod.py:326-465) of a batch of B sub-ring tables that share
+ * their radii: sr (B, 16, nsub) float64, out (B, by, bx, nz) float32 yxz, accumulated with += (zero it first).
+ * chunks: blocks per (sub-ring, radial sample); ceil(max naz / 256) gives one azimuth per thread, any value
+ * >= 1 is correct. */
+int bb_galmod_analytic(const bb_model* m, bb_box box, const double* sr, int nsub, int B, int chunks, float* out,
+                       void* stream);
+
+/* 'Same' 2D beam convolution of every channel (FFTConvolver.__call__, bbarolo/smooth.py:145-165; _conv_sparse,
+ * galmod.py:974-1037), values below 1e-12 set to zero, then the scale applied (DiskModel.convolve,
+ * galmod.py:1273-1283). in/out (B, by, bx, nz) float32 yxz, distinct buffers. */
+int bb_conv2d_same(const bb_model* m, const float* in, float* out, int B, int by, int bx, void* stream);
+
+/* Masked Gaussian residual sums (_sumsq_residuals / _residuals_fracerr, bbarolo/bayesian/likelihood.py:22-77)
+ * of B models (B, nvox) float32 against dvalid (nvox) float32 in the same voxel order: the data of the voxels
+ * that enter the reference's sum (region, likelihood mask and blanks applied), NaN for the others.
+ * eps2 <= 0: out_ss[b] = sum (d - m)^2 (out_lg may be NULL). eps2 > 0: out_ss[b] = sum r^2 / v and
+ * out_lg[b] = sum_{m != 0} log(v / sigma2), v = sigma2 + eps2 m^2 (sigma2 where m = 0).
+ * work: device float64 scratch of bb_residual_work_size(B, nvox) elements. Results are FP64, deterministic. */
+int64_t bb_residual_work_size(int B, int64_t nvox);
+int bb_residual_gauss(const float* model, const float* dvalid, int B, int64_t nvox, double sigma2, double eps2,
+                      double* work, int64_t work_len, double* out_ss, double* out_lg, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

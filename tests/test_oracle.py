@@ -1,0 +1,162 @@
+"""Pins of the oracle (CPU): the FP64 C restatement checked against the invariants and closed forms of the
+reference's tests, so that a plausible mistake (a dropped term, a wrong sign or index) fails here."""
+import math
+
+import numpy as np
+import pytest
+from scipy.special import erf
+from scipy.signal import fftconvolve
+
+import oracle
+from bbarolo2_amd.galmod import GalmodSetup, GalmodPar, model_rings, nzq_for, PHI_TAB, model_box
+from bbarolo2_amd.header import emptyheader
+from bbarolo2_amd.rings import Rings
+from bbarolo2_amd.smooth import beam_kernel
+from conftest import small_rings
+
+
+def analytic(setup, rings, blo=None, bhi=None, final=True, mode="CENTRED"):
+    if final:
+        rings = model_rings(rings, mode, setup.radsep)
+    sr, _ = setup.subrings(rings)
+    if blo is None:
+        blo, bhi = (0, 0), (setup.dimx, setup.dimy)
+    return oracle.galmod_analytic(setup, sr, blo, bhi, nzq_for(sr), PHI_TAB), sr
+
+
+def test_phi_table_closed_form():
+    t = np.linspace(-5.9, 5.9, 1001)
+    x = (t + 6.0) * 512.0
+    k = x.astype(int)
+    lin = PHI_TAB[k] + (x - k) * (PHI_TAB[k + 1] - PHI_TAB[k])
+    assert np.abs(lin - 0.5 * (1 + erf(t / math.sqrt(2)))).max() < 5e-7
+
+
+def test_analytic_total_flux(small_header):
+    """Total flux of a disk inside the cube with lines inside the band: sum over sub-rings of
+    dens x ring area (pixels) x pixel area x cd2i (each sub-ring's samples carry its whole flux)."""
+    st = GalmodSetup(small_header)
+    st.zstep = 0.25
+    r = small_rings(nr=5, vrot=100., dens=np.linspace(2., 1., 5))
+    mod, sr = analytic(st, r)
+    expect = np.sum(sr[5] * 2 * np.pi * sr[0] * st.radsep) * st.cd2i[0]
+    assert abs(mod.sum() / expect - 1) < 1e-6
+
+
+def test_analytic_vertical_profile_erf():
+    """Edge-on Gaussian layer vs the erf profile within 1% of the peak (test_bbarolo.py:5063-5075)."""
+    h = emptyheader([128, 128, 40], [-1 / 3600., 1 / 3600., 10.], beam=3 / 3600.)
+    st = GalmodSetup(h, GalmodPar(LTYPE=1))
+    st.zstep = 0.25
+    r = Rings().setRings(radii=[10., 50.], xpos=63.5, ypos=63.5, vsys=0., vrot=150., vdisp=10., z0=5., dens=1.,
+                         inc=90., phi=0.)
+    mod, _ = analytic(st, r, final=False)
+    prof = mod.sum(axis=2)[60:68].sum(axis=0)
+    x = np.arange(128) - 63.5
+    g = erf((x + 0.5) / (5 * np.sqrt(2))) - erf((x - 0.5) / (5 * np.sqrt(2)))
+    assert np.abs(prof / prof.max() - g / g.max()).max() < 0.01
+
+
+@pytest.mark.parametrize("ltype", [1, 4])
+def test_analytic_box_is_crop(ltype):
+    """A box model is the crop of the full model (test_bbarolo.py:5076-5088): <= 1e-9 of the peak."""
+    h = emptyheader([128, 128, 240], [-10 / 3600., 10 / 3600., 5.], beam=30 / 3600.)
+    st = GalmodSetup(h, GalmodPar(LTYPE=ltype))
+    st.zstep = 0.25
+    r = Rings().setRings(radii=[50., 70.], radsep=20., xpos=63.5, ypos=63.5, vsys=0., vrot=150., vdisp=8., z0=10.,
+                         dens=1., inc=70., phi=45.)
+    blo, bhi = model_box(h, r)
+    full, _ = analytic(st, r, final=False)
+    box, _ = analytic(st, r, blo, bhi, final=False)
+    assert np.abs(box - full[blo[1]:bhi[1], blo[0]:bhi[0]]).max() < 1e-9 * full.max()
+
+
+def _weights(st, rings, mode="CENTRED"):
+    """Interpolation weights of the ring densities in the sub-rings (DiskModel._ring_weights,
+    galmod.py:1285-1314), from unit-density rings."""
+    cols = []
+    for k in range(rings.nr):
+        rr = rings.copy()
+        rr.dens = np.zeros(rings.nr)
+        rr.dens[k] = 1.
+        cols.append(st.subrings(model_rings(rr, mode, st.radsep))[0][5])
+    W = np.array(cols).T
+    widx = np.full((W.shape[0], 2), -1, np.int64)
+    wval = np.zeros((W.shape[0], 2))
+    for s in range(W.shape[0]):
+        nz = np.flatnonzero(W[s] != 0)
+        assert len(nz) <= 2
+        widx[s, :len(nz)] = nz
+        wval[s, :len(nz)] = W[s, nz]
+    return widx, wval
+
+
+@pytest.mark.parametrize("z0,dvdz", [(0., 0.), (10., 0.), (15., 0.5)])
+def test_ring_maps_linearity(small_header, z0, dvdz):
+    """The total intensity map of an ANALYTIC model is sum_r dens_r K_r (test_bbarolo.py:90-104, 5089-5093),
+    before the convolution and the channel factor: <= 1e-6 of the peak."""
+    st = GalmodSetup(small_header)
+    st.zstep = 0.25
+    r = small_rings(z0=z0, dvdz=dvdz, dens=np.linspace(3., 0.5, 8), inc=np.linspace(50., 70., 8))
+    blo, bhi = (5, 8), (60, 58)
+    mod, _ = analytic(st, r, blo, bhi)
+    mr = model_rings(r, "CENTRED", st.radsep).copy()
+    mr.dens = np.ones(mr.nr)
+    sr, _ = st.subrings(mr)
+    widx, wval = _weights(st, r)
+    K = oracle.ring_maps_analytic(st, sr, widx, wval, r.nr, blo, bhi, nzq_for(sr))
+    mom0 = mod.sum(axis=2)
+    assert np.abs(np.tensordot(r.dens, K, axes=1) * st.cd2i[0] - mom0).max() < 1e-6 * mom0.max()
+
+
+def test_conv_same_definitions(small_header):
+    """orc_conv_same = the direct-sum 'same' convolution (brute force on a tiny cube), = the FFT one, and
+    conserves the flux of a model away from the edges (test_bbarolo.py:240-248)."""
+    k, _ = beam_kernel(small_header)
+    rng = np.random.default_rng(1)
+    a = rng.random((9, 11, 3))
+    out = oracle.conv_same(a, k)
+    ky, kx = k.shape
+    bf = np.zeros_like(a)
+    for y in range(9):
+        for x in range(11):
+            for j in range(ky):
+                for i in range(kx):
+                    yy, xx = y + ky // 2 - j, x + kx // 2 - i
+                    if 0 <= yy < 9 and 0 <= xx < 11:
+                        bf[y, x] += k[j, i] * a[yy, xx]
+    assert np.abs(out - bf).max() < 1e-14
+    kr = rng.random((5, 7))
+    big = rng.random((30, 40, 4))
+    o2 = oracle.conv_same(big, kr, scale=2.5)
+    ff = fftconvolve(big, kr[:, :, None], mode="same")
+    assert np.abs(o2 - 2.5 * ff).max() < 1e-10
+    pt = np.zeros((40, 40, 2))
+    pt[20, 20, 1] = 3.
+    assert abs(oracle.conv_same(pt, k).sum() - 3.) < 1e-12
+    tiny = np.full((5, 5, 1), 1e-13)
+    assert oracle.conv_same(tiny, k).max() == 0.0          # below the 1e-12 threshold
+    # the FFT path of the CPU baseline (FFTConvolver) gives the direct sum
+    big[:, :, 2] = 0.                                     # (an empty plane: skipped by the FFT path)
+    o3 = oracle.conv_same(big, kr, scale=2.5)
+    assert np.abs(oracle.conv_fft(big, kr, 2.5) - o3).max() < 1e-12 * o3.max()
+
+
+def test_residuals_closed_form():
+    """_sumsq_residuals and _residuals_fracerr on hand-built cubes: blanks (0, NaN) skipped, maskmode 2 keeps only
+    the mask, maskmode 1 zeroes the data outside it, region restricts the pixels."""
+    d = np.array([[[1., 0., 2.], [np.nan, 3., 4.]]], np.float32)          # (1, 2, 3)
+    m = np.array([[[0.5, 7., 1.], [9., 1., 1.]]])
+    mask = np.array([[[1, 1, 0], [1, 1, 1]]], bool)
+    region = np.array([[1, 1, 1], [1, 0, 1]], bool)
+    # maskmode 0: voxels (0,0),(0,2),(1,1),(1,2)
+    assert oracle.sumsq_residuals(m, d, None) == pytest.approx(0.25 + 1 + 4 + 9)
+    # maskmode 2: (0,2) out of the mask
+    assert oracle.sumsq_residuals(m, d, mask, 2) == pytest.approx(0.25 + 4 + 9)
+    # maskmode 1: (0,2) counts with d = 0
+    assert oracle.sumsq_residuals(m, d, mask, 1) == pytest.approx(0.25 + 1 + 4 + 9)
+    # region drops (1,1)
+    assert oracle.sumsq_residuals(m, d, mask, 2, region) == pytest.approx(0.25 + 9)
+    s, lg = oracle.residuals_fracerr(m, d, mask, 2, region, 2.0, 0.25)
+    v1, v2 = 2 + 0.25 * 0.25, 2 + 0.25 * 1
+    assert s == pytest.approx(0.25 / v1 + 9 / v2) and lg == pytest.approx(math.log(v1 / 2) + math.log(v2 / 2))
