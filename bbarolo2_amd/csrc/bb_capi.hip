@@ -10,8 +10,13 @@
 #include "../../include/bbgpu.h"
 
 extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const double* sr, int nsub, int B,
-                                         int chunks, const double* cd2i, const double* vedges, const double* ptab,
-                                         int ntab, float* out, hipStream_t st);
+                                         const int* naz, const int4* chunks, int nchunks, const double2* az,
+                                         int TX, int TY, double* sub, double* wtab, short4* bbox, const double* cd2i,
+                                         const double* vedges, const double* ptab, int ntab, float* out,
+                                         int accumulate, hipStream_t st);
+extern "C" size_t bb_analytic_sub_bytes();
+extern "C" size_t bb_analytic_wt_bytes();
+extern "C" int bb_analytic_chunk();
 extern "C" int bb_launch_conv2d_same(const float* in, float* out, int B, int by, int bx, int nz, const float* kern,
                                      const int* rowspan, int ky, int kx, float scale, hipStream_t st);
 extern "C" int bb_residual_blocks(int64_t nvox);
@@ -47,6 +52,20 @@ struct bb_model {
     float* d_kern;
     int* d_span;
     double scale;
+    int TX, TY;        // pixel tile of the ANALYTIC deposit (its spectra fit the LDS budget)
+};
+
+// The azimuth plan of a set of sub-ring radii (ANALYTIC samples): naz per sub-ring, the chunks of BB_CS
+// azimuths of every (sub-ring, radial sample) and the cosine / sine of every azimuth, built on the host with
+// the reference's recurrence (galmod.py:372-388) so that the device rounds pixels as the reference does.
+struct bb_azplan {
+    int device;
+    int nsub;
+    int nchunks;
+    std::vector<double> R0;
+    int* d_naz;
+    int4* d_chunks;
+    double2* d_az;
 };
 
 extern "C" const char* bb_last_error(void) { return g_err; }
@@ -84,14 +103,14 @@ static int upload(T** dst, const T* src, size_t n, const char* what) {
 extern "C" int bb_model_destroy(bb_model* m) {
     if (!m) return 0;
     int dev = 0;
-    hipGetDevice(&dev);
-    hipSetDevice(m->device);
-    hipFree(m->d_cd2i);
-    hipFree(m->d_vedges);
-    hipFree(m->d_ptab);
-    hipFree(m->d_kern);
-    hipFree(m->d_span);
-    hipSetDevice(dev);
+    (void)hipGetDevice(&dev);
+    (void)hipSetDevice(m->device);
+    (void)hipFree(m->d_cd2i);
+    (void)hipFree(m->d_vedges);
+    (void)hipFree(m->d_ptab);
+    (void)hipFree(m->d_kern);
+    (void)hipFree(m->d_span);
+    (void)hipSetDevice(dev);
     delete m;
     return 0;
 }
@@ -110,7 +129,7 @@ extern "C" int bb_model_create(const bb_grid* g, const double* cd2i, const doubl
     m->g = *g;
     m->ntab = ntab;
     m->scale = scale;
-    hipGetDevice(&m->device);
+    (void)hipGetDevice(&m->device);
     rc = upload(&m->d_cd2i, cd2i, (size_t)g->nz, "upload cd2i");
     if (!rc) rc = upload(&m->d_vedges, vedges, (size_t)g->nlines * (g->nz + 1), "upload vedges");
     if (!rc) rc = upload(&m->d_ptab, ptab, (size_t)ntab, "upload Phi table");
@@ -136,12 +155,97 @@ extern "C" int bb_model_create(const bb_grid* g, const double* cd2i, const doubl
         rc = upload(&m->d_kern, kf.data(), kf.size(), "upload kernel");
         if (!rc) rc = upload(&m->d_span, span.data(), span.size(), "upload kernel spans");
     }
+    if (!rc) {
+        // the largest pixel tile whose spectra (float32) fit 62 KB of LDS
+        static const int cand[][2] = {{16, 16}, {16, 8}, {8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
+        m->TX = 0;
+        for (auto& c : cand)
+            if ((size_t)c[0] * c[1] * g->nz * sizeof(float) <= 62 * 1024) {
+                m->TX = c[0];
+                m->TY = c[1];
+                break;
+            }
+        if (!m->TX) rc = fail(BB_ERR_ARG, "bb_model_create: nz = %d channels do not fit the LDS tile", g->nz);
+    }
     if (rc) {
         bb_model_destroy(m);
         return rc;
     }
     *out = m;
     return 0;
+}
+
+extern "C" int bb_azplan_destroy(bb_azplan* p) {
+    if (!p) return 0;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipSetDevice(p->device);
+    (void)hipFree(p->d_naz);
+    (void)hipFree(p->d_chunks);
+    (void)hipFree(p->d_az);
+    (void)hipSetDevice(dev);
+    delete p;
+    return 0;
+}
+
+extern "C" int bb_azplan_create(const bb_model* m, const double* R0, int nsub, bb_azplan** out) {
+    if (!out) return fail(BB_ERR_ARG, "bb_azplan_create: out is NULL");
+    *out = nullptr;
+    if (!m || (nsub > 0 && !R0) || nsub < 0) return fail(BB_ERR_ARG, "bb_azplan_create: bad arguments");
+    const int CS = bb_analytic_chunk();
+    std::vector<int> naz(nsub);
+    std::vector<int4> chunks;
+    std::vector<double2> az;
+    for (int s = 0; s < nsub; s++) {
+        if (!(R0[s] >= 0) || !(R0[s] < 1e6)) return fail(BB_ERR_ARG, "bb_azplan_create: radius %d = %g", s, R0[s]);
+        double circ_pix = 2 * M_PI * R0[s] / sqrt(m->g.pixarea);
+        long long n = (long long)(3 * circ_pix + 0.5);
+        if (n < 16) n = 16;
+        if (n > (1 << 24)) return fail(BB_ERR_ARG, "bb_azplan_create: %lld azimuths in sub-ring %d", n, s);
+        naz[s] = (int)n;
+        const double daz = 2 * M_PI / (double)n;
+        const double cd = cos(daz), sd = sin(daz);
+        for (int kr = 0; kr < 2; kr++) {
+            const int off = (int)az.size();
+            const double az0 = 2 * M_PI * (0.5 * ((kr + 1) % 2) + 0.25) / (double)n;
+            double caz = cos(az0), saz = sin(az0);
+            for (long long ka = 0; ka < n; ka++) {
+                if (ka > 0) {
+                    double c2 = caz * cd - saz * sd, s2 = saz * cd + caz * sd;
+                    caz = c2;
+                    saz = s2;
+                }
+                az.push_back(make_double2(caz, saz));
+            }
+            for (int ka0 = 0; ka0 < n; ka0 += CS)
+                chunks.push_back(make_int4(2 * s + kr, ka0, (int)((n - ka0) < CS ? (n - ka0) : CS), off + ka0));
+        }
+    }
+    if (az.size() > (size_t)INT32_MAX) return fail(BB_ERR_ARG, "bb_azplan_create: too many azimuth samples");
+    bb_azplan* p = new bb_azplan();
+    p->nsub = nsub;
+    p->nchunks = (int)chunks.size();
+    p->R0.assign(R0, R0 + nsub);
+    p->d_naz = nullptr;
+    p->d_chunks = nullptr;
+    p->d_az = nullptr;
+    (void)hipGetDevice(&p->device);
+    int rc = upload(&p->d_naz, naz.data(), naz.size(), "upload naz");
+    if (!rc) rc = upload(&p->d_chunks, chunks.data(), chunks.size(), "upload chunks");
+    if (!rc) rc = upload(&p->d_az, az.data(), az.size(), "upload azimuths");
+    if (rc) {
+        bb_azplan_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return 0;
+}
+
+extern "C" int64_t bb_analytic_work_size(const bb_azplan* p, int B) {
+    if (!p || B < 0) return -1;
+    size_t per = (size_t)p->nsub * (bb_analytic_sub_bytes() + bb_analytic_wt_bytes()) +
+                 (size_t)p->nchunks * sizeof(short4);
+    return (int64_t)(per * B + 256);
 }
 
 static int check_box(const bb_model* m, bb_box box) {
@@ -151,16 +255,31 @@ static int check_box(const bb_model* m, bb_box box) {
     return 0;
 }
 
-extern "C" int bb_galmod_analytic(const bb_model* m, bb_box box, const double* sr, int nsub, int B, int chunks,
-                                  float* out, void* stream) {
-    if (!m) return fail(BB_ERR_ARG, "bb_galmod_analytic: model is NULL");
+extern "C" int bb_galmod_analytic(const bb_model* m, const bb_azplan* p, bb_box box, const double* sr, int nsub,
+                                  int B, void* work, int64_t work_bytes, float* out, int accumulate, void* stream) {
+    if (!m || !p) return fail(BB_ERR_ARG, "bb_galmod_analytic: model or plan is NULL");
     int rc = check_box(m, box);
     if (rc) return rc;
-    if (nsub < 0 || B < 0 || chunks < 1 || chunks > 65535 || 2 * nsub > 65535 || B > 65535)
-        return fail(BB_ERR_ARG, "bb_galmod_analytic: nsub=%d B=%d chunks=%d out of range", nsub, B, chunks);
+    if (nsub != p->nsub) return fail(BB_ERR_ARG, "bb_galmod_analytic: nsub %d != plan's %d", nsub, p->nsub);
+    if (B < 0 || B > 65535) return fail(BB_ERR_ARG, "bb_galmod_analytic: B=%d out of range", B);
+    if (box.bhi0 - box.blo0 > 32767 || box.bhi1 - box.blo1 > 32767)
+        return fail(BB_ERR_ARG, "bb_galmod_analytic: box larger than 32767 pixels");
     if (nsub == 0 || B == 0) return 0;
-    if (!sr || !out) return fail(BB_ERR_ARG, "bb_galmod_analytic: NULL array");
-    rc = bb_launch_galmod_analytic(&m->g, box, sr, nsub, B, chunks, m->d_cd2i, m->d_vedges, m->d_ptab, m->ntab, out,
+    if (!sr || !out || !work) return fail(BB_ERR_ARG, "bb_galmod_analytic: NULL array");
+    if (work_bytes < bb_analytic_work_size(p, B))
+        return fail(BB_ERR_ARG, "bb_galmod_analytic: work holds %lld bytes, needs %lld", (long long)work_bytes,
+                    (long long)bb_analytic_work_size(p, B));
+    const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
+    const long long ntiles = (long long)((bx + m->TX - 1) / m->TX) * ((by + m->TY - 1) / m->TY);
+    if (ntiles > INT32_MAX) return fail(BB_ERR_ARG, "bb_galmod_analytic: too many tiles");
+    char* w = (char*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
+    double* sub = (double*)w;
+    w += (size_t)B * nsub * bb_analytic_sub_bytes();
+    double* wtab = (double*)w;
+    w += (size_t)B * nsub * bb_analytic_wt_bytes();
+    short4* bbox = (short4*)w;
+    rc = bb_launch_galmod_analytic(&m->g, box, sr, nsub, B, p->d_naz, p->d_chunks, p->nchunks, p->d_az, m->TX, m->TY,
+                                   sub, wtab, bbox, m->d_cd2i, m->d_vedges, m->d_ptab, m->ntab, out, accumulate,
                                    (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_galmod_analytic launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }

@@ -40,9 +40,9 @@ __device__ __forceinline__ double bb_lagged(double vrot, double dvdz, double dz)
 // Linear interpolation in the 1/512 table of Phi over [-6, 6] (galmod.py:183-191).
 __device__ __forceinline__ double bb_phi(double t, const double* __restrict__ tab, int ntab) {
     double x = (t + 6.0) * 512.0;
-    if (x <= 0.0) return 0.0;
+    if (!(x > 0.0)) return 0.0;                     // (also NaN: never an index outside the table)
+    if (!(x < (double)(ntab - 1))) return 1.0;
     int64_t k = (int64_t)x;
-    if (k >= ntab - 1) return 1.0;
     double a = tab[k];
     return a + (x - (double)k) * (tab[k + 1] - a);
 }

@@ -59,6 +59,7 @@ class DiskModel:
                 self.scalefactor = 1.0 if self.setup.densIsFlux else sc
         self.setup.zstep = zstep_for(header, self.setup, self.cfield is not None)
         self._handle = None
+        self._plans = None
         self._device = device
 
     # ------------------------------------------------------------------ device objects
@@ -105,12 +106,33 @@ class DiskModel:
             self._handle[key] = h
         return h
 
+    def azplan(self, R0, nzq=ZSAMPLES_MAX):
+        """The bb_azplan of the sub-ring radii R0 (arcmin) on the current device (cached by radii)."""
+        torch = _torch()
+        R0 = np.ascontiguousarray(R0, np.float64)
+        key = (torch.cuda.current_device(), int(nzq), R0.tobytes())
+        if self._plans is None:
+            self._plans = {}
+        p = self._plans.get(key)
+        if p is None:
+            h = self.handle(nzq)
+            out = C.c_void_p()
+            _lib.check(_lib.lib().bb_azplan_create(h, R0.ctypes.data, len(R0), C.byref(out)))
+            if len(self._plans) > 32:
+                for old in self._plans.values():
+                    _lib.lib().bb_azplan_destroy(old)
+                self._plans = {}
+            p = out
+            self._plans[key] = p
+        return p
+
     def __del__(self):
         try:
-            if self._handle:
-                L = _lib.lib()
-                for h in self._handle.values():
-                    L.bb_model_destroy(h)
+            L = _lib.lib()
+            for p in (self._plans or {}).values():
+                L.bb_azplan_destroy(p)
+            for h in (self._handle or {}).values():
+                L.bb_model_destroy(h)
         except Exception:
             pass
 
@@ -145,17 +167,25 @@ class DiskModel:
         B, nrow, nsub = sr.shape
         if nrow != 16:
             raise ValueError("sub-ring tables must have 16 rows")
+        if not np.all(np.isfinite(sr)):
+            raise ValueError("sub-ring tables hold NaN or infinite values")
         by, bx, nz = bhi[1] - blo[1], bhi[0] - blo[0], self.setup.nsubs
         dev = torch.device("cuda", torch.cuda.current_device())
         nzq = nzq_for(sr.reshape(16, -1))
         h = self.handle(nzq)
         st = _stream_ptr(torch, stream)
-        dep = torch.zeros((B, by, bx, nz), dtype=torch.float32, device=dev) if work is None else work.zero_()
+        dep = torch.empty((B, by, bx, nz), dtype=torch.float32, device=dev) if work is None else work
         if nsub:
             d_sr = torch.from_numpy(sr).to(dev, non_blocking=False)
-            chunks = int(max(1, math.ceil(int(naz_of(sr[0, 0], self.setup.pixarea).max()) / AN_THREADS)))
+            p = self.azplan(sr[0, 0], nzq)
+            L = _lib.lib()
+            wb = int(L.bb_analytic_work_size(p, B))
+            ws = torch.empty(wb, dtype=torch.uint8, device=dev)
             box = _lib.bb_box(blo[0], blo[1], bhi[0], bhi[1])
-            _lib.check(_lib.lib().bb_galmod_analytic(h, box, d_sr.data_ptr(), nsub, B, chunks, dep.data_ptr(), st))
+            _lib.check(L.bb_galmod_analytic(h, p, box, d_sr.data_ptr(), nsub, B, ws.data_ptr(), wb, dep.data_ptr(),
+                                            0, st))
+        else:
+            dep.zero_()
         if not convolve or self.cfield is None:
             return dep
         res = torch.empty_like(dep) if out is None else out

@@ -137,9 +137,11 @@ def main():
     by, bx, nz = bhi[1] - blo[1], bhi[0] - blo[0], dm.setup.nsubs
     nvox = by * bx * nz
     L = _lib.lib()
-    h = dm.handle(1 if not np.any(sr[:, 6] > 0) else 1024)
-    from bbarolo2_amd.engine import naz_of, AN_THREADS
-    chunks = int(max(1, math.ceil(int(naz_of(sr[0, 0], dm.setup.pixarea).max()) / AN_THREADS)))
+    nzq = 1 if not np.any(sr[:, 6] > 0) else 1024
+    h = dm.handle(nzq)
+    plan = dm.azplan(sr[0, 0], nzq)
+    wb = int(L.bb_analytic_work_size(plan, B))
+    ws = torch.empty(wb, dtype=torch.uint8, device="cuda")
     box = _lib.bb_box(blo[0], blo[1], bhi[0], bhi[1])
     s2 = like.sigma ** 2
     stream = torch.cuda.current_stream()
@@ -148,10 +150,10 @@ def main():
     stage = np.zeros(3)
 
     def step(timed):
-        bufs["dep"].zero_()
         if timed:
             ev[0].record(stream)
-        _lib.check(L.bb_galmod_analytic(h, box, d_sr.data_ptr(), sr.shape[2], B, chunks, bufs["dep"].data_ptr(), sp))
+        _lib.check(L.bb_galmod_analytic(h, plan, box, d_sr.data_ptr(), sr.shape[2], B, ws.data_ptr(), wb,
+                                        bufs["dep"].data_ptr(), 0, sp))
         if timed:
             ev[1].record(stream)
         _lib.check(L.bb_conv2d_same(h, bufs["dep"].data_ptr(), bufs["mod"].data_ptr(), B, by, bx, sp))

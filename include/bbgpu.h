@@ -57,12 +57,21 @@ int bb_model_create(const bb_grid* g, const double* cd2i, const double* vedges, 
                     const double* kern, int ky, int kx, double scale, bb_model** out);
 int bb_model_destroy(bb_model* m);
 
-/* ANALYTIC GALMOD (_galmod_analytic, bbarolo/galmod.py:326-465) of a batch of B sub-ring tables that share
- * their radii: sr (B, 16, nsub) float64, out (B, by, bx, nz) float32 yxz, accumulated with += (zero it first).
- * chunks: blocks per (sub-ring, radial sample); ceil(max naz / 256) gives one azimuth per thread, any value
- * >= 1 is correct. */
-int bb_galmod_analytic(const bb_model* m, bb_box box, const double* sr, int nsub, int B, int chunks, float* out,
-                       void* stream);
+/* The azimuth plan of a set of sub-ring radii R0 (arcmin, HOST array, nsub): the ANALYTIC samples of every
+ * sub-ring (naz = max(int(3 circumference / pixel + 0.5), 16), two radial samples, galmod.py:339-345) and the
+ * cosine and sine of every azimuth from the reference's rotation recurrence (galmod.py:372-388), computed on
+ * the host once per set of radii and copied to the current device. */
+typedef struct bb_azplan bb_azplan;
+int bb_azplan_create(const bb_model* m, const double* R0, int nsub, bb_azplan** out);
+int bb_azplan_destroy(bb_azplan* p);
+
+/* ANALYTIC GALMOD (_galmod_analytic, bbarolo/galmod.py:326-465) of a batch of B sub-ring tables whose radii are
+ * those of the plan: sr (B, 16, nsub) float64 (rows as GalmodSetup.subrings, galmod.py:761-851), out
+ * (B, by, bx, nz) float32 yxz over the box. accumulate = 0 writes the models, 1 adds them to out (the numba
+ * kernels accumulate with +=). work: device scratch of bb_analytic_work_size(plan, B) bytes. */
+int64_t bb_analytic_work_size(const bb_azplan* p, int B);
+int bb_galmod_analytic(const bb_model* m, const bb_azplan* p, bb_box box, const double* sr, int nsub, int B,
+                       void* work, int64_t work_bytes, float* out, int accumulate, void* stream);
 
 /* 'Same' 2D beam convolution of every channel (FFTConvolver.__call__, bbarolo/smooth.py:145-165; _conv_sparse,
  * galmod.py:974-1037), values below 1e-12 set to zero, then the scale applied (DiskModel.convolve,

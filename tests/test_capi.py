@@ -27,7 +27,9 @@ def test_argument_errors_before_device_calls():
     rc = L.bb_model_create(C.byref(g), None, None, None, 0, None, 0, 0, 1.0, C.byref(out))
     assert rc != 0 and b"nz" in L.bb_last_error() and not out.value
     # null model handles
-    assert L.bb_galmod_analytic(None, _lib.bb_box(0, 0, 4, 4), None, 1, 1, 1, None, None) != 0
+    assert L.bb_galmod_analytic(None, None, _lib.bb_box(0, 0, 4, 4), None, 1, 1, None, 0, None, 0, None) != 0
+    assert L.bb_azplan_create(None, None, 3, None) != 0
+    assert L.bb_analytic_work_size(None, 4) < 0
     assert L.bb_conv2d_same(None, None, None, 1, 4, 4, None) != 0
     # residual: too small a work buffer is rejected
     assert L.bb_residual_work_size(3, 10000) > 0
