@@ -11,14 +11,17 @@
 
 extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const double* sr, int nsub, int B,
                                          const int* naz, const int4* chunks, int nchunks, const double2* az,
-                                         int TX, int TY, double* sub, double* wtab, short4* bbox, const double* cd2i,
-                                         const double* vedges, const double* ptab, int ntab, float* out,
-                                         int accumulate, hipStream_t st);
+                                         int TX, int TY, double* sub, double* wtab, short4* bbox, short4* sbox,
+                                         const double* cd2i, const double* vedges, const double* ptab, int ntab,
+                                         float* out, int accumulate, hipStream_t st);
 extern "C" size_t bb_analytic_sub_bytes();
 extern "C" size_t bb_analytic_wt_bytes();
 extern "C" int bb_analytic_chunk();
+extern "C" size_t bb_analytic_lds_bytes(int nz, int nlines, int TX, int TY, int ntab);
+extern "C" size_t bb_analytic_lds_static();
 extern "C" int bb_launch_conv2d_same(const float* in, float* out, int B, int by, int bx, int nz, const float* kern,
-                                     const int* rowspan, int ky, int kx, float scale, hipStream_t st);
+                                     int ky, int kx, float scale, hipStream_t st);
+extern "C" int bb_conv_max_kx();
 extern "C" int bb_residual_blocks(int64_t nvox);
 extern "C" int bb_launch_residual(const float* model, const float* dvalid, int B, int64_t nvox, double sigma2,
                                   double eps2, double* partial, double* out_ss, double* out_lg, hipStream_t st);
@@ -35,6 +38,7 @@ static int fail(int code, const char* fmt, ...) {
 
 #define BB_ERR_ARG 1
 #define BB_ERR_HIP 2
+#define BB_LDS_MAX (160 * 1024)   // LDS of a gfx950 CU, all usable by one workgroup
 
 static int hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) return fail(BB_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
@@ -50,7 +54,6 @@ struct bb_model {
     double* d_ptab;
     int ky, kx;
     float* d_kern;
-    int* d_span;
     double scale;
     int TX, TY;        // pixel tile of the ANALYTIC deposit (its spectra fit the LDS budget)
 };
@@ -109,7 +112,6 @@ extern "C" int bb_model_destroy(bb_model* m) {
     (void)hipFree(m->d_vedges);
     (void)hipFree(m->d_ptab);
     (void)hipFree(m->d_kern);
-    (void)hipFree(m->d_span);
     (void)hipSetDevice(dev);
     delete m;
     return 0;
@@ -122,8 +124,9 @@ extern "C" int bb_model_create(const bb_grid* g, const double* cd2i, const doubl
     int rc = check_grid(g);
     if (rc) return rc;
     if (!cd2i || !vedges || !ptab || ntab < 2) return fail(BB_ERR_ARG, "bb_model_create: missing setup arrays");
-    if (kern && (ky < 1 || kx < 1 || ky % 2 == 0 || kx % 2 == 0 || ky > 511 || ky * kx > 4096))
-        return fail(BB_ERR_ARG, "bb_model_create: kernel %dx%d must be odd-sized, ky <= 511, <= 4096 taps", ky, kx);
+    if (kern && (ky < 1 || kx < 1 || ky % 2 == 0 || kx % 2 == 0 || ky > 511 || kx > bb_conv_max_kx()))
+        return fail(BB_ERR_ARG, "bb_model_create: kernel %dx%d must be odd-sized with kx <= %d", ky, kx,
+                    bb_conv_max_kx());
     bb_model* m = new bb_model();
     memset(m, 0, sizeof *m);
     m->g = *g;
@@ -134,33 +137,18 @@ extern "C" int bb_model_create(const bb_grid* g, const double* cd2i, const doubl
     if (!rc) rc = upload(&m->d_vedges, vedges, (size_t)g->nlines * (g->nz + 1), "upload vedges");
     if (!rc) rc = upload(&m->d_ptab, ptab, (size_t)ntab, "upload Phi table");
     if (!rc && kern) {
-        // float32 taps and the non-zero span [ilo, ihi] of every kernel row (ihi < ilo: empty row)
         std::vector<float> kf((size_t)ky * kx);
-        std::vector<int> span(2 * (size_t)ky);
-        for (int j = 0; j < ky; j++) {
-            int lo = kx, hi = -1;
-            for (int i = 0; i < kx; i++) {
-                double v = kern[(size_t)j * kx + i];
-                kf[(size_t)j * kx + i] = (float)v;
-                if (v != 0) {
-                    if (lo == kx) lo = i;
-                    hi = i;
-                }
-            }
-            span[2 * j] = lo;
-            span[2 * j + 1] = hi;
-        }
+        for (size_t i = 0; i < kf.size(); i++) kf[i] = (float)kern[i];
         m->ky = ky;
         m->kx = kx;
         rc = upload(&m->d_kern, kf.data(), kf.size(), "upload kernel");
-        if (!rc) rc = upload(&m->d_span, span.data(), span.size(), "upload kernel spans");
     }
     if (!rc) {
-        // the largest pixel tile whose spectra (float32) fit 62 KB of LDS
+        // the largest pixel tile whose spectra (float32), with the channel and Phi tables, fit the LDS of a CU
         static const int cand[][2] = {{16, 16}, {16, 8}, {8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
         m->TX = 0;
         for (auto& c : cand)
-            if ((size_t)c[0] * c[1] * g->nz * sizeof(float) <= 62 * 1024) {
+            if (bb_analytic_lds_bytes(g->nz, g->nlines, c[0], c[1], ntab) + bb_analytic_lds_static() <= BB_LDS_MAX) {
                 m->TX = c[0];
                 m->TY = c[1];
                 break;
@@ -172,6 +160,22 @@ extern "C" int bb_model_create(const bb_grid* g, const double* cd2i, const doubl
         return rc;
     }
     *out = m;
+    return 0;
+}
+
+extern "C" int bb_model_set_tile(bb_model* m, int TX, int TY) {
+    if (!m || TX < 1 || TY < 1 || TX > 64 || TY > 64) return fail(BB_ERR_ARG, "bb_model_set_tile: bad tile %dx%d", TX, TY);
+    if (bb_analytic_lds_bytes(m->g.nz, m->g.nlines, TX, TY, m->ntab) + bb_analytic_lds_static() > BB_LDS_MAX)
+        return fail(BB_ERR_ARG, "bb_model_set_tile: a %dx%d tile of %d channels exceeds the LDS", TX, TY, m->g.nz);
+    m->TX = TX;
+    m->TY = TY;
+    return 0;
+}
+
+extern "C" int bb_model_get_tile(const bb_model* m, int* TX, int* TY) {
+    if (!m || !TX || !TY) return fail(BB_ERR_ARG, "bb_model_get_tile: NULL argument");
+    *TX = m->TX;
+    *TY = m->TY;
     return 0;
 }
 
@@ -244,7 +248,7 @@ extern "C" int bb_azplan_create(const bb_model* m, const double* R0, int nsub, b
 extern "C" int64_t bb_analytic_work_size(const bb_azplan* p, int B) {
     if (!p || B < 0) return -1;
     size_t per = (size_t)p->nsub * (bb_analytic_sub_bytes() + bb_analytic_wt_bytes()) +
-                 (size_t)p->nchunks * sizeof(short4);
+                 (size_t)p->nchunks * (1 + bb_analytic_chunk()) * sizeof(short4);
     return (int64_t)(per * B + 256);
 }
 
@@ -278,8 +282,10 @@ extern "C" int bb_galmod_analytic(const bb_model* m, const bb_azplan* p, bb_box 
     double* wtab = (double*)w;
     w += (size_t)B * nsub * bb_analytic_wt_bytes();
     short4* bbox = (short4*)w;
+    w += (size_t)B * p->nchunks * sizeof(short4);
+    short4* sbox = (short4*)w;
     rc = bb_launch_galmod_analytic(&m->g, box, sr, nsub, B, p->d_naz, p->d_chunks, p->nchunks, p->d_az, m->TX, m->TY,
-                                   sub, wtab, bbox, m->d_cd2i, m->d_vedges, m->d_ptab, m->ntab, out, accumulate,
+                                   sub, wtab, bbox, sbox, m->d_cd2i, m->d_vedges, m->d_ptab, m->ntab, out, accumulate,
                                    (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_galmod_analytic launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
@@ -291,8 +297,8 @@ extern "C" int bb_conv2d_same(const bb_model* m, const float* in, float* out, in
     if (B < 0 || by < 1 || bx < 1 || B > 65535) return fail(BB_ERR_ARG, "bb_conv2d_same: bad sizes");
     if (B == 0) return 0;
     if (!in || !out || in == out) return fail(BB_ERR_ARG, "bb_conv2d_same: in/out must be distinct, non-NULL");
-    int rc = bb_launch_conv2d_same(in, out, B, by, bx, m->g.nz, m->d_kern, m->d_span, m->ky, m->kx,
-                                   (float)m->scale, (hipStream_t)stream);
+    int rc = bb_launch_conv2d_same(in, out, B, by, bx, m->g.nz, m->d_kern, m->ky, m->kx, (float)m->scale,
+                                   (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_conv2d_same launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
 

@@ -2,75 +2,115 @@
 // kernel, then values below 1e-12 set to zero and the flux scale applied (FFTConvolver and
 // DiskModel.convolve, bbarolo/smooth.py:145-165 and galmod.py:1273-1283; _conv_sparse galmod.py:974-1037).
 //
-// Layout yxz (spectra contiguous), so a wavefront's 64 lanes read 64 consecutive channels of one pixel
-// (256-byte coalesced loads). Each thread computes BB_CV_RX consecutive output pixels of one row for one
-// channel, sliding along the kernel row, so every input value loaded serves up to RX outputs. The kernel
-// taps live in LDS as float32; rows are restricted to their non-zero span (the beam is cut below 1e-4 of
-// its peak, an ellipse inside the (ky, kx) box).
+// Layout yxz (spectra contiguous): the lanes of a wavefront run along the channels, so every load is a
+// contiguous segment and all lanes follow the same control flow; the kernel taps are wave-uniform (scalar
+// loads). Each thread computes a block of RY x RX output pixels of one channel: it walks the RY + KY - 1
+// input rows that feed the block, keeps one input row segment (RX + KX - 1 values) in registers and adds it,
+// through every kernel row that links it to an output row, to the RY x RX accumulators. The kernel width KX
+// is a template parameter (odd, <= 25) so that the row loops unroll into registers; taps outside the beam's
+// 1e-4 cut are zero and simply add nothing.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define BB_CV_ZL 64      // channels per block (lanes)
-#define BB_CV_PY 4       // output pixel groups per block (waves)
-#define BB_CV_RX 8       // consecutive output pixels per thread
-#define BB_CV_KMAX 4096  // kernel taps held in LDS
+#define BB_CV_RY 4
+#define BB_CV_RX 4
+#define BB_CV_THREADS 256
 
-__global__ void __launch_bounds__(BB_CV_ZL * BB_CV_PY)
-k_conv2d_same(const float* __restrict__ inall, float* __restrict__ outall, int by, int bx, int nz,
-              const float* __restrict__ kern, const int* __restrict__ rowspan, int ky, int kx, float scale) {
-    __shared__ float sk[BB_CV_KMAX];
-    __shared__ int sspan[2 * 512];
-    const int nt = blockDim.x * blockDim.y;
-    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-    for (int i = tid; i < ky * kx; i += nt) sk[i] = kern[i];
-    for (int i = tid; i < 2 * ky; i += nt) sspan[i] = rowspan[i];
-    __syncthreads();
+template <int KX>
+__global__ void __launch_bounds__(BB_CV_THREADS)
+k_conv2d_same(const float* __restrict__ inall, float* __restrict__ outall, int by, int bx, int nz, int ZL,
+              const float* __restrict__ kern, int ky, float scale) {
+    constexpr int W = BB_CV_RX + KX - 1;
+    constexpr int hx = KX / 2;
+    const int hy = ky / 2;
     const int b = blockIdx.z;
-    const int z = blockIdx.y * BB_CV_ZL + threadIdx.x;
-    const int nxg = (bx + BB_CV_RX - 1) / BB_CV_RX;
-    const int grp = blockIdx.x * BB_CV_PY + threadIdx.y;
-    if (z >= nz || grp >= by * nxg) return;
-    const int y = grp / nxg, x0 = (grp % nxg) * BB_CV_RX;
-    const float* in = inall + (size_t)b * by * bx * nz;
-    float* out = outall + (size_t)b * by * bx * nz;
-    const int hy = ky / 2, hx = kx / 2;
-    float acc[BB_CV_RX];
+    const int zl = threadIdx.x % ZL, grp = threadIdx.x / ZL, ngrp = blockDim.x / ZL;
+    const int z = blockIdx.y * ZL + zl;
+    const int nbx = (bx + BB_CV_RX - 1) / BB_CV_RX, nby = (by + BB_CV_RY - 1) / BB_CV_RY;
+    const int blk = blockIdx.x * ngrp + grp;
+    if (z >= nz || blk >= nbx * nby) return;
+    const int y0 = (blk / nbx) * BB_CV_RY, x0 = (blk % nbx) * BB_CV_RX;
+    const float* in = inall + (size_t)b * by * bx * nz + z;
+    float acc[BB_CV_RY][BB_CV_RX];
 #pragma unroll
-    for (int r = 0; r < BB_CV_RX; r++) acc[r] = 0.f;
-    for (int j = 0; j < ky; j++) {
-        const int yy = y + hy - j;         // input row feeding output row y through kernel row j
-        if (yy < 0 || yy >= by) continue;
-        const int ilo = sspan[2 * j], ihi = sspan[2 * j + 1];
-        if (ihi < ilo) continue;
-        const float* row = in + (size_t)yy * bx * nz + z;
-        const float* kr = sk + j * kx;
-        // output x0 + r uses input column x0 + r + hx - i, i in [ilo, ihi]
-        const int clo = x0 + hx - ihi, chi = x0 + BB_CV_RX - 1 + hx - ilo;
-        for (int c = clo; c <= chi; c++) {
-            if (c < 0 || c >= bx) continue;
-            const float v = row[(size_t)c * nz];
+    for (int r = 0; r < BB_CV_RY; r++)
 #pragma unroll
-            for (int r = 0; r < BB_CV_RX; r++) {
-                const int i = x0 + r + hx - c;
-                if (i >= ilo && i <= ihi) acc[r] = fmaf(kr[i], v, acc[r]);
+        for (int c = 0; c < BB_CV_RX; c++) acc[r][c] = 0.f;
+    // output (y, x) = sum_{j,i} K[j][i] in[y + hy - j][x + hx - i]
+    const int iylo = y0 - hy, iyhi = y0 + BB_CV_RY - 1 + hy;
+    for (int iy = iylo; iy <= iyhi; iy++) {
+        if (iy < 0 || iy >= by) continue;
+        float row[W];
+        const float* ir = in + (size_t)iy * bx * nz;
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            const int ix = x0 - hx + w;
+            row[w] = (ix >= 0 && ix < bx) ? ir[(size_t)ix * nz] : 0.f;
+        }
+#pragma unroll
+        for (int r = 0; r < BB_CV_RY; r++) {
+            const int j = y0 + r + hy - iy;
+            if (j < 0 || j >= ky) continue;
+            const float* kr = kern + j * KX;
+#pragma unroll
+            for (int i = 0; i < KX; i++) {
+                const float kv = kr[i];
+#pragma unroll
+                for (int c = 0; c < BB_CV_RX; c++) acc[r][c] = fmaf(kv, row[c + KX - 1 - i], acc[r][c]);
             }
         }
     }
+    float* out = outall + (size_t)b * by * bx * nz + z;
 #pragma unroll
-    for (int r = 0; r < BB_CV_RX; r++) {
-        const int x = x0 + r;
-        if (x >= bx) break;
-        float v = acc[r];
-        v = v < 1e-12f ? 0.f : v * scale;
-        out[((size_t)y * bx + x) * nz + z] = v;
+    for (int r = 0; r < BB_CV_RY; r++) {
+        const int y = y0 + r;
+        if (y >= by) break;
+#pragma unroll
+        for (int c = 0; c < BB_CV_RX; c++) {
+            const int x = x0 + c;
+            if (x < bx) {
+                float v = acc[r][c];
+                out[((size_t)y * bx + x) * nz] = v < 1e-12f ? 0.f : v * scale;
+            }
+        }
     }
 }
 
+// Channel lanes per group: the width in {64, 32, 16, 8} that leaves the fewest idle lanes.
+static int pick_zl(int nz) {
+    int best = 64, waste = 1 << 30;
+    for (int zl : {64, 32, 16, 8}) {
+        int w = ((nz + zl - 1) / zl) * zl - nz;
+        if (w < waste) {
+            waste = w;
+            best = zl;
+        }
+    }
+    return best;
+}
+
+template <int KX>
+static void launch(const float* in, float* out, int B, int by, int bx, int nz, const float* kern, int ky, float scale,
+                   hipStream_t st) {
+    const int ZL = pick_zl(nz);
+    const int ngrp = BB_CV_THREADS / ZL;
+    const int nblk = ((bx + BB_CV_RX - 1) / BB_CV_RX) * ((by + BB_CV_RY - 1) / BB_CV_RY);
+    dim3 grid((nblk + ngrp - 1) / ngrp, (nz + ZL - 1) / ZL, B);
+    hipLaunchKernelGGL(k_conv2d_same<KX>, grid, dim3(BB_CV_THREADS), 0, st, in, out, by, bx, nz, ZL, kern, ky, scale);
+}
+
+extern "C" int bb_conv_max_kx() { return 25; }
+
+// kern: (ky, kx) float32 taps on the device, kx odd <= 25.
 extern "C" int bb_launch_conv2d_same(const float* in, float* out, int B, int by, int bx, int nz, const float* kern,
-                                     const int* rowspan, int ky, int kx, float scale, hipStream_t st) {
-    const int nxg = (bx + BB_CV_RX - 1) / BB_CV_RX;
-    dim3 grid((by * nxg + BB_CV_PY - 1) / BB_CV_PY, (nz + BB_CV_ZL - 1) / BB_CV_ZL, B);
-    hipLaunchKernelGGL(k_conv2d_same, grid, dim3(BB_CV_ZL, BB_CV_PY), 0, st, in, out, by, bx, nz, kern, rowspan, ky,
-                       kx, scale);
+                                     int ky, int kx, float scale, hipStream_t st) {
+    switch (kx) {
+#define BB_CASE(K) \
+    case K: launch<K>(in, out, B, by, bx, nz, kern, ky, scale, st); break;
+        BB_CASE(1) BB_CASE(3) BB_CASE(5) BB_CASE(7) BB_CASE(9) BB_CASE(11) BB_CASE(13) BB_CASE(15) BB_CASE(17)
+        BB_CASE(19) BB_CASE(21) BB_CASE(23) BB_CASE(25)
+#undef BB_CASE
+        default: return (int)hipErrorInvalidValue;
+    }
     return (int)hipGetLastError();
 }

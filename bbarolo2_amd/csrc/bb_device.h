@@ -9,8 +9,15 @@
 #define BB_PI 3.141592653589793
 
 // Round half away from zero; NaN or |x| >= 1e18 -> a sentinel far outside any grid (galmod.py:38-46).
+// Values within +-1e9 (every pixel or channel of a cube, and far beyond) round with 32-bit conversions; the
+// others take the 64-bit path, so the result is the reference's for every input.
 __device__ __forceinline__ int64_t bb_lround(double x) {
-    if (!(fabs(x) < 1e18)) return -((int64_t)1 << 62);
+    const double a = fabs(x);
+    if (a < 1e9) {
+        const int r = (int)(a + 0.5);
+        return x >= 0 ? (int64_t)r : -(int64_t)r;
+    }
+    if (!(a < 1e18)) return -((int64_t)1 << 62);
     if (x >= 0) return (int64_t)(x + 0.5);
     return -(int64_t)(-x + 0.5);
 }

@@ -24,8 +24,8 @@
 #define BB_CS 32      // azimuth samples per chunk
 #define BB_NSF 32     // doubles per sub-ring record
 #define BB_ZCAP 64    // vertical weights tabulated per sub-ring (beyond: computed on the fly)
-#define BB_RMAX 8     // vertical runs per sample held in registers (beyond: deposited one by one)
-#define BB_TILE_THREADS 256
+#define BB_RMAX 16    // runs of a sample inside a tile held in registers (beyond: deposited one by one)
+#define BB_TILE_THREADS 512
 
 enum {
     F_R0 = 0, F_R1, F_XPOS, F_YPOS, F_SPA, F_CPA, F_SINC, F_CINC, F_VSYS, F_VROT, F_VRAD, F_VVERT, F_SIG, F_K,
@@ -148,19 +148,6 @@ k_an_prep(bb_grid g, bb_box box, const double* __restrict__ srall, int nsub, con
     q[F_ZDEP] = zdep ? 1.0 : 0.0;
 }
 
-// Weight of vertical sample kz (the flux of the layer in its bin).
-__device__ __forceinline__ double wq_at(const double* __restrict__ q, const double* __restrict__ wt, int kz,
-                                        int ltype) {
-    const int nzz = (int)q[F_NZZ];
-    if (nzz == 1) return 1.0;
-    const int half = (int)q[F_HALF];
-    const int j = zbin_of(kz, half);
-    if (j < BB_ZCAP) return wt[j];
-    double lo, hi;
-    zbin(j, (int)q[F_NIN], q[F_STEP], q[F_ZLIM], lo, hi);
-    return bb_layer_cdf(ltype, hi) - bb_layer_cdf(ltype, lo);
-}
-
 __device__ __forceinline__ double z_at(const double* __restrict__ q, int kz) {
     const int nzz = (int)q[F_NZZ];
     if (nzz <= 1) return 0.0;
@@ -178,186 +165,333 @@ __device__ __forceinline__ void pix_at(const double* __restrict__ q, const bb_gr
 // ---------------------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_an_bbox(bb_grid g, bb_box box, const int4* __restrict__ chunks, int nchunks, const double2* __restrict__ az,
-          const double* __restrict__ suball, int nsub, short4* __restrict__ bball) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
-    if (c >= nchunks) return;
-    const int4 ch = chunks[c];
+          const double* __restrict__ suball, int nsub, short4* __restrict__ bball, short4* __restrict__ sball) {
+    // one thread per azimuth sample (BB_CS lanes per chunk): the box of its vertical segment, then the box of
+    // the chunk by a reduction over its lanes
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+    const int c = t / BB_CS, lane = t % BB_CS;
+    const bool valid = c < nchunks;
+    const int4 ch = valid ? chunks[c] : make_int4(0, 0, 0, 0);
     const int s = ch.x >> 1, kr = ch.x & 1;
-    const double* q = suball + ((size_t)b * nsub + s) * BB_NSF;
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
-    int64_t x0 = INT64_MAX, x1 = INT64_MIN, y0 = INT64_MAX, y1 = INT64_MIN;
-    if (q[F_DENS] != 0) {
-        const double R = kr ? q[F_R1] : q[F_R0];
-        const int nzz = (int)q[F_NZZ];
-        const double za = z_at(q, 0), zb = z_at(q, nzz - 1);
-        for (int i = 0; i < ch.z; i++) {
-            const double2 cs = az[ch.w + i];
+    int x0 = 1 << 30, x1 = -(1 << 30), y0 = 1 << 30, y1 = -(1 << 30);
+    if (valid && lane < ch.z) {
+        const double* q = suball + ((size_t)b * nsub + s) * BB_NSF;
+        if (q[F_DENS] != 0) {
+            const double R = kr ? q[F_R1] : q[F_R0];
+            const int nzz = (int)q[F_NZZ];
+            const double2 cs = az[ch.w + lane];
             const double caz = cs.x, saz = cs.y;
             const double wside = caz >= 0.0 ? q[F_WREC] : q[F_WAPP];
-            if (wside == 0.0) continue;
-            const double x = R * caz, yb = R * saz * q[F_CINC], xs = x * q[F_SPA], xc = x * q[F_CPA];
-            int64_t g0, g1;
-            pix_at(q, g, xs, xc, yb, za, g0, g1);
-            x0 = g0 < x0 ? g0 : x0; x1 = g0 > x1 ? g0 : x1; y0 = g1 < y0 ? g1 : y0; y1 = g1 > y1 ? g1 : y1;
-            pix_at(q, g, xs, xc, yb, zb, g0, g1);
-            x0 = g0 < x0 ? g0 : x0; x1 = g0 > x1 ? g0 : x1; y0 = g1 < y0 ? g1 : y0; y1 = g1 > y1 ? g1 : y1;
+            if (wside != 0.0) {
+                const double x = R * caz, yb = R * saz * q[F_CINC], xs = x * q[F_SPA], xc = x * q[F_CPA];
+                int64_t a0, a1, e0, e1;
+                pix_at(q, g, xs, xc, yb, z_at(q, 0), a0, a1);
+                pix_at(q, g, xs, xc, yb, z_at(q, nzz - 1), e0, e1);
+                // box-relative, clipped to the box (the pixel of a sample is monotone in its height)
+                int64_t lx = (a0 < e0 ? a0 : e0) - box.blo0, hx = (a0 > e0 ? a0 : e0) - box.blo0;
+                int64_t ly = (a1 < e1 ? a1 : e1) - box.blo1, hy = (a1 > e1 ? a1 : e1) - box.blo1;
+                lx = lx < 0 ? 0 : lx;
+                ly = ly < 0 ? 0 : ly;
+                hx = hx > bx - 1 ? bx - 1 : hx;
+                hy = hy > by - 1 ? by - 1 : hy;
+                if (lx <= hx && ly <= hy) {
+                    x0 = (int)lx;
+                    x1 = (int)hx;
+                    y0 = (int)ly;
+                    y1 = (int)hy;
+                }
+            }
         }
     }
-    short4 r = make_short4(1, 0, 1, 0);       // empty
-    if (x0 <= x1) {
-        // box-relative, widened by one pixel, clipped to the box
-        int64_t a0 = x0 - box.blo0 - 1, a1 = x1 - box.blo0 + 1, c0 = y0 - box.blo1 - 1, c1 = y1 - box.blo1 + 1;
-        a0 = a0 < 0 ? 0 : a0;
-        c0 = c0 < 0 ? 0 : c0;
-        a1 = a1 > bx - 1 ? bx - 1 : a1;
-        c1 = c1 > by - 1 ? by - 1 : c1;
-        if (a0 <= a1 && c0 <= c1) r = make_short4((short)a0, (short)a1, (short)c0, (short)c1);
+    if (valid) sball[((size_t)b * nchunks + c) * BB_CS + lane] = x0 <= x1 ? make_short4((short)x0, (short)x1, (short)y0, (short)y1)
+                                                                          : make_short4(1, 0, 1, 0);
+    for (int off = BB_CS / 2; off > 0; off >>= 1) {
+        const int ox0 = __shfl_xor(x0, off, BB_CS), ox1 = __shfl_xor(x1, off, BB_CS);
+        const int oy0 = __shfl_xor(y0, off, BB_CS), oy1 = __shfl_xor(y1, off, BB_CS);
+        x0 = ox0 < x0 ? ox0 : x0;
+        x1 = ox1 > x1 ? ox1 : x1;
+        y0 = oy0 < y0 ? oy0 : y0;
+        y1 = oy1 > y1 ? oy1 : y1;
     }
-    bball[(size_t)b * nchunks + c] = r;
+    if (valid && lane == 0)
+        bball[(size_t)b * nchunks + c] = x0 <= x1 ? make_short4((short)x0, (short)x1, (short)y0, (short)y1)
+                                                  : make_short4(1, 0, 1, 0);
 }
 
 // ---------------------------------------------------------------------------------------------------------
-// Add aw * the channel-integrated line (centre v) to the spectrum sp of the LDS tile (galmod.py:194-214).
-__device__ __forceinline__ void lds_line(float* sp, double aw, double v, double sig, const double* __restrict__ ve,
-                                         const double* __restrict__ cd2i, const double* __restrict__ ptab, int ntab,
-                                         int64_t kc, int64_t K, int nz) {
+// The deposit of one tile. Per-sub-ring constants are held in registers (Sub), the vertical weights of the
+// chunk being processed, the channel edges and the channel factors in LDS; the line profile is built from
+// the Phi table (galmod.py:183-214) and added to the LDS spectra of the runs kept.
+struct Sub {
+    double R, xpos, ypos, spa, cpa, sinc, cinc, vsys, vrot, vrad, vvert, sig, flux, wrec, wapp, z0, dvdz, zcyl, step,
+        zlim;
+    int64_t K;
+    int zdep, nzz, nin, half;
+};
+
+__device__ __forceinline__ void load_sub(Sub& u, const double* __restrict__ q, int kr) {
+    u.R = kr ? q[F_R1] : q[F_R0];
+    u.xpos = q[F_XPOS];
+    u.ypos = q[F_YPOS];
+    u.spa = q[F_SPA];
+    u.cpa = q[F_CPA];
+    u.sinc = q[F_SINC];
+    u.cinc = q[F_CINC];
+    u.vsys = q[F_VSYS];
+    u.vrot = q[F_VROT];
+    u.vrad = q[F_VRAD];
+    u.vvert = q[F_VVERT];
+    u.sig = q[F_SIG];
+    u.flux = q[F_FLUX];
+    u.wrec = q[F_WREC];
+    u.wapp = q[F_WAPP];
+    u.z0 = q[F_Z0];
+    u.dvdz = q[F_DVDZ];
+    u.zcyl = q[F_ZCYL];
+    u.step = q[F_STEP];
+    u.zlim = q[F_ZLIM];
+    u.K = (int64_t)q[F_K];
+    u.zdep = q[F_ZDEP] != 0.0;
+    u.nzz = (int)q[F_NZZ];
+    u.nin = (int)q[F_NIN];
+    u.half = (int)q[F_HALF];
+}
+
+__device__ __forceinline__ double uz(const Sub& u, int kz) {
+    return u.nzz <= 1 ? 0.0 : zq_at(kz, u.half, u.nin, u.step, u.zlim) * u.z0;
+}
+
+__device__ __forceinline__ void upix(const Sub& u, const bb_grid& g, double xs, double xc, double yb, double z,
+                                     int64_t& g0, int64_t& g1) {
+    double y = yb - z * u.sinc;
+    g0 = bb_lround(u.xpos + (xs - y * u.cpa) / g.cdelt0);
+    g1 = bb_lround(u.ypos + (xc + y * u.spa) / g.cdelt1);
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Accumulation in the LDS tile is fixed point: every contribution x (>= 0) is stored as round(x / q) in a
+// 64-bit integer, q = amax / 2^31 with amax a bound of any single contribution of the parameter vector
+// (largest relint x flux x side weight over its sub-rings, times the largest channel factor), so one
+// contribution is < 2^31 units and a voxel holds 2^33 of them. Integer LDS atomics run at full rate on
+// gfx950 (float ones are serialised per lane), and the sums are exact: the model does not depend on the
+// order of the atomics.
+struct Acc {
+    unsigned long long* tile;
+    float inv_q;        // 1 / q
+    int nzp;
+    __device__ __forceinline__ void add(int slot, int k, float x) const {
+        const unsigned int u = (unsigned int)fmaf(x, inv_q, 0.5f);
+        atomicAdd(tile + slot * nzp + k, (unsigned long long)u);
+    }
+};
+
+// Phi(t) from the float32 copy of the table in LDS (same interpolation as galmod.py:183-191).
+__device__ __forceinline__ float lds_phi(double t, const float* ph, int ntab) {
+    const double x = (t + 6.0) * 512.0;
+    if (!(x > 0.0)) return 0.f;
+    if (!(x < (double)(ntab - 1))) return 1.f;
+    const int k = (int)x;
+    const float fr = (float)(x - (double)k);
+    const float a = ph[k];
+    return fmaf(fr, ph[k + 1] - a, a);
+}
+
+// aw x the channel-integrated line of centre v (galmod.py:194-214) into pixel slot of the tile: the rare
+// paths (a height-dependent velocity, more runs of a sample in the tile than BB_RMAX).
+__device__ __noinline__ void acc_line(const Acc& A, int slot, double aw, double v, double sig, const double* ve,
+                                      const double* cd, const float* sph, int ntab, int64_t kc, int64_t K, int nz) {
     int64_t k0 = kc - K;
     if (k0 < 0) k0 = 0;
     int64_t k1 = kc + K + 1;
     if (k1 > nz) k1 = nz;
     if (k1 <= k0) return;
     const double inv = 1.0 / sig;
-    double pp = bb_phi((ve[k0] - v) * inv, ptab, ntab);
-    for (int64_t k = k0; k < k1; k++) {
-        double pn = bb_phi((ve[k + 1] - v) * inv, ptab, ntab);
-        double f = pn - pp;
+    float pp = lds_phi((ve[k0] - v) * inv, sph, ntab);
+    for (int k = (int)k0; k < (int)k1; k++) {
+        const float pn = lds_phi((ve[k + 1] - v) * inv, sph, ntab);
+        const float f = pn - pp;
         pp = pn;
-        atomicAdd(sp + k, (float)(aw * ((f >= 0 ? f : -f) * cd2i[k])));
+        A.add(slot, k, (float)aw * (fabsf(f) * (float)cd[k]));
     }
 }
 
-struct TileCtx {
-    int tx0, ty0, TX, TY, bx, by, nz;
-    __device__ __forceinline__ int slot(int64_t g0, int64_t g1, const bb_box& box) const {
-        int64_t rx = g0 - box.blo0 - tx0, ry = g1 - box.blo1 - ty0;
-        if (rx < 0 || rx >= TX || ry < 0 || ry >= TY) return -1;
-        if (rx + tx0 >= bx || ry + ty0 >= by) return -1;
-        return (int)(ry * TX + rx);
+__device__ __noinline__ double uw_cold(const Sub& u, int j, int ltype) {
+    double lo, hi;
+    zbin(j, u.nin, u.step, u.zlim, lo, hi);
+    return bb_layer_cdf(ltype, hi) - bb_layer_cdf(ltype, lo);
+}
+
+// weight of vertical sample kz: the LDS copy of the sub-ring's bins (wl, BB_WCAP of them) when there is
+// one, else the global table (BB_ZCAP bins), else the layer CDF
+#define BB_WCAP 16
+__device__ __forceinline__ double uw_(const Sub& u, const float* wl, const double* __restrict__ wt, int kz,
+                                      int ltype) {
+    if (u.nzz <= 1) return 1.0;
+    const int j = zbin_of(kz, u.half);
+    if (wl && j < BB_WCAP) return (double)wl[j];
+    if (j < BB_ZCAP) return wt[j];
+    return uw_cold(u, j, ltype);
+}
+
+#define BB_LIST_CAP 1024     // chunks meeting the tile, gathered before their samples are tested
+#define BB_SAMP_CAP 2048     // samples meeting the tile, gathered before they are deposited
+#define BB_NGRP (BB_TILE_THREADS / BB_CS)
+#define BB_WLDS_MAX 4096     // vertical weights (float) kept in LDS for all the sub-rings of a vector
+
+// Odd stride of the pixel spectra in the LDS tile: consecutive pixels start in different banks.
+__device__ __host__ __forceinline__ int bb_tile_stride(int nz) { return nz | 1; }
+
+struct TileGeo {
+    int64_t ax0, ax1, ay0, ay1;     // the tile in absolute pixels (inclusive)
+    int TX;
+    __device__ __forceinline__ int slot(int64_t p0, int64_t p1) const {
+        if (p0 < ax0 || p0 > ax1 || p1 < ay0 || p1 > ay1) return -1;
+        return (int)((p1 - ay0) * TX + (p0 - ax0));
     }
 };
 
-// One azimuth sample of a chunk: its vertical runs inside the tile, deposited into the LDS tile.
-__device__ void tile_sample(const bb_grid& g, const bb_box& box, const TileCtx& t, float* tile, const double* q,
-                            const double* wt, double R, double caz, double saz, const double* __restrict__ cd2i,
-                            const double* __restrict__ vedges, const double* __restrict__ ptab, int ntab) {
-    const double wside = caz >= 0.0 ? q[F_WREC] : q[F_WAPP];
-    if (wside == 0.0) return;
-    const int nz = t.nz, nzz = (int)q[F_NZZ];
-    const double x = R * caz, yb = R * saz * q[F_CINC], xs = x * q[F_SPA], xc = x * q[F_CPA];
-    const double sinc = q[F_SINC], cinc = q[F_CINC];
-    const double vplane = q[F_VSYS] + (q[F_VROT] * caz + q[F_VRAD] * saz) * sinc + q[F_VVERT] * cinc;
-    const double sig = q[F_SIG];
-    const int64_t K = (int64_t)q[F_K];
-    const double afl = q[F_FLUX] * wside;
-    if (q[F_ZDEP] == 0.0) {
-        // runs of consecutive vertical samples in the same pixel (galmod.py:414-431)
-        int rslot[BB_RMAX];
-        double rws[BB_RMAX];
-        int nr = 0;
-        int64_t c0, c1;
-        pix_at(q, g, xs, xc, yb, z_at(q, 0), c0, c1);
-        double ws = wq_at(q, wt, 0, g.ltype);
-        for (int kz = 1; kz <= nzz; kz++) {
-            int64_t h0 = 0, h1 = 0;
-            bool same = false;
-            if (kz < nzz) {
-                pix_at(q, g, xs, xc, yb, z_at(q, kz), h0, h1);
-                same = h0 == c0 && h1 == c1;
-            }
-            if (same) {
-                ws += wq_at(q, wt, kz, g.ltype);
-                continue;
-            }
-            // the run [.., kz) ends: keep it if its pixel is in this tile
-            int sl = t.slot(c0, c1, box);
-            if (sl >= 0) {
-                if (nr < BB_RMAX) {
-#pragma unroll
-                    for (int r = 0; r < BB_RMAX; r++)
-                        if (r == nr) { rslot[r] = sl; rws[r] = ws; }
-                    nr++;
-                } else {
-                    for (int nl = 0; nl < g.nlines; nl++) {
-                        double v = vplane;
-                        int64_t kc = bb_lround(bb_velgrid(v, g.axtyp, g.velsys, g.lfreq0[nl], g.crval3, g.cdelt3,
-                                                          g.crpix3));
-                        lds_line(tile + (size_t)sl * nz, g.relint[nl] * afl * ws, v, sig,
-                                 vedges + (size_t)nl * (nz + 1), cd2i, ptab, ntab, kc, K, nz);
-                    }
-                }
-            }
-            if (kz < nzz) {
-                c0 = h0;
-                c1 = h1;
-                ws = wq_at(q, wt, kz, g.ltype);
-            }
-        }
-        if (nr == 0) return;
-        // one line profile per emission line, shared by the runs kept (galmod.py:411-413)
-        for (int nl = 0; nl < g.nlines; nl++) {
-            const double* ve = vedges + (size_t)nl * (nz + 1);
-            const double a = g.relint[nl] * afl;
-            const double v = vplane;
-            const int64_t kc = bb_lround(bb_velgrid(v, g.axtyp, g.velsys, g.lfreq0[nl], g.crval3, g.cdelt3, g.crpix3));
-            int64_t k0 = kc - K;
-            if (k0 < 0) k0 = 0;
-            int64_t k1 = kc + K + 1;
-            if (k1 > nz) k1 = nz;
-            if (k1 <= k0) continue;
-            const double inv = 1.0 / sig;
-            double pp = bb_phi((ve[k0] - v) * inv, ptab, ntab);
-            for (int64_t k = k0; k < k1; k++) {
-                double pn = bb_phi((ve[k + 1] - v) * inv, ptab, ntab);
-                double f = pn - pp;
-                pp = pn;
-                const double prof = (f >= 0 ? f : -f) * cd2i[k];
-#pragma unroll
-                for (int r = 0; r < BB_RMAX; r++)
-                    if (r < nr) atomicAdd(tile + (size_t)rslot[r] * nz + k, (float)((a * rws[r]) * prof));
-            }
-        }
-        return;
-    }
-    // the velocity depends on |z|: samples at +z and -z share their line (galmod.py:432-465)
-    for (int kz = 0; kz < (nzz + 1) / 2; kz++) {
-        const int kz2 = nzz - 1 - kz;
-        const double za = z_at(q, kz), zb = z_at(q, kz2);
+// A height-dependent velocity (galmod.py:432-465): every vertical pair has its own line.
+__device__ __noinline__ void an_sample_lag(const bb_grid& g, const bb_box& box, const TileGeo& T, const Acc& A,
+                                           const double* sve, const double* scd, const float* sph, int ntab,
+                                           const Sub& u, const float* wl, const double* __restrict__ wt, double caz,
+                                           double saz) {
+    const int nz = g.nz;
+    const double wside = caz >= 0.0 ? u.wrec : u.wapp;
+    const double x = u.R * caz, yb = u.R * saz * u.cinc, xs = x * u.spa, xc = x * u.cpa;
+    const double vplane = u.vsys + (u.vrot * caz + u.vrad * saz) * u.sinc + u.vvert * u.cinc;
+    const double afl = u.flux * wside;
+    for (int kz = 0; kz < (u.nzz + 1) / 2; kz++) {
+        const int kz2 = u.nzz - 1 - kz;
+        const double za = uz(u, kz), zb = uz(u, kz2);
         int64_t ga0, ga1, gb0, gb1;
-        pix_at(q, g, xs, xc, yb, za, ga0, ga1);
-        pix_at(q, g, xs, xc, yb, zb, gb0, gb1);
+        upix(u, g, xs, xc, yb, za, ga0, ga1);
+        upix(u, g, xs, xc, yb, zb, gb0, gb1);
         const bool in1 = !(ga0 < box.blo0 || ga0 >= box.bhi0 || ga1 < box.blo1 || ga1 >= box.bhi1);
         const bool in2 = kz2 != kz && !(gb0 < box.blo0 || gb0 >= box.bhi0 || gb1 < box.blo1 || gb1 >= box.bhi1);
-        const int s1 = in1 ? t.slot(ga0, ga1, box) : -1;
-        const int s2 = in2 ? t.slot(gb0, gb1, box) : -1;
+        const int s1 = in1 ? T.slot(ga0, ga1) : -1;
+        const int s2 = in2 ? T.slot(gb0, gb1) : -1;
         if (s1 < 0 && s2 < 0) continue;
         const double az_ = fabs(za);
         double v;
-        if (az_ > q[F_ZCYL])
-            v = q[F_VSYS] + (bb_lagged(q[F_VROT], q[F_DVDZ], az_ - q[F_ZCYL]) * caz + q[F_VRAD] * saz) * sinc +
-                q[F_VVERT] * cinc;
+        if (az_ > u.zcyl)
+            v = u.vsys + (bb_lagged(u.vrot, u.dvdz, az_ - u.zcyl) * caz + u.vrad * saz) * u.sinc + u.vvert * u.cinc;
         else
             v = vplane;
-        const double w1 = wq_at(q, wt, kz, g.ltype), w2 = wq_at(q, wt, kz2, g.ltype);
+        const double w1 = uw_(u, wl, wt, kz, g.ltype), w2 = uw_(u, wl, wt, kz2, g.ltype);
         for (int nl = 0; nl < g.nlines; nl++) {
-            const double* ve = vedges + (size_t)nl * (nz + 1);
+            const double* ve = sve + (size_t)nl * (nz + 1);
             const double a = g.relint[nl] * afl;
             const int64_t kc = bb_lround(bb_velgrid(v, g.axtyp, g.velsys, g.lfreq0[nl], g.crval3, g.cdelt3, g.crpix3));
             if (in1 && in2 && ga0 == gb0 && ga1 == gb1) {
-                lds_line(tile + (size_t)s1 * nz, a * (w1 + w2), v, sig, ve, cd2i, ptab, ntab, kc, K, nz);
+                acc_line(A, s1, a * (w1 + w2), v, u.sig, ve, scd, sph, ntab, kc, u.K, nz);
                 continue;
             }
-            if (s1 >= 0) lds_line(tile + (size_t)s1 * nz, a * w1, v, sig, ve, cd2i, ptab, ntab, kc, K, nz);
-            if (s2 >= 0) lds_line(tile + (size_t)s2 * nz, a * w2, v, sig, ve, cd2i, ptab, ntab, kc, K, nz);
+            if (s1 >= 0) acc_line(A, s1, a * w1, v, u.sig, ve, scd, sph, ntab, kc, u.K, nz);
+            if (s2 >= 0) acc_line(A, s2, a * w2, v, u.sig, ve, scd, sph, ntab, kc, u.K, nz);
+        }
+    }
+}
+
+// Deposit the runs of one azimuth sample that fall in the tile (galmod.py:389-431 for one sample).
+// Only the vertical samples that can reach the tile are placed exactly: a continuous estimate of the
+// (monotone) pixel track picks them, widened by one sample on each side, and those neighbours are outside the
+// tile, so every run inside it is complete.
+__device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, const TileGeo& T, const Acc& A,
+                                          const double* sve, const double* scd, const float* sph, int ntab,
+                                          const Sub& u, const float* wl, const double* __restrict__ wt, double caz,
+                                          double saz) {
+    if (u.zdep) {
+        an_sample_lag(g, box, T, A, sve, scd, sph, ntab, u, wl, wt, caz, saz);
+        return;
+    }
+    const int nz = g.nz;
+    const double wside = caz >= 0.0 ? u.wrec : u.wapp;
+    const double x = u.R * caz, yb = u.R * saz * u.cinc, xs = x * u.spa, xc = x * u.cpa;
+    const double vplane = u.vsys + (u.vrot * caz + u.vrad * saz) * u.sinc + u.vvert * u.cinc;
+    const double afl = u.flux * wside;
+    int ka = 0, kb = u.nzz - 1;
+    if (u.nzz > 2) {
+        const double X0 = u.xpos + (xs - yb * u.cpa) / g.cdelt0, dX = u.sinc * u.cpa / g.cdelt0;
+        const double Y0 = u.ypos + (xc + yb * u.spa) / g.cdelt1, dY = -u.sinc * u.spa / g.cdelt1;
+        const double m = 1e-6;
+        int first = u.nzz, last = -1;
+        for (int kz = 0; kz < u.nzz; kz++) {
+            const double z = uz(u, kz), X = X0 + z * dX, Y = Y0 + z * dY;
+            if (X >= (double)T.ax0 - 0.5 - m && X < (double)T.ax1 + 0.5 + m && Y >= (double)T.ay0 - 0.5 - m &&
+                Y < (double)T.ay1 + 0.5 + m) {
+                first = kz < first ? kz : first;
+                last = kz;
+            }
+        }
+        if (last < 0) return;
+        ka = first > 0 ? first - 1 : 0;
+        kb = last < u.nzz - 1 ? last + 1 : u.nzz - 1;
+    }
+    int rslot[BB_RMAX];
+    float rws[BB_RMAX];
+    int nr = 0;
+    int64_t c0, c1;
+    upix(u, g, xs, xc, yb, uz(u, ka), c0, c1);
+    double ws = uw_(u, wl, wt, ka, g.ltype);
+    for (int kz = ka + 1; kz <= kb + 1; kz++) {
+        int64_t h0 = 0, h1 = 0;
+        bool same = false;
+        if (kz <= kb) {
+            upix(u, g, xs, xc, yb, uz(u, kz), h0, h1);
+            same = h0 == c0 && h1 == c1;
+        }
+        if (same) {
+            ws += uw_(u, wl, wt, kz, g.ltype);
+            continue;
+        }
+        const int sl = T.slot(c0, c1);
+        if (sl >= 0) {
+            if (nr < BB_RMAX) {
+#pragma unroll
+                for (int r = 0; r < BB_RMAX; r++)
+                    if (r == nr) {
+                        rslot[r] = sl;
+                        rws[r] = (float)ws;
+                    }
+                nr++;
+            } else {
+                for (int nl = 0; nl < g.nlines; nl++) {
+                    int64_t kc = bb_lround(bb_velgrid(vplane, g.axtyp, g.velsys, g.lfreq0[nl], g.crval3, g.cdelt3,
+                                                      g.crpix3));
+                    acc_line(A, sl, g.relint[nl] * afl * ws, vplane, u.sig, sve + (size_t)nl * (nz + 1), scd, sph,
+                             ntab, kc, u.K, nz);
+                }
+            }
+        }
+        if (kz <= kb) {
+            c0 = h0;
+            c1 = h1;
+            ws = uw_(u, wl, wt, kz, g.ltype);
+        }
+    }
+    // one line profile per emission line, shared by the runs kept (galmod.py:411-431)
+    for (int nl = 0; nl < g.nlines && nr > 0; nl++) {
+        const double* ve = sve + (size_t)nl * (nz + 1);
+        const float a = (float)(g.relint[nl] * afl);
+        const int64_t kc = bb_lround(bb_velgrid(vplane, g.axtyp, g.velsys, g.lfreq0[nl], g.crval3, g.cdelt3, g.crpix3));
+        int64_t k0 = kc - u.K;
+        if (k0 < 0) k0 = 0;
+        int64_t k1 = kc + u.K + 1;
+        if (k1 > nz) k1 = nz;
+        const double inv = 1.0 / u.sig;
+        float aw[BB_RMAX];
+#pragma unroll
+        for (int r = 0; r < BB_RMAX; r++) aw[r] = a * rws[r] * A.inv_q;
+        float pp = k1 > k0 ? lds_phi((ve[k0] - vplane) * inv, sph, ntab) : 0.f;
+        for (int k = (int)k0; k < (int)k1; k++) {
+            const float pn = lds_phi((ve[k + 1] - vplane) * inv, sph, ntab);
+            const float prof = fabsf(pn - pp) * (float)scd[k];
+            pp = pn;
+#pragma unroll
+            for (int r = 0; r < BB_RMAX; r++)
+                if (r < nr)
+                    atomicAdd(A.tile + rslot[r] * A.nzp + k, (unsigned long long)(unsigned int)fmaf(aw[r], prof, 0.5f));
         }
     }
 }
@@ -365,78 +499,168 @@ __device__ void tile_sample(const bb_grid& g, const bb_box& box, const TileCtx& 
 __global__ void __launch_bounds__(BB_TILE_THREADS)
 k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks, int nchunks,
           const double2* __restrict__ az, const double* __restrict__ suball, const double* __restrict__ wtaball,
-          int nsub, const short4* __restrict__ bball, const double* __restrict__ cd2i,
-          const double* __restrict__ vedges, const double* __restrict__ ptab, int ntab, float* __restrict__ outall,
-          int accumulate) {
-    extern __shared__ float tile[];
-    __shared__ int list[BB_TILE_THREADS];
-    __shared__ int nlist;
+          int nsub, const short4* __restrict__ bball, const short4* __restrict__ sball,
+          const double* __restrict__ cd2i, const double* __restrict__ vedges, const double* __restrict__ ptab,
+          int ntab, float* __restrict__ outall, int accumulate) {
+    // dynamic LDS: [vedges (nlines, nz+1) f64][cd2i (nz) f64][Phi (ntab) f32][weights f32][tile u64]
+    extern __shared__ double dyn[];
+    __shared__ int clist[BB_LIST_CAP];
+    __shared__ int slist[BB_SAMP_CAP];
+    __shared__ int nlist, nsamp;
+    __shared__ double red[BB_TILE_THREADS / 64];
     const int b = blockIdx.y;
+    const int nz = g.nz, nl_ = g.nlines, nzp = bb_tile_stride(nz);
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
     const int ntx = (bx + TX - 1) / TX;
-    TileCtx t;
-    t.TX = TX;
-    t.TY = TY;
-    t.tx0 = (blockIdx.x % ntx) * TX;
-    t.ty0 = (blockIdx.x / ntx) * TY;
-    t.bx = bx;
-    t.by = by;
-    t.nz = g.nz;
-    const int nz = g.nz;
-    const int ntile = TX * TY * nz;
-    for (int i = threadIdx.x; i < ntile; i += blockDim.x) tile[i] = 0.f;
-    const short4* bb = bball + (size_t)b * nchunks;
+    const int tx0 = (blockIdx.x % ntx) * TX, ty0 = (blockIdx.x / ntx) * TY;
+    const int tx1 = tx0 + TX - 1, ty1 = ty0 + TY - 1;
+    const bool wlds = nsub * BB_WCAP <= BB_WLDS_MAX;
+    double* sve = dyn;
+    double* scd = dyn + (size_t)nl_ * (nz + 1);
+    float* sph = (float*)(scd + nz);
+    float* swl = sph + ((ntab + 1) & ~1);
+    unsigned long long* tile = (unsigned long long*)(swl + (wlds ? BB_WLDS_MAX : 0));
+    const int ntile = TX * TY * nzp;
     const double* sub = suball + (size_t)b * nsub * BB_NSF;
     const double* wtab = wtaball + (size_t)b * nsub * BB_ZCAP;
-    const int tx1 = t.tx0 + TX - 1, ty1 = t.ty0 + TY - 1;
-    const int lane = threadIdx.x % BB_CS, grp = threadIdx.x / BB_CS, ngrp = blockDim.x / BB_CS;
-    for (int base = 0; base < nchunks; base += blockDim.x) {
-        if (threadIdx.x == 0) nlist = 0;
-        __syncthreads();
+    for (int i = threadIdx.x; i < ntile; i += blockDim.x) tile[i] = 0ull;
+    for (int i = threadIdx.x; i < nl_ * (nz + 1); i += blockDim.x) sve[i] = vedges[i];
+    for (int i = threadIdx.x; i < nz; i += blockDim.x) scd[i] = cd2i[i];
+    for (int i = threadIdx.x; i < ntab; i += blockDim.x) sph[i] = (float)ptab[i];
+    if (wlds)
+        for (int i = threadIdx.x; i < nsub * BB_WCAP; i += blockDim.x)
+            swl[i] = (float)wtab[(size_t)(i / BB_WCAP) * BB_ZCAP + i % BB_WCAP];
+    // the fixed-point scale of this vector: amax bounds any single contribution
+    double am = 0.;
+    for (int s = threadIdx.x; s < nsub; s += blockDim.x) {
+        const double* q = sub + (size_t)s * BB_NSF;
+        const double a = fabs(q[F_FLUX]) * fmax(fabs(q[F_WREC]), fabs(q[F_WAPP]));
+        am = a > am ? a : am;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_down(am, off, 64);
+        am = o > am ? o : am;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = am;
+    if (threadIdx.x == 0) {
+        nlist = 0;
+        nsamp = 0;
+    }
+    __syncthreads();
+    double amax = 0.;
+    for (int w = 0; w < BB_TILE_THREADS / 64; w++) amax = red[w] > amax ? red[w] : amax;
+    double rmax = 0., cmax = 0.;
+    for (int nl = 0; nl < nl_; nl++) rmax = fabs(g.relint[nl]) > rmax ? fabs(g.relint[nl]) : rmax;
+    for (int k = 0; k < nz; k++) cmax = fabs(scd[k]) > cmax ? fabs(scd[k]) : cmax;
+    amax *= rmax * cmax;
+    const double q = amax > 0. ? amax * 0x1p-31 : 1.0;
+    Acc A;
+    A.tile = tile;
+    A.nzp = nzp;
+    A.inv_q = (float)(1.0 / q);
+    const short4* bb = bball + (size_t)b * nchunks;
+    const short4* sb = sball + (size_t)b * nchunks * BB_CS;
+    const int lane = threadIdx.x % BB_CS, grp = threadIdx.x / BB_CS;
+    TileGeo T;
+    T.ax0 = box.blo0 + tx0;
+    T.ax1 = box.blo0 + (tx1 < bx - 1 ? tx1 : bx - 1);
+    T.ay0 = box.blo1 + ty0;
+    T.ay1 = box.blo1 + (ty1 < by - 1 ? ty1 : by - 1);
+    T.TX = TX;
+    for (int base = 0; base < nchunks && amax > 0.; base += blockDim.x) {
+        // 1. the chunks whose bounding box meets the tile
         const int c = base + threadIdx.x;
         if (c < nchunks) {
             const short4 r = bb[c];
-            if (r.x <= r.y && r.x <= tx1 && r.y >= t.tx0 && r.z <= ty1 && r.w >= t.ty0) list[atomicAdd(&nlist, 1)] = c;
+            if (r.x <= r.y && r.x <= tx1 && r.y >= tx0 && r.z <= ty1 && r.w >= ty0) clist[atomicAdd(&nlist, 1)] = c;
         }
         __syncthreads();
         const int n = nlist;
-        for (int j = grp; j < n; j += ngrp) {
-            const int4 ch = chunks[list[j]];
-            if (lane < ch.z) {
-                const int s = ch.x >> 1, kr = ch.x & 1;
-                const double* q = sub + (size_t)s * BB_NSF;
-                const double2 cs = az[ch.w + lane];
-                tile_sample(g, box, t, tile, q, wtab + (size_t)s * BB_ZCAP, kr ? q[F_R1] : q[F_R0], cs.x, cs.y,
-                            cd2i, vedges, ptab, ntab);
+        __syncthreads();                            // (every thread has read n before the list changes)
+        if (!(n > BB_LIST_CAP - (int)blockDim.x || base + (int)blockDim.x >= nchunks)) continue;
+        for (int j0 = 0; j0 < n; j0 += BB_NGRP) {
+            // 2. their samples whose vertical segment meets the tile (one lane per azimuth)
+            const int j = j0 + grp;
+            if (j < n) {
+                const int cj = clist[j];
+                const short4 r = sb[(size_t)cj * BB_CS + lane];
+                if (r.x <= r.y && r.x <= tx1 && r.y >= tx0 && r.z <= ty1 && r.w >= ty0)
+                    slist[atomicAdd(&nsamp, 1)] = (cj << 5) | lane;
+            }
+            __syncthreads();
+            const int ns = nsamp;
+            __syncthreads();
+            if (ns > BB_SAMP_CAP - (int)blockDim.x || j0 + BB_NGRP >= n) {
+                // 3. deposit the samples gathered, one per thread. The list holds neighbouring azimuths of one
+                // sub-ring next to each other; lane l of wavefront w takes entry l * W + w of each window of
+                // blockDim entries (W wavefronts), so the lanes of one LDS atomic are W azimuths apart (different
+                // pixels: same-address atomics of a wavefront serialise) on neighbouring sub-rings (similar work).
+                const int W = blockDim.x >> 6;
+                for (int base = 0; base < ns; base += blockDim.x) {
+                    const int idx = base + (threadIdx.x & 63) * W + (threadIdx.x >> 6);
+                    if (idx >= ns) continue;
+                    const int e = slist[idx];
+                    const int4 ch = chunks[e >> 5];
+                    const int s = ch.x >> 1;
+                    Sub u;
+                    load_sub(u, sub + (size_t)s * BB_NSF, ch.x & 1);
+                    const double2 cs = az[ch.w + (e & 31)];
+                    an_sample(g, box, T, A, sve, scd, sph, ntab, u, wlds ? swl + s * BB_WCAP : nullptr,
+                              wtab + (size_t)s * BB_ZCAP, cs.x, cs.y);
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) nsamp = 0;
+                __syncthreads();
             }
         }
+        if (threadIdx.x == 0) nlist = 0;
         __syncthreads();
     }
+    __syncthreads();
     float* out = outall + (size_t)b * by * bx * nz;
-    for (int i = threadIdx.x; i < ntile; i += blockDim.x) {
+    for (int i = threadIdx.x; i < TX * TY * nz; i += blockDim.x) {
         const int p = i / nz, k = i - p * nz;
-        const int rx = t.tx0 + p % TX, ry = t.ty0 + p / TX;
+        const int rx = tx0 + p % TX, ry = ty0 + p / TX;
         if (rx < bx && ry < by) {
             float* o = out + ((size_t)ry * bx + rx) * nz + k;
-            *o = accumulate ? *o + tile[i] : tile[i];
+            const float v = (float)((double)tile[(size_t)p * nzp + k] * q);
+            *o = accumulate ? *o + v : v;
         }
     }
+}
+
+// LDS bytes of the dynamic part of k_an_tile: channel edges and factors (f64), Phi table and vertical weights
+// (f32), the tile (u64).
+extern "C" size_t bb_analytic_lds_bytes(int nz, int nlines, int TX, int TY, int ntab) {
+    return ((size_t)nlines * (nz + 1) + nz) * sizeof(double) + (size_t)((ntab + 1) & ~1) * sizeof(float) +
+           (size_t)BB_WLDS_MAX * sizeof(float) + (size_t)TX * TY * bb_tile_stride(nz) * sizeof(unsigned long long);
+}
+
+extern "C" size_t bb_analytic_lds_static() {
+    return (BB_LIST_CAP + BB_SAMP_CAP) * sizeof(int) + (BB_TILE_THREADS / 64) * sizeof(double) + 64;
 }
 
 // ---------------------------------------------------------------------------------------------------------
 extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const double* sr, int nsub, int B,
                                          const int* naz, const int4* chunks, int nchunks, const double2* az,
-                                         int TX, int TY, double* sub, double* wtab, short4* bbox, const double* cd2i,
-                                         const double* vedges, const double* ptab, int ntab, float* out,
-                                         int accumulate, hipStream_t st) {
+                                         int TX, int TY, double* sub, double* wtab, short4* bbox, short4* sbox,
+                                         const double* cd2i, const double* vedges, const double* ptab, int ntab,
+                                         float* out, int accumulate, hipStream_t st) {
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
     hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, naz, sub, wtab);
-    hipLaunchKernelGGL(k_an_bbox, dim3((nchunks + 255) / 256, B), dim3(256), 0, st, *g, box, chunks, nchunks, az, sub,
-                       nsub, bbox);
+    hipLaunchKernelGGL(k_an_bbox, dim3((nchunks * BB_CS + 255) / 256, B), dim3(256), 0, st, *g, box, chunks, nchunks,
+                       az, sub, nsub, bbox, sbox);
     const int ntiles = ((bx + TX - 1) / TX) * ((by + TY - 1) / TY);
-    const size_t lds = (size_t)TX * TY * g->nz * sizeof(float);
+    const size_t lds = bb_analytic_lds_bytes(g->nz, g->nlines, TX, TY, ntab);
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_an_tile, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(160 * 1024 - bb_analytic_lds_static()));
+        if (e != hipSuccess) return (int)e;
+        attr = true;
+    }
     hipLaunchKernelGGL(k_an_tile, dim3(ntiles, B), dim3(BB_TILE_THREADS), lds, st, *g, box, TX, TY, chunks, nchunks,
-                       az, sub, wtab, nsub, bbox, cd2i, vedges, ptab, ntab, out, accumulate);
+                       az, sub, wtab, nsub, bbox, sbox, cd2i, vedges, ptab, ntab, out, accumulate);
     return (int)hipGetLastError();
 }
 

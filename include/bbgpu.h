@@ -51,11 +51,17 @@ int bb_device_count(int* n);
 /* The per-fit model object: the header-derived setup of GalmodSetup (bbarolo/galmod.py:609-758) and the beam
  * kernel of DiskModel (galmod.py:1199-1214, FFTConvolver(kernel) of smooth.py:107-143), copied to the current
  * device. cd2i (nz), vedges (nlines, nz+1), ptab (ntab) and kern (ky, kx) are HOST float64 arrays; kern may be
- * NULL (no smoothing). scale: the flux scaling after the convolution (1 for DENSFLUX models). */
+ * NULL (no smoothing); its sizes are odd with kx <= 25. scale: the flux scaling after the convolution (1 for
+ * DENSFLUX models). */
 typedef struct bb_model bb_model;
 int bb_model_create(const bb_grid* g, const double* cd2i, const double* vedges, const double* ptab, int ntab,
                     const double* kern, int ky, int kx, double scale, bb_model** out);
 int bb_model_destroy(bb_model* m);
+
+/* The pixel tile (TX x TY) of the ANALYTIC deposit: by default the largest whose spectra fit the LDS of a CU;
+ * set_tile lets a tuner choose another (rejected if it does not fit). No counterpart in the reference. */
+int bb_model_set_tile(bb_model* m, int TX, int TY);
+int bb_model_get_tile(const bb_model* m, int* TX, int* TY);
 
 /* The azimuth plan of a set of sub-ring radii R0 (arcmin, HOST array, nsub): the ANALYTIC samples of every
  * sub-ring (naz = max(int(3 circumference / pixel + 0.5), 16), two radial samples, galmod.py:339-345) and the
