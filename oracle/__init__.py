@@ -53,7 +53,7 @@ def lib():
         L.orc_ring_maps_analytic.restype = None
         L.orc_zgrid.argtypes = [P, P, C.c_int, C.c_int] + [C.c_double] * 11 + [C.c_int64] * 4 + [C.c_double]
         L.orc_zgrid.restype = C.c_int
-        L.orc_conv_same.argtypes = [P, P, C.c_int, C.c_int, C.c_int, P, C.c_int, C.c_int, C.c_double]
+        L.orc_conv_same.argtypes = [P, P, C.c_int, C.c_int, C.c_int, P, C.c_int, C.c_int, C.c_double, C.c_int]
         L.orc_conv_same.restype = None
         L.orc_sumsq_residuals.argtypes = [P, P, P, C.c_int, P, C.c_int, C.c_int, C.c_int]
         L.orc_sumsq_residuals.restype = C.c_double
@@ -118,13 +118,13 @@ def zgrid(nmax, ltype, z0, sinc, pixsize, dvdz, sig, R0, radsep, xpos, ypos, cde
     return zq[:n], wq[:n]
 
 
-def conv_same(model_yxz, kern, scale=1.0):
-    """orc_conv_same: beam-convolved model (ny, nx, nz) float64, thresholded at 1e-12 and scaled."""
+def conv_same(model_yxz, kern, scale=1.0, thresh=True):
+    """orc_conv_same: beam-convolved model (ny, nx, nz) float64, thresholded at 1e-12 (thresh) and scaled."""
     m = np.ascontiguousarray(model_yxz, np.float64)
     k = np.ascontiguousarray(kern, np.float64)
     out = np.empty_like(m)
     ny, nx, nz = m.shape
-    lib().orc_conv_same(_ptr(m), _ptr(out), ny, nx, nz, _ptr(k), k.shape[0], k.shape[1], float(scale))
+    lib().orc_conv_same(_ptr(m), _ptr(out), ny, nx, nz, _ptr(k), k.shape[0], k.shape[1], float(scale), int(bool(thresh)))
     return out
 
 

@@ -365,10 +365,11 @@ void orc_ring_maps_analytic(double* out, int nr, const double* sr, int nsub, con
 
 /* 'Same' 2D convolution of each channel of a yxz model with kern (ky, kx): the direct-sum definition of
  * FFTConvolver (smooth.py:145-165: full linear convolution cropped at [k//2, k//2 + n)), then values below
- * 1e-12 set to zero and the scale applied (galmod.py:1273-1283, and _conv_sparse galmod.py:974-1037).
+ * 1e-12 set to zero (thresh != 0; the ring maps of DiskModel.ring_maps, galmod.py:1342-1345, are not
+ * thresholded) and the scale applied (galmod.py:1273-1283, and _conv_sparse galmod.py:974-1037).
  * in/out (ny, nx, nz) float64. */
 void orc_conv_same(const double* in, double* out, int ny, int nx, int nz, const double* kern, int ky, int kx,
-                   double scale) {
+                   double scale, int thresh) {
     int hy = ky / 2, hx = kx / 2;
     memset(out, 0, sizeof(double) * (size_t)ny * nx * nz);
     for (int y = 0; y < ny; y++)
@@ -390,7 +391,7 @@ void orc_conv_same(const double* in, double* out, int ny, int nx, int nz, const 
     size_t n = (size_t)ny * nx * nz;
     for (size_t i = 0; i < n; i++) {
         double v = out[i];
-        if (v < 1.E-12) out[i] = 0.;
+        if (thresh && v < 1.E-12) out[i] = 0.;
         else if (scale != 1.0) out[i] = v * scale;
     }
 }
