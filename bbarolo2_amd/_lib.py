@@ -29,6 +29,12 @@ class bb_box(C.Structure):
     _fields_ = [("blo0", C.c_int32), ("blo1", C.c_int32), ("bhi0", C.c_int32), ("bhi1", C.c_int32)]
 
 
+class bb_profile_desc(C.Structure):
+    _fields_ = [("nr", C.c_int32), ("npix", C.c_int32), ("nok", C.c_int32), ("pad0", C.c_int32),
+                ("smooth", C.c_double), ("corr", C.c_double), ("floor", C.c_double), ("norm", C.c_double),
+                ("dmed", C.c_double)]
+
+
 _P = C.c_void_p
 _SIGS = {
     "bb_last_error": (C.c_char_p, []),
@@ -46,6 +52,15 @@ _SIGS = {
     "bb_residual_work_size": (C.c_int64, [C.c_int, C.c_int64]),
     "bb_residual_gauss": (C.c_int, [_P, _P, C.c_int, C.c_int64, C.c_double, C.c_double, _P, C.c_int64, _P, _P,
                                     _P]),
+    "bb_ring_maps_work_size": (C.c_int64, [_P, C.c_int]),
+    "bb_ring_maps_deposit": (C.c_int, [_P, _P, bb_box, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, C.c_double,
+                                       _P, C.c_int64, _P, _P]),
+    "bb_ring_maps_finish": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _P, _P]),
+    "bb_profile_create": (C.c_int, [C.POINTER(bb_profile_desc), _P, _P, _P, _P, _P, C.POINTER(_P)]),
+    "bb_profile_destroy": (C.c_int, [_P]),
+    "bb_profile_work_size": (C.c_int64, [_P, C.c_int]),
+    "bb_profile_solve": (C.c_int, [_P, _P, C.c_int, _P, C.c_int64, _P, _P]),
+    "bb_profile_set_dens": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P]),
 }
 
 _lib = None

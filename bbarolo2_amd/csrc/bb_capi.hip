@@ -23,6 +23,24 @@ extern "C" int bb_launch_conv2d_same(const float* in, float* out, int B, int by,
                                      int ky, int kx, float scale, hipStream_t st);
 extern "C" int bb_conv_max_kx();
 extern "C" int bb_residual_blocks(int64_t nvox);
+extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* sr, int nsub, int B, const int* naz,
+                                   const int4* chunks, int nchunks, const double2* az, double* sub, double* wtab,
+                                   const int* widx, const double* wval, int nmap, int map0, double qunit,
+                                   unsigned long long* counts, hipStream_t st);
+extern "C" size_t bb_rm_finish_lds(int ky, int kx);
+extern "C" int bb_launch_rm_finish(const unsigned long long* counts, double* maps, int P, int by, int bx,
+                                   const double* kern, int ky, int kx, double unit, double scale, int convolve,
+                                   hipStream_t st);
+extern "C" int bb_gram_blocks(int npix);
+extern "C" size_t bb_gram_entries(int n);
+extern "C" size_t bb_nnls_lds(int n);
+extern "C" int bb_nnls_max();
+extern "C" int bb_launch_profile_solve(int n, int nr, int nok, double smooth, double corr, double floor_, double norm,
+                                       double dmed, const double* maps, const double* wpix, const double* ypix,
+                                       int npix, const double* lcoef, const int* lcol, const int* near, int B,
+                                       double* partial, double* gram, double* Mall, double* x, hipStream_t st);
+extern "C" int bb_launch_set_dens(const double* x, int B, int n, int nr, const int* widx, const double* wval, int nsub,
+                                  double* sr, float* dens, hipStream_t st);
 extern "C" int bb_launch_residual(const float* model, const float* dvalid, int B, int64_t nvox, double sigma2,
                                   double eps2, double* partial, double* out_ss, double* out_lg, hipStream_t st);
 
@@ -54,6 +72,7 @@ struct bb_model {
     double* d_ptab;
     int ky, kx;
     float* d_kern;
+    double* d_kern64;
     double scale;
     int TX, TY;        // pixel tile of the ANALYTIC deposit (its spectra fit the LDS budget)
 };
@@ -112,6 +131,7 @@ extern "C" int bb_model_destroy(bb_model* m) {
     (void)hipFree(m->d_vedges);
     (void)hipFree(m->d_ptab);
     (void)hipFree(m->d_kern);
+    (void)hipFree(m->d_kern64);
     (void)hipSetDevice(dev);
     delete m;
     return 0;
@@ -142,6 +162,7 @@ extern "C" int bb_model_create(const bb_grid* g, const double* cd2i, const doubl
         m->ky = ky;
         m->kx = kx;
         rc = upload(&m->d_kern, kf.data(), kf.size(), "upload kernel");
+        if (!rc) rc = upload(&m->d_kern64, kern, kf.size(), "upload kernel");
     }
     if (!rc) {
         // the largest pixel tile whose spectra (float32), with the channel and Phi tables, fit the LDS of a CU
@@ -319,4 +340,142 @@ extern "C" int bb_residual_gauss(const float* model, const float* dvalid, int B,
                     (long long)bb_residual_work_size(B, nvox));
     int rc = bb_launch_residual(model, dvalid, B, nvox, sigma2, eps2, work, out_ss, out_lg, (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_residual_gauss launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Ring maps and the deconvolved density profile.
+extern "C" int64_t bb_ring_maps_work_size(const bb_azplan* p, int B) {
+    if (!p || B < 0) return -1;
+    return (int64_t)((size_t)B * p->nsub * (bb_analytic_sub_bytes() + bb_analytic_wt_bytes()) + 256);
+}
+
+extern "C" int bb_ring_maps_deposit(const bb_model* m, const bb_azplan* p, bb_box box, const double* sr, int nsub,
+                                    int B, const int32_t* widx, const double* wval, int nmap, int map0, double qunit,
+                                    void* work, int64_t work_bytes, uint64_t* counts, void* stream) {
+    if (!m || !p) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: model or plan is NULL");
+    int rc = check_box(m, box);
+    if (rc) return rc;
+    if (nsub != p->nsub) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: nsub %d != plan's %d", nsub, p->nsub);
+    if (B < 0 || B > 65535) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: B=%d out of range", B);
+    if (nmap < 1 || map0 < 0 || map0 >= nmap) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: map0 %d / nmap %d", map0, nmap);
+    if (!(qunit > 0) || !isfinite(qunit)) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: qunit must be > 0");
+    if (nsub == 0 || B == 0) return 0;
+    if (!sr || !widx || !wval || !work || !counts) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: NULL array");
+    if (work_bytes < bb_ring_maps_work_size(p, B))
+        return fail(BB_ERR_ARG, "bb_ring_maps_deposit: work holds %lld bytes, needs %lld", (long long)work_bytes,
+                    (long long)bb_ring_maps_work_size(p, B));
+    char* w = (char*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
+    double* sub = (double*)w;
+    double* wtab = (double*)(w + (size_t)B * nsub * bb_analytic_sub_bytes());
+    rc = bb_launch_ring_maps(&m->g, box, sr, nsub, B, p->d_naz, p->d_chunks, p->nchunks, p->d_az, sub, wtab,
+                             (const int*)widx, wval, nmap, map0, qunit, (unsigned long long*)counts,
+                             (hipStream_t)stream);
+    return rc ? fail(BB_ERR_HIP, "bb_ring_maps_deposit launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
+}
+
+extern "C" int bb_ring_maps_finish(const bb_model* m, const uint64_t* counts, int P, int by, int bx, double unit,
+                                   int convolve, double* maps, void* stream) {
+    if (!m) return fail(BB_ERR_ARG, "bb_ring_maps_finish: model is NULL");
+    if (P < 0 || P > 65535 || by < 1 || bx < 1) return fail(BB_ERR_ARG, "bb_ring_maps_finish: bad sizes");
+    if (convolve && !m->d_kern64) return fail(BB_ERR_ARG, "bb_ring_maps_finish: the model has no beam kernel");
+    if (convolve && bb_rm_finish_lds(m->ky, m->kx) > BB_LDS_MAX)
+        return fail(BB_ERR_ARG, "bb_ring_maps_finish: a %dx%d kernel is too large", m->ky, m->kx);
+    if (P == 0) return 0;
+    if (!counts || !maps) return fail(BB_ERR_ARG, "bb_ring_maps_finish: NULL array");
+    int rc = bb_launch_rm_finish((const unsigned long long*)counts, maps, P, by, bx, m->d_kern64, m->ky, m->kx, unit,
+                                 m->scale, convolve, (hipStream_t)stream);
+    return rc ? fail(BB_ERR_HIP, "bb_ring_maps_finish launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
+}
+
+struct bb_profile {
+    bb_profile_desc d;
+    int device;
+    double* d_wpix;
+    double* d_ypix;
+    double* d_lcoef;
+    int* d_lcol;
+    int* d_near;
+};
+
+extern "C" int bb_profile_destroy(bb_profile* p) {
+    if (!p) return 0;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipSetDevice(p->device);
+    (void)hipFree(p->d_wpix);
+    (void)hipFree(p->d_ypix);
+    (void)hipFree(p->d_lcoef);
+    (void)hipFree(p->d_lcol);
+    (void)hipFree(p->d_near);
+    (void)hipSetDevice(dev);
+    delete p;
+    return 0;
+}
+
+extern "C" int bb_profile_create(const bb_profile_desc* d, const double* wpix, const double* ypix, const double* lcoef,
+                                 const int32_t* lcol, const int32_t* near, bb_profile** out) {
+    if (!out) return fail(BB_ERR_ARG, "bb_profile_create: out is NULL");
+    *out = nullptr;
+    if (!d || !wpix || !ypix) return fail(BB_ERR_ARG, "bb_profile_create: NULL argument");
+    const int n = d->nr + 1;
+    if (d->nr < 1 || n > bb_nnls_max()) return fail(BB_ERR_ARG, "bb_profile_create: nr = %d (1..%d)", d->nr, bb_nnls_max() - 1);
+    if (d->npix < 1 || d->nok < 1 || d->nok > d->npix) return fail(BB_ERR_ARG, "bb_profile_create: bad pixel counts");
+    const bool prior = d->smooth > 0 && n >= 3;
+    if (prior && (!lcoef || !lcol || !near)) return fail(BB_ERR_ARG, "bb_profile_create: the prior needs its rows");
+    if (prior && (!(d->norm > 0) || !(d->dmed > 0) || !(d->floor >= 0)))
+        return fail(BB_ERR_ARG, "bb_profile_create: norm and dmed must be > 0, floor >= 0");
+    if (prior)
+        for (int i = 0; i < 3 * (n - 2); i++)
+            if (lcol[i] < 0 || lcol[i] >= n) return fail(BB_ERR_ARG, "bb_profile_create: column %d out of range", lcol[i]);
+    bb_profile* p = new bb_profile();
+    memset(p, 0, sizeof *p);
+    p->d = *d;
+    (void)hipGetDevice(&p->device);
+    int rc = upload(&p->d_wpix, wpix, (size_t)d->npix, "upload pixel weights");
+    if (!rc) rc = upload(&p->d_ypix, ypix, (size_t)d->npix, "upload map");
+    if (!rc && prior) rc = upload(&p->d_lcoef, lcoef, (size_t)3 * (n - 2), "upload curvature rows");
+    if (!rc && prior) rc = upload(&p->d_lcol, (const int*)lcol, (size_t)3 * (n - 2), "upload curvature rows");
+    if (!rc && prior) rc = upload(&p->d_near, (const int*)near, (size_t)(n - 2), "upload curvature rows");
+    if (rc) {
+        bb_profile_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return 0;
+}
+
+extern "C" int64_t bb_profile_work_size(const bb_profile* p, int B) {
+    if (!p || B < 0) return -1;
+    const int n = p->d.nr + 1, m = n + 1;
+    const size_t dbl = (size_t)B * ((size_t)bb_gram_blocks(p->d.npix) * bb_gram_entries(n) + (size_t)m * m + (size_t)n * n);
+    return (int64_t)(dbl * sizeof(double) + 256);
+}
+
+extern "C" int bb_profile_solve(const bb_profile* p, const double* maps, int B, void* work, int64_t work_bytes,
+                                double* x, void* stream) {
+    if (!p) return fail(BB_ERR_ARG, "bb_profile_solve: profile is NULL");
+    if (B < 0 || B > 65535) return fail(BB_ERR_ARG, "bb_profile_solve: B=%d out of range", B);
+    if (B == 0) return 0;
+    if (!maps || !work || !x) return fail(BB_ERR_ARG, "bb_profile_solve: NULL array");
+    if (work_bytes < bb_profile_work_size(p, B))
+        return fail(BB_ERR_ARG, "bb_profile_solve: work holds %lld bytes, needs %lld", (long long)work_bytes,
+                    (long long)bb_profile_work_size(p, B));
+    const int n = p->d.nr + 1, m = n + 1;
+    double* partial = (double*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
+    double* gram = partial + (size_t)B * bb_gram_blocks(p->d.npix) * bb_gram_entries(n);
+    double* Mall = gram + (size_t)B * m * m;
+    const bool prior = p->d.smooth > 0 && n >= 3;
+    int rc = bb_launch_profile_solve(n, p->d.nr, p->d.nok, prior ? p->d.smooth : 0.0, p->d.corr, p->d.floor, p->d.norm,
+                                     p->d.dmed, maps, p->d_wpix, p->d_ypix, p->d.npix, p->d_lcoef, p->d_lcol, p->d_near,
+                                     B, partial, gram, Mall, x, (hipStream_t)stream);
+    return rc ? fail(BB_ERR_HIP, "bb_profile_solve launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
+}
+
+extern "C" int bb_profile_set_dens(const double* x, int B, int n, int nr, const int32_t* widx, const double* wval,
+                                   int nsub, double* sr, float* dens, void* stream) {
+    if (B < 0 || B > 65535 || nr < 1 || n < nr || nsub < 0) return fail(BB_ERR_ARG, "bb_profile_set_dens: bad sizes");
+    if (B == 0 || nsub == 0) return 0;
+    if (!x || !widx || !wval || !sr) return fail(BB_ERR_ARG, "bb_profile_set_dens: NULL array");
+    int rc = bb_launch_set_dens(x, B, n, nr, (const int*)widx, wval, nsub, sr, dens, (hipStream_t)stream);
+    return rc ? fail(BB_ERR_HIP, "bb_profile_set_dens launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }

@@ -53,14 +53,14 @@ __device__ __forceinline__ double zq_at(int kz, int half, int nin, double step, 
 // ---------------------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(64)
 k_an_prep(bb_grid g, bb_box box, const double* __restrict__ srall, int nsub, const int* __restrict__ naz_tab,
-          double* __restrict__ suball, double* __restrict__ wtaball) {
+          double* __restrict__ suball, double* __restrict__ wtaball, int unit_dens) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
     if (s >= nsub) return;
     const double* sr = srall + (size_t)b * 16 * nsub;
     double* q = suball + ((size_t)b * nsub + s) * BB_NSF;
     double* wt = wtaball + ((size_t)b * nsub + s) * BB_ZCAP;
     auto S = [&](int row) { return sr[(size_t)row * nsub + s]; };
-    const double R0 = S(0), dens = S(5);
+    const double R0 = S(0), dens = unit_dens ? 1.0 : S(5);    // (unit_dens: the ring maps)
     const int64_t naz = naz_tab[s];
     const double ring_area_pix = 2 * BB_PI * R0 * g.radsep / g.pixarea;
     const double r_inner = R0 - 0.5 * g.radsep;
@@ -647,7 +647,7 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
                                          const double* cd2i, const double* vedges, const double* ptab, int ntab,
                                          float* out, int accumulate, hipStream_t st) {
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
-    hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, naz, sub, wtab);
+    hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, naz, sub, wtab, 0);
     hipLaunchKernelGGL(k_an_bbox, dim3((nchunks * BB_CS + 255) / 256, B), dim3(256), 0, st, *g, box, chunks, nchunks,
                        az, sub, nsub, bbox, sbox);
     const int ntiles = ((bx + TX - 1) / TX) * ((by + TY - 1) / TY);
@@ -667,3 +667,78 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
 extern "C" size_t bb_analytic_sub_bytes() { return BB_NSF * sizeof(double); }
 extern "C" size_t bb_analytic_wt_bytes() { return BB_ZCAP * sizeof(double); }
 extern "C" int bb_analytic_chunk() { return BB_CS; }
+
+// ---------------------------------------------------------------------------------------------------------
+// Unit-density ring maps (galmod.py:468-536): the samples of the deposit above summed over the channels.
+// Every run of equal-pixel vertical samples adds fluxsc x (its vertical weights) x (the side weight) x wval to
+// the total-intensity map of each ring its sub-ring interpolates. The sub-ring records come from k_an_prep with
+// unit densities, so their flux field is fluxsc. Accumulation is 64-bit fixed point in global
+// memory (a contribution x adds round(x / qunit)): exact sums, independent of the order of the atomics. One
+// thread per azimuth sample; thread t takes chunk t mod nchunks, so the lanes of one wavefront hold samples
+// of different chunks and rarely meet on one pixel.
+__global__ void __launch_bounds__(256)
+k_rm_deposit(bb_grid g, bb_box box, const int4* __restrict__ chunks, int nchunks, const double2* __restrict__ az,
+             const double* __restrict__ suball, const double* __restrict__ wtaball, int nsub,
+             const int2* __restrict__ widx, const double2* __restrict__ wval, int nmap, int map0, double inv_q,
+             unsigned long long* __restrict__ counts) {
+    const int b = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nchunks * BB_CS) return;
+    const int c = (int)(t % nchunks), lane = (int)(t / nchunks);
+    const int4 ch = chunks[c];
+    if (lane >= ch.z) return;
+    const int s = ch.x >> 1, kr = ch.x & 1;
+    int2 wi = widx[s];
+    if (wi.x >= nmap - map0) wi.x = -1;      // (out-of-range ring indices are ignored)
+    if (wi.y >= nmap - map0) wi.y = -1;
+    if (wi.x < 0 && wi.y < 0) return;
+    const double* q = suball + ((size_t)b * nsub + s) * BB_NSF;
+    const double* wt = wtaball + ((size_t)b * nsub + s) * BB_ZCAP;
+    Sub u;
+    load_sub(u, q, kr);
+    const double2 cs = az[ch.w + lane];
+    const double wside = cs.x >= 0.0 ? u.wrec : u.wapp;
+    if (wside == 0.0) return;
+    const double2 wv = wval[s];
+    const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
+    const size_t plane = (size_t)by * bx;
+    unsigned long long* o0 = wi.x >= 0 ? counts + ((size_t)b * nmap + map0 + wi.x) * plane : nullptr;
+    unsigned long long* o1 = wi.y >= 0 ? counts + ((size_t)b * nmap + map0 + wi.y) * plane : nullptr;
+    const double x = u.R * cs.x, yb = u.R * cs.y * u.cinc, xs = x * u.spa, xc = x * u.cpa;
+    int64_t p0, p1;
+    upix(u, g, xs, xc, yb, uz(u, 0), p0, p1);
+    double ws = uw_(u, nullptr, wt, 0, g.ltype);
+    for (int kz = 1; kz <= u.nzz; kz++) {
+        int64_t n0 = 0, n1 = 0;
+        const bool more = kz < u.nzz;
+        if (more) {
+            upix(u, g, xs, xc, yb, uz(u, kz), n0, n1);
+            if (n0 == p0 && n1 == p1) {
+                ws += uw_(u, nullptr, wt, kz, g.ltype);
+                continue;
+            }
+        }
+        if (!(p0 < box.blo0 || p0 >= box.bhi0 || p1 < box.blo1 || p1 >= box.bhi1)) {
+            const size_t pix = (size_t)(p1 - box.blo1) * bx + (size_t)(p0 - box.blo0);
+            const double base = u.flux * ws;
+            if (o0) atomicAdd(o0 + pix, (unsigned long long)(base * wv.x * wside * inv_q + 0.5));
+            if (o1) atomicAdd(o1 + pix, (unsigned long long)(base * wv.y * wside * inv_q + 0.5));
+        }
+        if (more) {
+            p0 = n0;
+            p1 = n1;
+            ws = uw_(u, nullptr, wt, kz, g.ltype);
+        }
+    }
+}
+
+extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* sr, int nsub, int B, const int* naz,
+                                   const int4* chunks, int nchunks, const double2* az, double* sub, double* wtab,
+                                   const int* widx, const double* wval, int nmap, int map0, double qunit,
+                                   unsigned long long* counts, hipStream_t st) {
+    hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, naz, sub, wtab, 1);
+    const int64_t nt = (int64_t)nchunks * BB_CS;
+    hipLaunchKernelGGL(k_rm_deposit, dim3((unsigned)((nt + 255) / 256), B), dim3(256), 0, st, *g, box, chunks, nchunks,
+                       az, sub, wtab, nsub, (const int2*)widx, (const double2*)wval, nmap, map0, 1.0 / qunit, counts);
+    return (int)hipGetLastError();
+}

@@ -1,0 +1,463 @@
+// The deconvolved density profile on the GPU (bbarolo/profile.py:33-138, bayesian/density.py:13-88): the
+// per-evaluation densities of the rings when the geometry is free and DENS is not.
+//
+//   k_rm_finish   fixed-point ring-map counts -> FP64 maps, convolved with the beam ('same', no 1e-12 cut) and
+//                 scaled (DiskModel.ring_maps, galmod.py:1340-1345). One workgroup per 16 x 16 output tile of
+//                 one map; the input tile with its halo and the taps live in LDS.
+//   k_gram        the Gram matrix of the weighted system [A | b], A = K / s and b = y / s on the usable pixels
+//                 (weight 1/s, 0 elsewhere): G = A^T A, h = A^T b and b^T b in one (n+1) x (n+1) matrix. Each
+//                 workgroup sums a range of pixels staged through LDS; k_gram_reduce adds the partial matrices
+//                 in a fixed order (deterministic).
+//   k_nnls        one wavefront per evaluation: solve_profile on the normal equations. The QR of the
+//                 reference reduces the least squares to the n x n triangle R with R^T R = A^T A and
+//                 R^T c = A^T b, so min |R x - c|^2 + |w L x|^2 and min x^T (G + L^T W^2 L) x - 2 h^T x have the
+//                 same minimiser over x >= 0; it is found with the Lawson-Hanson active set, the passive
+//                 systems by Cholesky in LDS. First solution with the relative prior, misfit from the Gram
+//                 matrix (b^T b - 2 h^T x + x^T G x), curvature weights from it, final solution.
+//   k_set_dens    the float32 densities (the ring table of Bayesian3DFIT stores float32, bayesian/rings.py:54-61)
+//                 interpolated into the density row of the model's sub-ring tables.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#define BB_RM_T 16           // output tile side of k_rm_finish
+#define BB_GR_PIX 128        // pixels staged per pass of k_gram
+#define BB_GR_THREADS 256
+#define BB_NN_MAX 128        // largest system (rings + annulus) of k_nnls
+
+// ---------------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(BB_RM_T * BB_RM_T)
+k_rm_finish(const unsigned long long* __restrict__ counts, double* __restrict__ maps, int by, int bx,
+            const double* __restrict__ kern, int ky, int kx, double unit, double scale, int convolve) {
+    extern __shared__ double rm_lds[];
+    const int p = blockIdx.z;
+    const int tx = threadIdx.x % BB_RM_T, ty = threadIdx.x / BB_RM_T;
+    const int x0 = blockIdx.x * BB_RM_T, y0 = blockIdx.y * BB_RM_T;
+    const unsigned long long* in = counts + (size_t)p * by * bx;
+    double* out = maps + (size_t)p * by * bx;
+    if (!convolve) {
+        const int x = x0 + tx, y = y0 + ty;
+        if (x < bx && y < by) out[(size_t)y * bx + x] = (double)in[(size_t)y * bx + x] * unit;
+        return;
+    }
+    const int hy = ky / 2, hx = kx / 2;
+    const int W = BB_RM_T + kx - 1, H = BB_RM_T + ky - 1;
+    double* tile = rm_lds;
+    double* tk = rm_lds + W * H;
+    // input rows y0 - (ky-1-hy) .. y0 + T-1 + hy, columns likewise: out(y, x) = sum K[j][i] in[y + hy - j][x + hx - i]
+    const int iy0 = y0 + hy - (ky - 1), ix0 = x0 + hx - (kx - 1);
+    for (int e = threadIdx.x; e < W * H; e += blockDim.x) {
+        const int r = e / W, c = e % W;
+        const int iy = iy0 + r, ix = ix0 + c;
+        tile[e] = (iy >= 0 && iy < by && ix >= 0 && ix < bx) ? (double)in[(size_t)iy * bx + ix] : 0.0;
+    }
+    for (int e = threadIdx.x; e < ky * kx; e += blockDim.x) tk[e] = kern[e];
+    __syncthreads();
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= bx || y >= by) return;
+    double acc = 0.0;
+    for (int j = 0; j < ky; j++) {
+        const double* row = tile + (ty + ky - 1 - j) * W + tx + kx - 1;
+        const double* kr = tk + j * kx;
+        for (int i = 0; i < kx; i++) acc = fma(kr[i], row[-i], acc);
+    }
+    out[(size_t)y * bx + x] = acc * unit * scale;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Entry e of the upper triangle (i <= j) of an m x m matrix, row by row.
+__device__ __forceinline__ void tri_ij(int e, int m, int& i, int& j) {
+    int r = 0, rem = e;
+    while (rem >= m - r) {
+        rem -= m - r;
+        r++;
+    }
+    i = r;
+    j = r + rem;
+}
+
+// partial[b][blk][e]: sum over the pixels of block blk of A~_pi A~_pj, A~ = [K_0 w, ..., K_{n-1} w, y w].
+__global__ void __launch_bounds__(BB_GR_THREADS)
+k_gram(const double* __restrict__ maps, const double* __restrict__ wpix, const double* __restrict__ ypix, int n,
+       int npix, int ppb, double* __restrict__ partial) {
+    extern __shared__ double gr_lds[];
+    const int m = n + 1, ne = m * (m + 1) / 2;
+    const int b = blockIdx.y, blk = blockIdx.x, nblk = gridDim.x;
+    double* st = gr_lds;                        // [BB_GR_PIX][m + 1] (odd stride)
+    const int sm = m | 1;
+    constexpr int EPT = 8;                      // entries per thread per sweep
+    double acc[EPT];
+    int ei[EPT], ej[EPT];
+    const int p0 = blk * ppb, p1 = min(npix, p0 + ppb);
+    for (int sweep = 0; sweep * BB_GR_THREADS * EPT < ne; sweep++) {
+#pragma unroll
+        for (int k = 0; k < EPT; k++) {
+            acc[k] = 0.0;
+            const int e = (sweep * EPT + k) * BB_GR_THREADS + threadIdx.x;
+            if (e < ne) tri_ij(e, m, ei[k], ej[k]);
+            else ei[k] = -1;
+        }
+        for (int pa = p0; pa < p1; pa += BB_GR_PIX) {
+            const int np = min(BB_GR_PIX, p1 - pa);
+            __syncthreads();
+            for (int e = threadIdx.x; e < np * m; e += BB_GR_THREADS) {
+                const int c = e / np, q = e % np;     // column c, pixel q: coalesced reads of one map
+                const int pix = pa + q;
+                const double w = wpix[pix];
+                double v;
+                if (c < n) v = w != 0.0 ? maps[((size_t)b * n + c) * npix + pix] * w : 0.0;
+                else v = ypix[pix];
+                st[q * sm + c] = v;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < EPT; k++) {
+                if (ei[k] < 0) continue;
+                const double* a = st + ei[k];
+                const double* c = st + ej[k];
+                double s = acc[k];
+                for (int q = 0; q < np; q++) s = fma(a[q * sm], c[q * sm], s);
+                acc[k] = s;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < EPT; k++) {
+            const int e = (sweep * EPT + k) * BB_GR_THREADS + threadIdx.x;
+            if (e < ne) partial[((size_t)b * nblk + blk) * ne + e] = acc[k];
+        }
+    }
+}
+
+// gram[b] (m x m, symmetric) = sum over the blocks of partial[b], in block order.
+__global__ void k_gram_reduce(const double* __restrict__ partial, int n, int nblk, double* __restrict__ gram) {
+    const int m = n + 1, ne = m * (m + 1) / 2;
+    const int b = blockIdx.y;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    double s = 0.0;
+    for (int k = 0; k < nblk; k++) s += partial[((size_t)b * nblk + k) * ne + e];
+    int i, j;
+    tri_ij(e, m, i, j);
+    gram[(size_t)b * m * m + (size_t)i * m + j] = s;
+    gram[(size_t)b * m * m + (size_t)j * m + i] = s;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Non-negative least squares on the normal equations, one wavefront (64 lanes) per system. M (n x n, row
+// major, symmetric positive definite) in global memory, h (n) in LDS; the result in x (LDS).
+struct NnLds {
+    double* C;      // n x n Cholesky factor of the passive system
+    double* z;      // n: passive solution
+    double* x;      // n: current solution
+    double* g;      // n: gradient h - M x
+    double* h;      // n
+    int* P;         // n: passive flags
+    int* ban;       // n: excluded from the next choice
+    int* sh;        // scratch ints: [0] chosen index, [1] flag
+    double* shd;    // scratch doubles
+};
+
+// z = M_PP^-1 h_P (0 outside P) by Cholesky; false if a pivot is not positive.
+__device__ bool nn_passive_solve(const double* __restrict__ M, const NnLds& L, int n, int lane) {
+    double* C = L.C;
+    for (int e = lane; e < n * n; e += 64) {
+        const int i = e / n, j = e % n;
+        C[e] = (L.P[i] && L.P[j]) ? M[e] : (i == j ? 1.0 : 0.0);
+    }
+    for (int i = lane; i < n; i += 64) L.z[i] = L.P[i] ? L.h[i] : 0.0;
+    __syncthreads();
+    bool ok = true;
+    for (int k = 0; k < n; k++) {
+        const double d2 = C[k * n + k];
+        if (!(d2 > 0.0)) ok = false;
+        const double d = sqrt(d2 > 0.0 ? d2 : 1.0);
+        __syncthreads();
+        for (int i = k + 1 + lane; i < n; i += 64) C[i * n + k] /= d;
+        if (lane == 0) C[k * n + k] = d;
+        __syncthreads();
+        const int r = n - k - 1;
+        for (int e = lane; e < r * r; e += 64) {
+            const int i = k + 1 + e / r, j = k + 1 + e % r;
+            if (j <= i) C[i * n + j] -= C[i * n + k] * C[j * n + k];
+        }
+        __syncthreads();
+    }
+    // forward: C y = z
+    for (int k = 0; k < n; k++) {
+        const double yk = L.z[k] / C[k * n + k];
+        __syncthreads();
+        for (int i = k + 1 + lane; i < n; i += 64) L.z[i] -= C[i * n + k] * yk;
+        if (lane == 0) L.z[k] = yk;
+        __syncthreads();
+    }
+    // backward: C^T z = y
+    for (int k = n - 1; k >= 0; k--) {
+        const double zk = L.z[k] / C[k * n + k];
+        __syncthreads();
+        for (int i = lane; i < k; i += 64) L.z[i] -= C[k * n + i] * zk;
+        if (lane == 0) L.z[k] = zk;
+        __syncthreads();
+    }
+    return ok;
+}
+
+// Lawson-Hanson active set: returns false if the iteration limits were reached or a pivot failed.
+__device__ bool nn_solve(const double* __restrict__ M, const NnLds& L, int n, int lane) {
+    for (int i = lane; i < n; i += 64) {
+        L.x[i] = 0.0;
+        L.P[i] = 0;
+        L.ban[i] = 0;
+    }
+    __syncthreads();
+    const int maxit = 3 * n + 10;
+    int it = 0;
+    for (; it < maxit; it++) {
+        // gradient h - M x and the size of its round-off
+        for (int i = lane; i < n; i += 64) {
+            double s = L.h[i], a = fabs(L.h[i]);
+            const double* Mi = M + (size_t)i * n;
+            for (int j = 0; j < n; j++) {
+                const double t = Mi[j] * L.x[j];
+                s -= t;
+                a += fabs(t);
+            }
+            L.g[i] = s;
+            L.shd[1 + i] = a;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            double amax = 0.0;
+            for (int i = 0; i < n; i++) amax = fmax(amax, L.shd[1 + i]);
+            double gb = 64.0 * (double)n * 2.220446049250313e-16 * amax;
+            int best = -1;
+            for (int i = 0; i < n; i++)
+                if (!L.P[i] && !L.ban[i] && L.g[i] > gb) {
+                    gb = L.g[i];
+                    best = i;
+                }
+            L.sh[0] = best;
+            if (best >= 0) L.P[best] = 1;
+        }
+        __syncthreads();
+        const int j = L.sh[0];
+        if (j < 0) break;
+        for (int inner = 0;; inner++) {
+            if (inner >= maxit) return false;
+            if (!nn_passive_solve(M, L, n, lane)) return false;
+            if (lane == 0) {
+                int state;          // 0: feasible, 1: the new index cannot enter, 2: step to the boundary
+                if (inner == 0 && !(L.z[j] > 0.0)) {
+                    L.P[j] = 0;
+                    L.ban[j] = 1;
+                    state = 1;
+                } else {
+                    double alpha = 2.0;
+                    int imin = -1;
+                    for (int i = 0; i < n; i++)
+                        if (L.P[i] && !(L.z[i] > 0.0)) {
+                            const double a = L.x[i] / (L.x[i] - L.z[i]);
+                            if (imin < 0 || a < alpha) {
+                                alpha = a;
+                                imin = i;
+                            }
+                        }
+                    if (imin < 0) {
+                        for (int i = 0; i < n; i++) {
+                            L.x[i] = L.z[i];
+                            L.ban[i] = 0;
+                        }
+                        state = 0;
+                    } else {
+                        for (int i = 0; i < n; i++) {
+                            L.ban[i] = 0;
+                            if (!L.P[i]) continue;
+                            const double xi = L.x[i] + alpha * (L.z[i] - L.x[i]);
+                            if (i == imin || !(xi > 0.0)) {
+                                L.x[i] = 0.0;
+                                L.P[i] = 0;
+                            } else {
+                                L.x[i] = xi;
+                            }
+                        }
+                        state = 2;
+                    }
+                }
+                L.sh[1] = state;
+            }
+            __syncthreads();
+            if (L.sh[1] != 2) break;
+        }
+    }
+    return it < maxit;
+}
+
+struct ProfConst {
+    int n, nr, nok;
+    double smooth, corr, floor, norm, dmed;
+};
+
+// M = G + sum_r w2[r] l_r^T l_r over the nrows curvature rows (lcoef / lcol: 3 per row; w2 NULL: wconst2).
+__device__ void nn_build(double* __restrict__ M, const double* __restrict__ G, int n, const double* __restrict__ lcoef,
+                         const int* __restrict__ lcol, const double* w2, double wconst2, int nrows, int lane) {
+    __syncthreads();
+    for (int e = lane; e < n * n; e += 64) {
+        const int i = e / n, j = e % n;
+        double s = G[i * (n + 1) + j];
+        for (int r = 0; r < nrows; r++) {
+            const int* c = lcol + 3 * r;
+            const int a = c[0] == i ? 0 : c[1] == i ? 1 : c[2] == i ? 2 : -1;
+            const int d = c[0] == j ? 0 : c[1] == j ? 1 : c[2] == j ? 2 : -1;
+            if (a >= 0 && d >= 0) s += (w2 ? w2[r] : wconst2) * lcoef[3 * r + a] * lcoef[3 * r + d];
+        }
+        M[e] = s;
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(64)
+k_nnls(ProfConst pc, const double* __restrict__ gram, const double* __restrict__ lcoef, const int* __restrict__ lcol,
+       const int* __restrict__ near, double* __restrict__ Mall, double* __restrict__ xout) {
+    extern __shared__ double nn_lds[];
+    const int n = pc.n, m = n + 1, lane = threadIdx.x, b = blockIdx.x;
+    NnLds L;
+    L.C = nn_lds;
+    L.z = L.C + n * n;
+    L.x = L.z + n;
+    L.g = L.x + n;
+    L.h = L.g + n;
+    L.shd = L.h + n;                 // 1 + n doubles
+    double* w2 = L.shd + 1 + n;      // n doubles
+    double* x0 = w2 + n;             // n doubles
+    L.P = (int*)(x0 + n);
+    L.ban = L.P + n;
+    L.sh = L.ban + n;
+    const double* G = gram + (size_t)b * m * m;
+    double* M = Mall + (size_t)b * n * n;
+    for (int i = lane; i < n; i += 64) L.h[i] = G[i * m + n];
+    __syncthreads();
+    bool ok = true;
+    if (!(pc.smooth > 0.0) || n < 3) {
+        nn_build(M, G, n, lcoef, lcol, nullptr, 0.0, 0, lane);
+        ok = nn_solve(M, L, n, lane);
+    } else {
+        double tr = 0.0;
+        for (int i = 0; i < n; i++) tr += G[i * m + i];
+        const double srel = sqrt(0.1 * tr / fmax(pc.norm, 1e-300));
+        nn_build(M, G, n, lcoef, lcol, nullptr, srel * srel, n - 2, lane);
+        ok = nn_solve(M, L, n, lane);
+        for (int i = lane; i < n; i += 64) x0[i] = L.x[i];
+        __syncthreads();
+        double xm = 0.0;
+        for (int i = 0; i < n - 1; i++) xm += x0[i];
+        xm /= (double)(n - 1);
+        if (ok && xm > 0.0) {
+            // misfit of the first solution: b^T b - 2 h^T x0 + x0^T G x0
+            double q = G[n * m + n];
+            for (int i = 0; i < n; i++) {
+                double gx = 0.0;
+                for (int j = 0; j < n; j++) gx += G[i * m + j] * x0[j];
+                q += x0[i] * (gx - 2.0 * L.h[i]);
+            }
+            const int dof = pc.nok - n > 1 ? pc.nok - n : 1;
+            const double chi2 = q / (double)dof;
+            const double infl = fmax(pc.corr, 1.0) * fmax(chi2, 1.0);
+            const double wfl = sqrt(pc.floor) * srel;
+            for (int r = lane; r < n - 2; r += 64) {
+                double w;
+                if (near[r]) {
+                    w = srel;
+                } else {
+                    const double xc = fmax(x0[lcol[3 * r + 1]], 0.1 * xm);
+                    w = fmax(sqrt(infl) * pc.dmed * pc.dmed / (pc.smooth * xc), wfl);
+                }
+                w2[r] = w * w;
+            }
+            __syncthreads();
+            nn_build(M, G, n, lcoef, lcol, w2, 0.0, n - 2, lane);
+            ok = nn_solve(M, L, n, lane);
+        }
+    }
+    for (int i = lane; i < n; i += 64) xout[(size_t)b * n + i] = ok ? L.x[i] : __builtin_nan("");
+}
+
+// dens (B, nr) = float32(x[:, :nr]); sr[b, 5, s] = sum_m wval[s][m] dens[b][widx[s][m]].
+__global__ void k_set_dens(const double* __restrict__ x, int n, int nr, const int2* __restrict__ widx,
+                           const double2* __restrict__ wval, int nsub, double* __restrict__ sr, float* __restrict__ dens) {
+    const int b = blockIdx.y;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const double* xb = x + (size_t)b * n;
+    if (dens && s < nr) dens[(size_t)b * nr + s] = (float)xb[s];
+    if (s >= nsub) return;
+    const int2 wi = widx[s];
+    const double2 wv = wval[s];
+    double d = 0.0;
+    if (wi.x >= 0 && wi.x < nr) d += wv.x * (double)(float)xb[wi.x];
+    if (wi.y >= 0 && wi.y < nr) d += wv.y * (double)(float)xb[wi.y];
+    sr[((size_t)b * 16 + 5) * nsub + s] = d;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+extern "C" size_t bb_rm_finish_lds(int ky, int kx) {
+    return (size_t)((BB_RM_T + ky - 1) * (BB_RM_T + kx - 1) + ky * kx) * sizeof(double);
+}
+
+extern "C" int bb_launch_rm_finish(const unsigned long long* counts, double* maps, int P, int by, int bx,
+                                   const double* kern, int ky, int kx, double unit, double scale, int convolve,
+                                   hipStream_t st) {
+    dim3 grid((bx + BB_RM_T - 1) / BB_RM_T, (by + BB_RM_T - 1) / BB_RM_T, P);
+    const size_t lds = convolve ? bb_rm_finish_lds(ky, kx) : 0;
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_rm_finish, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           160 * 1024);
+        if (e != hipSuccess) return (int)e;
+    }
+    hipLaunchKernelGGL(k_rm_finish, grid, dim3(BB_RM_T * BB_RM_T), lds, st, counts, maps, by, bx, kern, ky, kx, unit,
+                       scale, convolve);
+    return (int)hipGetLastError();
+}
+
+extern "C" int bb_gram_blocks(int npix) { return (npix + 2047) / 2048; }
+extern "C" size_t bb_gram_entries(int n) { return (size_t)(n + 1) * (n + 2) / 2; }
+extern "C" size_t bb_nnls_lds(int n) { return (size_t)(n * n + 8 * n + 1) * sizeof(double) + 3 * n * sizeof(int) + 16; }
+extern "C" int bb_nnls_max() { return BB_NN_MAX; }
+
+extern "C" int bb_launch_profile_solve(int n, int nr, int nok, double smooth, double corr, double floor_, double norm,
+                                       double dmed, const double* maps, const double* wpix, const double* ypix,
+                                       int npix, const double* lcoef, const int* lcol, const int* near, int B,
+                                       double* partial, double* gram, double* Mall, double* x, hipStream_t st) {
+    const int nblk = bb_gram_blocks(npix);
+    const int ppb = 2048;
+    const int m = n + 1;
+    const size_t lds = (size_t)BB_GR_PIX * (m | 1) * sizeof(double);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_gram, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return (int)e;
+    }
+    hipLaunchKernelGGL(k_gram, dim3(nblk, B), dim3(BB_GR_THREADS), lds, st, maps, wpix, ypix, n, npix, ppb, partial);
+    const int ne = (int)bb_gram_entries(n);
+    hipLaunchKernelGGL(k_gram_reduce, dim3((ne + 255) / 256, B), dim3(256), 0, st, partial, n, nblk, gram);
+    ProfConst pc;
+    pc.n = n;
+    pc.nr = nr;
+    pc.nok = nok;
+    pc.smooth = smooth;
+    pc.corr = corr;
+    pc.floor = floor_;
+    pc.norm = norm;
+    pc.dmed = dmed;
+    const size_t nl = bb_nnls_lds(n);
+    if (nl > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_nnls, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return (int)e;
+    }
+    hipLaunchKernelGGL(k_nnls, dim3(B), dim3(64), nl, st, pc, gram, lcoef, lcol, near, Mall, x);
+    return (int)hipGetLastError();
+}
+
+extern "C" int bb_launch_set_dens(const double* x, int B, int n, int nr, const int* widx, const double* wval, int nsub,
+                                  double* sr, float* dens, hipStream_t st) {
+    const int nt = nsub > nr ? nsub : nr;
+    hipLaunchKernelGGL(k_set_dens, dim3((nt + 255) / 256, B), dim3(256), 0, st, x, n, nr, (const int2*)widx,
+                       (const double2*)wval, nsub, sr, dens);
+    return (int)hipGetLastError();
+}

@@ -159,9 +159,55 @@ class DiskModel:
         a = m[0].double().cpu().numpy()
         return np.ascontiguousarray(a.transpose(2, 0, 1)) if layout == "zyx" else a
 
-    def array_batch(self, sr, blo, bhi, convolve=True, out=None, work=None, stream=None):
+    def ring_maps(self, rings, blo=None, bhi=None, ringmode=None):
+        """Total-intensity maps (nr, by, bx) float64 of the rings with unit density, beam-convolved when smoothing
+        is on (DiskModel.ring_maps, galmod.py:1317-1345): sum_r dens_r maps[r] is the total-intensity map of the
+        model of the rings. Computed on the device with the ANALYTIC samples."""
+        from .profile import weights_of, QBITS
+        torch = _torch()
+        if blo is None or bhi is None:
+            blo, bhi = (0, 0), (self.setup.dimx, self.setup.dimy)
+        st = self.setup
+        mode = self.ringmode if ringmode is None else ringmode_name(ringmode)
+        mr = model_rings(rings, mode, st.radsep).copy()
+        mr.dens = np.ones(mr.nr)
+        sr = st.subrings(mr)[0]
+        cols = []
+        for r in range(rings.nr):
+            unit = rings.copy()
+            unit.dens = np.eye(rings.nr)[r]
+            cols.append(st.subrings(model_rings(unit, mode, st.radsep))[0][5])
+        widx, wval = weights_of(np.stack(cols, axis=1))
+        by, bx = bhi[1] - blo[1], bhi[0] - blo[0]
+        dev = torch.device("cuda", torch.cuda.current_device())
+        counts = torch.zeros((rings.nr, by, bx), dtype=torch.int64, device=dev)
+        maps = torch.empty((rings.nr, by, bx), dtype=torch.float64, device=dev)
+        nzq = nzq_for(sr)
+        h = self.handle(nzq)
+        L = _lib.lib()
+        stp = _stream_ptr(torch)
+        nsub = sr.shape[1]
+        flux = [2 * math.pi * R * st.radsep / st.pixarea * st.pixarea / (naz_of(R, st.pixarea) * 2) for R in sr[0]]
+        qunit = max(max(flux, default=1.), 1e-300) * 2. ** -QBITS
+        if nsub:
+            p = self.azplan(sr[0], nzq)
+            wb = int(L.bb_ring_maps_work_size(p, 1))
+            ws = torch.empty(wb, dtype=torch.uint8, device=dev)
+            d_sr = torch.from_numpy(np.ascontiguousarray(sr)).to(dev)
+            d_wi = torch.from_numpy(widx).to(dev)
+            d_wv = torch.from_numpy(wval).to(dev)
+            box = _lib.bb_box(blo[0], blo[1], bhi[0], bhi[1])
+            _lib.check(L.bb_ring_maps_deposit(h, p, box, d_sr.data_ptr(), nsub, 1, d_wi.data_ptr(), d_wv.data_ptr(),
+                                              rings.nr, 0, qunit, ws.data_ptr(), wb, counts.data_ptr(), stp))
+        unit = qunit * float(np.mean(st.cd2i)) * float(np.sum(st.relint))
+        _lib.check(L.bb_ring_maps_finish(h, counts.data_ptr(), rings.nr, by, bx, unit, int(self.cfield is not None),
+                                         maps.data_ptr(), stp))
+        return maps.cpu().numpy()
+
+    def array_batch(self, sr, blo, bhi, convolve=True, out=None, work=None, stream=None, d_sr=None):
         """ANALYTIC models of B sub-ring tables sr (B, 16, nsub) sharing their radii, in the box [blo, bhi):
-        a (B, by, bx, nz) float32 device tensor (yxz), convolved with the beam if convolve."""
+        a (B, by, bx, nz) float32 device tensor (yxz), convolved with the beam if convolve. d_sr: the tables
+        already on the device (sr then gives their radii and heights: the density row may differ)."""
         torch = _torch()
         sr = np.ascontiguousarray(sr, np.float64)
         B, nrow, nsub = sr.shape
@@ -176,7 +222,8 @@ class DiskModel:
         st = _stream_ptr(torch, stream)
         dep = torch.empty((B, by, bx, nz), dtype=torch.float32, device=dev) if work is None else work
         if nsub:
-            d_sr = torch.from_numpy(sr).to(dev, non_blocking=False)
+            if d_sr is None:
+                d_sr = torch.from_numpy(sr).to(dev, non_blocking=False)
             p = self.azplan(sr[0, 0], nzq)
             L = _lib.lib()
             wb = int(L.bb_analytic_work_size(p, B))

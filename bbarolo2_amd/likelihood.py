@@ -24,7 +24,11 @@ import numpy as np
 from . import _lib
 from .engine import _torch, _stream_ptr
 from .galmod import SubringPlan
+from .profile import DensityProfile, SMOOTH_DEFAULT
 from .rings import Rings
+
+# free parameters that change the deconvolved density profile (fitter.py:614-615)
+PROFILE_KEYS = ('inc', 'phi', 'xpos', 'ypos', 'radmax', 'z0', 'fthick')
 
 # ring parameters that make the likelihood zero when negative (likelihood.py:532-535)
 POSITIVE = ('vrot', 'vdisp', 'inc', 'xpos', 'ypos', 'z0', 'dens', 'zcyl', 'z0thick', 'fthick', 'vdispthick')
@@ -89,7 +93,8 @@ class GpuLikelihood:
     fracerr: fixed fractional model error (None: 0)."""
 
     def __init__(self, dm, data, rings0, free, likemask=None, region=None, likebox=None, sigma=1.0,
-                 likefactor=1.0, fracerr=None, maskmode=2, chunk=256):
+                 likefactor=1.0, fracerr=None, maskmode=2, chunk=256, densprof_smooth=SMOOTH_DEFAULT,
+                 likefactor2d=None, beam_arcsec=None):
         torch = _torch()
         self.torch = torch
         self.dm = dm
@@ -140,6 +145,33 @@ class GpuLikelihood:
         self.nvox_box = int(np.prod(dbox.shape))
         self.plan = SubringPlan(dm.setup, self.rings0, dm.ringmode, final=True)
         self._bufs = {}
+        # DENS not fitted: the deconvolved profile of the initial rings, and of every parameter vector when the
+        # geometry is free (fitter.py:606-615)
+        self.useNorm = "dens" not in free.index
+        self.update_prof = self.useNorm and any(k in free.index for k in PROFILE_KEYS)
+        self.profile = None
+        self.dens0 = None
+        if self.useNorm:
+            v = lm & ~blank
+            ymap = np.where(v, data, 0.).astype(np.float64).sum(axis=0)
+            nvmap = v.sum(axis=0)
+            smap = self.sigma * np.sqrt(nvmap.astype(np.float64))
+            sl = (slice(Lb[1], Hb[1]), slice(Lb[0], Hb[0]))
+            ok = region[sl] & (nvmap[sl] > 0)
+            if beam_arcsec is None:
+                bm = dm.header.Bmaj()
+                beam_arcsec = abs(bm) * 3600. if bm else None
+            l2d = self.likefactor if likefactor2d is None else float(likefactor2d)
+            self.profile = DensityProfile(dm, self.rings0, self.plan, (Lb, Hb), ymap[sl], smap[sl], ok,
+                                          smooth=densprof_smooth, corr=1. / l2d, beam_arcsec=beam_arcsec)
+            sr = self.plan.tables({})[0]
+            sr[:, 5, :] = 1.
+            d_sr = torch.from_numpy(np.ascontiguousarray(sr)).to(self.dev)
+            dens = self.profile.run(sr, d_sr, {}).cpu().numpy()[0].astype(np.float64)
+            if not np.all(np.isfinite(dens)):
+                raise RuntimeError("the density profile of the initial rings did not converge")
+            self.dens0 = dens
+            self.rings0.dens = dens.copy()
 
     def _buffers(self, B):
         b = self._bufs.get(B)
@@ -157,22 +189,39 @@ class GpuLikelihood:
         return b
 
     def tables(self, theta):
+        """(sr (B, 16, nsub), values) of the rows of theta: the sub-ring tables of the model (density row 1 when
+        the profile is computed per vector) and the ring values of the free keys."""
         vals = self.free.values(theta)
+        if self.useNorm and not self.update_prof:
+            vals["dens"] = self.dens0
         sr, _ = self.plan.tables(vals)
-        return sr
+        if self.update_prof:
+            sr[:, 5, :] = 1.
+        return sr, vals
 
-    def residuals(self, sr):
-        """(S_box, lg) per table: device launches for one chunk of sub-ring tables (B, 16, nsub)."""
+    def residuals(self, sr, vals=None):
+        """(S_box, lg) per table: device launches for one chunk of sub-ring tables (B, 16, nsub); with a
+        per-vector profile, vals are the ring values of the free keys (for the annulus)."""
         torch = self.torch
         B = sr.shape[0]
         bufs = self._buffers(B)
-        mod = self.dm.array_batch(sr, self.blo, self.bhi, convolve=True, out=bufs["mod"], work=bufs["dep"])
+        d_sr = torch.from_numpy(np.ascontiguousarray(sr)).to(self.dev)
+        dens = None
+        if self.update_prof:
+            dens = self.profile.run(sr, d_sr, vals if vals is not None else {})
+        mod = self.dm.array_batch(sr, self.blo, self.bhi, convolve=True, out=bufs["mod"], work=bufs["dep"],
+                                  d_sr=d_sr)
         st = _stream_ptr(torch)
         s2 = self.sigma ** 2
         _lib.check(_lib.lib().bb_residual_gauss(mod.data_ptr(), self.dvalid.data_ptr(), B, self.nvox_box, s2,
                                                 self.eps ** 2, bufs["work"].data_ptr(), bufs["wl"],
                                                 bufs["ss"].data_ptr(), bufs["lg"].data_ptr(), st))
-        return bufs["ss"].cpu().numpy().copy(), bufs["lg"].cpu().numpy().copy()
+        ss, lg = bufs["ss"].cpu().numpy().copy(), bufs["lg"].cpu().numpy().copy()
+        if dens is not None:
+            self.last_dens = dens.cpu().numpy().copy()
+            if not np.all(np.isfinite(self.last_dens)):
+                raise RuntimeError("the density profile did not converge (non-negative least squares)")
+        return ss, lg
 
     def loglike(self, theta):
         """log-likelihood of each row of theta (B, ndim) -> (B,) float64 (-inf where a positive parameter
@@ -184,7 +233,7 @@ class GpuLikelihood:
         s2 = self.sigma ** 2
         for c in range(0, len(idx), self.chunk):
             ii = idx[c:c + self.chunk]
-            ss, lg = self.residuals(self.tables(theta[ii]))
+            ss, lg = self.residuals(*self.tables(theta[ii]))
             if self.eps > 0:
                 nll = 0.5 * (ss + self.ss_out / s2) + 0.5 * self.nvox * math.log(2 * math.pi * s2) + 0.5 * lg
             else:

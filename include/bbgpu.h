@@ -94,6 +94,56 @@ int64_t bb_residual_work_size(int B, int64_t nvox);
 int bb_residual_gauss(const float* model, const float* dvalid, int B, int64_t nvox, double sigma2, double eps2,
                       double* work, int64_t work_len, double* out_ss, double* out_lg, void* stream);
 
+/* Unit-density ring maps (DiskModel.ring_maps / _ring_maps_analytic, bbarolo/galmod.py:1317-1345, 468-536) in
+ * two steps. bb_ring_maps_deposit adds to counts (B, nmap, by, bx) uint64 the total-intensity maps, summed over
+ * the channels, of the ANALYTIC samples of the sub-ring tables sr (B, 16, nsub) (their density row is not read): sub-ring s
+ * adds to map map0 + widx[s][m] (m = 0, 1; -1 or beyond nmap: none) with weight wval[s][m] (device arrays
+ * (nsub, 2) int32 / float64: the interpolation of the ring densities, DiskModel._ring_weights,
+ * galmod.py:1285-1314). Fixed point: a contribution x adds round(x / qunit), so the sums are exact and
+ * deterministic; the caller picks qunit so that no sum exceeds 2^64 units (the contributions are <= fluxsc,
+ * the largest 2 pi R0 radsep / (2 naz) of the tables). work: bb_ring_maps_work_size(plan, B) bytes. */
+int64_t bb_ring_maps_work_size(const bb_azplan* p, int B);
+int bb_ring_maps_deposit(const bb_model* m, const bb_azplan* p, bb_box box, const double* sr, int nsub, int B,
+                         const int32_t* widx, const double* wval, int nmap, int map0, double qunit, void* work,
+                         int64_t work_bytes, uint64_t* counts, void* stream);
+/* maps (P, by, bx) float64 = counts x unit, convolved with the beam ('same', without the 1e-12 cut of the
+ * models) and multiplied by the model's scale when convolve != 0 (galmod.py:1342-1345). unit = qunit x
+ * mean(cd2i) x sum(relint). */
+int bb_ring_maps_finish(const bb_model* m, const uint64_t* counts, int P, int by, int bx, double unit, int convolve,
+                        double* maps, void* stream);
+
+/* The deconvolved density profile of the rings (profile_system / solve_profile, bbarolo/profile.py:33-138; the
+ * per-evaluation density of Bayesian3DFIT, bayesian/density.py:13-88, likelihood.py:595-596). The per-fit
+ * constants: the pixels of the map box (wpix = 1 / map noise where usable, else 0; ypix = data map x wpix), the
+ * second-difference rows of the curvature prior on the radii of the rings and of the exterior annulus (3
+ * coefficients and columns per row, profile.py:81-99) and which of them cross two radii closer than a quarter
+ * of the ring spacing (near). */
+typedef struct bb_profile_desc {
+    int32_t nr;          /* rings; the system has n = nr + 1 columns (the rings, then the exterior annulus) */
+    int32_t npix;        /* pixels of the map box */
+    int32_t nok;         /* usable pixels (wpix > 0) */
+    int32_t pad0;
+    double smooth;       /* width of the curvature prior (like_kwargs densprof_smooth; 0: none) */
+    double corr;         /* pixels per independent noise value of the map (1 / likefactor2d) */
+    double floor;        /* lowest prior weight as a fraction of the relative prior's (DENSPROF_FLOOR) */
+    double norm;         /* sum of the squared row coefficients, spacings >= dmin (profile.py:141-150) */
+    double dmed;         /* median spacing of the ring radii */
+} bb_profile_desc;
+typedef struct bb_profile bb_profile;
+int bb_profile_create(const bb_profile_desc* d, const double* wpix, const double* ypix, const double* lcoef,
+                      const int32_t* lcol, const int32_t* near, bb_profile** out);
+int bb_profile_destroy(bb_profile* p);
+/* x (B, n) float64 = solve_profile of the maps (B, n, by, bx) of bb_ring_maps_finish: the two non-negative
+ * solutions (relative prior, then the density-scaled prior) on the normal equations of the weighted system; NaN
+ * where the active set did not converge. work: bb_profile_work_size(p, B) bytes. */
+int64_t bb_profile_work_size(const bb_profile* p, int B);
+int bb_profile_solve(const bb_profile* p, const double* maps, int B, void* work, int64_t work_bytes, double* x,
+                     void* stream);
+/* The densities into the model: dens (B, nr) float32 = x[:, :nr] (the float32 ring table,
+ * bayesian/rings.py:54-61; may be NULL) and row 5 of sr (B, 16, nsub) = sum_m wval[s][m] dens[widx[s][m]]. */
+int bb_profile_set_dens(const double* x, int B, int n, int nr, const int32_t* widx, const double* wval, int nsub,
+                        double* sr, float* dens, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -153,7 +153,7 @@ def residuals_fracerr(model_zyx, data_zyx, mask, maskmode, region, sigma2, eps2)
     return s, lg.value
 
 
-def conv_fft(model_yxz, kern, scale=1.0):
+def conv_fft(model_yxz, kern, scale=1.0, thresh=True):
     """The FFT 'same' convolution of the reference (FFTConvolver.__call__, bbarolo/smooth.py:107-165: rfft2 of
     the non-empty planes padded to next_fast_len(n + k//2), times the kernel transform, irfft2, crop at k//2),
     then the 1e-12 threshold and the scale (DiskModel.convolve, galmod.py:1273-1283). (ny, nx, nz) -> (ny, nx, nz).
@@ -171,7 +171,8 @@ def conv_fft(model_yxz, kern, scale=1.0):
         f = sfft.rfft2(d[full], s=(Ny, Nx))
         f *= sfft.rfft2(k, s=(Ny, Nx))
         out[full] = sfft.irfft2(f, s=(Ny, Nx))[..., ky // 2:ky // 2 + ny, kx // 2:kx // 2 + nx]
-    out[out < 1.E-12] = 0.
+    if thresh:
+        out[out < 1.E-12] = 0.
     if scale != 1:
         out *= scale
     return np.ascontiguousarray(out.transpose(1, 2, 0))

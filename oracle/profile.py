@@ -50,9 +50,9 @@ def interp_weights(setup, rings, mode):
     return widx, wval
 
 
-def ring_maps(setup, kern, scalefactor, rings, blo, bhi, mode="CENTRED"):
+def ring_maps(setup, kern, scalefactor, rings, blo, bhi, mode="CENTRED", fft=False):
     """(nr, by, bx) FP64 total-intensity maps of the rings with unit density, beam-convolved when kern is
-    given (DiskModel.ring_maps, galmod.py:1317-1345)."""
+    given (DiskModel.ring_maps, galmod.py:1317-1345); fft: by FFT as the reference (else direct sums)."""
     units = model_rings(rings, mode, setup.radsep).copy()
     units.dens = np.ones(units.nr)
     sr = setup.subrings(units)[0]
@@ -61,7 +61,8 @@ def ring_maps(setup, kern, scalefactor, rings, blo, bhi, mode="CENTRED"):
     maps = maps * (float(np.mean(setup.cd2i)) * float(np.sum(setup.relint)))
     if kern is None:
         return maps
-    yxr = oracle.conv_same(np.ascontiguousarray(np.moveaxis(maps, 0, -1)), kern, 1.0, thresh=False)
+    conv = oracle.conv_fft if fft else oracle.conv_same
+    yxr = conv(np.ascontiguousarray(np.moveaxis(maps, 0, -1)), kern, 1.0, thresh=False)
     return np.ascontiguousarray(np.moveaxis(yxr, -1, 0)) * scalefactor
 
 
@@ -85,13 +86,13 @@ def annulus(setup, rings, mode, beam_arcsec):
     return ext, rext
 
 
-def system(setup, kern, scalefactor, rings, box, ymap, smap, okmap, beam_arcsec=None, mode="CENTRED"):
+def system(setup, kern, scalefactor, rings, box, ymap, smap, okmap, beam_arcsec=None, mode="CENTRED", fft=False):
     """(A, b, ok, radii): the columns of the rings and of the annulus on the usable pixels, divided by the map
     noise, or None with fewer than 2 nr usable pixels (profile.py:33-70)."""
     blo, bhi = box
     ext, rext = annulus(setup, rings, mode, beam_arcsec)
-    K = np.concatenate([ring_maps(setup, kern, scalefactor, rings, blo, bhi, mode),
-                        ring_maps(setup, kern, scalefactor, ext, blo, bhi, "CENTRED")])
+    K = np.concatenate([ring_maps(setup, kern, scalefactor, rings, blo, bhi, mode, fft),
+                        ring_maps(setup, kern, scalefactor, ext, blo, bhi, "CENTRED", fft)])
     ok = okmap & (smap > 0) & np.isfinite(ymap)
     if np.count_nonzero(ok) < 2 * rings.nr:
         return None
@@ -153,9 +154,9 @@ def solve(A, b, radii, smooth=SMOOTH_DEFAULT, corr=1., floor=None):
 
 
 def deconvolved(setup, kern, scalefactor, rings, box, ymap, smap, okmap, smooth=SMOOTH_DEFAULT, beam_arcsec=None,
-                corr=1., mode="CENTRED"):
+                corr=1., mode="CENTRED", fft=False):
     """The densities of the rings (profile.py:188-197), or None with too few pixels."""
-    s = system(setup, kern, scalefactor, rings, box, ymap, smap, okmap, beam_arcsec, mode)
+    s = system(setup, kern, scalefactor, rings, box, ymap, smap, okmap, beam_arcsec, mode, fft)
     if s is None:
         return None
     A, b, _, radii = s

@@ -147,3 +147,23 @@ def test_fillgauss_scalefactor_point():
     amplitude 1: for a tiny old beam the convolution at the centre is K(0) = peak of the normalised kernel."""
     g = fillgauss2d((3., 3., 0.), (1., 1.), 1.0, norm=False)
     assert g[g.shape[0] // 2, g.shape[1] // 2] == 1.0
+
+
+def test_profile_prior_rows_match_oracle():
+    """The per-fit curvature rows of the device profile equal the oracle's (dense L rebuilt from the 3-column
+    form), as do the relative-prior norm, the median spacing and the flags of near radii."""
+    from bbarolo2_amd.profile import prior_rows
+    from oracle import profile as op
+    for R in ([15., 45., 75., 105., 135., 170.], [1., 2., 3., 4., 5., 4.5], [1., 2., 3., 3.],
+              list(np.arange(40) * 30. + 15.) + [1230.]):
+        lcoef, lcol, near, norm, dmed = prior_rows(R)
+        L = np.zeros((len(R) - 2, len(R)))
+        for r in range(len(R) - 2):
+            L[r, lcol[r]] = lcoef[r]
+        Lo, order, Rs, tiny, dmin = op.curvature(R)
+        assert np.allclose(L, Lo, rtol=1e-13, atol=0)
+        _, s, _ = op.relative_scale(R, 7.0, 0.1)
+        assert np.isclose(np.sqrt(0.1 * 7.0 / norm), s, rtol=1e-13)
+        assert dmed == np.median(np.diff(np.sort(np.asarray(R)[:-1])))
+        gaps = np.diff(Rs)
+        assert np.array_equal(near.astype(bool), np.minimum(gaps[:-1], gaps[1:]) < 0.25 * dmed)
