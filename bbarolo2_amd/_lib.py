@@ -53,8 +53,8 @@ _SIGS = {
     "bb_residual_gauss": (C.c_int, [_P, _P, C.c_int, C.c_int64, C.c_double, C.c_double, _P, C.c_int64, _P, _P,
                                     _P]),
     "bb_ring_maps_work_size": (C.c_int64, [_P, C.c_int]),
-    "bb_ring_maps_deposit": (C.c_int, [_P, _P, bb_box, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, C.c_double,
-                                       _P, C.c_int64, _P, _P]),
+    "bb_ring_maps_deposit": (C.c_int, [_P, _P, bb_box, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, C.c_int,
+                                       C.c_double, _P, C.c_int64, _P, _P]),
     "bb_ring_maps_finish": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _P, _P]),
     "bb_profile_create": (C.c_int, [C.POINTER(bb_profile_desc), _P, _P, _P, _P, _P, C.POINTER(_P)]),
     "bb_profile_destroy": (C.c_int, [_P]),

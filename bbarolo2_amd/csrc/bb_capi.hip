@@ -24,9 +24,12 @@ extern "C" int bb_launch_conv2d_same(const float* in, float* out, int B, int by,
 extern "C" int bb_conv_max_kx();
 extern "C" int bb_residual_blocks(int64_t nvox);
 extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* sr, int nsub, int B, const int* naz,
-                                   const int4* chunks, int nchunks, const double2* az, double* sub, double* wtab,
-                                   const int* widx, const double* wval, int nmap, int map0, double qunit,
-                                   unsigned long long* counts, hipStream_t st);
+                                   const int4* chunks, int nchunks, const double2* az, int TX, int TY, double* sub,
+                                   double* wtab, short4* bbox, short4* sbox, const int* widx, const double* wval,
+                                   int nmap, int map0, int nw, double qunit, unsigned long long* counts,
+                                   hipStream_t st);
+extern "C" size_t bb_rm_tile_lds(int nw, int TX, int TY);
+extern "C" size_t bb_rm_tile_static();
 extern "C" size_t bb_rm_finish_lds(int ky, int kx);
 extern "C" int bb_launch_rm_finish(const unsigned long long* counts, double* maps, int P, int by, int bx,
                                    const double* kern, int ky, int kx, double unit, double scale, int convolve,
@@ -344,31 +347,55 @@ extern "C" int bb_residual_gauss(const float* model, const float* dvalid, int B,
 
 // ---------------------------------------------------------------------------------------------------------
 // Ring maps and the deconvolved density profile.
-extern "C" int64_t bb_ring_maps_work_size(const bb_azplan* p, int B) {
-    if (!p || B < 0) return -1;
-    return (int64_t)((size_t)B * p->nsub * (bb_analytic_sub_bytes() + bb_analytic_wt_bytes()) + 256);
-}
+extern "C" int64_t bb_ring_maps_work_size(const bb_azplan* p, int B) { return bb_analytic_work_size(p, B); }
 
 extern "C" int bb_ring_maps_deposit(const bb_model* m, const bb_azplan* p, bb_box box, const double* sr, int nsub,
-                                    int B, const int32_t* widx, const double* wval, int nmap, int map0, double qunit,
-                                    void* work, int64_t work_bytes, uint64_t* counts, void* stream) {
+                                    int B, const int32_t* widx, const double* wval, int nmap, int map0, int nw,
+                                    double qunit, void* work, int64_t work_bytes, uint64_t* counts, void* stream) {
     if (!m || !p) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: model or plan is NULL");
     int rc = check_box(m, box);
     if (rc) return rc;
     if (nsub != p->nsub) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: nsub %d != plan's %d", nsub, p->nsub);
     if (B < 0 || B > 65535) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: B=%d out of range", B);
-    if (nmap < 1 || map0 < 0 || map0 >= nmap) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: map0 %d / nmap %d", map0, nmap);
+    if (box.bhi0 - box.blo0 > 32767 || box.bhi1 - box.blo1 > 32767)
+        return fail(BB_ERR_ARG, "bb_ring_maps_deposit: box larger than 32767 pixels");
+    if (nw < 1 || map0 < 0 || map0 + nw > nmap)
+        return fail(BB_ERR_ARG, "bb_ring_maps_deposit: maps %d..%d outside nmap %d", map0, map0 + nw - 1, nmap);
     if (!(qunit > 0) || !isfinite(qunit)) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: qunit must be > 0");
-    if (nsub == 0 || B == 0) return 0;
-    if (!sr || !widx || !wval || !work || !counts) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: NULL array");
-    if (work_bytes < bb_ring_maps_work_size(p, B))
+    if (B == 0) return 0;
+    if (!counts) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: NULL array");
+    if (nsub > 0 && (!sr || !widx || !wval || !work)) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: NULL array");
+    if (nsub > 0 && work_bytes < bb_ring_maps_work_size(p, B))
         return fail(BB_ERR_ARG, "bb_ring_maps_deposit: work holds %lld bytes, needs %lld", (long long)work_bytes,
                     (long long)bb_ring_maps_work_size(p, B));
+    // the largest pixel tile whose nw maps fit the LDS of a CU
+    static const int cand[][2] = {{16, 16}, {16, 8}, {8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
+    int TX = 0, TY = 0;
+    for (auto& c : cand)
+        if (bb_rm_tile_lds(nw, c[0], c[1]) + bb_rm_tile_static() <= BB_LDS_MAX) {
+            TX = c[0];
+            TY = c[1];
+            break;
+        }
+    if (!TX) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: %d maps do not fit the LDS tile", nw);
+    const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
+    if (nsub == 0) {
+        const size_t plane = (size_t)by * bx;
+        for (int b = 0; b < B; b++)
+            rc = hip_check(hipMemsetAsync(counts + ((size_t)b * nmap + map0) * plane, 0, plane * nw * sizeof(uint64_t),
+                                          (hipStream_t)stream), "bb_ring_maps_deposit");
+        return rc;
+    }
     char* w = (char*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
     double* sub = (double*)w;
-    double* wtab = (double*)(w + (size_t)B * nsub * bb_analytic_sub_bytes());
-    rc = bb_launch_ring_maps(&m->g, box, sr, nsub, B, p->d_naz, p->d_chunks, p->nchunks, p->d_az, sub, wtab,
-                             (const int*)widx, wval, nmap, map0, qunit, (unsigned long long*)counts,
+    w += (size_t)B * nsub * bb_analytic_sub_bytes();
+    double* wtab = (double*)w;
+    w += (size_t)B * nsub * bb_analytic_wt_bytes();
+    short4* bbox = (short4*)w;
+    w += (size_t)B * p->nchunks * sizeof(short4);
+    short4* sbox = (short4*)w;
+    rc = bb_launch_ring_maps(&m->g, box, sr, nsub, B, p->d_naz, p->d_chunks, p->nchunks, p->d_az, TX, TY, sub, wtab,
+                             bbox, sbox, (const int*)widx, wval, nmap, map0, nw, qunit, (unsigned long long*)counts,
                              (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_ring_maps_deposit launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }

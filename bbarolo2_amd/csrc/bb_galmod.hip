@@ -393,24 +393,13 @@ __device__ __noinline__ void an_sample_lag(const bb_grid& g, const bb_box& box, 
     }
 }
 
-// Deposit the runs of one azimuth sample that fall in the tile (galmod.py:389-431 for one sample).
-// Only the vertical samples that can reach the tile are placed exactly: a continuous estimate of the
-// (monotone) pixel track picks them, widened by one sample on each side, and those neighbours are outside the
-// tile, so every run inside it is complete.
-__device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, const TileGeo& T, const Acc& A,
-                                          const double* sve, const double* scd, const float* sph, int ntab,
-                                          const Sub& u, const float* wl, const double* __restrict__ wt, double caz,
-                                          double saz) {
-    if (u.zdep) {
-        an_sample_lag(g, box, T, A, sve, scd, sph, ntab, u, wl, wt, caz, saz);
-        return;
-    }
-    const int nz = g.nz;
-    const double wside = caz >= 0.0 ? u.wrec : u.wapp;
-    const double x = u.R * caz, yb = u.R * saz * u.cinc, xs = x * u.spa, xc = x * u.cpa;
-    const double vplane = u.vsys + (u.vrot * caz + u.vrad * saz) * u.sinc + u.vvert * u.cinc;
-    const double afl = u.flux * wside;
-    int ka = 0, kb = u.nzz - 1;
+// The vertical samples of one azimuth sample that can fall in the tile, [ka, kb]; false if none. Only they are
+// placed exactly: a continuous estimate of the (monotone) pixel track picks them, widened by one sample on each
+// side, and those neighbours are outside the tile, so every run of equal pixels inside it is complete.
+__device__ __forceinline__ bool tile_zrange(const Sub& u, const bb_grid& g, const TileGeo& T, double xs, double xc,
+                                            double yb, int& ka, int& kb) {
+    ka = 0;
+    kb = u.nzz - 1;
     if (u.nzz > 2) {
         const double X0 = u.xpos + (xs - yb * u.cpa) / g.cdelt0, dX = u.sinc * u.cpa / g.cdelt0;
         const double Y0 = u.ypos + (xc + yb * u.spa) / g.cdelt1, dY = -u.sinc * u.spa / g.cdelt1;
@@ -424,10 +413,29 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
                 last = kz;
             }
         }
-        if (last < 0) return;
+        if (last < 0) return false;
         ka = first > 0 ? first - 1 : 0;
         kb = last < u.nzz - 1 ? last + 1 : u.nzz - 1;
     }
+    return true;
+}
+
+// Deposit the runs of one azimuth sample that fall in the tile (galmod.py:389-431 for one sample).
+__device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, const TileGeo& T, const Acc& A,
+                                          const double* sve, const double* scd, const float* sph, int ntab,
+                                          const Sub& u, const float* wl, const double* __restrict__ wt, double caz,
+                                          double saz) {
+    if (u.zdep) {
+        an_sample_lag(g, box, T, A, sve, scd, sph, ntab, u, wl, wt, caz, saz);
+        return;
+    }
+    const int nz = g.nz;
+    const double wside = caz >= 0.0 ? u.wrec : u.wapp;
+    const double x = u.R * caz, yb = u.R * saz * u.cinc, xs = x * u.spa, xc = x * u.cpa;
+    const double vplane = u.vsys + (u.vrot * caz + u.vrad * saz) * u.sinc + u.vvert * u.cinc;
+    const double afl = u.flux * wside;
+    int ka, kb;
+    if (!tile_zrange(u, g, T, xs, xc, yb, ka, kb)) return;
     int rslot[BB_RMAX];
     float rws[BB_RMAX];
     int nr = 0;
@@ -496,6 +504,58 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
     }
 }
 
+// The samples of a batch vector that meet the tile [tx0, tx1] x [ty0, ty1] (box-relative): the chunks whose
+// box meets it are gathered in clist, then their samples whose vertical segment meets it in slist, and dep(e)
+// is called for each, e = (chunk << 5) | azimuth lane, one per thread. The list holds neighbouring azimuths of
+// one sub-ring next to each other; lane l of wavefront w takes entry l * W + w of each window of blockDim
+// entries (W wavefronts), so the lanes of one LDS atomic are W azimuths apart (different pixels: same-address
+// atomics of a wavefront serialise) on neighbouring sub-rings (similar work). Every thread of the block calls
+// it (it synchronises).
+template <class Dep>
+__device__ __forceinline__ void tile_samples(int nchunks, const short4* __restrict__ bb, const short4* __restrict__ sb,
+                                             int tx0, int tx1, int ty0, int ty1, int* clist, int* slist, int& nlist,
+                                             int& nsamp, bool active, Dep&& dep) {
+    const int lane = threadIdx.x % BB_CS, grp = threadIdx.x / BB_CS;
+    for (int base = 0; base < nchunks && active; base += blockDim.x) {
+        // 1. the chunks whose bounding box meets the tile
+        const int c = base + threadIdx.x;
+        if (c < nchunks) {
+            const short4 r = bb[c];
+            if (r.x <= r.y && r.x <= tx1 && r.y >= tx0 && r.z <= ty1 && r.w >= ty0) clist[atomicAdd(&nlist, 1)] = c;
+        }
+        __syncthreads();
+        const int n = nlist;
+        __syncthreads();                            // (every thread has read n before the list changes)
+        if (!(n > BB_LIST_CAP - (int)blockDim.x || base + (int)blockDim.x >= nchunks)) continue;
+        for (int j0 = 0; j0 < n; j0 += BB_NGRP) {
+            // 2. their samples whose vertical segment meets the tile (one lane per azimuth)
+            const int j = j0 + grp;
+            if (j < n) {
+                const int cj = clist[j];
+                const short4 r = sb[(size_t)cj * BB_CS + lane];
+                if (r.x <= r.y && r.x <= tx1 && r.y >= tx0 && r.z <= ty1 && r.w >= ty0)
+                    slist[atomicAdd(&nsamp, 1)] = (cj << 5) | lane;
+            }
+            __syncthreads();
+            const int ns = nsamp;
+            __syncthreads();
+            if (ns > BB_SAMP_CAP - (int)blockDim.x || j0 + BB_NGRP >= n) {
+                // 3. the samples gathered, one per thread
+                const int W = blockDim.x >> 6;
+                for (int base2 = 0; base2 < ns; base2 += blockDim.x) {
+                    const int idx = base2 + (threadIdx.x & 63) * W + (threadIdx.x >> 6);
+                    if (idx < ns) dep(slist[idx]);
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) nsamp = 0;
+                __syncthreads();
+            }
+        }
+        if (threadIdx.x == 0) nlist = 0;
+        __syncthreads();
+    }
+}
+
 __global__ void __launch_bounds__(BB_TILE_THREADS)
 k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks, int nchunks,
           const double2* __restrict__ az, const double* __restrict__ suball, const double* __restrict__ wtaball,
@@ -560,62 +620,21 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     A.inv_q = (float)(1.0 / q);
     const short4* bb = bball + (size_t)b * nchunks;
     const short4* sb = sball + (size_t)b * nchunks * BB_CS;
-    const int lane = threadIdx.x % BB_CS, grp = threadIdx.x / BB_CS;
     TileGeo T;
     T.ax0 = box.blo0 + tx0;
     T.ax1 = box.blo0 + (tx1 < bx - 1 ? tx1 : bx - 1);
     T.ay0 = box.blo1 + ty0;
     T.ay1 = box.blo1 + (ty1 < by - 1 ? ty1 : by - 1);
     T.TX = TX;
-    for (int base = 0; base < nchunks && amax > 0.; base += blockDim.x) {
-        // 1. the chunks whose bounding box meets the tile
-        const int c = base + threadIdx.x;
-        if (c < nchunks) {
-            const short4 r = bb[c];
-            if (r.x <= r.y && r.x <= tx1 && r.y >= tx0 && r.z <= ty1 && r.w >= ty0) clist[atomicAdd(&nlist, 1)] = c;
-        }
-        __syncthreads();
-        const int n = nlist;
-        __syncthreads();                            // (every thread has read n before the list changes)
-        if (!(n > BB_LIST_CAP - (int)blockDim.x || base + (int)blockDim.x >= nchunks)) continue;
-        for (int j0 = 0; j0 < n; j0 += BB_NGRP) {
-            // 2. their samples whose vertical segment meets the tile (one lane per azimuth)
-            const int j = j0 + grp;
-            if (j < n) {
-                const int cj = clist[j];
-                const short4 r = sb[(size_t)cj * BB_CS + lane];
-                if (r.x <= r.y && r.x <= tx1 && r.y >= tx0 && r.z <= ty1 && r.w >= ty0)
-                    slist[atomicAdd(&nsamp, 1)] = (cj << 5) | lane;
-            }
-            __syncthreads();
-            const int ns = nsamp;
-            __syncthreads();
-            if (ns > BB_SAMP_CAP - (int)blockDim.x || j0 + BB_NGRP >= n) {
-                // 3. deposit the samples gathered, one per thread. The list holds neighbouring azimuths of one
-                // sub-ring next to each other; lane l of wavefront w takes entry l * W + w of each window of
-                // blockDim entries (W wavefronts), so the lanes of one LDS atomic are W azimuths apart (different
-                // pixels: same-address atomics of a wavefront serialise) on neighbouring sub-rings (similar work).
-                const int W = blockDim.x >> 6;
-                for (int base = 0; base < ns; base += blockDim.x) {
-                    const int idx = base + (threadIdx.x & 63) * W + (threadIdx.x >> 6);
-                    if (idx >= ns) continue;
-                    const int e = slist[idx];
-                    const int4 ch = chunks[e >> 5];
-                    const int s = ch.x >> 1;
-                    Sub u;
-                    load_sub(u, sub + (size_t)s * BB_NSF, ch.x & 1);
-                    const double2 cs = az[ch.w + (e & 31)];
-                    an_sample(g, box, T, A, sve, scd, sph, ntab, u, wlds ? swl + s * BB_WCAP : nullptr,
-                              wtab + (size_t)s * BB_ZCAP, cs.x, cs.y);
-                }
-                __syncthreads();
-                if (threadIdx.x == 0) nsamp = 0;
-                __syncthreads();
-            }
-        }
-        if (threadIdx.x == 0) nlist = 0;
-        __syncthreads();
-    }
+    tile_samples(nchunks, bb, sb, tx0, tx1, ty0, ty1, clist, slist, nlist, nsamp, amax > 0., [&](int e) {
+        const int4 ch = chunks[e >> 5];
+        const int s = ch.x >> 1;
+        Sub u;
+        load_sub(u, sub + (size_t)s * BB_NSF, ch.x & 1);
+        const double2 cs = az[ch.w + (e & 31)];
+        an_sample(g, box, T, A, sve, scd, sph, ntab, u, wlds ? swl + s * BB_WCAP : nullptr, wtab + (size_t)s * BB_ZCAP,
+                  cs.x, cs.y);
+    });
     __syncthreads();
     float* out = outall + (size_t)b * by * bx * nz;
     for (int i = threadIdx.x; i < TX * TY * nz; i += blockDim.x) {
@@ -672,73 +691,119 @@ extern "C" int bb_analytic_chunk() { return BB_CS; }
 // Unit-density ring maps (galmod.py:468-536): the samples of the deposit above summed over the channels.
 // Every run of equal-pixel vertical samples adds fluxsc x (its vertical weights) x (the side weight) x wval to
 // the total-intensity map of each ring its sub-ring interpolates. The sub-ring records come from k_an_prep with
-// unit densities, so their flux field is fluxsc. Accumulation is 64-bit fixed point in global
-// memory (a contribution x adds round(x / qunit)): exact sums, independent of the order of the atomics. One
-// thread per azimuth sample; thread t takes chunk t mod nchunks, so the lanes of one wavefront hold samples
-// of different chunks and rarely meet on one pixel.
-__global__ void __launch_bounds__(256)
-k_rm_deposit(bb_grid g, bb_box box, const int4* __restrict__ chunks, int nchunks, const double2* __restrict__ az,
-             const double* __restrict__ suball, const double* __restrict__ wtaball, int nsub,
-             const int2* __restrict__ widx, const double2* __restrict__ wval, int nmap, int map0, double inv_q,
-             unsigned long long* __restrict__ counts) {
+// unit densities (their flux field is fluxsc) and the sample boxes from k_an_bbox on the box of the maps. One
+// workgroup per (vector, tile of TX x TY pixels) holds the tile's nw maps in LDS as 64-bit fixed point (a
+// contribution x adds round(x / qunit): exact sums, independent of the order of the atomics) and writes them
+// once: every count is written by exactly one workgroup.
+__global__ void __launch_bounds__(BB_TILE_THREADS)
+k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks, int nchunks,
+          const double2* __restrict__ az, const double* __restrict__ suball, const double* __restrict__ wtaball,
+          int nsub, const short4* __restrict__ bball, const short4* __restrict__ sball, const int2* __restrict__ widx,
+          const double2* __restrict__ wval, int nmap, int map0, int nw, double inv_q,
+          unsigned long long* __restrict__ counts) {
+    extern __shared__ double dyn[];
+    __shared__ int clist[BB_LIST_CAP];
+    __shared__ int slist[BB_SAMP_CAP];
+    __shared__ int nlist, nsamp;
     const int b = blockIdx.y;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nchunks * BB_CS) return;
-    const int c = (int)(t % nchunks), lane = (int)(t / nchunks);
-    const int4 ch = chunks[c];
-    if (lane >= ch.z) return;
-    const int s = ch.x >> 1, kr = ch.x & 1;
-    int2 wi = widx[s];
-    if (wi.x >= nmap - map0) wi.x = -1;      // (out-of-range ring indices are ignored)
-    if (wi.y >= nmap - map0) wi.y = -1;
-    if (wi.x < 0 && wi.y < 0) return;
-    const double* q = suball + ((size_t)b * nsub + s) * BB_NSF;
-    const double* wt = wtaball + ((size_t)b * nsub + s) * BB_ZCAP;
-    Sub u;
-    load_sub(u, q, kr);
-    const double2 cs = az[ch.w + lane];
-    const double wside = cs.x >= 0.0 ? u.wrec : u.wapp;
-    if (wside == 0.0) return;
-    const double2 wv = wval[s];
+    const int nwp = nw | 1;
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
-    const size_t plane = (size_t)by * bx;
-    unsigned long long* o0 = wi.x >= 0 ? counts + ((size_t)b * nmap + map0 + wi.x) * plane : nullptr;
-    unsigned long long* o1 = wi.y >= 0 ? counts + ((size_t)b * nmap + map0 + wi.y) * plane : nullptr;
-    const double x = u.R * cs.x, yb = u.R * cs.y * u.cinc, xs = x * u.spa, xc = x * u.cpa;
-    int64_t p0, p1;
-    upix(u, g, xs, xc, yb, uz(u, 0), p0, p1);
-    double ws = uw_(u, nullptr, wt, 0, g.ltype);
-    for (int kz = 1; kz <= u.nzz; kz++) {
-        int64_t n0 = 0, n1 = 0;
-        const bool more = kz < u.nzz;
-        if (more) {
-            upix(u, g, xs, xc, yb, uz(u, kz), n0, n1);
-            if (n0 == p0 && n1 == p1) {
+    const int ntx = (bx + TX - 1) / TX;
+    const int tx0 = (blockIdx.x % ntx) * TX, ty0 = (blockIdx.x / ntx) * TY;
+    const int tx1 = tx0 + TX - 1, ty1 = ty0 + TY - 1;
+    unsigned long long* tile = (unsigned long long*)dyn;
+    for (int i = threadIdx.x; i < TX * TY * nwp; i += blockDim.x) tile[i] = 0ull;
+    if (threadIdx.x == 0) {
+        nlist = 0;
+        nsamp = 0;
+    }
+    __syncthreads();
+    const double* sub = suball + (size_t)b * nsub * BB_NSF;
+    const double* wtab = wtaball + (size_t)b * nsub * BB_ZCAP;
+    TileGeo T;
+    T.ax0 = box.blo0 + tx0;
+    T.ax1 = box.blo0 + (tx1 < bx - 1 ? tx1 : bx - 1);
+    T.ay0 = box.blo1 + ty0;
+    T.ay1 = box.blo1 + (ty1 < by - 1 ? ty1 : by - 1);
+    T.TX = TX;
+    tile_samples(nchunks, bball + (size_t)b * nchunks, sball + (size_t)b * nchunks * BB_CS, tx0, tx1, ty0, ty1, clist,
+                 slist, nlist, nsamp, true, [&](int e) {
+        const int4 ch = chunks[e >> 5];
+        const int s = ch.x >> 1;
+        int2 wi = widx[s];
+        if (wi.x >= nw) wi.x = -1;      // (out-of-range ring indices are ignored)
+        if (wi.y >= nw) wi.y = -1;
+        if (wi.x < 0 && wi.y < 0) return;
+        Sub u;
+        load_sub(u, sub + (size_t)s * BB_NSF, ch.x & 1);
+        const double* wt = wtab + (size_t)s * BB_ZCAP;
+        const double2 cs = az[ch.w + (e & 31)];
+        const double wside = cs.x >= 0.0 ? u.wrec : u.wapp;
+        if (wside == 0.0) return;
+        const double2 wv = wval[s];
+        const double x = u.R * cs.x, yb = u.R * cs.y * u.cinc, xs = x * u.spa, xc = x * u.cpa;
+        int ka, kb;
+        if (!tile_zrange(u, g, T, xs, xc, yb, ka, kb)) return;
+        int64_t c0, c1;
+        upix(u, g, xs, xc, yb, uz(u, ka), c0, c1);
+        double ws = uw_(u, nullptr, wt, ka, g.ltype);
+        for (int kz = ka + 1; kz <= kb + 1; kz++) {
+            int64_t h0 = 0, h1 = 0;
+            bool same = false;
+            if (kz <= kb) {
+                upix(u, g, xs, xc, yb, uz(u, kz), h0, h1);
+                same = h0 == c0 && h1 == c1;
+            }
+            if (same) {
                 ws += uw_(u, nullptr, wt, kz, g.ltype);
                 continue;
             }
+            const int sl = T.slot(c0, c1);
+            if (sl >= 0) {
+                const double base = u.flux * ws;
+                if (wi.x >= 0) atomicAdd(tile + sl * nwp + wi.x, (unsigned long long)(base * wv.x * wside * inv_q + 0.5));
+                if (wi.y >= 0) atomicAdd(tile + sl * nwp + wi.y, (unsigned long long)(base * wv.y * wside * inv_q + 0.5));
+            }
+            if (kz <= kb) {
+                c0 = h0;
+                c1 = h1;
+                ws = uw_(u, nullptr, wt, kz, g.ltype);
+            }
         }
-        if (!(p0 < box.blo0 || p0 >= box.bhi0 || p1 < box.blo1 || p1 >= box.bhi1)) {
-            const size_t pix = (size_t)(p1 - box.blo1) * bx + (size_t)(p0 - box.blo0);
-            const double base = u.flux * ws;
-            if (o0) atomicAdd(o0 + pix, (unsigned long long)(base * wv.x * wside * inv_q + 0.5));
-            if (o1) atomicAdd(o1 + pix, (unsigned long long)(base * wv.y * wside * inv_q + 0.5));
-        }
-        if (more) {
-            p0 = n0;
-            p1 = n1;
-            ws = uw_(u, nullptr, wt, kz, g.ltype);
-        }
+    });
+    __syncthreads();
+    for (int i = threadIdx.x; i < nw * TX * TY; i += blockDim.x) {
+        const int m = i / (TX * TY), p = i - m * (TX * TY);
+        const int rx = tx0 + p % TX, ry = ty0 + p / TX;
+        if (rx < bx && ry < by) counts[(((size_t)b * nmap + map0 + m) * by + ry) * bx + rx] = tile[(size_t)p * nwp + m];
     }
 }
 
+extern "C" size_t bb_rm_tile_lds(int nw, int TX, int TY) {
+    return (size_t)TX * TY * (nw | 1) * sizeof(unsigned long long);
+}
+
+extern "C" size_t bb_rm_tile_static() { return (BB_LIST_CAP + BB_SAMP_CAP) * sizeof(int) + 64; }
+
 extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* sr, int nsub, int B, const int* naz,
-                                   const int4* chunks, int nchunks, const double2* az, double* sub, double* wtab,
-                                   const int* widx, const double* wval, int nmap, int map0, double qunit,
-                                   unsigned long long* counts, hipStream_t st) {
+                                   const int4* chunks, int nchunks, const double2* az, int TX, int TY, double* sub,
+                                   double* wtab, short4* bbox, short4* sbox, const int* widx, const double* wval,
+                                   int nmap, int map0, int nw, double qunit, unsigned long long* counts,
+                                   hipStream_t st) {
+    const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
     hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, naz, sub, wtab, 1);
-    const int64_t nt = (int64_t)nchunks * BB_CS;
-    hipLaunchKernelGGL(k_rm_deposit, dim3((unsigned)((nt + 255) / 256), B), dim3(256), 0, st, *g, box, chunks, nchunks,
-                       az, sub, wtab, nsub, (const int2*)widx, (const double2*)wval, nmap, map0, 1.0 / qunit, counts);
+    hipLaunchKernelGGL(k_an_bbox, dim3((nchunks * BB_CS + 255) / 256, B), dim3(256), 0, st, *g, box, chunks, nchunks,
+                       az, sub, nsub, bbox, sbox);
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_rm_tile, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(160 * 1024 - bb_rm_tile_static()));
+        if (e != hipSuccess) return (int)e;
+        attr = true;
+    }
+    const int ntiles = ((bx + TX - 1) / TX) * ((by + TY - 1) / TY);
+    hipLaunchKernelGGL(k_rm_tile, dim3(ntiles, B), dim3(BB_TILE_THREADS), bb_rm_tile_lds(nw, TX, TY), st, *g, box, TX,
+                       TY, chunks, nchunks, az, sub, wtab, nsub, bbox, sbox, (const int2*)widx, (const double2*)wval,
+                       nmap, map0, nw, 1.0 / qunit, counts);
     return (int)hipGetLastError();
 }

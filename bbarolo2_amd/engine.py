@@ -180,7 +180,7 @@ class DiskModel:
         widx, wval = weights_of(np.stack(cols, axis=1))
         by, bx = bhi[1] - blo[1], bhi[0] - blo[0]
         dev = torch.device("cuda", torch.cuda.current_device())
-        counts = torch.zeros((rings.nr, by, bx), dtype=torch.int64, device=dev)
+        counts = torch.empty((rings.nr, by, bx), dtype=torch.int64, device=dev)
         maps = torch.empty((rings.nr, by, bx), dtype=torch.float64, device=dev)
         nzq = nzq_for(sr)
         h = self.handle(nzq)
@@ -189,16 +189,15 @@ class DiskModel:
         nsub = sr.shape[1]
         flux = [2 * math.pi * R * st.radsep / st.pixarea * st.pixarea / (naz_of(R, st.pixarea) * 2) for R in sr[0]]
         qunit = max(max(flux, default=1.), 1e-300) * 2. ** -QBITS
-        if nsub:
-            p = self.azplan(sr[0], nzq)
-            wb = int(L.bb_ring_maps_work_size(p, 1))
-            ws = torch.empty(wb, dtype=torch.uint8, device=dev)
-            d_sr = torch.from_numpy(np.ascontiguousarray(sr)).to(dev)
-            d_wi = torch.from_numpy(widx).to(dev)
-            d_wv = torch.from_numpy(wval).to(dev)
-            box = _lib.bb_box(blo[0], blo[1], bhi[0], bhi[1])
-            _lib.check(L.bb_ring_maps_deposit(h, p, box, d_sr.data_ptr(), nsub, 1, d_wi.data_ptr(), d_wv.data_ptr(),
-                                              rings.nr, 0, qunit, ws.data_ptr(), wb, counts.data_ptr(), stp))
+        p = self.azplan(sr[0], nzq)
+        wb = int(L.bb_ring_maps_work_size(p, 1))
+        ws = torch.empty(wb, dtype=torch.uint8, device=dev)
+        d_sr = torch.from_numpy(np.ascontiguousarray(sr)).to(dev)
+        d_wi = torch.from_numpy(widx).to(dev)
+        d_wv = torch.from_numpy(wval).to(dev)
+        box = _lib.bb_box(blo[0], blo[1], bhi[0], bhi[1])
+        _lib.check(L.bb_ring_maps_deposit(h, p, box, d_sr.data_ptr(), nsub, 1, d_wi.data_ptr(), d_wv.data_ptr(),
+                                          rings.nr, 0, rings.nr, qunit, ws.data_ptr(), wb, counts.data_ptr(), stp))
         unit = qunit * float(np.mean(st.cd2i)) * float(np.sum(st.relint))
         _lib.check(L.bb_ring_maps_finish(h, counts.data_ptr(), rings.nr, by, bx, unit, int(self.cfield is not None),
                                          maps.data_ptr(), stp))

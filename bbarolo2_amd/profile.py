@@ -178,21 +178,26 @@ class DensityProfile:
             self._bufs = {B: b}
         return b
 
-    def maps(self, sr, d_sr, sr_ext, stream=None):
+    def ext_tables(self, values, B):
+        """Host sub-ring tables (B, 16, nsub_ext) of the annulus of B parameter vectors."""
+        sr_ext = self.plan_ext.tables(self.annulus_values(values))[0]
+        if sr_ext.shape[0] != B:
+            sr_ext = np.ascontiguousarray(np.broadcast_to(sr_ext, (B,) + sr_ext.shape[1:]))
+        return sr_ext
+
+    def maps(self, sr, d_sr, sr_ext, stream=None, d_sr_ext=None):
         """Device ring maps (B, nr+1, by, bx) float64 of B parameter vectors: sr / d_sr the host / device sub-ring
-        tables of the model with density row 1, sr_ext the host tables of the annulus."""
+        tables of the model (their density row is not read), sr_ext (d_sr_ext) the host (device) tables of the
+        annulus."""
         torch = self.torch
         L = _lib.lib()
         B = sr.shape[0]
         b = self._buffers(B, sr_ext.shape[2])
         st = _stream_ptr(torch, stream)
         box = _lib.bb_box(self.blo[0], self.blo[1], self.bhi[0], self.bhi[1])
-        b["counts"].zero_()
-        for tab, dtab, widx, wval, map0 in ((sr, d_sr, self.d_widx, self.d_wval, 0),
-                                            (sr_ext, None, self.d_widx_e, self.d_wval_e, self.nr)):
+        for tab, dtab, widx, wval, map0, nw in ((sr, d_sr, self.d_widx, self.d_wval, 0, self.nr),
+                                                (sr_ext, d_sr_ext, self.d_widx_e, self.d_wval_e, self.nr, 1)):
             nsub = tab.shape[2]
-            if nsub == 0:
-                continue
             if dtab is None:
                 dtab = torch.from_numpy(np.ascontiguousarray(tab)).to(self.dev)
             nzq = nzq_for(tab.reshape(16, -1))
@@ -201,7 +206,8 @@ class DensityProfile:
             wb = int(L.bb_ring_maps_work_size(p, B))
             ws = torch.empty(wb, dtype=torch.uint8, device=self.dev)
             _lib.check(L.bb_ring_maps_deposit(h, p, box, dtab.data_ptr(), nsub, B, widx.data_ptr(), wval.data_ptr(),
-                                              self.n, map0, self.qunit, ws.data_ptr(), wb, b["counts"].data_ptr(), st))
+                                              self.n, map0, nw, self.qunit, ws.data_ptr(), wb, b["counts"].data_ptr(),
+                                              st))
         h = self.dm.handle(nzq_for(sr.reshape(16, -1)))
         _lib.check(L.bb_ring_maps_finish(h, b["counts"].data_ptr(), b["P"], self.by, self.bx, self.unit,
                                          int(self.dm.cfield is not None), b["maps"].data_ptr(), st))
@@ -217,18 +223,20 @@ class DensityProfile:
                                                b["work"].numel(), b["x"].data_ptr(), st))
         return b["x"]
 
+    def set_dens(self, x, d_sr, stream=None):
+        """float32 densities of the solutions x into row 5 of the device tables d_sr (B, 16, nsub) of the model;
+        returns them (B, nr) (device)."""
+        B = x.shape[0]
+        b = self._buffers(B, 0)
+        _lib.check(_lib.lib().bb_profile_set_dens(x.data_ptr(), B, self.n, self.nr, self.d_widx.data_ptr(),
+                                                  self.d_wval.data_ptr(), d_sr.shape[2], d_sr.data_ptr(),
+                                                  b["dens"].data_ptr(), _stream_ptr(self.torch, stream)))
+        return b["dens"]
+
     def run(self, sr, d_sr, values, stream=None):
         """The profile of B parameter vectors written into row 5 of the device tables d_sr (B, 16, nsub) of the
         model; returns the float32 densities (B, nr) (device)."""
-        torch = self.torch
-        sr_ext = self.plan_ext.tables(self.annulus_values(values))[0]
-        if sr_ext.shape[0] != sr.shape[0]:
-            sr_ext = np.ascontiguousarray(np.broadcast_to(sr_ext, (sr.shape[0],) + sr_ext.shape[1:]))
-        sr_ext[:, 5, :] = 1.
+        sr_ext = self.ext_tables(values, sr.shape[0])
         maps = self.maps(sr, d_sr, sr_ext, stream)
         x = self.solve(maps, stream)
-        b = self._buffers(sr.shape[0], 0)
-        _lib.check(_lib.lib().bb_profile_set_dens(x.data_ptr(), sr.shape[0], self.n, self.nr, self.d_widx.data_ptr(),
-                                                  self.d_wval.data_ptr(), sr.shape[2], d_sr.data_ptr(),
-                                                  b["dens"].data_ptr(), _stream_ptr(torch, stream)))
-        return b["dens"]
+        return self.set_dens(x, d_sr, stream)

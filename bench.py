@@ -1,8 +1,9 @@
 """Benchmark: likelihood evaluations per second of the 160x160x96 GLOBAL configuration on one GPU.
 
 One step = one batch of B parameter vectors (free VROT, VDISP per ring, INC, PA: 82-D) evaluated by the
-model-cube likelihood (ANALYTIC GALMOD -> beam convolution -> masked Gaussian residual), inputs resident on
-the device. Prints one JSON line (see --help).
+model-cube likelihood, inputs resident on the device: the deconvolved density profile of every vector (ring maps,
+non-negative least squares), then ANALYTIC GALMOD -> beam convolution -> masked Gaussian residual. Prints one
+JSON line (see --help).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
 """
@@ -55,24 +56,51 @@ _W = {}
 
 
 def _cpu_worker_init(wl):
+    # one core per worker: no BLAS / OpenMP threads of its own (the pool is the parallelism)
+    from threadpoolctl import threadpool_limits
+    _W["limits"] = threadpool_limits(1)
     _W.update(wl)
 
 
-def _cpu_eval(i):
-    """One likelihood evaluation with the FP64 oracle (model, beam, residual); returns its wall time."""
-    import oracle
-    from bbarolo2_amd.galmod import GalmodSetup, SubringPlan, nzq_for, PHI_TAB, zstep_for
+def _cpu_setup(w):
+    """Per-worker constants of the oracle leg: the setup, the sub-ring plan and the data maps of the profile
+    (likemask voxels, noise sigma sqrt(n), region), as the likelihood prepares them."""
+    from bbarolo2_amd.galmod import GalmodSetup, SubringPlan, zstep_for
     from bbarolo2_amd.likelihood import float32_rings
+    st = GalmodSetup(w["h"], w["par"])
+    st.zstep = zstep_for(w["h"], st, True)
+    w["st"] = st
+    w["r32"] = float32_rings(w["truth"])
+    w["plan"] = SubringPlan(st, w["r32"], "CENTRED")
+    data, lm = w["data"], w["likemask"]
+    v = lm & np.isfinite(data) & (data != 0)
+    w["ymap"] = np.where(v, data, 0.).astype(np.float64).sum(axis=0)
+    nv = v.sum(axis=0)
+    w["smap"] = w["sigma"] * np.sqrt(nv.astype(np.float64))
+    w["okmap"] = w["region"] & (nv > 0)
+
+
+def _cpu_eval(i):
+    """One likelihood evaluation with the FP64 oracle (deconvolved profile with FFT convolutions, model, beam,
+    residual); returns its wall time."""
+    import oracle
+    from oracle import profile as op
+    from bbarolo2_amd.galmod import nzq_for, PHI_TAB
     w = _W
     if "st" not in w:
-        st = GalmodSetup(w["h"], w["par"])
-        st.zstep = zstep_for(w["h"], st, True)
-        w["st"] = st
-        w["plan"] = SubringPlan(st, float32_rings(w["truth"]), "CENTRED")
+        _cpu_setup(w)
     st, plan = w["st"], w["plan"]
     t0 = time.perf_counter()
-    sr, _ = plan.tables(w["free"].values(w["theta"][i % len(w["theta"])][None]))
+    vals = w["free"].values(w["theta"][i % len(w["theta"])][None])
+    rr = w["r32"].copy()
+    for k, v in vals.items():
+        setattr(rr, k, v[0])
     blo, bhi = w["box"]
+    sl = (slice(blo[1], bhi[1]), slice(blo[0], bhi[0]))
+    dens = op.deconvolved(st, w["kern"], 1.0, rr, (blo, bhi), w["ymap"][sl], w["smap"][sl], w["okmap"][sl],
+                          beam_arcsec=30., corr=1. / w["likefactor"], fft=True)
+    vals["dens"] = dens.astype(np.float32).astype(np.float64)[None]
+    sr, _ = plan.tables(vals)
     m = oracle.galmod_analytic(st, sr[0], blo, bhi, nzq_for(sr[0]), PHI_TAB)
     m = oracle.conv_fft(m, w["kern"])
     d = w["data"][:, blo[1]:bhi[1], blo[0]:bhi[0]]
@@ -88,7 +116,8 @@ def cpu_baseline(wl, per_worker=2):
     import multiprocessing as mp
     P = max(1, min(16, os.cpu_count() or 1))
     ctx = mp.get_context("fork")
-    small = {k: wl[k] for k in ("h", "truth", "par", "data", "likemask", "region", "free", "theta", "box", "kern")}
+    small = {k: wl[k] for k in ("h", "truth", "par", "data", "likemask", "region", "free", "theta", "box", "kern",
+                                "sigma", "likefactor")}
     with ctx.Pool(P, initializer=_cpu_worker_init, initargs=(small,)) as pool:
         pool.map(_cpu_eval, range(P), chunksize=1)
         t0 = time.perf_counter()
@@ -96,7 +125,8 @@ def cpu_baseline(wl, per_worker=2):
         wall = time.perf_counter() - t0
     med = float(np.median(times))
     return dict(value=P / med, unit="evals/s", cores=P, kind="port",
-                sample=f"{P * per_worker} FP64 oracle likelihood evaluations (ANALYTIC model, beam, residual) "
+                sample=f"{P * per_worker} FP64 oracle likelihood evaluations (deconvolved density profile, ANALYTIC "
+                       f"model, beam, residual) "
                        f"of the 160x160x96 workload on {P} worker processes, 1 untimed each; median "
                        f"{med * 1e3:.0f} ms/eval/core, wall {wall:.1f} s; single core {1 / med:.2f} evals/s",
                 single_core=1 / med)
@@ -129,9 +159,14 @@ def main():
     dm = DiskModel(wl["h"], wl["par"], galmodtype="ANALYTIC")
     like = GpuLikelihood(dm, wl["data"], wl["truth"], wl["free"], likemask=wl["likemask"], region=wl["region"],
                          likebox=wl["box"], sigma=wl["sigma"], likefactor=wl["likefactor"], chunk=B)
+    if like.profile is None or not like.update_prof:
+        raise RuntimeError("the workload fits the geometry without DENS: the profile must be per vector")
     theta = np.resize(wl["theta"], (B, wl["free"].ndim))
-    sr = like.tables(theta)
+    sr, vals = like.tables(theta)
+    prof = like.profile
     d_sr = torch.from_numpy(np.ascontiguousarray(sr)).cuda()
+    sr_ext = prof.ext_tables(vals, B)
+    d_sr_ext = torch.from_numpy(np.ascontiguousarray(sr_ext)).cuda()
     bufs = like._buffers(B)
     blo, bhi = like.blo, like.bhi
     by, bx, nz = bhi[1] - blo[1], bhi[0] - blo[0], dm.setup.nsubs
@@ -146,24 +181,28 @@ def main():
     s2 = like.sigma ** 2
     stream = torch.cuda.current_stream()
     sp = _stream_ptr(torch)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    stage = np.zeros(3)
+    names = ["ring_maps", "profile_solve", "galmod_analytic", "conv2d_same", "residual"]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
+    stage = np.zeros(len(names))
 
     def step(timed):
-        if timed:
-            ev[0].record(stream)
+        def mark(k):
+            if timed:
+                ev[k].record(stream)
+        mark(0)
+        maps = prof.maps(sr, d_sr, sr_ext, d_sr_ext=d_sr_ext)
+        mark(1)
+        prof.set_dens(prof.solve(maps), d_sr)
+        mark(2)
         _lib.check(L.bb_galmod_analytic(h, plan, box, d_sr.data_ptr(), sr.shape[2], B, ws.data_ptr(), wb,
                                         bufs["dep"].data_ptr(), 0, sp))
-        if timed:
-            ev[1].record(stream)
+        mark(3)
         _lib.check(L.bb_conv2d_same(h, bufs["dep"].data_ptr(), bufs["mod"].data_ptr(), B, by, bx, sp))
-        if timed:
-            ev[2].record(stream)
+        mark(4)
         _lib.check(L.bb_residual_gauss(bufs["mod"].data_ptr(), like.dvalid.data_ptr(), B, nvox, s2, 0.0,
                                        bufs["work"].data_ptr(), bufs["wl"], bufs["ss"].data_ptr(),
                                        bufs["lg"].data_ptr(), sp))
-        if timed:
-            ev[3].record(stream)
+        mark(5)
 
     for _ in range(a.warmup):
         step(False)
@@ -171,17 +210,20 @@ def main():
     t0 = time.perf_counter()
     for _ in range(a.steps):
         step(True)
-        ev[3].synchronize()
-        stage += [ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), ev[2].elapsed_time(ev[3])]
+        ev[-1].synchronize()
+        stage += [ev[k].elapsed_time(ev[k + 1]) for k in range(len(names))]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     stage /= a.steps
     ms = dt * 1e3 / a.steps
     value = B * a.steps / dt
-    # parity of this batch's log-likelihoods with the host path (the same launches through GpuLikelihood)
+    # the same vectors through the host path (GpuLikelihood.loglike), for the record
     ll = like.loglike(theta[:4])
-    names = ["galmod_analytic", "conv2d_same", "residual"]
-    bytes_per_eval = {"galmod_analytic": 4 * nvox, "conv2d_same": 8 * nvox, "residual": 8 * nvox}
+    nmap = prof.n * prof.by * prof.bx
+    # algorithmic HBM bytes per evaluation: the cube written once (deposit), read and written (convolution), read
+    # with the data (residual); the maps: counts written and read, FP64 maps written (finish) and read (Gram)
+    bytes_per_eval = {"ring_maps": 8 * nmap * 3, "profile_solve": 8 * nmap, "galmod_analytic": 4 * nvox,
+                      "conv2d_same": 8 * nvox, "residual": 8 * nvox}
     dom = names[int(np.argmax(stage))]
     ach = bytes_per_eval[dom] * B / (stage[names.index(dom)] * 1e-3)
     out = {
@@ -189,7 +231,8 @@ def main():
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 (f64 geometry and sums)", "data": "synthetic",
         "config": {"workload": "GLOBAL likelihood, 160x160x96, 8\" pixels, 5.2 km/s, beam 30\", 40 rings, "
-                               "ANALYTIC, free VROT VDISP INC PA (82-D), DENS fixed", "batch": B,
+                               "ANALYTIC, free VROT VDISP INC PA (82-D), DENS from the deconvolved profile of "
+                               "every vector", "batch": B,
                    "model_box": [int(blo[0]), int(blo[1]), int(bhi[0]), int(bhi[1])]},
         "stages_ms_per_step": dict(zip(names, [round(float(x), 4) for x in stage])),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",

@@ -95,17 +95,18 @@ int bb_residual_gauss(const float* model, const float* dvalid, int B, int64_t nv
                       double* work, int64_t work_len, double* out_ss, double* out_lg, void* stream);
 
 /* Unit-density ring maps (DiskModel.ring_maps / _ring_maps_analytic, bbarolo/galmod.py:1317-1345, 468-536) in
- * two steps. bb_ring_maps_deposit adds to counts (B, nmap, by, bx) uint64 the total-intensity maps, summed over
- * the channels, of the ANALYTIC samples of the sub-ring tables sr (B, 16, nsub) (their density row is not read): sub-ring s
- * adds to map map0 + widx[s][m] (m = 0, 1; -1 or beyond nmap: none) with weight wval[s][m] (device arrays
- * (nsub, 2) int32 / float64: the interpolation of the ring densities, DiskModel._ring_weights,
- * galmod.py:1285-1314). Fixed point: a contribution x adds round(x / qunit), so the sums are exact and
- * deterministic; the caller picks qunit so that no sum exceeds 2^64 units (the contributions are <= fluxsc,
- * the largest 2 pi R0 radsep / (2 naz) of the tables). work: bb_ring_maps_work_size(plan, B) bytes. */
+ * two steps. bb_ring_maps_deposit writes the planes map0 .. map0+nw-1 of counts (B, nmap, by, bx) uint64: the
+ * total-intensity maps, summed over the channels, of the ANALYTIC samples of the sub-ring tables sr (B, 16, nsub;
+ * their density row is not read) with the radii of the plan. Sub-ring s adds to map map0 + widx[s][m] (m = 0, 1;
+ * -1 or >= nw: none) with weight wval[s][m] (device arrays (nsub, 2) int32 / float64: the interpolation of the
+ * ring densities, DiskModel._ring_weights, galmod.py:1285-1314). Fixed point: a contribution x adds
+ * round(x / qunit), so the sums are exact and deterministic; the caller picks qunit so that no sum exceeds 2^64
+ * units (every contribution is <= fluxsc = 2 pi R0 radsep / (2 naz) of its sub-ring).
+ * work: bb_ring_maps_work_size(plan, B) bytes. */
 int64_t bb_ring_maps_work_size(const bb_azplan* p, int B);
 int bb_ring_maps_deposit(const bb_model* m, const bb_azplan* p, bb_box box, const double* sr, int nsub, int B,
-                         const int32_t* widx, const double* wval, int nmap, int map0, double qunit, void* work,
-                         int64_t work_bytes, uint64_t* counts, void* stream);
+                         const int32_t* widx, const double* wval, int nmap, int map0, int nw, double qunit,
+                         void* work, int64_t work_bytes, uint64_t* counts, void* stream);
 /* maps (P, by, bx) float64 = counts x unit, convolved with the beam ('same', without the 1e-12 cut of the
  * models) and multiplied by the model's scale when convolve != 0 (galmod.py:1342-1345). unit = qunit x
  * mean(cd2i) x sum(relint). */
