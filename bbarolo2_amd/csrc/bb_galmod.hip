@@ -269,6 +269,37 @@ __device__ __forceinline__ void upix(const Sub& u, const bb_grid& g, double xs, 
     g1 = bb_lround(u.ypos + (xc + y * u.spa) / g.cdelt1);
 }
 
+// The pixel track of an azimuth sample is linear in its height: X(z) = X0 + z dX, Y(z) = Y0 + z dY (the same
+// sums as upix, regrouped). Its value is within ~1e-12 pixel of upix's, so it rounds to the same pixel except
+// within 1e-9 of a half-integer, where upix computes the pixel exactly as the reference does.
+struct Track {
+    double X0, dX, Y0, dY;
+};
+
+__device__ __forceinline__ Track utrack(const Sub& u, const bb_grid& g, double xs, double xc, double yb) {
+    Track t;
+    t.X0 = u.xpos + (xs - yb * u.cpa) / g.cdelt0;
+    t.dX = u.sinc * u.cpa / g.cdelt0;
+    t.Y0 = u.ypos + (xc + yb * u.spa) / g.cdelt1;
+    t.dY = -u.sinc * u.spa / g.cdelt1;
+    return t;
+}
+
+__device__ __forceinline__ bool near_half(double X) {
+    return !(fabs(X) < 1e9) || fabs(X - floor(X) - 0.5) < 1e-9;
+}
+
+__device__ __forceinline__ void upix_fast(const Sub& u, const bb_grid& g, const Track& t, double xs, double xc,
+                                          double yb, double z, int64_t& g0, int64_t& g1) {
+    const double X = fma(z, t.dX, t.X0), Y = fma(z, t.dY, t.Y0);
+    if (near_half(X) || near_half(Y)) {
+        upix(u, g, xs, xc, yb, z, g0, g1);
+    } else {
+        g0 = (int64_t)rint(X);
+        g1 = (int64_t)rint(Y);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------------------
 // Accumulation in the LDS tile is fixed point: every contribution x (>= 0) is stored as round(x / q) in a
 // 64-bit integer, q = amax / 2^31 with amax a bound of any single contribution of the parameter vector
@@ -295,6 +326,18 @@ __device__ __forceinline__ float lds_phi(double t, const float* ph, int ntab) {
     const float fr = (float)(x - (double)k);
     const float a = ph[k];
     return fmaf(fr, ph[k + 1] - a, a);
+}
+
+// The same with the argument in float32: t is a difference of velocities over sigma (the subtraction is FP64),
+// and x = 512 (t + 6) keeps 2.4e-4 of a table step, far below the table's own interpolation error.
+__device__ __forceinline__ float lds_phi_f(float t, const float* ph, int ntab) {
+    const float x = fmaf(t, 512.0f, 3072.0f);
+    const float xc = fminf(fmaxf(x, 0.0f), (float)(ntab - 2));     // (branch-free: the ends are selected below)
+    const int k = (int)xc;
+    const float fr = xc - (float)k;
+    const float a = ph[k];
+    const float v = fmaf(fr, ph[k + 1] - a, a);
+    return x > 0.0f ? (x < (float)(ntab - 1) ? v : 1.0f) : 0.0f;
 }
 
 // aw x the channel-integrated line of centre v (galmod.py:194-214) into pixel slot of the tile: the rare
@@ -396,13 +439,11 @@ __device__ __noinline__ void an_sample_lag(const bb_grid& g, const bb_box& box, 
 // The vertical samples of one azimuth sample that can fall in the tile, [ka, kb]; false if none. Only they are
 // placed exactly: a continuous estimate of the (monotone) pixel track picks them, widened by one sample on each
 // side, and those neighbours are outside the tile, so every run of equal pixels inside it is complete.
-__device__ __forceinline__ bool tile_zrange(const Sub& u, const bb_grid& g, const TileGeo& T, double xs, double xc,
-                                            double yb, int& ka, int& kb) {
+__device__ __forceinline__ bool tile_zrange(const Sub& u, const Track& tr, const TileGeo& T, int& ka, int& kb) {
     ka = 0;
     kb = u.nzz - 1;
     if (u.nzz > 2) {
-        const double X0 = u.xpos + (xs - yb * u.cpa) / g.cdelt0, dX = u.sinc * u.cpa / g.cdelt0;
-        const double Y0 = u.ypos + (xc + yb * u.spa) / g.cdelt1, dY = -u.sinc * u.spa / g.cdelt1;
+        const double X0 = tr.X0, dX = tr.dX, Y0 = tr.Y0, dY = tr.dY;
         const double m = 1e-6;
         int first = u.nzz, last = -1;
         for (int kz = 0; kz < u.nzz; kz++) {
@@ -422,7 +463,8 @@ __device__ __forceinline__ bool tile_zrange(const Sub& u, const bb_grid& g, cons
 
 // Deposit the runs of one azimuth sample that fall in the tile (galmod.py:389-431 for one sample).
 __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, const TileGeo& T, const Acc& A,
-                                          const double* sve, const double* scd, const float* sph, int ntab,
+                                          const double* sve, const double* scd, const float* scf, const float* sph,
+                                          int ntab,
                                           const Sub& u, const float* wl, const double* __restrict__ wt, double caz,
                                           double saz) {
     if (u.zdep) {
@@ -434,19 +476,20 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
     const double x = u.R * caz, yb = u.R * saz * u.cinc, xs = x * u.spa, xc = x * u.cpa;
     const double vplane = u.vsys + (u.vrot * caz + u.vrad * saz) * u.sinc + u.vvert * u.cinc;
     const double afl = u.flux * wside;
+    const Track tr = utrack(u, g, xs, xc, yb);
     int ka, kb;
-    if (!tile_zrange(u, g, T, xs, xc, yb, ka, kb)) return;
+    if (!tile_zrange(u, tr, T, ka, kb)) return;
     int rslot[BB_RMAX];
     float rws[BB_RMAX];
     int nr = 0;
     int64_t c0, c1;
-    upix(u, g, xs, xc, yb, uz(u, ka), c0, c1);
+    upix_fast(u, g, tr, xs, xc, yb, uz(u, ka), c0, c1);
     double ws = uw_(u, wl, wt, ka, g.ltype);
     for (int kz = ka + 1; kz <= kb + 1; kz++) {
         int64_t h0 = 0, h1 = 0;
         bool same = false;
         if (kz <= kb) {
-            upix(u, g, xs, xc, yb, uz(u, kz), h0, h1);
+            upix_fast(u, g, tr, xs, xc, yb, uz(u, kz), h0, h1);
             same = h0 == c0 && h1 == c1;
         }
         if (same) {
@@ -487,19 +530,27 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
         if (k0 < 0) k0 = 0;
         int64_t k1 = kc + u.K + 1;
         if (k1 > nz) k1 = nz;
-        const double inv = 1.0 / u.sig;
+        const float inv = (float)(1.0 / u.sig);
         float aw[BB_RMAX];
+        int rb[BB_RMAX];
 #pragma unroll
-        for (int r = 0; r < BB_RMAX; r++) aw[r] = a * rws[r] * A.inv_q;
-        float pp = k1 > k0 ? lds_phi((ve[k0] - vplane) * inv, sph, ntab) : 0.f;
+        for (int r = 0; r < BB_RMAX; r++) {
+            aw[r] = a * rws[r] * A.inv_q;
+            rb[r] = rslot[r] * A.nzp;
+        }
+        float pp = k1 > k0 ? lds_phi_f((float)(ve[k0] - vplane) * inv, sph, ntab) : 0.f;
         for (int k = (int)k0; k < (int)k1; k++) {
-            const float pn = lds_phi((ve[k + 1] - vplane) * inv, sph, ntab);
-            const float prof = fabsf(pn - pp) * (float)scd[k];
+            // (k opaque to the optimiser: the slot addresses are formed per run used, not carried for all slots)
+            asm volatile("" : "+v"(k));
+            const float pn = lds_phi_f((float)(ve[k + 1] - vplane) * inv, sph, ntab);
+            const float prof = fabsf(pn - pp) * scf[k];
             pp = pn;
 #pragma unroll
-            for (int r = 0; r < BB_RMAX; r++)
-                if (r < nr)
-                    atomicAdd(A.tile + rslot[r] * A.nzp + k, (unsigned long long)(unsigned int)fmaf(aw[r], prof, 0.5f));
+            for (int r = 0; r < BB_RMAX; r++) {
+                const bool on = r < nr;
+                if (__ballot(on) == 0) break;      // (no active lane has a run r: the wavefront leaves the loop)
+                if (on) atomicAdd(A.tile + rb[r] + k, (unsigned long long)(unsigned int)fmaf(aw[r], prof, 0.5f));
+            }
         }
     }
 }
@@ -562,7 +613,7 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
           int nsub, const short4* __restrict__ bball, const short4* __restrict__ sball,
           const double* __restrict__ cd2i, const double* __restrict__ vedges, const double* __restrict__ ptab,
           int ntab, float* __restrict__ outall, int accumulate) {
-    // dynamic LDS: [vedges (nlines, nz+1) f64][cd2i (nz) f64][Phi (ntab) f32][weights f32][tile u64]
+    // dynamic LDS: [vedges (nlines, nz+1) f64][cd2i (nz) f64][cd2i (nz) f32][Phi (ntab) f32][weights f32][tile u64]
     extern __shared__ double dyn[];
     __shared__ int clist[BB_LIST_CAP];
     __shared__ int slist[BB_SAMP_CAP];
@@ -577,7 +628,8 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     const bool wlds = nsub * BB_WCAP <= BB_WLDS_MAX;
     double* sve = dyn;
     double* scd = dyn + (size_t)nl_ * (nz + 1);
-    float* sph = (float*)(scd + nz);
+    float* scf = (float*)(scd + nz);
+    float* sph = scf + ((nz + 1) & ~1);
     float* swl = sph + ((ntab + 1) & ~1);
     unsigned long long* tile = (unsigned long long*)(swl + (wlds ? BB_WLDS_MAX : 0));
     const int ntile = TX * TY * nzp;
@@ -585,7 +637,10 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     const double* wtab = wtaball + (size_t)b * nsub * BB_ZCAP;
     for (int i = threadIdx.x; i < ntile; i += blockDim.x) tile[i] = 0ull;
     for (int i = threadIdx.x; i < nl_ * (nz + 1); i += blockDim.x) sve[i] = vedges[i];
-    for (int i = threadIdx.x; i < nz; i += blockDim.x) scd[i] = cd2i[i];
+    for (int i = threadIdx.x; i < nz; i += blockDim.x) {
+        scd[i] = cd2i[i];
+        scf[i] = (float)cd2i[i];
+    }
     for (int i = threadIdx.x; i < ntab; i += blockDim.x) sph[i] = (float)ptab[i];
     if (wlds)
         for (int i = threadIdx.x; i < nsub * BB_WCAP; i += blockDim.x)
@@ -632,8 +687,8 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
         Sub u;
         load_sub(u, sub + (size_t)s * BB_NSF, ch.x & 1);
         const double2 cs = az[ch.w + (e & 31)];
-        an_sample(g, box, T, A, sve, scd, sph, ntab, u, wlds ? swl + s * BB_WCAP : nullptr, wtab + (size_t)s * BB_ZCAP,
-                  cs.x, cs.y);
+        an_sample(g, box, T, A, sve, scd, scf, sph, ntab, u, wlds ? swl + s * BB_WCAP : nullptr,
+                  wtab + (size_t)s * BB_ZCAP, cs.x, cs.y);
     });
     __syncthreads();
     float* out = outall + (size_t)b * by * bx * nz;
@@ -651,7 +706,7 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
 // LDS bytes of the dynamic part of k_an_tile: channel edges and factors (f64), Phi table and vertical weights
 // (f32), the tile (u64).
 extern "C" size_t bb_analytic_lds_bytes(int nz, int nlines, int TX, int TY, int ntab) {
-    return ((size_t)nlines * (nz + 1) + nz) * sizeof(double) + (size_t)((ntab + 1) & ~1) * sizeof(float) +
+    return ((size_t)nlines * (nz + 1) + nz) * sizeof(double) + (size_t)(((nz + 1) & ~1) + ((ntab + 1) & ~1)) * sizeof(float) +
            (size_t)BB_WLDS_MAX * sizeof(float) + (size_t)TX * TY * bb_tile_stride(nz) * sizeof(unsigned long long);
 }
 
@@ -742,16 +797,17 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
         if (wside == 0.0) return;
         const double2 wv = wval[s];
         const double x = u.R * cs.x, yb = u.R * cs.y * u.cinc, xs = x * u.spa, xc = x * u.cpa;
+        const Track tr = utrack(u, g, xs, xc, yb);
         int ka, kb;
-        if (!tile_zrange(u, g, T, xs, xc, yb, ka, kb)) return;
+        if (!tile_zrange(u, tr, T, ka, kb)) return;
         int64_t c0, c1;
-        upix(u, g, xs, xc, yb, uz(u, ka), c0, c1);
+        upix_fast(u, g, tr, xs, xc, yb, uz(u, ka), c0, c1);
         double ws = uw_(u, nullptr, wt, ka, g.ltype);
         for (int kz = ka + 1; kz <= kb + 1; kz++) {
             int64_t h0 = 0, h1 = 0;
             bool same = false;
             if (kz <= kb) {
-                upix(u, g, xs, xc, yb, uz(u, kz), h0, h1);
+                upix_fast(u, g, tr, xs, xc, yb, uz(u, kz), h0, h1);
                 same = h0 == c0 && h1 == c1;
             }
             if (same) {
