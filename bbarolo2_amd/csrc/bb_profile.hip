@@ -46,15 +46,23 @@ k_rm_finish(const unsigned long long* __restrict__ counts, double* __restrict__ 
     double* tk = rm_lds + W * H;
     // input rows y0 - (ky-1-hy) .. y0 + T-1 + hy, columns likewise: out(y, x) = sum K[j][i] in[y + hy - j][x + hx - i]
     const int iy0 = y0 + hy - (ky - 1), ix0 = x0 + hx - (kx - 1);
+    int any = 0;
     for (int e = threadIdx.x; e < W * H; e += blockDim.x) {
         const int r = e / W, c = e % W;
         const int iy = iy0 + r, ix = ix0 + c;
-        tile[e] = (iy >= 0 && iy < by && ix >= 0 && ix < bx) ? (double)in[(size_t)iy * bx + ix] : 0.0;
+        const unsigned long long v = (iy >= 0 && iy < by && ix >= 0 && ix < bx) ? in[(size_t)iy * bx + ix] : 0ull;
+        any |= v != 0ull;
+        tile[e] = (double)v;
     }
     for (int e = threadIdx.x; e < ky * kx; e += blockDim.x) tk[e] = kern[e];
-    __syncthreads();
+    // a map is non-zero only near its ring: tiles whose input is all zero are written as zeros
+    const int nonzero = __syncthreads_or(any);
     const int x = x0 + tx, y = y0 + ty;
     if (x >= bx || y >= by) return;
+    if (!nonzero) {
+        out[(size_t)y * bx + x] = 0.0;
+        return;
+    }
     double acc = 0.0;
     for (int j = 0; j < ky; j++) {
         const double* row = tile + (ty + ky - 1 - j) * W + tx + kx - 1;
@@ -202,13 +210,31 @@ __device__ bool nn_passive_solve(const double* __restrict__ M, const NnLds& L, i
 }
 
 // Lawson-Hanson active set: returns false if the iteration limits were reached or a pivot failed.
-__device__ bool nn_solve(const double* __restrict__ M, const NnLds& L, int n, int lane) {
+// warm (optional, n): a guess of the positive set. If the passive solution on it is feasible the active set
+// starts there (a few iterations when the guess is the solution of a nearby problem), else from x = 0.
+__device__ bool nn_solve(const double* __restrict__ M, const NnLds& L, int n, int lane, const double* warm = nullptr) {
     for (int i = lane; i < n; i += 64) {
         L.x[i] = 0.0;
-        L.P[i] = 0;
+        L.P[i] = warm ? (warm[i] > 0.0) : 0;
         L.ban[i] = 0;
     }
     __syncthreads();
+    if (warm) {
+        bool ok = nn_passive_solve(M, L, n, lane);
+        if (lane == 0) {
+            int feas = ok;
+            for (int i = 0; i < n && feas; i++)
+                if (L.P[i] && !(L.z[i] > 0.0)) feas = 0;
+            L.sh[1] = feas;
+        }
+        __syncthreads();
+        const int feas = L.sh[1];
+        for (int i = lane; i < n; i += 64) {
+            if (feas) L.x[i] = L.z[i];
+            else L.P[i] = 0;
+        }
+        __syncthreads();
+    }
     const int maxit = 3 * n + 10;
     int it = 0;
     for (; it < maxit; it++) {
@@ -374,7 +400,7 @@ k_nnls(ProfConst pc, const double* __restrict__ gram, const double* __restrict__
             }
             __syncthreads();
             nn_build(M, G, n, lcoef, lcol, w2, 0.0, n - 2, lane);
-            ok = nn_solve(M, L, n, lane);
+            ok = nn_solve(M, L, n, lane, x0);
         }
     }
     for (int i = lane; i < n; i += 64) xout[(size_t)b * n + i] = ok ? L.x[i] : __builtin_nan("");
