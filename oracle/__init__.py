@@ -31,6 +31,15 @@ class orc_grid(C.Structure):
                 ("chwidth", C.c_double), ("zstep", C.c_double)]
 
 
+class orc_mcgrid(C.Structure):
+    _fields_ = [("nz", C.c_int32), ("ltype", C.c_int32), ("axtyp", C.c_int32), ("velsys", C.c_int32),
+                ("nlines", C.c_int32), ("cmode", C.c_int32), ("seed", C.c_uint32), ("pad0", C.c_int32),
+                ("radsep", C.c_double), ("cdens", C.c_double), ("pixarea", C.c_double), ("cdelt0", C.c_double),
+                ("cdelt1", C.c_double), ("crota2", C.c_double), ("sig_instr", C.c_double), ("crval3", C.c_double),
+                ("cdelt3", C.c_double), ("crpix3", C.c_double),
+                ("blo0", C.c_int64), ("blo1", C.c_int64), ("bhi0", C.c_int64), ("bhi1", C.c_int64)]
+
+
 _lib = None
 
 
@@ -60,6 +69,18 @@ def lib():
         L.orc_residuals_fracerr.argtypes = [P, P, P, C.c_int, P, C.c_int, C.c_int, C.c_int, C.c_double,
                                             C.c_double, C.POINTER(C.c_double)]
         L.orc_residuals_fracerr.restype = C.c_double
+        U4 = C.c_uint32 * 4
+        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.orc_philox4x32_10.restype = None
+        L.orc_galmod_mc.argtypes = [P, P, P, C.c_int, P, C.POINTER(orc_mcgrid), P, P, P]
+        L.orc_galmod_mc.restype = C.c_double
+        for f in ("orc_mc_log", "orc_mc_sqrt", "orc_mc_rcp"):
+            getattr(L, f).argtypes = [C.c_float]
+            getattr(L, f).restype = C.c_float
+        L.orc_mc_sincos_turn.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_mc_layer_dev.argtypes = [C.c_int, C.c_uint32, C.c_uint32]
+        L.orc_mc_layer_dev.restype = C.c_float
+        L.orc_mc_box_muller.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         _lib = L
     return _lib
 
@@ -176,3 +197,39 @@ def conv_fft(model_yxz, kern, scale=1.0, thresh=True):
     if scale != 1:
         out *= scale
     return np.ascontiguousarray(out.transpose(1, 2, 0))
+
+
+# ---- counter-based Monte-Carlo GALMOD (mc.c) -----------------------------------------------------------------
+def philox(ctr, key):
+    """Philox4x32-10 of a counter (4 uint32) and key (2 uint32) -> 4 uint32."""
+    c = (C.c_uint32 * 4)(*[int(x) & 0xFFFFFFFF for x in ctr])
+    k = (C.c_uint32 * 2)(*[int(x) & 0xFFFFFFFF for x in key])
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def make_mcgrid(setup, blo, bhi, seed):
+    g = orc_mcgrid()
+    g.nz, g.ltype, g.axtyp, g.velsys = setup.nsubs, setup.ltype, setup.axtyp, setup.velsys
+    g.nlines, g.cmode, g.seed = len(setup.lfreq0), int(setup.cmode), int(seed) & 0xFFFFFFFF
+    g.radsep, g.cdens, g.pixarea = setup.radsep, setup.cdens, setup.pixarea
+    g.cdelt0, g.cdelt1, g.crota2 = setup.cdelt[0], setup.cdelt[1], setup.crota2
+    g.sig_instr, g.crval3, g.cdelt3, g.crpix3 = setup.sig_instr, setup.crval3, setup.cdelt3, setup.crpix3
+    g.blo0, g.blo1, g.bhi0, g.bhi1 = blo[0], blo[1], bhi[0], bhi[1]
+    return g
+
+
+def galmod_mc(setup, sr, nv, blo, bhi, seed):
+    """orc_galmod_mc: (counts (by, bx, nz) uint64, model (by, bx, nz) float64, q) of the counter-based MC model."""
+    sr = np.ascontiguousarray(sr, np.float64)
+    nv = np.ascontiguousarray(nv, np.int64)
+    g = make_mcgrid(setup, blo, bhi, seed)
+    counts = np.zeros((bhi[1] - blo[1], bhi[0] - blo[0], setup.nsubs), np.uint64)
+    out = np.zeros(counts.shape)
+    ri = np.ascontiguousarray(setup.relint, np.float64)
+    cd = np.ascontiguousarray(setup.cd2i, np.float64)
+    lf = np.ascontiguousarray(setup.lfreq0, np.float64)
+    q = lib().orc_galmod_mc(_ptr(counts), _ptr(out), _ptr(sr), sr.shape[1], _ptr(nv), C.byref(g), _ptr(ri),
+                            _ptr(cd), _ptr(lf))
+    return counts, out, q

@@ -1,0 +1,346 @@
+/*
+ * ORACLE -- test infrastructure only. CPU restatement of the counter-based Monte-Carlo GALMOD of this project
+ * (the reference's MC kernel _galmod_mc, bbarolo/galmod.py:92-171, with its xoroshiro128+/ziggurat stream
+ * replaced by Philox4x32-10 counters, so that every cloud's draws are a function of (seed, sub-ring, cloud,
+ * block) and the GPU can draw them in any order).
+ *
+ * The specification both sides follow:
+ *   - per sub-ring (FP64, as galmod.py:103-117): ring_area_pix = 2 pi R0 radsep / pixarea,
+ *     nc = lround(cdens dens^cmode ring_area_pix), at least lround(10 ring_area_pix) when
+ *     lround(nc / ring_area_pix) < 10; fluxsc = float32(dens ring_area_pix pixarea / (nc nv));
+ *   - per cloud ic of sub-ring s: x = philox4x32_10(ctr {ic, 0, 0, 0}, key {seed, s});
+ *     R = sqrt(r_in^2 + 2 radsep R0 |2 u(x0) - 1|), azimuth |2 u(x1) - 1| turns, height deviate from (x2, x3)
+ *     by the layer type (1 Gaussian: Box-Muller; 2 sech2: atanh; 3 exponential: -+log|.|; 4 Lorentzian: tan;
+ *     else uniform), all in FP32 with the functions below (+, -, *, fmaf and integer bit operations only, so
+ *     the CPU and the GPU round identically);
+ *   - sub-cloud iv: normal deviate iv % 4 of block 1 + iv / 4 (ctr {ic, 1 + iv / 4, 0, 0}), two Box-Muller
+ *     pairs per block; channel lround(c0 + n sg) on a linear velocity axis with one line, else
+ *     lround(velgrid(vs + n vdisp)) per line (FP64, galmod.py:49-62);
+ *   - each sub-cloud adds u = (uint32) fmaf(relint_f[nl] fluxsc wside, 1/q, 0.5) to its voxel, q = the largest
+ *     relint fluxsc wside of the vector x 2^-30; the model is count x q x cd2i[k].
+ * The integer count cube is the bit-exact contract between this file and the device kernels.
+ *
+ * Pinned by tests/test_oracle_mc.py: Philox4x32-10 known-answer vectors (Salmon et al., SC'11, Random123),
+ * the deviates' moments and ranges, the functions against libm, MC vs ANALYTIC total flux within 2% and
+ * spectrum correlation > 0.99 (tests/test_bbarolo.py:78-87), box model = crop of the full model (exact).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---- Philox4x32-10 ---------------------------------------------------------------------------------- */
+static inline void mulhilo32(uint32_t a, uint32_t b, uint32_t* hi, uint32_t* lo) {
+    const uint64_t p = (uint64_t)a * (uint64_t)b;
+    *hi = (uint32_t)(p >> 32);
+    *lo = (uint32_t)p;
+}
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint32_t hi0, lo0, hi1, lo1;
+        mulhilo32(0xD2511F53u, c0, &hi0, &lo0);
+        mulhilo32(0xCD9E8D57u, c2, &hi1, &lo1);
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+/* ---- FP32 functions from +, -, *, fmaf ------------------------------------------------------------- */
+static inline float u01(uint32_t x) { return ((float)(x >> 9) + 0.5f) * 0x1p-23f; } /* (0, 1), exact */
+
+static inline float f_of(uint32_t i) { float f; memcpy(&f, &i, 4); return f; }
+static inline uint32_t i_of(float f) { uint32_t i; memcpy(&i, &f, 4); return i; }
+
+/* 1/x, x > 0: bit estimate, then Newton steps */
+static inline float rcp_f(float x) {
+    float y = f_of(0x7EF311C3u - i_of(x));
+    for (int i = 0; i < 4; i++) y = fmaf(y, fmaf(-x, y, 1.0f), y);
+    return y;
+}
+
+/* sqrt(x), x >= 0: inverse square root by Newton, then one correction of x y */
+static inline float sqrt_f(float x) {
+    if (!(x > 0.0f)) return 0.0f;
+    float y = f_of(0x5F375A86u - (i_of(x) >> 1));
+    for (int i = 0; i < 4; i++) y = y * fmaf(-0.5f * x * y, y, 1.5f);
+    const float s = x * y;
+    return fmaf(0.5f * y, fmaf(-s, s, x), s);
+}
+
+/* log(x), x > 0 normal: x = m 2^e, m in [sqrt(1/2), sqrt(2)), log(1 + y) = y P(y) (least-squares fit) */
+static inline float log_f(float x) {
+    uint32_t i = i_of(x);
+    int e = (int)((i >> 23) & 0xFF) - 127;
+    float m = f_of((i & 0x007FFFFFu) | 0x3F800000u);
+    if (m > 1.41421356f) {
+        m *= 0.5f;
+        e += 1;
+    }
+    const float y = m - 1.0f;
+    float p = -7.638592273e-02f;
+    p = fmaf(p, y, 1.291417032e-01f);
+    p = fmaf(p, y, -1.324084252e-01f);
+    p = fmaf(p, y, 1.418007910e-01f);
+    p = fmaf(p, y, -1.660912633e-01f);
+    p = fmaf(p, y, 2.000207156e-01f);
+    p = fmaf(p, y, -2.500158548e-01f);
+    p = fmaf(p, y, 3.333332539e-01f);
+    p = fmaf(p, y, -4.999998808e-01f);
+    p = fmaf(p, y, 1.0f);
+    return fmaf((float)e, 0.693147182f, y * p);
+}
+
+/* (cos, sin) of 2 pi t, t in [0, 1]: quadrant of 4 t, Taylor series of the remaining quarter turn */
+static inline void sincos_turn(float t, float* c, float* s) {
+    const float t4 = 4.0f * t;
+    const float qf = floorf(t4);
+    const float f = t4 - qf;          /* exact */
+    const int q = ((int)qf) & 3;
+    const float f2 = f * f;
+    /* sin(pi f / 2) and cos(pi f / 2): Taylor coefficients (-1)^k (pi/2)^n / n! */
+    float ps = 5.69217278e-08f;
+    ps = fmaf(ps, f2, -3.59884325e-06f);
+    ps = fmaf(ps, f2, 1.60441181e-04f);
+    ps = fmaf(ps, f2, -4.68175393e-03f);
+    ps = fmaf(ps, f2, 7.96926245e-02f);
+    ps = fmaf(ps, f2, -6.45964086e-01f);
+    ps = fmaf(ps, f2, 1.57079637e+00f);
+    const float sv = ps * f;
+    float pc = -6.38660325e-09f;
+    pc = fmaf(pc, f2, 4.71087475e-07f);
+    pc = fmaf(pc, f2, -2.52020418e-05f);
+    pc = fmaf(pc, f2, 9.19260259e-04f);
+    pc = fmaf(pc, f2, -2.08634809e-02f);
+    pc = fmaf(pc, f2, 2.53669500e-01f);
+    pc = fmaf(pc, f2, -1.23370051e+00f);
+    pc = fmaf(pc, f2, 1.0f);
+    const float cv = pc;
+    switch (q) {
+        case 0: *c = cv; *s = sv; break;
+        case 1: *c = -sv; *s = cv; break;
+        case 2: *c = -cv; *s = -sv; break;
+        default: *c = sv; *s = -cv; break;
+    }
+}
+
+/* two standard normal deviates from two uniforms (Box-Muller) */
+static inline void box_muller(uint32_t a, uint32_t b, float* n0, float* n1) {
+    const float r = sqrt_f(-2.0f * log_f(u01(a)));
+    float c, s;
+    sincos_turn(u01(b), &c, &s);
+    *n0 = r * c;
+    *n1 = r * s;
+}
+
+/* vertical deviate of the layer (galmod.py:76-89 with these draws) */
+static inline float layer_dev(int ltype, uint32_t a, uint32_t b) {
+    if (ltype == 1) {
+        float n0, n1;
+        box_muller(a, b, &n0, &n1);
+        return n0;
+    }
+    const float x = 2.0f * u01(a) - 1.0f;
+    if (ltype == 2) return 0.5f * (log_f(1.0f + x) - log_f(1.0f - x));
+    if (ltype == 3) return x >= 0.0f ? -log_f(x) : log_f(-x);
+    if (ltype == 4) {
+        float c, s;
+        sincos_turn(fabsf(x) * 0.25f, &c, &s);
+        const float t = s * rcp_f(c);
+        return x >= 0.0f ? t : -t;
+    }
+    return x;
+}
+
+static inline int64_t lround_f(float x) {
+    const float a = fabsf(x);
+    if (!(a < 1e9f)) return -((int64_t)1 << 62);
+    const int64_t r = (int64_t)(a + 0.5f);
+    return x >= 0.0f ? r : -r;
+}
+
+static inline int64_t lround_d(double x) {
+    if (!(fabs(x) < 1e18)) return -((int64_t)1 << 62);
+    if (x >= 0) return (int64_t)(x + 0.5);
+    return -(int64_t)(-x + 0.5);
+}
+
+static inline float lag_f(float vrot, float dvdz, float dz) {
+    const float v = vrot - dvdz * dz;
+    if (vrot >= 0.0f) return v > 0.0f ? v : 0.0f;
+    return v < 0.0f ? v : 0.0f;
+}
+
+static inline double velgrid_d(double v, int axtyp, int velsys, double freq0, double crval3, double cdelt3,
+                               double crpix3) {
+    const double C = 2.99792458e8;
+    double w;
+    if (axtyp == 4) {
+        w = v;
+    } else {
+        if (velsys == 1) w = freq0 * (1 - v / C);
+        else if (velsys == 2) w = freq0 * (C / (C + v));
+        else w = freq0 * sqrt((C - v) / (C + v));
+        if (axtyp == 2) w = C / w;
+    }
+    return (w - crval3) / cdelt3 + crpix3 - 1;
+}
+
+/* ---- the model ----------------------------------------------------------------------------------- */
+typedef struct orc_mcgrid {
+    int32_t nz, ltype, axtyp, velsys, nlines, cmode;
+    uint32_t seed;
+    int32_t pad0;
+    double radsep, cdens, pixarea, cdelt0, cdelt1, crota2, sig_instr, crval3, cdelt3, crpix3;
+    int64_t blo0, blo1, bhi0, bhi1;
+} orc_mcgrid;
+
+typedef struct mc_sub {
+    int64_t nc;
+    float fluxsc, r_in2, r_cross, sinc, cinc, spa, cpa, xpos, ypos, vsys, vrot, vrad, vvert, vdisp, z0, dvdz,
+        zcyl, wrec, wapp;
+    int nv;
+} mc_sub;
+
+/* the per-sub-ring quantities; nc = 0 for a sub-ring without clouds */
+static void mc_prep(mc_sub* q, const double* sr, int nsub, int s, const int64_t* nv, const orc_mcgrid* g) {
+    const double R0 = sr[s], dens = sr[5 * nsub + s];
+    memset(q, 0, sizeof *q);
+    if (dens == 0) return;
+    const double ring_area_pix = 2 * M_PI * R0 * g->radsep / g->pixarea;
+    double dc = g->cmode == 1 ? dens : g->cmode == 2 ? dens * dens : pow(dens, (double)g->cmode);
+    int64_t nc = lround_d(g->cdens * dc * ring_area_pix);
+    if ((double)lround_d((double)nc / ring_area_pix) < 10.) nc = lround_d(10 * ring_area_pix);
+    if (nc <= 0) return;
+    q->nc = nc;
+    q->nv = (int)nv[s];
+    q->fluxsc = (float)(dens * ring_area_pix * g->pixarea / ((double)nc * (double)nv[s]));
+    const double r_inner = R0 - 0.5 * g->radsep;
+    q->r_in2 = (float)(r_inner * r_inner);
+    q->r_cross = (float)(2.0 * g->radsep * R0);
+    const double TURN = 1.0 / (2 * M_PI);
+    float c, sn;
+    double ti = sr[9 * nsub + s] * TURN;
+    ti -= floor(ti);
+    sincos_turn((float)ti, &c, &sn);
+    q->sinc = sn;
+    q->cinc = c;
+    double tp = (sr[10 * nsub + s] + g->crota2) * TURN;
+    tp -= floor(tp);
+    sincos_turn((float)tp, &c, &sn);
+    q->spa = sn;
+    q->cpa = c;
+    q->xpos = (float)sr[11 * nsub + s];
+    q->ypos = (float)sr[12 * nsub + s];
+    q->vsys = (float)sr[13 * nsub + s];
+    q->vrot = (float)sr[1 * nsub + s];
+    q->vrad = (float)sr[2 * nsub + s];
+    q->vvert = (float)sr[3 * nsub + s];
+    q->vdisp = (float)sqrt(sr[4 * nsub + s] * sr[4 * nsub + s] + g->sig_instr * g->sig_instr);
+    q->z0 = (float)sr[6 * nsub + s];
+    q->dvdz = (float)sr[7 * nsub + s];
+    q->zcyl = (float)sr[8 * nsub + s];
+    q->wrec = (float)sr[14 * nsub + s];
+    q->wapp = (float)sr[15 * nsub + s];
+}
+
+/* counts (by, bx, nz) uint64 +=; out (by, bx, nz) float64 = counts x q x cd2i (if not NULL); returns q */
+double orc_galmod_mc(uint64_t* counts, double* out, const double* sr, int nsub, const int64_t* nv,
+                     const orc_mcgrid* g, const double* relint, const double* cd2i, const double* lfreq0) {
+    const int nz = g->nz, nlines = g->nlines;
+    const int64_t bx = g->bhi0 - g->blo0, by = g->bhi1 - g->blo1;
+    mc_sub* subs = (mc_sub*)calloc(nsub > 0 ? nsub : 1, sizeof(mc_sub));
+    double amax = 0, rmax = 0;
+    float relf[8];
+    for (int nl = 0; nl < nlines; nl++) {
+        relf[nl] = (float)relint[nl];
+        rmax = fabs((double)relf[nl]) > rmax ? fabs((double)relf[nl]) : rmax;
+    }
+    for (int s = 0; s < nsub; s++) {
+        mc_prep(&subs[s], sr, nsub, s, nv, g);
+        const double w = fabs((double)subs[s].wrec) > fabs((double)subs[s].wapp) ? fabs((double)subs[s].wrec)
+                                                                                : fabs((double)subs[s].wapp);
+        const double a = subs[s].nc > 0 ? fabs((double)subs[s].fluxsc) * w : 0.0;
+        amax = a > amax ? a : amax;
+    }
+    amax *= rmax;
+    const double q = amax > 0 ? amax * 0x1p-30 : 1.0;
+    const float inv_q = (float)(1.0 / q);
+    const float icd0 = (float)(1.0 / g->cdelt0), icd1 = (float)(1.0 / g->cdelt1);
+    const int linear = g->axtyp == 4 && nlines == 1;
+    const float ia = (float)(1.0 / g->cdelt3), ib = (float)(-g->crval3 / g->cdelt3 + g->crpix3 - 1);
+    for (int s = 0; s < nsub; s++) {
+        const mc_sub* u = &subs[s];
+        const uint32_t key[2] = {g->seed, (uint32_t)s};
+        for (int64_t ic = 0; ic < u->nc; ic++) {
+            uint32_t ctr[4] = {(uint32_t)ic, 0u, 0u, 0u}, x[4];
+            orc_philox4x32_10(ctr, key, x);
+            const float R = sqrt_f(fmaf(u->r_cross, fabsf(2.0f * u01(x[0]) - 1.0f), u->r_in2));
+            float caz, saz;
+            sincos_turn(fabsf(2.0f * u01(x[1]) - 1.0f), &caz, &saz);
+            const float z = layer_dev(g->ltype, x[2], x[3]) * u->z0;
+            const float xx = R * caz;
+            const float yy = R * saz * u->cinc - z * u->sinc;
+            const int64_t g0 = lround_f(u->xpos + (xx * u->spa - yy * u->cpa) * icd0);
+            const int64_t g1 = lround_f(u->ypos + (xx * u->cpa + yy * u->spa) * icd1);
+            const float wside = caz >= 0.0f ? u->wrec : u->wapp;
+            if (wside == 0.0f || g0 < g->blo0 || g0 >= g->bhi0 || g1 < g->blo1 || g1 >= g->bhi1) continue;
+            const float az_ = fabsf(z);
+            float vs;
+            if (az_ > u->zcyl) vs = u->vsys + (lag_f(u->vrot, u->dvdz, az_ - u->zcyl) * caz + u->vrad * saz) * u->sinc;
+            else vs = u->vsys + (u->vrot * caz + u->vrad * saz) * u->sinc;
+            vs += u->vvert * u->cinc;
+            const float aw = u->fluxsc * wside;
+            uint64_t* cp = counts + ((g1 - g->blo1) * bx + (g0 - g->blo0)) * (int64_t)nz;
+            const float c0 = vs * ia + ib, sg = u->vdisp * ia;
+            for (int iv = 0; iv < u->nv; iv += 4) {
+                uint32_t cb[4] = {(uint32_t)ic, 1u + (uint32_t)(iv / 4), 0u, 0u}, y[4];
+                orc_philox4x32_10(cb, key, y);
+                float n[4];
+                box_muller(y[0], y[1], &n[0], &n[1]);
+                box_muller(y[2], y[3], &n[2], &n[3]);
+                for (int j = 0; j < 4 && iv + j < u->nv; j++) {
+                    if (linear) {
+                        const int64_t k = lround_f(fmaf(n[j], sg, c0));
+                        if (k < 0 || k >= nz) continue;
+                        cp[k] += (uint64_t)(uint32_t)fmaf(relf[0] * aw, inv_q, 0.5f);
+                    } else {
+                        const double v = (double)fmaf(n[j], u->vdisp, vs);
+                        for (int nl = 0; nl < nlines; nl++) {
+                            const int64_t k = lround_d(velgrid_d(v, g->axtyp, g->velsys, lfreq0[nl], g->crval3,
+                                                                 g->cdelt3, g->crpix3));
+                            if (k < 0 || k >= nz) continue;
+                            cp[k] += (uint64_t)(uint32_t)fmaf(relf[nl] * aw, inv_q, 0.5f);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    (void)by;
+    if (out)
+        for (int64_t p = 0; p < by * bx; p++)
+            for (int k = 0; k < nz; k++) out[p * nz + k] = (double)counts[p * nz + k] * q * cd2i[k];
+    free(subs);
+    return q;
+}
+
+/* the FP32 functions, exported for their tests */
+float orc_mc_log(float x) { return log_f(x); }
+float orc_mc_sqrt(float x) { return sqrt_f(x); }
+float orc_mc_rcp(float x) { return rcp_f(x); }
+void orc_mc_sincos_turn(float t, float* c, float* s) { sincos_turn(t, c, s); }
+float orc_mc_layer_dev(int ltype, uint32_t a, uint32_t b) { return layer_dev(ltype, a, b); }
+void orc_mc_box_muller(uint32_t a, uint32_t b, float* n0, float* n1) { box_muller(a, b, n0, n1); }
