@@ -29,6 +29,14 @@ extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* s
                                    int nmap, int map0, int nw, double qunit, unsigned long long* counts,
                                    hipStream_t st);
 extern "C" size_t bb_rm_tile_lds(int nw, int TX, int TY);
+extern "C" size_t bb_mc_sub_bytes();
+extern "C" size_t bb_mc_rec_bytes();
+extern "C" size_t bb_mc_lds_bytes(int nz, int TX, int TY);
+extern "C" int bb_launch_galmod_mc(const bb_grid* g, bb_box box, const double* sr, int nsub, int B, int nvpar,
+                                   uint32_t seed, int TX, int TY, int64_t cap, void* subs, double* qv, float* invq,
+                                   int64_t* total, int* tcount, int64_t* toff, int* tcursor, void* recs, int* overflow,
+                                   const double* cd2i, float* out, unsigned long long* counts, int accumulate,
+                                   hipStream_t st);
 extern "C" size_t bb_rm_tile_static();
 extern "C" size_t bb_rm_finish_lds(int ky, int kx);
 extern "C" int bb_launch_rm_finish(const unsigned long long* counts, double* maps, int P, int by, int bx,
@@ -505,4 +513,75 @@ extern "C" int bb_profile_set_dens(const double* x, int B, int n, int nr, const 
     if (!x || !widx || !wval || !sr) return fail(BB_ERR_ARG, "bb_profile_set_dens: NULL array");
     int rc = bb_launch_set_dens(x, B, n, nr, (const int*)widx, wval, nsub, sr, dens, (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_profile_set_dens launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Monte-Carlo GALMOD.
+static size_t mc_align(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static void mc_tiles(const bb_model* m, bb_box box, int* ntiles) {
+    const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
+    *ntiles = ((bx + m->TX - 1) / m->TX) * ((by + m->TY - 1) / m->TY);
+}
+
+extern "C" int64_t bb_mc_work_size(const bb_model* m, bb_box box, int nsub, int B, int64_t max_clouds) {
+    if (!m || nsub < 0 || B < 0 || max_clouds < 0 || box.bhi0 <= box.blo0 || box.bhi1 <= box.blo1) return -1;
+    int nt;
+    mc_tiles(m, box, &nt);
+    size_t w = mc_align((size_t)B * nsub * bb_mc_sub_bytes()) + mc_align((size_t)B * (sizeof(double) + sizeof(float) +
+                                                                                      sizeof(int64_t))) +
+               mc_align((size_t)B * nt * sizeof(int)) * 2 + mc_align((size_t)B * (nt + 1) * sizeof(int64_t)) +
+               mc_align(sizeof(int)) + mc_align((size_t)B * max_clouds * bb_mc_rec_bytes()) + 256;
+    return (int64_t)w;
+}
+
+extern "C" int bb_galmod_mc(const bb_model* m, bb_box box, const double* sr, int nsub, int B, int nvpar, uint32_t seed,
+                            int64_t max_clouds, void* work, int64_t work_bytes, float* out, uint64_t* counts,
+                            int* overflow, int accumulate, void* stream) {
+    if (!m) return fail(BB_ERR_ARG, "bb_galmod_mc: model is NULL");
+    int rc = check_box(m, box);
+    if (rc) return rc;
+    if (B < 0 || B > 65535 || nsub < 0 || nsub > 65535) return fail(BB_ERR_ARG, "bb_galmod_mc: bad sizes");
+    if (nvpar < 1 || nvpar > 65535) return fail(BB_ERR_ARG, "bb_galmod_mc: nv = %d", nvpar);
+    if (max_clouds < 0) return fail(BB_ERR_ARG, "bb_galmod_mc: max_clouds < 0");
+    if (box.bhi0 - box.blo0 > 32767 || box.bhi1 - box.blo1 > 32767)
+        return fail(BB_ERR_ARG, "bb_galmod_mc: box larger than 32767 pixels");
+    if (bb_mc_lds_bytes(m->g.nz, m->TX, m->TY) > BB_LDS_MAX)
+        return fail(BB_ERR_ARG, "bb_galmod_mc: the %dx%d tile of %d channels exceeds the LDS", m->TX, m->TY, m->g.nz);
+    if (B == 0) return 0;
+    if (!out || !work || !overflow || (nsub > 0 && !sr)) return fail(BB_ERR_ARG, "bb_galmod_mc: NULL array");
+    if (work_bytes < bb_mc_work_size(m, box, nsub, B, max_clouds))
+        return fail(BB_ERR_ARG, "bb_galmod_mc: work holds %lld bytes, needs %lld", (long long)work_bytes,
+                    (long long)bb_mc_work_size(m, box, nsub, B, max_clouds));
+    int nt;
+    mc_tiles(m, box, &nt);
+    char* w = (char*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
+    void* subs = w;
+    w += mc_align((size_t)B * nsub * bb_mc_sub_bytes());
+    double* qv = (double*)w;
+    float* invq = (float*)(qv + B);
+    int64_t* total = (int64_t*)(((uintptr_t)(invq + B) + 7) & ~(uintptr_t)7);
+    w += mc_align((size_t)B * (sizeof(double) + sizeof(float) + sizeof(int64_t)));
+    int* tcount = (int*)w;
+    w += mc_align((size_t)B * nt * sizeof(int));
+    int* tcursor = (int*)w;
+    w += mc_align((size_t)B * nt * sizeof(int));
+    int64_t* toff = (int64_t*)w;
+    w += mc_align((size_t)B * (nt + 1) * sizeof(int64_t));
+    w += mc_align(sizeof(int));
+    void* recs = w;
+    hipStream_t st = (hipStream_t)stream;
+    rc = hip_check(hipMemsetAsync(tcount, 0, (size_t)B * nt * sizeof(int), st), "bb_galmod_mc");
+    if (!rc) rc = hip_check(hipMemsetAsync(overflow, 0, sizeof(int), st), "bb_galmod_mc");
+    if (rc) return rc;
+    if (nsub == 0) {
+        const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
+        if (!accumulate) rc = hip_check(hipMemsetAsync(out, 0, (size_t)B * by * bx * m->g.nz * sizeof(float), st), "bb_galmod_mc");
+        if (!rc && counts) rc = hip_check(hipMemsetAsync(counts, 0, (size_t)B * by * bx * m->g.nz * 8, st), "bb_galmod_mc");
+        return rc;
+    }
+    rc = bb_launch_galmod_mc(&m->g, box, sr, nsub, B, nvpar, seed, m->TX, m->TY, max_clouds, subs, qv, invq, total,
+                             tcount, toff, tcursor, recs, overflow, m->d_cd2i, out, (unsigned long long*)counts,
+                             accumulate, st);
+    return rc ? fail(BB_ERR_HIP, "bb_galmod_mc launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }

@@ -3,7 +3,9 @@
 DiskModel(header, par, ringmode, densflux, galmodtype, smooth) computes the header-derived setup and the
 beam kernel once, copies them to the device (bb_model_create), and builds models of rings:
   array(rings, blo, bhi, final, layout)   float64 numpy array (nz, ny, nx) or (ny, nx, nz), as the reference;
-  array_batch(tables, box)                a (B, by, bx, nz) float32 device tensor (yxz) of B sub-ring tables.
+  array_batch(tables, box)                a (B, by, bx, nz) float32 device tensor (yxz) of B sub-ring tables
+                                          (ANALYTIC); array_batch_mc the same with the Monte-Carlo model;
+  ring_maps(rings, box)                   the unit-density total-intensity maps of the rings.
 Device memory and streams come from torch; the arithmetic runs in the HIP kernels. There is no CPU path:
 without a GPU or without libbbgpu.so these methods raise.
 """
@@ -15,7 +17,7 @@ import numpy as np
 
 from . import _lib
 from .galmod import (GalmodSetup, GalmodPar, model_rings, model_box, ringmode_name, zstep_for, nzq_for,
-                     PHI_TAB, ZSAMPLES_MAX)
+                     PHI_TAB, ZSAMPLES_MAX, seed32)
 from .smooth import beam_kernel
 
 AN_THREADS = 256
@@ -145,17 +147,19 @@ class DiskModel:
             rings = model_rings(rings, self.ringmode if ringmode is None else ringmode, self.setup.radsep)
         return self.setup.subrings(rings)
 
-    def array(self, rings, blo=None, bhi=None, final=True, ringmode=None, galmodtype=None, layout="zyx"):
+    def array(self, rings, blo=None, bhi=None, final=True, ringmode=None, galmodtype=None, layout="zyx", seed=None):
         """The model of the rings (convolved with the beam if smoothing is on) in the box [blo, bhi), as a
         float64 array (nz, ny, nx), or (ny, nx, nz) with layout='yxz' (DiskModel.array, galmod.py:1242-1271).
-        final=False (models of single rings in the 3DFIT ring fits) uses the rings as nodes."""
+        final=False (models of single rings in the 3DFIT ring fits) uses the rings as nodes. seed: of the MC
+        model (default ISEED)."""
         gtype = (galmodtype or self.setup.galmodtype).upper()
-        if gtype != "ANALYTIC":
-            raise NotImplementedError("GALMODTYPE MC models are not built by this engine yet")
         if blo is None or bhi is None:
             blo, bhi = (0, 0), (self.setup.dimx, self.setup.dimy)
         sr, _ = self.subrings(rings, final, ringmode)
-        m = self.array_batch(sr[None], blo, bhi, convolve=self.cfield is not None)
+        if gtype == "ANALYTIC":
+            m = self.array_batch(sr[None], blo, bhi, convolve=self.cfield is not None)
+        else:
+            m = self.array_batch_mc(sr[None], blo, bhi, convolve=self.cfield is not None, seed=seed)
         a = m[0].double().cpu().numpy()
         return np.ascontiguousarray(a.transpose(2, 0, 1)) if layout == "zyx" else a
 
@@ -202,6 +206,60 @@ class DiskModel:
         _lib.check(L.bb_ring_maps_finish(h, counts.data_ptr(), rings.nr, by, bx, unit, int(self.cfield is not None),
                                          maps.data_ptr(), stp))
         return maps.cpu().numpy()
+
+    def mc_clouds(self, sr):
+        """The clouds of each table (B, 16, nsub) of the MC model (galmod.py:103-108): (B,) int64 upper bounds
+        (the clouds inside any box are at most these)."""
+        st = self.setup
+        sr = np.asarray(sr, np.float64)
+        R0, dens = sr[:, 0], sr[:, 5]
+        area = 2 * math.pi * R0 * st.radsep / st.pixarea
+        dc = dens if st.cmode == 1 else dens * dens if st.cmode == 2 else dens ** st.cmode
+        nc = np.floor(st.cdens * dc * area + 0.5)
+        nc = np.where(np.floor(nc / np.maximum(area, 1e-300) + 0.5) < 10., np.floor(10 * area + 0.5), nc)
+        nc = np.where(dens != 0, nc, 0)
+        return (nc.sum(axis=1) + 2 * sr.shape[2]).astype(np.int64)
+
+    def array_batch_mc(self, sr, blo, bhi, convolve=True, out=None, work=None, stream=None, d_sr=None, seed=None,
+                       counts=None, max_clouds=None):
+        """Monte-Carlo models (counter-based stream, bbgpu.h) of B sub-ring tables sr (B, 16, nsub) in the box
+        [blo, bhi): a (B, by, bx, nz) float32 device tensor (yxz), convolved with the beam if convolve. counts:
+        optional (B, by, bx, nz) int64 device tensor of the integer sums. max_clouds: clouds per vector to hold
+        (default: the bound of mc_clouds on sr)."""
+        torch = _torch()
+        sr = np.ascontiguousarray(sr, np.float64)
+        B, nrow, nsub = sr.shape
+        if nrow != 16:
+            raise ValueError("sub-ring tables must have 16 rows")
+        if not np.all(np.isfinite(sr)):
+            raise ValueError("sub-ring tables hold NaN or infinite values")
+        st = self.setup
+        by, bx, nz = bhi[1] - blo[1], bhi[0] - blo[0], st.nsubs
+        dev = torch.device("cuda", torch.cuda.current_device())
+        h = self.handle(nzq_for(sr.reshape(16, -1)))
+        sp = _stream_ptr(torch, stream)
+        L = _lib.lib()
+        box = _lib.bb_box(blo[0], blo[1], bhi[0], bhi[1])
+        cap = int(self.mc_clouds(sr).max()) if max_clouds is None else int(max_clouds)
+        wb = int(L.bb_mc_work_size(h, box, nsub, B, cap))
+        if wb < 0:
+            raise _lib.BBError("bb_mc_work_size: bad arguments")
+        ws = torch.empty(wb, dtype=torch.uint8, device=dev)
+        flag = torch.zeros(1, dtype=torch.int32, device=dev)
+        dep = torch.empty((B, by, bx, nz), dtype=torch.float32, device=dev) if work is None else work
+        if d_sr is None:
+            d_sr = torch.from_numpy(sr).to(dev)
+        nv = st.nvpar if st.nvpar >= 1 else st.nsubs
+        s32 = seed32(st.iseed if seed is None else seed)
+        _lib.check(L.bb_galmod_mc(h, box, d_sr.data_ptr(), nsub, B, nv, s32, cap, ws.data_ptr(), wb, dep.data_ptr(),
+                                  None if counts is None else counts.data_ptr(), flag.data_ptr(), 0, sp))
+        if int(flag.item()) != 0:
+            raise _lib.BBError(f"bb_galmod_mc: more than {cap} clouds of a vector inside the box")
+        if not convolve or self.cfield is None:
+            return dep
+        res = torch.empty_like(dep) if out is None else out
+        _lib.check(L.bb_conv2d_same(h, dep.data_ptr(), res.data_ptr(), B, by, bx, sp))
+        return res
 
     def array_batch(self, sr, blo, bhi, convolve=True, out=None, work=None, stream=None, d_sr=None):
         """ANALYTIC models of B sub-ring tables sr (B, 16, nsub) sharing their radii, in the box [blo, bhi):

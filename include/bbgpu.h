@@ -94,6 +94,19 @@ int64_t bb_residual_work_size(int B, int64_t nvox);
 int bb_residual_gauss(const float* model, const float* dvalid, int B, int64_t nvox, double sigma2, double eps2,
                       double* work, int64_t work_len, double* out_ss, double* out_lg, void* stream);
 
+/* Monte-Carlo GALMOD (_galmod_mc, bbarolo/galmod.py:92-171) of B sub-ring tables sr (B, 16, nsub) with a
+ * counter-based stream: the draws of cloud ic of sub-ring s are Philox4x32-10 of counters (ic, block) under the
+ * key (seed, s) (the reference's xoroshiro128+ / ziggurat stream is sequential), so the model of a box is the crop
+ * of the full model and every vector of a batch has the same draws (common random numbers, as ISEED). nvpar:
+ * sub-clouds per cloud (NV, or the channel count; 1 where VDISP = 0). out (B, by, bx, nz) float32 yxz, written
+ * (accumulate = 0) or added to; counts (optional, (B, by, bx, nz) uint64): the integer sums of the sub-clouds in
+ * units of q (the model is count x q x cd2i). max_clouds bounds the clouds of one vector inside the box; if it
+ * is exceeded *overflow (device int) is set and the model is incomplete. work: bb_mc_work_size bytes. */
+int64_t bb_mc_work_size(const bb_model* m, bb_box box, int nsub, int B, int64_t max_clouds);
+int bb_galmod_mc(const bb_model* m, bb_box box, const double* sr, int nsub, int B, int nvpar, uint32_t seed,
+                 int64_t max_clouds, void* work, int64_t work_bytes, float* out, uint64_t* counts, int* overflow,
+                 int accumulate, void* stream);
+
 /* Unit-density ring maps (DiskModel.ring_maps / _ring_maps_analytic, bbarolo/galmod.py:1317-1345, 468-536) in
  * two steps. bb_ring_maps_deposit writes the planes map0 .. map0+nw-1 of counts (B, nmap, by, bx) uint64: the
  * total-intensity maps, summed over the channels, of the ANALYTIC samples of the sub-ring tables sr (B, 16, nsub;
