@@ -87,7 +87,7 @@ def float32_rings(rings):
 class GpuLikelihood:
     """Batched log-likelihood of a cube for the free parameters `free` around the rings `rings0`.
 
-    dm: engine.DiskModel of the cube (ANALYTIC). data (nz, ny, nx) float32. likemask (nz, ny, nx) bool
+    dm: engine.DiskModel of the cube (ANALYTIC or MC). data (nz, ny, nx) float32. likemask (nz, ny, nx) bool
     (maskmode 2: only those voxels; None: all). region (ny, nx) bool (None: all). likebox ((blo), (bhi)):
     the likelihood region's box. sigma: noise per voxel. likefactor: 1 / correlation volume.
     fracerr: fixed fractional model error (None: 0)."""
@@ -209,8 +209,18 @@ class GpuLikelihood:
         dens = None
         if self.update_prof:
             dens = self.profile.run(sr, d_sr, vals if vals is not None else {})
-        mod = self.dm.array_batch(sr, self.blo, self.bhi, convolve=True, out=bufs["mod"], work=bufs["dep"],
-                                  d_sr=d_sr)
+        if self.dm.setup.galmodtype == "MC":
+            tab = sr
+            if dens is not None:
+                # the clouds of the MC model follow the densities: bound them with those of the profile
+                v2 = dict(vals or {})
+                v2["dens"] = dens.cpu().numpy().astype(np.float64)
+                tab = self.plan.tables(v2)[0]
+            mod = self.dm.array_batch_mc(sr, self.blo, self.bhi, convolve=True, out=bufs["mod"], work=bufs["dep"],
+                                         d_sr=d_sr, max_clouds=int(self.dm.mc_clouds(tab).max()))
+        else:
+            mod = self.dm.array_batch(sr, self.blo, self.bhi, convolve=True, out=bufs["mod"], work=bufs["dep"],
+                                      d_sr=d_sr)
         st = _stream_ptr(torch)
         s2 = self.sigma ** 2
         _lib.check(_lib.lib().bb_residual_gauss(mod.data_ptr(), self.dvalid.data_ptr(), B, self.nvox_box, s2,

@@ -217,6 +217,23 @@ def main():
     stage /= a.steps
     ms = dt * 1e3 / a.steps
     value = B * a.steps / dt
+    # the Monte-Carlo model (GALMODTYPE MC, counter-based stream) of the same vectors and densities, timed alone
+    dens = prof.set_dens(prof.solve(prof.maps(sr, d_sr, sr_ext, d_sr_ext=d_sr_ext)), d_sr).cpu().numpy()
+    v2 = dict(vals)
+    v2["dens"] = dens.astype(np.float64)
+    sr_mc = like.plan.tables(v2)[0]
+    dm_mc = DiskModel(wl["h"], wl["par"], galmodtype="MC")
+    cap = int(dm_mc.mc_clouds(sr_mc).max())
+    dm_mc.array_batch_mc(sr_mc, blo, bhi, convolve=False, work=bufs["dep"], d_sr=d_sr, max_clouds=cap)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    nmc = 3
+    for _ in range(nmc):
+        dm_mc.array_batch_mc(sr_mc, blo, bhi, convolve=False, work=bufs["dep"], d_sr=d_sr, max_clouds=cap)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    mc_ms = e0.elapsed_time(e1) / nmc
     # the same vectors through the host path (GpuLikelihood.loglike), for the record
     ll = like.loglike(theta[:4])
     nmap = prof.n * prof.by * prof.bx
@@ -235,6 +252,7 @@ def main():
                                "every vector", "batch": B,
                    "model_box": [int(blo[0]), int(blo[1]), int(bhi[0]), int(bhi[1])]},
         "stages_ms_per_step": dict(zip(names, [round(float(x), 4) for x in stage])),
+        "galmod_mc_ms_per_step": round(float(mc_ms), 4),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": ach / PEAK_HBM, "traffic": None},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},

@@ -82,3 +82,45 @@ def test_mc_batch_and_crop(small_header):
         assert np.array_equal(one[0], cfull[b])
     cbox, _ = _gpu(dm, sr, (5, 8), (60, 58), 7)
     assert np.array_equal(cbox, cfull[:, 8:58, 5:60])
+
+
+def test_likelihood_mc_with_profile():
+    """GpuLikelihood with the MC model, INC and PA free and DENS not fitted: the log-likelihood of every vector
+    agrees with the oracle pipeline (ANALYTIC ring maps -> profile -> float32 densities -> MC model -> beam ->
+    residual)."""
+    import math
+    from oracle import profile as op
+    from bbarolo2_amd.galmod import nzq_for, PHI_TAB
+    from bbarolo2_amd.likelihood import GpuLikelihood, FreePars, float32_rings
+    h = emptyheader([64, 64, 40], [-10. / 3600., 10. / 3600., 10.], beam=30. / 3600., bunit="JY/BEAM", obj="MOCK")
+    dm = _dm(h, par=GalmodPar(GALMODTYPE="MC", ISEED=11))
+    st, kern, sf = dm.setup, dm.cfield, dm.scalefactor
+    r = small_rings(8, dens=2. * np.exp(-(np.arange(8) * 20. + 10.) / 50.), z0=5.)
+    sr, _ = st.subrings(model_rings(r, "CENTRED", st.radsep))
+    m = oracle.conv_same(oracle.galmod_analytic(st, sr, (0, 0), (64, 64), nzq_for(sr), PHI_TAB), kern, sf)
+    sig = m.max() / 100.
+    data = (m + np.random.default_rng(3).normal(0., sig, m.shape)).transpose(2, 0, 1).astype(np.float32)
+    free = FreePars([("vrot", "single"), ("inc", "single"), ("phi", "single")], r.nr)
+    like = GpuLikelihood(dm, data, r, free, sigma=sig, likefactor=1., densprof_smooth=0.3)
+    theta = np.array([[150., 60., 45.], [140., 55., 50.]])
+    ll = like.loglike(theta)
+    nz, ny, nx = data.shape
+    ymap = data.astype(np.float64).sum(axis=0)
+    smap = np.full((ny, nx), sig * math.sqrt(nz))
+    blo, bhi = like.blo, like.bhi
+    for i, th in enumerate(theta):
+        rr = float32_rings(r)
+        for k, v in free.values(th[None]).items():
+            setattr(rr, k, v[0])
+        d = op.deconvolved(st, kern, sf, rr, ((0, 0), (nx, ny)), ymap, smap, np.ones((ny, nx), bool), smooth=0.3,
+                           beam_arcsec=30.)
+        rr.dens = d.astype(np.float32).astype(np.float64)
+        sr, nv = st.subrings(model_rings(rr, "CENTRED", st.radsep))
+        _, mm, _ = oracle.galmod_mc(st, sr, nv, blo, bhi, 11)
+        mm = oracle.conv_same(mm, kern, sf)
+        dbox = data[:, blo[1]:bhi[1], blo[0]:bhi[0]]
+        S = oracle.sumsq_residuals(np.ascontiguousarray(mm.transpose(2, 0, 1)), dbox, np.ones(dbox.shape, bool), 2,
+                                   np.ones(dbox.shape[1:], bool)) + like.ss_out
+        ref = -(0.5 * S / sig ** 2 + 0.5 * like.nvox * math.log(2 * math.pi * sig ** 2))
+        print(f"MC loglike {i}: gpu {ll[i]:.6f} oracle {ref:.6f}", flush=True)
+        assert abs(ll[i] - ref) <= 1e-5 * 0.5 * S / sig ** 2
