@@ -81,6 +81,11 @@ def lib():
         L.orc_mc_layer_dev.argtypes = [C.c_int, C.c_uint32, C.c_uint32]
         L.orc_mc_layer_dev.restype = C.c_float
         L.orc_mc_box_muller.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_norm.argtypes = [P, C.c_int, C.c_int, C.c_int, P, P, C.c_int, C.c_int, C.c_int64, C.c_int64,
+                               C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                               C.c_int, C.c_int, P, C.c_double, C.c_double, C.c_int, P, P, P, C.c_int, P, C.c_int,
+                               C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.orc_norm.restype = C.c_double
         _lib = L
     return _lib
 
@@ -233,3 +238,29 @@ def galmod_mc(setup, sr, nv, blo, bhi, seed):
     q = lib().orc_galmod_mc(_ptr(counts), _ptr(out), _ptr(sr), sr.shape[1], _ptr(nv), C.byref(g), _ptr(ri),
                             _ptr(cd), _ptr(lf))
     return counts, out, q
+
+
+# ---- the 3DFIT ring residual (norm.c) ------------------------------------------------------------------------
+def norm3dfit(model_yxz, data_yxz, mask_yxz, blo, x0, y0, phi, inc, r1, r2, wpow, side, ftype, chan_noise,
+              data_noise, bweight, normtype, pre=None, ilo=None, ihi=None, thr=None, foot=0, hmax=0.):
+    """orc_norm: (value, model scaled in place (a copy), nblanks, ntot) of _norm (galfit.py:39-188)."""
+    m = np.array(model_yxz, np.float64, order="C", copy=True)
+    by, bx, nz = m.shape
+    d = np.ascontiguousarray(data_yxz, np.float32)
+    mk = np.ascontiguousarray(mask_yxz, np.uint8)
+    ny, nx = d.shape[0], d.shape[1]
+    cn = np.ascontiguousarray(chan_noise, np.float64)
+    if pre is not None:
+        pre = np.ascontiguousarray(pre, np.float64)
+        ilo = np.ascontiguousarray(ilo, np.int64)
+        ihi = np.ascontiguousarray(ihi, np.int64)
+        thr = np.ascontiguousarray(thr, np.float64)
+        ky = len(ilo)
+    else:
+        ky = 1
+    nb, nt = C.c_int64(), C.c_int64()
+    v = lib().orc_norm(_ptr(m), by, bx, nz, _ptr(d), _ptr(mk), ny, nx, int(blo[0]), int(blo[1]), float(x0), float(y0),
+                       float(phi), float(inc), float(r1), float(r2), float(wpow), int(side), int(ftype), _ptr(cn),
+                       float(data_noise), float(bweight), int(normtype), _ptr(pre), _ptr(ilo), _ptr(ihi), ky,
+                       _ptr(thr), int(foot), float(hmax), C.byref(nb), C.byref(nt))
+    return v, m, nb.value, nt.value
