@@ -35,6 +35,11 @@ class bb_profile_desc(C.Structure):
                 ("dmed", C.c_double)]
 
 
+class bb_norm_par(C.Structure):
+    _fields_ = [("side", C.c_int32), ("ftype", C.c_int32), ("normtype", C.c_int32), ("foot", C.c_int32),
+                ("wpow", C.c_double), ("data_noise", C.c_double), ("bweight", C.c_double), ("hmax", C.c_double)]
+
+
 _P = C.c_void_p
 _SIGS = {
     "bb_last_error": (C.c_char_p, []),
@@ -55,6 +60,9 @@ _SIGS = {
     "bb_mc_work_size": (C.c_int64, [_P, bb_box, C.c_int, C.c_int, C.c_int64]),
     "bb_galmod_mc": (C.c_int, [_P, bb_box, _P, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int64, _P, C.c_int64, _P,
                                _P, _P, C.c_int, _P]),
+    "bb_norm_work_size": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "bb_norm3dfit": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P,
+                               _P, C.POINTER(bb_norm_par), _P, _P, _P, C.c_int, _P, _P, C.c_int64, _P, _P, _P, _P]),
     "bb_ring_maps_work_size": (C.c_int64, [_P, C.c_int]),
     "bb_ring_maps_deposit": (C.c_int, [_P, _P, bb_box, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, C.c_int,
                                        C.c_double, _P, C.c_int64, _P, _P]),

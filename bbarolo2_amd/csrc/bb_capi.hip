@@ -29,6 +29,13 @@ extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* s
                                    int nmap, int map0, int nw, double qunit, unsigned long long* counts,
                                    hipStream_t st);
 extern "C" size_t bb_rm_tile_lds(int nw, int TX, int TY);
+extern "C" int bb_norm_blocks(int by, int bx);
+extern "C" int bb_launch_norm(int B, int by, int bx, int nz, int ny, int nx, int64_t blo0, int64_t blo1, double wpow,
+                              int side, int ftype, double data_noise, double bweight, int normtype, int foot,
+                              double hmax, const float* model, const float* data, const uint8_t* mask,
+                              const double* ring, const double* chan_noise, const float* pre, const int* ilo,
+                              const int* ihi, int ky, const double* thr, double* work, double* val, int64_t* nb,
+                              int64_t* nt, hipStream_t st);
 extern "C" size_t bb_mc_sub_bytes();
 extern "C" size_t bb_mc_rec_bytes();
 extern "C" size_t bb_mc_lds_bytes(int nz, int TX, int TY);
@@ -584,4 +591,40 @@ extern "C" int bb_galmod_mc(const bb_model* m, bb_box box, const double* sr, int
                              tcount, toff, tcursor, recs, overflow, m->d_cd2i, out, (unsigned long long*)counts,
                              accumulate, st);
     return rc ? fail(BB_ERR_HIP, "bb_galmod_mc launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// The 3DFIT ring residual.
+extern "C" int64_t bb_norm_work_size(int B, int by, int bx, int nz, int with_support) {
+    if (B < 0 || by < 1 || bx < 1 || nz < 1) return -1;
+    const int64_t nblk = bb_norm_blocks(by, bx);
+    int64_t n = (int64_t)B * nblk * 3 + B;
+    if (with_support) n += (int64_t)B * by * (bx + 1) * nz;
+    return n;
+}
+
+extern "C" int bb_norm3dfit(const float* model, int B, int by, int bx, int nz, const float* data, const uint8_t* mask,
+                            int ny, int nx, int blo0, int blo1, const double* ring, const double* chan_noise,
+                            const bb_norm_par* par, const float* pre, const int32_t* ilo, const int32_t* ihi, int ky,
+                            const double* thr, double* work, int64_t work_len, double* value, int64_t* nblank,
+                            int64_t* ntot, void* stream) {
+    if (!par) return fail(BB_ERR_ARG, "bb_norm3dfit: par is NULL");
+    if (B < 0 || B > 65535 || by < 1 || bx < 1 || nz < 1 || ny < 1 || nx < 1)
+        return fail(BB_ERR_ARG, "bb_norm3dfit: bad sizes");
+    if (blo0 < 0 || blo1 < 0 || blo0 + bx > nx || blo1 + by > ny)
+        return fail(BB_ERR_ARG, "bb_norm3dfit: the box [%d,%d)x[%d,%d) is outside the cube", blo0, blo0 + bx, blo1,
+                    blo1 + by);
+    if (par->ftype < 1 || par->ftype > 4) return fail(BB_ERR_ARG, "bb_norm3dfit: ftype = %d (1..4)", par->ftype);
+    if (par->normtype < 0 || par->normtype > 2) return fail(BB_ERR_ARG, "bb_norm3dfit: normtype = %d", par->normtype);
+    if (pre && (ky < 1 || ky > 4096 || !ilo || !ihi || !thr)) return fail(BB_ERR_ARG, "bb_norm3dfit: bad support");
+    if (B == 0) return 0;
+    if (!model || !data || !mask || !ring || !chan_noise || !work || !value)
+        return fail(BB_ERR_ARG, "bb_norm3dfit: NULL array");
+    if (work_len < bb_norm_work_size(B, by, bx, nz, pre != nullptr))
+        return fail(BB_ERR_ARG, "bb_norm3dfit: work holds %lld doubles, needs %lld", (long long)work_len,
+                    (long long)bb_norm_work_size(B, by, bx, nz, pre != nullptr));
+    int rc = bb_launch_norm(B, by, bx, nz, ny, nx, blo0, blo1, par->wpow, par->side, par->ftype, par->data_noise,
+                            par->bweight, par->normtype, par->foot, par->hmax, model, data, mask, ring, chan_noise, pre,
+                            (const int*)ilo, (const int*)ihi, ky, thr, work, value, nblank, ntot, (hipStream_t)stream);
+    return rc ? fail(BB_ERR_HIP, "bb_norm3dfit launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }

@@ -107,6 +107,31 @@ int bb_galmod_mc(const bb_model* m, bb_box box, const double* sr, int nsub, int 
                  int64_t max_clouds, void* work, int64_t work_bytes, float* out, uint64_t* counts, int* overflow,
                  int accumulate, void* stream);
 
+/* The 3DFIT residual of ring models (_norm + Galfit._norm_value / _support_args, bbarolo/galfit.py:39-188,
+ * 685-733): B trial models (B, by, bx, nz) float32 yxz (convolved) in the box [blo, blo + (bx, by)) of the data
+ * (ny, nx, nz) float32 yxz and mask (ny, nx, nz) uint8 (the transposed cube and mask of Galfit, data_T/mask_T);
+ * ring (B, 6) float64: x0, y0 (box-relative pixels), phi, inc (degrees), r1, r2 (pixels) of each trial;
+ * chan_noise (nz) float64. Optional support of ANALYTIC models: pre (B, by, bx, nz) float32 (unconvolved model in
+ * units of sub-clouds), ilo/ihi (ky) int32 footprint columns of the kernel rows, thr (nz) float64. value (B)
+ * float64 (+inf without voxels), nblank / ntot (B) int64 (may be NULL). All arrays on the device; the model is not
+ * scaled in place (the reference's in-place normalisation is internal to one evaluation).
+ * work: bb_norm_work_size(...) float64 elements. */
+typedef struct bb_norm_par {
+    int32_t side;        /* 0 both, 1 receding, 2 approaching (SIDE) */
+    int32_t ftype;       /* 1 chi^2/sqrt(obs), 2 |m-o|, 3 |m-o|/(m+o), 4 (m-o)^2 (FTYPE) */
+    int32_t normtype;    /* 0 LOCAL, 1 AZIM, 2 NONE (NORM) */
+    int32_t foot;        /* 0 annulus, 1 strip along the major axis (|y| <= hmax) */
+    double wpow;         /* weight |cos th|^wpow, |sin th|^|wpow| if negative (WFUNC) */
+    double data_noise;   /* obs where the data are <= 0 */
+    double bweight;      /* exponent of the blank penalty (BWEIGHT) */
+    double hmax;
+} bb_norm_par;
+int64_t bb_norm_work_size(int B, int by, int bx, int nz, int with_support);
+int bb_norm3dfit(const float* model, int B, int by, int bx, int nz, const float* data, const uint8_t* mask, int ny,
+                 int nx, int blo0, int blo1, const double* ring, const double* chan_noise, const bb_norm_par* par,
+                 const float* pre, const int32_t* ilo, const int32_t* ihi, int ky, const double* thr, double* work,
+                 int64_t work_len, double* value, int64_t* nblank, int64_t* ntot, void* stream);
+
 /* Unit-density ring maps (DiskModel.ring_maps / _ring_maps_analytic, bbarolo/galmod.py:1317-1345, 468-536) in
  * two steps. bb_ring_maps_deposit writes the planes map0 .. map0+nw-1 of counts (B, nmap, by, bx) uint64: the
  * total-intensity maps, summed over the channels, of the ANALYTIC samples of the sub-ring tables sr (B, 16, nsub;
