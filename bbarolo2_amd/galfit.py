@@ -1,0 +1,440 @@
+"""3DFIT CLASSIC ring fits with the models and residuals on the GPU (bbarolo/galfit.py).
+
+The host driver mirrors the reference's first-stage fit of each ring: the annulus of the ring (_make_dring,
+galfit.py:1006-1029), the clamping and redraws of the trial values (func3D, galfit.py:832-889), the downhill simplex
+(minimize and _mtry, galfit.py:891-1003), the pixel count that skips empty rings (_count_ring_pixels,
+galfit.py:199-216), and the generator of each ring fit seeded [12345, stage, ring] (galfit.py:1097-1100). Every
+evaluation is a device model of the ring in its own box (ANALYTIC or MC; bb_galmod_analytic / bb_galmod_mc, then the
+beam, as DiskModel.array with final=False, galmod.py:1242-1271) and its residual (bb_norm3dfit, with the support of
+ANALYTIC models, galfit.py:685-733). The points of a simplex that are independent (its first vertices, the
+shrink) are evaluated as one batch.
+"""
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .engine import _torch, _stream_ptr
+from .header import arcsconv
+from .rings import Rings
+
+VROT, VDISP, DENS, Z0, INC, PA, XPOS, YPOS, VSYS, VRAD, MAXPAR = range(11)
+NORMCODE = {"LOCAL": 0, "AZIM": 1, "NONE": 2}
+SIDECODE = {"R": 1, "A": 2}
+# expected number of MC sub-clouds below which a voxel of an ANALYTIC model is not a blank (galfit.py:191-194)
+MC_SUPPORT_NMIN = math.exp(-0.5772156649015329)
+KEYS = {VROT: "vrot", VDISP: "vdisp", DENS: "dens", Z0: "z0", INC: "inc", PA: "phi", XPOS: "xpos", YPOS: "ypos",
+        VSYS: "vsys", VRAD: "vrad"}
+
+
+@dataclass
+class Fit3DPar:
+    """The 3DFIT parameters of the ring fits (the defaults of bbarolo/params.py:170-200)."""
+    FREE: str = "VROT VDISP"
+    DELTAVROT: float = 1000.0
+    DELTAINC: float = 5.0
+    DELTAPHI: float = 15.0
+    MINVDISP: float = 0.1
+    MAXVDISP: float = 1000.0
+    FTYPE: int = 2
+    WFUNC: int = 2
+    TOL: float = 1e-3
+    BWEIGHT: float = 1.0
+    NORM: str = "LOCAL"
+    SIDE: str = "B"
+
+
+def free_flags(free):
+    """Galfit.setFree (galfit.py:658-674)."""
+    f = free.lower()
+    m = [False] * MAXPAR
+    for key, idx in (("vrot", VROT), ("disp", VDISP), ("z0", Z0), ("inc", INC), ("xpos", XPOS), ("ypos", YPOS),
+                     ("vsys", VSYS), ("vrad", VRAD)):
+        if key in f:
+            m[idx] = True
+    if "pa" in f or "phi" in f:
+        m[PA] = True
+    if "all" in f:
+        m = [True] * MAXPAR
+    if not any(m):
+        raise ValueError(" 3DFIT ERROR: NO free parameters!")
+    return m
+
+
+def count_ring_pixels(mask2d, blo, bhi, x0, y0, phi, inc, r1, r2):
+    """(pixels of the annulus in the box, those with some mask) (galfit.py:199-216; the box edges included)."""
+    F = math.pi / 180.
+    sphi, cphi, cinc = math.sin(F * phi), math.cos(F * phi), math.cos(F * inc)
+    ys, xs = np.mgrid[blo[1]:bhi[1] + 1, blo[0]:bhi[0] + 1]
+    xx, yy = xs - blo[0], ys - blo[1]
+    xr = -(xx - x0) * sphi + (yy - y0) * cphi
+    yr = (-(xx - x0) * cphi - (yy - y0) * sphi) / cinc
+    r = np.sqrt(xr * xr + yr * yr)
+    ring = (r >= r1) & (r <= r2)
+    inm = (ys < mask2d.shape[0]) & (xs < mask2d.shape[1])
+    m = np.zeros(ring.shape, bool)
+    m[inm] = mask2d[ys[inm], xs[inm]] > 0
+    return int(ring.sum()), int((ring & m).sum())
+
+
+class RingSimplex:
+    """The host side of the first-stage 3DFIT of the rings: bounds, trial clamps, the simplex and the ring loop.
+    The residuals come from values(rings) (a list of single-ring Rings -> array of residuals), which RingFit
+    evaluates on the device.
+
+    inr: the initial Rings; par: Fit3DPar; nx, ny: the cube's spatial size; mask2d (ny, nx): the number of masked
+    channels of each pixel; pixscale: arcsec per pixel; box: rings -> (blo, bhi), the box of a ring's model;
+    vsys_range: (min, max) of VSYS (the velocity range of the cube)."""
+
+    def __init__(self, inr, par, nx, ny, mask2d, pixscale, box, vsys_range=(-1e9, 1e9)):
+        self.par = par or Fit3DPar()
+        self.inr = inr
+        self.nx, self.ny = nx, ny
+        self.mask2d = np.asarray(mask2d)
+        self.pixscale = float(pixscale)
+        self.box = box
+        self.mpar = free_flags(self.par.FREE)
+        self.nfree = sum(self.mpar)
+        nm = (self.par.NORM.split() or ["LOCAL"])[0].upper()
+        nm = "AZIM" if nm == "AZIMUTHAL" else nm
+        self.norm = NORMCODE.get(nm, NORMCODE["LOCAL"])
+        p = self.par
+        self.maxs = np.zeros(MAXPAR)
+        self.mins = np.zeros(MAXPAR)
+        self.maxs[VROT] = inr.vrot.max() + p.DELTAVROT
+        self.mins[VROT] = max(inr.vrot.min() - p.DELTAVROT, 0)
+        self.maxs[VDISP], self.mins[VDISP] = p.MAXVDISP, p.MINVDISP
+        self.maxs[Z0], self.mins[Z0] = 1000, 0.
+        self.maxs[INC] = min(inr.inc.max() + p.DELTAINC, 90)
+        self.mins[INC] = max(inr.inc.min() - p.DELTAINC, 0)
+        self.maxs[PA] = min(inr.phi.max() + p.DELTAPHI, 360)
+        self.mins[PA] = max(inr.phi.min() - p.DELTAPHI, -360)
+        self.maxs[XPOS] = min(inr.xpos.max() + 10, nx)
+        self.mins[XPOS] = max(inr.xpos.min() - 10, 0)
+        self.maxs[YPOS] = min(inr.ypos.max() + 10, ny)
+        self.mins[YPOS] = max(inr.ypos.min() - 10, 0)
+        self.mins[VSYS], self.maxs[VSYS] = sorted(vsys_range)
+        self.maxs[VRAD], self.mins[VRAD] = 500, -500
+        self.rng = np.random.default_rng([12345, 0])
+        self.nevals = 0
+
+    def values(self, rings):
+        raise NotImplementedError
+
+    # ------------------------------------------------------------------ rings
+    def make_dring(self, ir, rings=None):
+        """Galfit._make_dring (galfit.py:1006-1029): the annulus halfway to the neighbouring radii."""
+        inr = self.inr
+        vr = inr if rings is None else rings
+        R = np.asarray(inr.radii, float)
+        if inr.nr == 1:
+            w1 = w2 = inr.radsep / 2. if inr.radsep > 0 else R[0] / 2.
+        elif ir == 0:
+            w1 = w2 = (R[1] - R[0]) / 2.
+        elif ir == inr.nr - 1:
+            w1 = w2 = (R[ir] - R[ir - 1]) / 2.
+        else:
+            w1, w2 = (R[ir] - R[ir - 1]) / 2., (R[ir + 1] - R[ir]) / 2.
+        w1, w2 = np.float32(w1), np.float32(w2)
+        drads = [max(float(np.float32(R[ir] - w1)), 0.), max(float(np.float32(R[ir] + w2)), 0.)]
+        vals = {k: getattr(vr, k)[ir] for k in ("xpos", "ypos", "vsys", "vrot", "vdisp", "vrad", "vvert", "dvdz", "zcyl",
+                                                "dens", "z0", "inc", "phi", "z0thick", "fthick", "vdispthick")}
+        d = Rings().setRings(radii=drads, **vals)
+        d.twolayer = vr.thick()
+        d.radsep = drads[1] - drads[0]
+        d.id = ir
+        return d
+
+    # ------------------------------------------------------------------ trials
+    def clamp(self, dring, zpar):
+        """func3D's bounds (galfit.py:832-889): the values of a trial point, clamped or redrawn, written back
+        into zpar; returns the dict of ring values."""
+        p, inr, mins, maxs, m = self.par, self.inr, self.mins, self.maxs, self.mpar
+        w = dring.id
+        fran = self.rng.random
+        vals = {k: float(getattr(dring, k)[0]) for k in KEYS.values()}
+        n = 0
+        if m[VROT]:
+            minv, maxv = max(inr.vrot[w] - p.DELTAVROT, mins[VROT]), min(inr.vrot[w] + p.DELTAVROT, maxs[VROT])
+            z = zpar[n]
+            v = maxv - fran() * p.DELTAVROT if z > maxv else (minv + fran() * p.DELTAVROT if z < minv else z)
+            vals["vrot"] = zpar[n] = v
+            n += 1
+        if m[VDISP]:
+            z = zpar[n]
+            span = min(inr.vdisp[w], maxs[VDISP] - mins[VDISP])
+            v = maxs[VDISP] - fran() * span if z > maxs[VDISP] else (mins[VDISP] + fran() * span
+                                                                        if z < mins[VDISP] else z)
+            vals["vdisp"] = zpar[n] = v
+            n += 1
+        if m[DENS]:
+            z = zpar[n]
+            vals["dens"] = zpar[n] = inr.dens[w] if z < 0 else z
+            n += 1
+        if m[Z0]:
+            z = zpar[n]
+            vals["z0"] = zpar[n] = inr.z0[w] if (z < mins[Z0] or z > maxs[Z0]) else z
+            n += 1
+        if m[INC]:
+            lo, hi = max(inr.inc[w] - p.DELTAINC, mins[INC]), min(inr.inc[w] + p.DELTAINC, maxs[INC])
+            z = zpar[n]
+            vals["inc"] = zpar[n] = hi if z > hi else (lo if z < lo else z)
+            n += 1
+        if m[PA]:
+            lo, hi = max(inr.phi[w] - p.DELTAPHI, mins[PA]), min(inr.phi[w] + p.DELTAPHI, maxs[PA])
+            z = zpar[n]
+            vals["phi"] = zpar[n] = hi if z > hi else (lo if z < lo else z)
+            n += 1
+        for idx, key in ((XPOS, "xpos"), (YPOS, "ypos"), (VSYS, "vsys"), (VRAD, "vrad")):
+            if m[idx]:
+                z = zpar[n]
+                vals[key] = zpar[n] = getattr(inr, key)[w] if (z < mins[idx] or z > maxs[idx]) else z
+                n += 1
+        return vals
+
+    def func3D_batch(self, dring, points):
+        """func3D of several points of one ring (the clamps in order, then one batch of device evaluations)."""
+        rings = []
+        for zpar in points:
+            vals = self.clamp(dring, zpar)
+            r = dring.copy()
+            r.id = dring.id
+            for k, v in vals.items():
+                getattr(r, k)[:] = v
+            rings.append(r)
+        return self.values(rings)
+
+    def func3D(self, dring, zpar):
+        return float(self.func3D_batch(dring, [zpar])[0])
+
+    # ------------------------------------------------------------------ minimisation
+    def minimize(self, dring):
+        """Downhill simplex of Galfit.minimize / _mtry (galfit.py:891-1003): (ok, minimum, pmin)."""
+        TINY = 1.0e-10
+        tol = self.par.TOL
+        ndim = self.nfree
+        m = self.mpar
+        NMAX = 200 * ndim
+        mpts = ndim + 1
+        point, dels = [], []
+        if m[VROT]:
+            dels.append(30. if dring.vrot[0] > 40 else 5)
+            point.append(dring.vrot[0])
+        if m[VDISP]:
+            v = dring.vdisp[0]
+            dels.append(20. if v > 80 else (10. if 20. < v <= 80. else (0.2 if v <= 2 else 5)))
+            point.append(v)
+        if m[DENS]:
+            dels.append(0.2 * dring.dens[0])
+            point.append(dring.dens[0])
+        if m[Z0]:
+            dels.append(0.2 * dring.z0[0])
+            point.append(dring.z0[0])
+        if m[INC]:
+            dels.append(5.)
+            point.append(dring.inc[0])
+        if m[PA]:
+            dels.append(10.)
+            point.append(dring.phi[0])
+        for idx, key in ((XPOS, "xpos"), (YPOS, "ypos"), (VSYS, "vsys")):
+            if m[idx]:
+                dels.append((self.maxs[idx] - self.mins[idx]) / 4.)
+                point.append(getattr(dring, key)[0])
+        if m[VRAD]:
+            dels.append(25)
+            point.append(dring.vrad[0])
+        p = np.tile(np.array(point, float), (mpts, 1))
+        for i in range(1, mpts):
+            p[i, i - 1] += dels[i - 1]
+        xs = [p[i].copy() for i in range(mpts)]
+        y = self.func3D_batch(dring, xs)
+        nfunc = 0
+        psum = p.sum(axis=0)
+        while True:
+            ilo = 0
+            ihi, inhi = (0, 1) if y[0] > y[1] else (1, 0)
+            for i in range(mpts):
+                if y[i] <= y[ilo]:
+                    ilo = i
+                if y[i] > y[ihi]:
+                    inhi = ihi
+                    ihi = i
+                elif y[i] > y[inhi] and i != ihi:
+                    inhi = i
+            rtol = 2.0 * abs(y[ihi] - y[ilo]) / (abs(y[ihi]) + abs(y[ilo]) + TINY)
+            if rtol < tol:
+                y[0], y[ilo] = y[ilo], y[0]
+                p[[0, ilo]] = p[[ilo, 0]]
+                return True, float(y[0]), p[0].copy()
+            if nfunc >= NMAX:
+                return False, float(y[ilo]), p[ilo].copy()
+            nfunc += 2
+            ytry = self._mtry(dring, p, y, psum, ihi, -1.0)
+            if ytry <= y[ilo]:
+                ytry = self._mtry(dring, p, y, psum, ihi, 2.0)
+            elif ytry >= y[inhi]:
+                ysave = y[ihi]
+                ytry = self._mtry(dring, p, y, psum, ihi, 0.5)
+                if ytry >= ysave:
+                    idx = [i for i in range(mpts) if i != ilo]
+                    for i in idx:
+                        p[i] = 0.5 * (p[i] + p[ilo])
+                    # (clamped copies are evaluated; the simplex keeps the unclamped points, galfit.py:978-981)
+                    y[idx] = self.func3D_batch(dring, [p[i].copy() for i in idx])
+                    nfunc += ndim
+                    psum = p.sum(axis=0)
+            else:
+                nfunc -= 1
+
+    def _mtry(self, dring, p, y, psum, ihi, fac):
+        ndim = self.nfree
+        fac1 = (1.0 - fac) / ndim
+        fac2 = fac1 - fac
+        ptry = psum * fac1 - p[ihi] * fac2
+        ytry = self.func3D(dring, ptry)
+        if ytry < y[ihi]:
+            y[ihi] = ytry
+            psum += ptry - p[ihi]
+            p[ihi] = ptry
+        return ytry
+
+    # ------------------------------------------------------------------ the fit
+    def fit_ring(self, ir, stage=1):
+        """The first-stage fit of ring ir (Galfit._fit_ring / _fit_one_ring, galfit.py:1097-1142): None if the ring
+        has too few pixels in the mask, else (ok, minimum, values of the free parameters)."""
+        self.rng = np.random.default_rng([12345, stage, ir])
+        dring = self.make_dring(ir)
+        blo, bhi = self.box(dring)
+        ps = self.pixscale
+        nTot, nIn = count_ring_pixels(self.mask2d, blo, bhi, dring.xpos[-1] - blo[0], dring.ypos[-1] - blo[1],
+                                      dring.phi[-1], dring.inc[-1], dring.radii[0] / ps, dring.radii[-1] / ps)
+        if nIn < 20 and (nIn == 0 or nIn < 0.7 * nTot):
+            return None
+        return self.minimize(dring)
+
+    def fit(self, stage=1):
+        """Fit every ring; returns (Rings with the fitted values, list of the ring results)."""
+        out = self.inr.copy()
+        res = []
+        for ir in range(self.inr.nr):
+            r = self.fit_ring(ir, stage)
+            res.append(r)
+            if r is None or not r[0]:
+                continue
+            n = 0
+            for idx in range(MAXPAR):
+                if self.mpar[idx]:
+                    getattr(out, KEYS[idx])[ir] = r[2][n]
+                    n += 1
+        return out, res
+
+
+class RingFit(RingSimplex):
+    """The first-stage 3DFIT of the rings of a cube, the residuals evaluated on the current device.
+
+    dm: the DiskModel (DENSFLUX models, as 3DFIT sets); data (nz, ny, nx) float32; mask (nz, ny, nx) bool; inr: the
+    initial Rings; par: Fit3DPar; data_noise: the spread of the data (obs where the data are <= 0); chan_noise (nz)
+    (default ones); vsys_range: (min, max) of VSYS (the velocity range of the cube)."""
+
+    def __init__(self, dm, data, mask, inr, par=None, data_noise=1.0, chan_noise=None, vsys_range=(-1e9, 1e9)):
+        torch = _torch()
+        nz, ny, nx = data.shape
+        h = dm.header
+        ps = (abs(h.Cdelt(0)) * arcsconv(h.Cunit(0)) + abs(h.Cdelt(1)) * arcsconv(h.Cunit(1))) / 2.
+        mask = np.asarray(mask, bool)
+        super().__init__(inr, par, nx, ny, mask.sum(axis=0), ps, lambda r: dm.box(r, final=False), vsys_range)
+        self.torch = torch
+        self.dm = dm
+        self.nz = nz
+        self.dev = torch.device("cuda", torch.cuda.current_device())
+        self.d_data = torch.from_numpy(np.ascontiguousarray(np.asarray(data, np.float32).transpose(1, 2, 0))).to(self.dev)
+        self.d_mask = torch.from_numpy(np.ascontiguousarray(mask.transpose(1, 2, 0)).astype(np.uint8)).to(self.dev)
+        cn = np.ones(nz) if chan_noise is None else np.asarray(chan_noise, float)
+        self.d_cn = torch.from_numpy(np.ascontiguousarray(cn, np.float64)).to(self.dev)
+        self.data_noise = float(data_noise)
+
+    # ------------------------------------------------------------------ device evaluation
+    def _support(self, unit):
+        """ilo, ihi (kernel rows' footprint where one sub-cloud stays above the 1e-12 cut) and thr of
+        Galfit._support_args (galfit.py:698-716), or None without an ANALYTIC support."""
+        dm, st = self.dm, self.dm.setup
+        if st.galmodtype != "ANALYTIC" or not unit > 0:
+            return None
+        cd = np.asarray(st.cd2i, float)
+        if dm.cfield is not None:
+            K = dm.cfield
+            kmin = 1.E-12 / (unit * float(cd.max()) * float(dm.scalefactor))
+            fp = K > max(kmin, 0.)
+            ky, kx = K.shape
+            hx = kx // 2
+            anyr = fp.any(axis=1)
+            i0 = np.where(anyr, fp.argmax(axis=1), 0)
+            i1 = np.where(anyr, kx - 1 - fp[:, ::-1].argmax(axis=1), -1)
+            ilo = np.where(anyr, hx - i1, 1).astype(np.int32)
+            ihi = np.where(anyr, hx - i0, 0).astype(np.int32)
+        else:
+            ilo = np.zeros(1, np.int32)
+            ihi = np.zeros(1, np.int32)
+        return ilo, ihi, MC_SUPPORT_NMIN * unit * cd
+
+    def values(self, rings):
+        """The 3DFIT residual of each ring of the list (Galfit.getFuncValue without the models of the other rings,
+        galfit.py:736-745), evaluated in batches of rings that share their box."""
+        out = np.empty(len(rings))
+        groups = {}
+        for i, r in enumerate(rings):
+            blo, bhi = self.box(r)
+            groups.setdefault((tuple(blo), tuple(bhi)), []).append(i)
+        for (blo, bhi), idx in groups.items():
+            out[idx] = self._values_box([rings[i] for i in idx], blo, bhi)
+        self.nevals += len(rings)
+        return out
+
+    def _values_box(self, rings, blo, bhi):
+        torch = self.torch
+        dm, st = self.dm, self.dm.setup
+        L = _lib.lib()
+        B = len(rings)
+        tabs = [st.subrings(r) for r in rings]
+        sr = np.stack([t[0] for t in tabs])
+        by, bx, nz = bhi[1] - blo[1], bhi[0] - blo[0], st.nsubs
+        stream = _stream_ptr(torch)
+        dep = torch.empty((B, by, bx, nz), dtype=torch.float32, device=self.dev)
+        if st.galmodtype == "MC":
+            mod = dm.array_batch_mc(sr, blo, bhi, convolve=dm.cfield is not None, work=dep)
+            sup = None
+        else:
+            mod = dm.array_batch(sr, blo, bhi, convolve=dm.cfield is not None, work=dep)
+            units = [st.subcloud_flux(t[0], t[1]) if t[0].shape[1] else 0. for t in tabs]
+            sup = self._support(units[0]) if len(set(units)) == 1 else None
+            if sup is None and any(u > 0 for u in units):
+                # (support units differ: evaluate one by one)
+                if B > 1:
+                    return np.array([self._values_box([r], blo, bhi)[0] for r in rings])
+                sup = self._support(units[0])
+        ps = self.pixscale
+        ring = np.array([[r.xpos[-1] - blo[0], r.ypos[-1] - blo[1], r.phi[-1], r.inc[-1], r.radii[0] / ps,
+                          r.radii[-1] / ps] for r in rings], np.float64)
+        d_ring = torch.from_numpy(ring).to(self.dev)
+        par = _lib.bb_norm_par(SIDECODE.get(self.par.SIDE, 0), int(self.par.FTYPE), self.norm, 0, float(self.par.WFUNC),
+                               self.data_noise, float(self.par.BWEIGHT), 0.)
+        wl = int(L.bb_norm_work_size(B, by, bx, nz, int(sup is not None)))
+        work = torch.empty(wl, dtype=torch.float64, device=self.dev)
+        val = torch.empty(B, dtype=torch.float64, device=self.dev)
+        if sup is not None:
+            ilo, ihi, thr = sup
+            d_ilo = torch.from_numpy(ilo).to(self.dev)
+            d_ihi = torch.from_numpy(ihi).to(self.dev)
+            d_thr = torch.from_numpy(np.ascontiguousarray(thr, np.float64)).to(self.dev)
+        _lib.check(L.bb_norm3dfit(mod.data_ptr(), B, by, bx, nz, self.d_data.data_ptr(), self.d_mask.data_ptr(), self.ny,
+                                  self.nx, int(blo[0]), int(blo[1]), d_ring.data_ptr(), self.d_cn.data_ptr(),
+                                  C.byref(par), dep.data_ptr() if sup is not None else None,
+                                  d_ilo.data_ptr() if sup is not None else None,
+                                  d_ihi.data_ptr() if sup is not None else None,
+                                  len(ilo) if sup is not None else 0,
+                                  d_thr.data_ptr() if sup is not None else None, work.data_ptr(), wl, val.data_ptr(),
+                                  None, None, stream))
+        return val.cpu().numpy()

@@ -1,0 +1,153 @@
+"""Host side of the 3DFIT ring fits (bbarolo2_amd/galfit.py, CPU): FREE parsing, the pixel count of a ring against
+lattice counts, the annuli of the rings, the bounds of the trial values, and the simplex on closed-form objectives
+(its minimum, its failure at NMAX, and batched evaluations giving the same iterates as one-by-one ones)."""
+import math
+
+import numpy as np
+import pytest
+
+from bbarolo2_amd import galfit as G
+from bbarolo2_amd.rings import Rings
+
+
+def test_free_flags():
+    m = G.free_flags("VROT VDISP")
+    assert m[G.VROT] and m[G.VDISP] and sum(m) == 2
+    m = G.free_flags("vrot pa inc")
+    assert m[G.PA] and m[G.INC] and sum(m) == 3
+    assert G.free_flags("phi")[G.PA]
+    assert all(G.free_flags("ALL"))
+    with pytest.raises(ValueError):
+        G.free_flags("dens")                 # (DENS is never free in 3DFIT: no flag, hence no free parameter)
+
+
+def test_count_ring_pixels_face_on():
+    """Face-on annulus: the lattice points with r1 <= r <= r2 of the box edges included; masked pixels among them."""
+    mask = np.zeros((50, 50), int)
+    mask[20:, :] = 3
+    blo, bhi = (5, 6), (40, 44)
+    x0, y0 = 20.3, 18.7
+    nt, ni = G.count_ring_pixels(mask, blo, bhi, x0, y0, 0., 0., 6., 11.)
+    ys, xs = np.mgrid[blo[1]:bhi[1] + 1, blo[0]:bhi[0] + 1]
+    r = np.hypot(xs - blo[0] - x0, ys - blo[1] - y0)
+    ring = (r >= 6) & (r <= 11)
+    assert nt == ring.sum() and ni == (ring & (ys >= 20)).sum() and 0 < ni < nt
+
+
+def test_count_ring_pixels_inclined():
+    """A disk (r1 = 0) at inclination i covers pi R^2 cos i pixels; the count is invariant under a rotation by
+    PA = 180 (the ellipse is symmetric); pixels outside the mask array count in nTot only."""
+    mask = np.ones((10, 10), int)
+    nt, ni = G.count_ring_pixels(mask, (0, 0), (120, 120), 60.2, 60.1, 30., 60., 0., 40.)
+    assert abs(nt - math.pi * 1600 * 0.5) < 0.02 * math.pi * 1600 * 0.5
+    assert nt == G.count_ring_pixels(mask, (0, 0), (120, 120), 60.2, 60.1, 210., 60., 0., 40.)[0]
+    assert ni == 0
+
+
+def _inr(radii, **kw):
+    d = dict(radii=radii, xpos=30., ypos=30., vsys=0., vrot=100., vdisp=10., z0=0., dens=1., inc=60., phi=45.)
+    d.update(kw)
+    return Rings().setRings(**d)
+
+
+class Quad(G.RingSimplex):
+    """A closed-form objective: sum of ((value - target) / scale)^2 over the free parameters, + 1."""
+
+    def __init__(self, inr, par, target, one_by_one=False, **kw):
+        super().__init__(inr, par, 64, 64, np.ones((64, 64)), 10., lambda r: ((0, 0), (64, 64)), **kw)
+        self.target = target
+        self.one_by_one = one_by_one
+        self.calls = []
+
+    def values(self, rings):
+        self.calls.append(len(rings))
+        self.nevals += len(rings)
+        out = []
+        for r in rings:
+            out.append(1. + sum(((getattr(r, k)[0] - t) / s) ** 2 for k, (t, s) in self.target.items()))
+        return np.array(out)
+
+    def func3D_batch(self, dring, points):
+        if not self.one_by_one:
+            return super().func3D_batch(dring, points)
+        return np.concatenate([super(Quad, self).func3D_batch(dring, [p]) for p in points])
+
+
+def test_make_dring():
+    inr = _inr([10.1, 30.3, 50.6, 70.2])
+    q = Quad(inr, G.Fit3DPar(), {})
+    r0, r1, r3 = q.make_dring(0), q.make_dring(1), q.make_dring(3)
+    f, d = np.float32, np.float64                     # (radius in FP64 +- a half width rounded to FP32)
+    assert r0.radii[0] == 0. and r0.radii[1] == float(f(d(10.1) + f(10.1)))
+    assert r1.radii[0] == float(f(d(30.3) - f(10.1))) and r1.radii[1] == float(f(d(30.3) + f((50.6 - 30.3) / 2.)))
+    assert r3.radii[1] == float(f(d(70.2) + f((70.2 - 50.6) / 2.)))
+    assert r1.radii[1] != float(f(30.3 + f((50.6 - 30.3) / 2.)))    # (not the all-FP32 sum)
+    assert r1.id == 1 and r1.radsep == r1.radii[1] - r1.radii[0] and r1.vrot[0] == 100.
+    one = Quad(_inr([20.]), G.Fit3DPar(), {}).make_dring(0)
+    assert one.radii[0] == 10. and one.radii[1] == 30.
+
+
+def test_clamp_bounds():
+    """Out-of-range trial values: VROT and VDISP redrawn inside their bands, INC and PA clamped to theirs, Z0 and
+    the centre back to the initial values; the trial vector gets the values used."""
+    inr = _inr([10., 30., 50.], z0=5.)
+    par = G.Fit3DPar(FREE="VROT VDISP Z0 INC PA XPOS", DELTAVROT=40., DELTAINC=5., DELTAPHI=15.)
+    q = Quad(inr, par, {})
+    d = q.make_dring(1)
+    z = np.array([500., -3., 2000., 80., 10., 1000.])
+    vals = q.clamp(d, z)
+    assert 100. < vals["vrot"] <= 140.               # maxv - u DELTAVROT, maxv = 100 + 40, u in [0, 1)
+    assert par.MINVDISP <= vals["vdisp"] <= par.MINVDISP + 10.
+    assert vals["z0"] == 5. and vals["inc"] == 65. and vals["phi"] == 30. and vals["xpos"] == 30.
+    assert np.array_equal(z, [vals[k] for k in ("vrot", "vdisp", "z0", "inc", "phi", "xpos")])
+    z = np.array([-100., 5000., 3., 40., 70., 31.])
+    vals = q.clamp(d, z)
+    assert 60. <= vals["vrot"] < 100. and par.MAXVDISP - 10. <= vals["vdisp"] <= par.MAXVDISP
+    assert (vals["z0"], vals["inc"], vals["phi"], vals["xpos"]) == (3., 55., 60., 31.)
+
+
+def test_simplex_minimum():
+    inr = _inr([10., 30., 50.])
+    par = G.Fit3DPar(FREE="VROT VDISP INC", TOL=1e-6)
+    q = Quad(inr, par, {"vrot": (123.4, 10.), "vdisp": (17.2, 3.), "inc": (62.5, 2.)})
+    ok, fmin, p = q.minimize(q.make_dring(1))
+    assert ok and fmin < 1. + 1e-4
+    assert abs(p[0] - 123.4) < 0.2 and abs(p[1] - 17.2) < 0.06 and abs(p[2] - 62.5) < 0.04
+    # first batch: the ndim + 1 vertices together
+    assert q.calls[0] == 4
+
+
+def test_simplex_nmax():
+    """A tolerance no simplex reaches: the fit stops after at most 200 x ndim function-value steps, not ok."""
+    inr = _inr([10., 30., 50.])
+    q = Quad(inr, G.Fit3DPar(FREE="VROT VDISP", TOL=-1.), {"vrot": (123.4, 10.), "vdisp": (17.2, 3.)})
+    ok, fmin, _ = q.minimize(q.make_dring(1))
+    assert not ok and fmin >= 1. and 400 <= q.nevals - 3 <= 400 + 2 + 2
+
+
+@pytest.mark.parametrize("free", ["VROT VDISP", "VROT VDISP INC PA", "VROT"])
+def test_batched_equals_one_by_one(free):
+    """Batched evaluations (first vertices, shrinks) draw the same random numbers in the same order as
+    one-by-one ones: identical iterates, results and evaluation counts, also with redraws at the bounds."""
+    inr = _inr([10., 30., 50.], vdisp=3.)
+    par = G.Fit3DPar(FREE=free, TOL=1e-5, DELTAVROT=8., DELTAINC=2., DELTAPHI=3.)
+    tgt = {"vrot": (109., 4.), "vdisp": (0.05, 1.), "inc": (70., 1.), "phi": (30., 1.)}
+    res = []
+    for obo in (False, True):
+        q = Quad(inr, par, tgt, one_by_one=obo)
+        q.rng = np.random.default_rng([12345, 1, 1])
+        r = q.minimize(q.make_dring(1))
+        res.append((r, q.nevals, q.rng.random()))
+    (a, na, ua), (b, nb, ub) = res
+    assert a[0] == b[0] and a[1] == b[1] and np.array_equal(a[2], b[2]) and na == nb and ua == ub
+
+
+def test_fit_skips_rings_outside_mask():
+    inr = _inr([10., 30., 50.])
+    q = Quad(inr, G.Fit3DPar(FREE="VROT"), {"vrot": (120., 5.)})
+    q.mask2d = np.zeros((64, 64))
+    out, res = q.fit()
+    assert res == [None, None, None] and np.array_equal(out.vrot, inr.vrot)
+    q.mask2d = np.ones((64, 64))
+    out, res = q.fit()
+    assert all(r[0] for r in res) and np.allclose(out.vrot, 120., atol=0.5)
