@@ -53,6 +53,7 @@ _SIGS = {
     "bb_azplan_destroy": (C.c_int, [_P]),
     "bb_analytic_work_size": (C.c_int64, [_P, C.c_int]),
     "bb_galmod_analytic": (C.c_int, [_P, _P, bb_box, _P, C.c_int, C.c_int, _P, C.c_int64, _P, C.c_int, _P]),
+    "bb_trace_deposit": (C.c_int, [_P, _P]),
     "bb_conv2d_same": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, _P]),
     "bb_residual_work_size": (C.c_int64, [C.c_int, C.c_int64]),
     "bb_residual_gauss": (C.c_int, [_P, _P, C.c_int, C.c_int64, C.c_double, C.c_double, _P, C.c_int64, _P, _P,

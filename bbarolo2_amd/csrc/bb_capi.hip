@@ -14,6 +14,7 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
                                          int TX, int TY, double* sub, double* wtab, short4* bbox, short4* sbox,
                                          const double* cd2i, const double* vedges, const double* ptab, int ntab,
                                          float* out, int accumulate, hipStream_t st);
+extern "C" void bb_set_tile_events(void* ev_begin, void* ev_end);
 extern "C" size_t bb_analytic_sub_bytes();
 extern "C" size_t bb_analytic_wt_bytes();
 extern "C" int bb_analytic_chunk();
@@ -327,6 +328,13 @@ extern "C" int bb_galmod_analytic(const bb_model* m, const bb_azplan* p, bb_box 
                                    sub, wtab, bbox, sbox, m->d_cd2i, m->d_vedges, m->d_ptab, m->ntab, out, accumulate,
                                    (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_galmod_analytic launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
+}
+
+extern "C" int bb_trace_deposit(void* ev_begin, void* ev_end) {
+    if ((ev_begin == nullptr) != (ev_end == nullptr))
+        return fail(BB_ERR_ARG, "bb_trace_deposit: give both events or neither");
+    bb_set_tile_events(ev_begin, ev_end);
+    return 0;
 }
 
 extern "C" int bb_conv2d_same(const bb_model* m, const float* in, float* out, int B, int by, int bx,

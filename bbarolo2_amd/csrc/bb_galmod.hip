@@ -25,6 +25,7 @@
 #define BB_NSF 32     // doubles per sub-ring record
 #define BB_ZCAP 64    // vertical weights tabulated per sub-ring (beyond: computed on the fly)
 #define BB_RMAX 16    // runs of a sample inside a tile held in registers (beyond: deposited one by one)
+#define BB_CB 8       // channels of a line profile computed together
 #define BB_TILE_THREADS 512
 
 enum {
@@ -341,9 +342,11 @@ __device__ __forceinline__ float lds_phi_f(float t, const float* ph, int ntab) {
 }
 
 // aw x the channel-integrated line of centre v (galmod.py:194-214) into pixel slot of the tile: the rare
-// paths (a height-dependent velocity, more runs of a sample in the tile than BB_RMAX).
-__device__ __noinline__ void acc_line(const Acc& A, int slot, double aw, double v, double sig, const double* ve,
-                                      const double* cd, const float* sph, int ntab, int64_t kc, int64_t K, int nz) {
+// paths (a height-dependent velocity, more runs of a sample in the tile than BB_RMAX). Scalar arguments only:
+// an argument passed by reference to a call that is not inlined would have to live in scratch memory.
+__device__ __noinline__ void acc_line(unsigned long long* tile, float inv_q, int nzp, int slot, double aw, double v,
+                                      double sig, const double* ve, const double* cd, const float* sph, int ntab,
+                                      int64_t kc, int64_t K, int nz) {
     int64_t k0 = kc - K;
     if (k0 < 0) k0 = 0;
     int64_t k1 = kc + K + 1;
@@ -355,13 +358,14 @@ __device__ __noinline__ void acc_line(const Acc& A, int slot, double aw, double 
         const float pn = lds_phi((ve[k + 1] - v) * inv, sph, ntab);
         const float f = pn - pp;
         pp = pn;
-        A.add(slot, k, (float)aw * (fabsf(f) * (float)cd[k]));
+        const unsigned int uq = (unsigned int)fmaf((float)aw * (fabsf(f) * (float)cd[k]), inv_q, 0.5f);
+        atomicAdd(tile + slot * nzp + k, (unsigned long long)uq);
     }
 }
 
-__device__ __noinline__ double uw_cold(const Sub& u, int j, int ltype) {
+__device__ __noinline__ double uw_cold(int j, int nin, double step, double zlim, int ltype) {
     double lo, hi;
-    zbin(j, u.nin, u.step, u.zlim, lo, hi);
+    zbin(j, nin, step, zlim, lo, hi);
     return bb_layer_cdf(ltype, hi) - bb_layer_cdf(ltype, lo);
 }
 
@@ -374,7 +378,7 @@ __device__ __forceinline__ double uw_(const Sub& u, const float* wl, const doubl
     const int j = zbin_of(kz, u.half);
     if (wl && j < BB_WCAP) return (double)wl[j];
     if (j < BB_ZCAP) return wt[j];
-    return uw_cold(u, j, ltype);
+    return uw_cold(j, u.nin, u.step, u.zlim, ltype);
 }
 
 #define BB_LIST_CAP 1024     // chunks meeting the tile, gathered before their samples are tested
@@ -395,7 +399,7 @@ struct TileGeo {
 };
 
 // A height-dependent velocity (galmod.py:432-465): every vertical pair has its own line.
-__device__ __noinline__ void an_sample_lag(const bb_grid& g, const bb_box& box, const TileGeo& T, const Acc& A,
+__device__ __forceinline__ void an_sample_lag(const bb_grid& g, const bb_box& box, const TileGeo& T, const Acc& A,
                                            const double* sve, const double* scd, const float* sph, int ntab,
                                            const Sub& u, const float* wl, const double* __restrict__ wt, double caz,
                                            double saz) {
@@ -427,11 +431,11 @@ __device__ __noinline__ void an_sample_lag(const bb_grid& g, const bb_box& box, 
             const double a = g.relint[nl] * afl;
             const int64_t kc = bb_lround(bb_velgrid(v, g.axtyp, g.velsys, g.lfreq0[nl], g.crval3, g.cdelt3, g.crpix3));
             if (in1 && in2 && ga0 == gb0 && ga1 == gb1) {
-                acc_line(A, s1, a * (w1 + w2), v, u.sig, ve, scd, sph, ntab, kc, u.K, nz);
+                acc_line(A.tile, A.inv_q, A.nzp, s1, a * (w1 + w2), v, u.sig, ve, scd, sph, ntab, kc, u.K, nz);
                 continue;
             }
-            if (s1 >= 0) acc_line(A, s1, a * w1, v, u.sig, ve, scd, sph, ntab, kc, u.K, nz);
-            if (s2 >= 0) acc_line(A, s2, a * w2, v, u.sig, ve, scd, sph, ntab, kc, u.K, nz);
+            if (s1 >= 0) acc_line(A.tile, A.inv_q, A.nzp, s1, a * w1, v, u.sig, ve, scd, sph, ntab, kc, u.K, nz);
+            if (s2 >= 0) acc_line(A.tile, A.inv_q, A.nzp, s2, a * w2, v, u.sig, ve, scd, sph, ntab, kc, u.K, nz);
         }
     }
 }
@@ -510,7 +514,7 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
                 for (int nl = 0; nl < g.nlines; nl++) {
                     int64_t kc = bb_lround(bb_velgrid(vplane, g.axtyp, g.velsys, g.lfreq0[nl], g.crval3, g.cdelt3,
                                                       g.crpix3));
-                    acc_line(A, sl, g.relint[nl] * afl * ws, vplane, u.sig, sve + (size_t)nl * (nz + 1), scd, sph,
+                    acc_line(A.tile, A.inv_q, A.nzp, sl, g.relint[nl] * afl * ws, vplane, u.sig, sve + (size_t)nl * (nz + 1), scd, sph,
                              ntab, kc, u.K, nz);
                 }
             }
@@ -539,17 +543,30 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
             rb[r] = rslot[r] * A.nzp;
         }
         float pp = k1 > k0 ? lds_phi_f((float)(ve[k0] - vplane) * inv, sph, ntab) : 0.f;
-        for (int k = (int)k0; k < (int)k1; k++) {
-            // (k opaque to the optimiser: the slot addresses are formed per run used, not carried for all slots)
-            asm volatile("" : "+v"(k));
-            const float pn = lds_phi_f((float)(ve[k + 1] - vplane) * inv, sph, ntab);
-            const float prof = fabsf(pn - pp) * scf[k];
-            pp = pn;
+        // blocks of BB_CB channels: the block's profile values first (independent table reads, one wait), then
+        // every run adds the block (one address per run, the channels as immediate offsets)
+        for (int kb = (int)k0; kb < (int)k1; kb += BB_CB) {
+            const int nk = (int)k1 - kb;
+            float prof[BB_CB];
+#pragma unroll
+            for (int j = 0; j < BB_CB; j++) {
+                const int k = kb + j < nz ? kb + j : nz - 1;      // (past k1: computed, not added)
+                const float pn = lds_phi_f((float)(ve[k + 1] - vplane) * inv, sph, ntab);
+                prof[j] = j < nk ? fabsf(pn - pp) * scf[k] : 0.f;
+                pp = pn;
+            }
 #pragma unroll
             for (int r = 0; r < BB_RMAX; r++) {
                 const bool on = r < nr;
                 if (__ballot(on) == 0) break;      // (no active lane has a run r: the wavefront leaves the loop)
-                if (on) atomicAdd(A.tile + rb[r] + k, (unsigned long long)(unsigned int)fmaf(aw[r], prof, 0.5f));
+                if (on) {
+                    // (channels past the window have prof 0 and add 0: no branch per channel; the tile has BB_CB
+                    // words of padding)
+                    unsigned long long* t = A.tile + rb[r] + kb;
+#pragma unroll
+                    for (int j = 0; j < BB_CB; j++)
+                        atomicAdd(t + j, (unsigned long long)(unsigned int)fmaf(aw[r], prof[j], 0.5f));
+                }
             }
         }
     }
@@ -707,11 +724,19 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
 // (f32), the tile (u64).
 extern "C" size_t bb_analytic_lds_bytes(int nz, int nlines, int TX, int TY, int ntab) {
     return ((size_t)nlines * (nz + 1) + nz) * sizeof(double) + (size_t)(((nz + 1) & ~1) + ((ntab + 1) & ~1)) * sizeof(float) +
-           (size_t)BB_WLDS_MAX * sizeof(float) + (size_t)TX * TY * bb_tile_stride(nz) * sizeof(unsigned long long);
+           (size_t)BB_WLDS_MAX * sizeof(float) + ((size_t)TX * TY * bb_tile_stride(nz) + BB_CB) * sizeof(unsigned long long);
 }
 
 extern "C" size_t bb_analytic_lds_static() {
     return (BB_LIST_CAP + BB_SAMP_CAP) * sizeof(int) + (BB_TILE_THREADS / 64) * sizeof(double) + 64;
+}
+
+// Events recorded around every k_an_tile launch of this host thread (measurement; null: none).
+static thread_local hipEvent_t t_tile_ev[2] = {nullptr, nullptr};
+
+extern "C" void bb_set_tile_events(void* ev_begin, void* ev_end) {
+    t_tile_ev[0] = (hipEvent_t)ev_begin;
+    t_tile_ev[1] = (hipEvent_t)ev_end;
 }
 
 // ---------------------------------------------------------------------------------------------------------
@@ -733,9 +758,15 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
         if (e != hipSuccess) return (int)e;
         attr = true;
     }
+    if (t_tile_ev[0] && t_tile_ev[1]) {
+        hipError_t e = hipEventRecord(t_tile_ev[0], st);
+        if (e != hipSuccess) return (int)e;
+    }
     hipLaunchKernelGGL(k_an_tile, dim3(ntiles, B), dim3(BB_TILE_THREADS), lds, st, *g, box, TX, TY, chunks, nchunks,
                        az, sub, wtab, nsub, bbox, sbox, cd2i, vedges, ptab, ntab, out, accumulate);
-    return (int)hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && t_tile_ev[0] && t_tile_ev[1]) e = hipEventRecord(t_tile_ev[1], st);
+    return (int)e;
 }
 
 extern "C" size_t bb_analytic_sub_bytes() { return BB_NSF * sizeof(double); }

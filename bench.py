@@ -8,6 +8,7 @@ JSON line (see --help).
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
 """
 import argparse
+import ctypes as C
 import json
 import math
 import os
@@ -20,6 +21,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM = 8.0e12          # MI355X HBM3E, bytes/s
+# HBM bytes per launch of k_an_tile at batch 256 on the 160x160x96 box, from two rocprofv3 counter passes
+# (FETCH_SIZE, WRITE_SIZE) scaled by the same counters of a plain 1 GiB copy (bench.py --copy-calibration)
+TRAFFIC_SHAPE = (256, 160, 160, 96)
+TRAFFIC_TILE_BYTES = (3827297.6 * 2.0 + 2457600.0 * 1.0) * 1024   # KiB x scale -> bytes
 
 
 def workload(n_theta, seed=1):
@@ -132,6 +137,18 @@ def cpu_baseline(wl, per_worker=2):
                 single_core=1 / med)
 
 
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain device copy of known size (an elementwise kernel that reads nbytes and writes nbytes): FETCH_SIZE and
+    WRITE_SIZE of its dispatches give the counters' scale."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,11 +156,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device buffers 5 times (the reference kernel of the HBM counter passes)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (the likelihood batches "
-              "shard across GPUs by parameter vectors; see DESIGN.md).", file=sys.stderr)
+              "shard across GPUs by parameter vectors, one process per GPU).", file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration()
+        return
     B = a.batch
     wl = workload(max(200, B))
     cpu = None
@@ -184,6 +206,12 @@ def main():
     names = ["ring_maps", "profile_solve", "galmod_analytic", "conv2d_same", "residual"]
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
     stage = np.zeros(len(names))
+    # the deposit kernel alone (k_an_tile), timed by events that the library records around its launch
+    tev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    for e in tev:
+        e.record(stream)               # (creates the HIP event)
+    torch.cuda.synchronize()
+    tile_ms = 0.0
 
     def step(timed):
         def mark(k):
@@ -207,13 +235,17 @@ def main():
     for _ in range(a.warmup):
         step(False)
     torch.cuda.synchronize()
+    _lib.check(L.bb_trace_deposit(C.c_void_p(tev[0].cuda_event), C.c_void_p(tev[1].cuda_event)))
     t0 = time.perf_counter()
     for _ in range(a.steps):
         step(True)
         ev[-1].synchronize()
         stage += [ev[k].elapsed_time(ev[k + 1]) for k in range(len(names))]
+        tile_ms += tev[0].elapsed_time(tev[1])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    _lib.check(L.bb_trace_deposit(None, None))
+    tile_ms /= a.steps
     stage /= a.steps
     ms = dt * 1e3 / a.steps
     value = B * a.steps / dt
@@ -241,8 +273,12 @@ def main():
     # with the data (residual); the maps: counts written and read, FP64 maps written (finish) and read (Gram)
     bytes_per_eval = {"ring_maps": 8 * nmap * 3, "profile_solve": 8 * nmap, "galmod_analytic": 4 * nvox,
                       "conv2d_same": 8 * nvox, "residual": 8 * nvox}
-    dom = names[int(np.argmax(stage))]
-    ach = bytes_per_eval[dom] * B / (stage[names.index(dom)] * 1e-3)
+    # the dominant kernel, k_an_tile: it writes the model cube once (4 B per voxel); everything else it touches
+    # (sub-ring records, boxes, LDS tables) is small and re-read from cache
+    ach = 4 * nvox * B / (tile_ms * 1e-3)
+    traffic = None
+    if (B, by, bx, nz) == TRAFFIC_SHAPE:
+        traffic = TRAFFIC_TILE_BYTES
     out = {
         "metric": "likelihood evals/s (160x160x96 cube)", "value": value, "unit": "evals/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -253,8 +289,9 @@ def main():
                    "model_box": [int(blo[0]), int(blo[1]), int(bhi[0]), int(bhi[1])]},
         "stages_ms_per_step": dict(zip(names, [round(float(x), 4) for x in stage])),
         "galmod_mc_ms_per_step": round(float(mc_ms), 4),
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                     "frac": ach / PEAK_HBM, "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": "k_an_tile", "kernel_ms": round(tile_ms, 4), "achieved": ach / 1e9,
+                     "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": ach / PEAK_HBM, "traffic": traffic,
+                     "traffic_unit": "bytes per launch", "algorithmic_bytes": 4 * nvox * B},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
         "loglike_sample": [float(x) for x in ll],
     }

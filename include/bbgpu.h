@@ -79,6 +79,11 @@ int64_t bb_analytic_work_size(const bb_azplan* p, int B);
 int bb_galmod_analytic(const bb_model* m, const bb_azplan* p, bb_box box, const double* sr, int nsub, int B,
                        void* work, int64_t work_bytes, float* out, int accumulate, void* stream);
 
+/* Measurement, no counterpart in the reference: two hipEvent_t (or both NULL to stop) that the calling host
+ * thread's ANALYTIC deposits record on their stream just before and just after the deposit kernel (the tile
+ * pass of bb_galmod_analytic), so that a benchmark times that kernel alone. */
+int bb_trace_deposit(void* ev_begin, void* ev_end);
+
 /* 'Same' 2D beam convolution of every channel (FFTConvolver.__call__, bbarolo/smooth.py:145-165; _conv_sparse,
  * galmod.py:974-1037), values below 1e-12 set to zero, then the scale applied (DiskModel.convolve,
  * galmod.py:1273-1283). in/out (B, by, bx, nz) float32 yxz, distinct buffers. */

@@ -38,3 +38,43 @@ def test_argument_errors_before_device_calls():
     assert rc != 0 and b"work" in L.bb_last_error()
     with pytest.raises(_lib.BBError):
         _lib.check(rc)
+
+
+def test_argument_errors_of_the_profile_mc_and_ring_entry_points():
+    L = _lib.lib()
+    box = _lib.bb_box(0, 0, 8, 8)
+    v = C.c_void_p(8)
+    # ring maps and the profile
+    assert L.bb_ring_maps_deposit(None, None, box, None, 1, 1, None, None, 2, 0, 2, 1.0, None, 0, None, None) != 0
+    assert L.bb_ring_maps_finish(None, None, 1, 4, 4, 1.0, 1, None, None) != 0
+    d = _lib.bb_profile_desc(0, 10, 10, 0, 0.3, 1.0, 0.05, 1.0, 1.0)
+    out = C.c_void_p()
+    rc = L.bb_profile_create(C.byref(d), v, v, None, None, None, C.byref(out))
+    assert rc != 0 and b"nr" in L.bb_last_error() and not out.value
+    d = _lib.bb_profile_desc(4, 10, 11, 0, 0.3, 1.0, 0.05, 1.0, 1.0)
+    assert L.bb_profile_create(C.byref(d), v, v, None, None, None, C.byref(out)) != 0
+    d = _lib.bb_profile_desc(4, 10, 10, 0, 0.3, 1.0, 0.05, 1.0, 1.0)
+    rc = L.bb_profile_create(C.byref(d), v, v, None, None, None, C.byref(out))
+    assert rc != 0 and b"prior" in L.bb_last_error()
+    assert L.bb_profile_work_size(None, 1) < 0
+    assert L.bb_profile_solve(None, None, 1, None, 0, None, None) != 0
+    assert L.bb_profile_set_dens(None, 1, 3, 0, None, None, 5, None, None, None) != 0
+    # Monte-Carlo GALMOD
+    assert L.bb_mc_work_size(None, box, 3, 1, 100) < 0
+    assert L.bb_galmod_mc(None, box, None, 3, 1, 1, 7, 100, None, 0, None, None, None, 0, None) != 0
+    # the 3DFIT ring residual
+    assert L.bb_norm_work_size(1, 0, 4, 4, 0) < 0
+    assert L.bb_norm_work_size(2, 4, 4, 3, 1) > L.bb_norm_work_size(2, 4, 4, 3, 0) > 0
+    args = [v, 1, 4, 4, 3, v, v, 10, 10, 0, 0, v, v]
+    tail = [None, None, None, 0, None, v, 100, v, None, None, None]
+    assert L.bb_norm3dfit(*args, None, *tail) != 0 and b"par" in L.bb_last_error()
+    p = _lib.bb_norm_par(0, 5, 0, 0, 2.0, 1.0, 1.0, 0.0)
+    assert L.bb_norm3dfit(*args, C.byref(p), *tail) != 0 and b"ftype" in L.bb_last_error()
+    p = _lib.bb_norm_par(0, 2, 0, 0, 2.0, 1.0, 1.0, 0.0)
+    bad = list(args)
+    bad[9] = 8                                   # box [8, 12) outside a 10-pixel cube
+    assert L.bb_norm3dfit(*bad, C.byref(p), *tail) != 0 and b"outside" in L.bb_last_error()
+    assert L.bb_norm3dfit(*args, C.byref(p), *tail[:-5], 1, *tail[-4:]) != 0 and b"work" in L.bb_last_error()
+    # deposit events: both or neither
+    assert L.bb_trace_deposit(v, None) != 0
+    assert L.bb_trace_deposit(None, None) == 0
