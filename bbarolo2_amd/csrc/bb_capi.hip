@@ -13,7 +13,7 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
                                          const int* naz, const int4* chunks, int nchunks, const double2* az,
                                          int TX, int TY, double* sub, double* wtab, short4* bbox, short4* sbox,
                                          const double* cd2i, const double* vedges, const double* ptab, int ntab,
-                                         float* out, int accumulate, hipStream_t st);
+                                         int lin_axis, float* out, int accumulate, hipStream_t st);
 extern "C" void bb_set_tile_events(void* ev_begin, void* ev_end);
 extern "C" size_t bb_analytic_sub_bytes();
 extern "C" size_t bb_analytic_wt_bytes();
@@ -94,6 +94,8 @@ struct bb_model {
     double* d_kern64;
     double scale;
     int TX, TY;        // pixel tile of the ANALYTIC deposit (its spectra fit the LDS budget)
+    int lin;           // linear velocity axis whose edges are exactly ((k - 1/2) + 1 - crpix3) cdelt3 + crval3 and
+                       // one channel factor cd2i for all channels: the deposit computes them instead of reading them
 };
 
 // The azimuth plan of a set of sub-ring radii (ANALYTIC samples): naz per sub-ring, the chunks of BB_CS
@@ -175,6 +177,17 @@ extern "C" int bb_model_create(const bb_grid* g, const double* cd2i, const doubl
     rc = upload(&m->d_cd2i, cd2i, (size_t)g->nz, "upload cd2i");
     if (!rc) rc = upload(&m->d_vedges, vedges, (size_t)g->nlines * (g->nz + 1), "upload vedges");
     if (!rc) rc = upload(&m->d_ptab, ptab, (size_t)ntab, "upload Phi table");
+    if (!rc && g->axtyp == 4) {
+        bool lin = true;
+        for (int k = 1; k < g->nz && lin; k++) lin = cd2i[k] == cd2i[0] && (float)cd2i[k] == (float)cd2i[0];
+        for (int l = 0; l < g->nlines && lin; l++)
+            for (int e = 0; e <= g->nz && lin; e++) {
+                volatile double t = (double)e + 0.5 - g->crpix3;       // (the operations of the device, unfused)
+                volatile double u = t * g->cdelt3;
+                lin = u + g->crval3 == vedges[(size_t)l * (g->nz + 1) + e];
+            }
+        m->lin = lin ? 1 : 0;
+    }
     if (!rc && kern) {
         std::vector<float> kf((size_t)ky * kx);
         for (size_t i = 0; i < kf.size(); i++) kf[i] = (float)kern[i];
@@ -325,8 +338,8 @@ extern "C" int bb_galmod_analytic(const bb_model* m, const bb_azplan* p, bb_box 
     w += (size_t)B * p->nchunks * sizeof(short4);
     short4* sbox = (short4*)w;
     rc = bb_launch_galmod_analytic(&m->g, box, sr, nsub, B, p->d_naz, p->d_chunks, p->nchunks, p->d_az, m->TX, m->TY,
-                                   sub, wtab, bbox, sbox, m->d_cd2i, m->d_vedges, m->d_ptab, m->ntab, out, accumulate,
-                                   (hipStream_t)stream);
+                                   sub, wtab, bbox, sbox, m->d_cd2i, m->d_vedges, m->d_ptab, m->ntab, m->lin, out,
+                                   accumulate, (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_galmod_analytic launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
 

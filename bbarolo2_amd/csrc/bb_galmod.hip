@@ -26,7 +26,8 @@
 #define BB_ZCAP 64    // vertical weights tabulated per sub-ring (beyond: computed on the fly)
 #define BB_RMAX 16    // runs of a sample inside a tile held in registers (beyond: deposited one by one)
 #define BB_CB 8       // channels of a line profile computed together
-#define BB_TILE_THREADS 512
+#define BB_TILE_THREADS 768     // 12 wavefronts: 3 per SIMD at the LDS of one workgroup per CU
+#define BB_RM_THREADS 512      // ring maps workgroup (1024 threads, 4 wavefronts per SIMD, measured no faster)
 
 enum {
     F_R0 = 0, F_R1, F_XPOS, F_YPOS, F_SPA, F_CPA, F_SINC, F_CINC, F_VSYS, F_VROT, F_VRAD, F_VVERT, F_SIG, F_K,
@@ -279,25 +280,33 @@ struct Track {
 
 __device__ __forceinline__ Track utrack(const Sub& u, const bb_grid& g, double xs, double xc, double yb) {
     Track t;
-    t.X0 = u.xpos + (xs - yb * u.cpa) / g.cdelt0;
-    t.dX = u.sinc * u.cpa / g.cdelt0;
-    t.Y0 = u.ypos + (xc + yb * u.spa) / g.cdelt1;
-    t.dY = -u.sinc * u.spa / g.cdelt1;
+    const double i0 = 1.0 / g.cdelt0, i1 = 1.0 / g.cdelt1;     // (an estimate: products, not quotients)
+    t.X0 = u.xpos + (xs - yb * u.cpa) * i0;
+    t.dX = u.sinc * u.cpa * i0;
+    t.Y0 = u.ypos + (xc + yb * u.spa) * i1;
+    t.dY = -u.sinc * u.spa * i1;
     return t;
 }
 
-__device__ __forceinline__ bool near_half(double X) {
-    return !(fabs(X) < 1e9) || fabs(X - floor(X) - 0.5) < 1e-9;
-}
+// A pixel index as int32: every box pixel fits; an index beyond +-2^30 (a sample far outside) becomes -2^30,
+// outside every tile.
+__device__ __forceinline__ int pix32(int64_t a) { return a > (1 << 30) || a < -(1 << 30) ? -(1 << 30) : (int)a; }
 
+// The pixel of the track at height z: the rounded estimate unless it is within 1e-9 of a half-integer (or
+// huge), where upix places the sample exactly as the reference does.
 __device__ __forceinline__ void upix_fast(const Sub& u, const bb_grid& g, const Track& t, double xs, double xc,
-                                          double yb, double z, int64_t& g0, int64_t& g1) {
+                                          double yb, double z, int& g0, int& g1) {
     const double X = fma(z, t.dX, t.X0), Y = fma(z, t.dY, t.Y0);
-    if (near_half(X) || near_half(Y)) {
-        upix(u, g, xs, xc, yb, z, g0, g1);
+    const double rx = rint(X), ry = rint(Y);
+    const double lim = 0.5 - 1e-9;
+    if (fabs(X) < 1e9 && fabs(Y) < 1e9 && fabs(X - rx) < lim && fabs(Y - ry) < lim) {
+        g0 = (int)rx;
+        g1 = (int)ry;
     } else {
-        g0 = (int64_t)rint(X);
-        g1 = (int64_t)rint(Y);
+        int64_t a, b;
+        upix(u, g, xs, xc, yb, z, a, b);
+        g0 = pix32(a);
+        g1 = pix32(b);
     }
 }
 
@@ -383,19 +392,19 @@ __device__ __forceinline__ double uw_(const Sub& u, const float* wl, const doubl
 
 #define BB_LIST_CAP 1024     // chunks meeting the tile, gathered before their samples are tested
 #define BB_SAMP_CAP 2048     // samples meeting the tile, gathered before they are deposited
-#define BB_NGRP (BB_TILE_THREADS / BB_CS)
 #define BB_WLDS_MAX 4096     // vertical weights (float) kept in LDS for all the sub-rings of a vector
 
 // Odd stride of the pixel spectra in the LDS tile: consecutive pixels start in different banks.
 __device__ __host__ __forceinline__ int bb_tile_stride(int nz) { return nz | 1; }
 
 struct TileGeo {
-    int64_t ax0, ax1, ay0, ay1;     // the tile in absolute pixels (inclusive)
+    int ax0, ax1, ay0, ay1;     // the tile in absolute pixels (inclusive)
     int TX;
-    __device__ __forceinline__ int slot(int64_t p0, int64_t p1) const {
+    __device__ __forceinline__ int slot(int p0, int p1) const {
         if (p0 < ax0 || p0 > ax1 || p1 < ay0 || p1 > ay1) return -1;
-        return (int)((p1 - ay0) * TX + (p0 - ax0));
+        return (p1 - ay0) * TX + (p0 - ax0);
     }
+    __device__ __forceinline__ int slot(int64_t p0, int64_t p1) const { return slot(pix32(p0), pix32(p1)); }
 };
 
 // A height-dependent velocity (galmod.py:432-465): every vertical pair has its own line.
@@ -465,7 +474,11 @@ __device__ __forceinline__ bool tile_zrange(const Sub& u, const Track& tr, const
     return true;
 }
 
-// Deposit the runs of one azimuth sample that fall in the tile (galmod.py:389-431 for one sample).
+// Deposit the runs of one azimuth sample that fall in the tile (galmod.py:389-431 for one sample). LIN: a linear
+// velocity axis (AXTYP 4) and one channel factor for all channels: the channel edges are computed as the host
+// computes them, ((k - 1/2) + 1 - crpix3) cdelt3 + crval3 in FP64 (the same operations, the same values), and
+// the channel factor is a constant, instead of being read from LDS.
+template <bool LIN>
 __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, const TileGeo& T, const Acc& A,
                                           const double* sve, const double* scd, const float* scf, const float* sph,
                                           int ntab,
@@ -486,11 +499,11 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
     int rslot[BB_RMAX];
     float rws[BB_RMAX];
     int nr = 0;
-    int64_t c0, c1;
+    int c0, c1;
     upix_fast(u, g, tr, xs, xc, yb, uz(u, ka), c0, c1);
     double ws = uw_(u, wl, wt, ka, g.ltype);
     for (int kz = ka + 1; kz <= kb + 1; kz++) {
-        int64_t h0 = 0, h1 = 0;
+        int h0 = 0, h1 = 0;
         bool same = false;
         if (kz <= kb) {
             upix_fast(u, g, tr, xs, xc, yb, uz(u, kz), h0, h1);
@@ -542,19 +555,29 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
             aw[r] = a * rws[r] * A.inv_q;
             rb[r] = rslot[r] * A.nzp;
         }
-        float pp = k1 > k0 ? lds_phi_f((float)(ve[k0] - vplane) * inv, sph, ntab) : 0.f;
+        auto edge = [&](int e) -> double {
+            return LIN ? ((double)e + 0.5 - g.crpix3) * g.cdelt3 + g.crval3 : ve[e];
+        };
+        const float cf0 = scf[0];
+        float pp = k1 > k0 ? lds_phi_f((float)(edge((int)k0) - vplane) * inv, sph, ntab) : 0.f;
         // blocks of BB_CB channels: the block's profile values first (independent table reads, one wait), then
         // every run adds the block (one address per run, the channels as immediate offsets)
         for (int kb = (int)k0; kb < (int)k1; kb += BB_CB) {
             const int nk = (int)k1 - kb;
-            float prof[BB_CB];
+            float pe[BB_CB], prof[BB_CB];
+#pragma unroll
+            for (int j = 0; j < BB_CB; j++) {            // (independent lookups: the loads issue together)
+                const int k = kb + j < nz ? kb + j : nz - 1;      // (past k1: computed, not added)
+                pe[j] = lds_phi_f((float)(edge(k + 1) - vplane) * inv, sph, ntab);
+            }
 #pragma unroll
             for (int j = 0; j < BB_CB; j++) {
-                const int k = kb + j < nz ? kb + j : nz - 1;      // (past k1: computed, not added)
-                const float pn = lds_phi_f((float)(ve[k + 1] - vplane) * inv, sph, ntab);
-                prof[j] = j < nk ? fabsf(pn - pp) * scf[k] : 0.f;
-                pp = pn;
+                const int k = kb + j < nz ? kb + j : nz - 1;
+                const float c = LIN ? cf0 : scf[k];
+                const float v = fabsf(pe[j] - (j ? pe[j - 1] : pp)) * c;
+                prof[j] = j < nk ? v : 0.f;
             }
+            pp = pe[BB_CB - 1];
 #pragma unroll
             for (int r = 0; r < BB_RMAX; r++) {
                 const bool on = r < nr;
@@ -595,7 +618,8 @@ __device__ __forceinline__ void tile_samples(int nchunks, const short4* __restri
         const int n = nlist;
         __syncthreads();                            // (every thread has read n before the list changes)
         if (!(n > BB_LIST_CAP - (int)blockDim.x || base + (int)blockDim.x >= nchunks)) continue;
-        for (int j0 = 0; j0 < n; j0 += BB_NGRP) {
+        const int ngrp = (int)blockDim.x / BB_CS;
+        for (int j0 = 0; j0 < n; j0 += ngrp) {
             // 2. their samples whose vertical segment meets the tile (one lane per azimuth)
             const int j = j0 + grp;
             if (j < n) {
@@ -607,7 +631,7 @@ __device__ __forceinline__ void tile_samples(int nchunks, const short4* __restri
             __syncthreads();
             const int ns = nsamp;
             __syncthreads();
-            if (ns > BB_SAMP_CAP - (int)blockDim.x || j0 + BB_NGRP >= n) {
+            if (ns > BB_SAMP_CAP - (int)blockDim.x || j0 + ngrp >= n) {
                 // 3. the samples gathered, one per thread
                 const int W = blockDim.x >> 6;
                 for (int base2 = 0; base2 < ns; base2 += blockDim.x) {
@@ -624,6 +648,7 @@ __device__ __forceinline__ void tile_samples(int nchunks, const short4* __restri
     }
 }
 
+template <bool LIN>
 __global__ void __launch_bounds__(BB_TILE_THREADS)
 k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks, int nchunks,
           const double2* __restrict__ az, const double* __restrict__ suball, const double* __restrict__ wtaball,
@@ -704,7 +729,7 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
         Sub u;
         load_sub(u, sub + (size_t)s * BB_NSF, ch.x & 1);
         const double2 cs = az[ch.w + (e & 31)];
-        an_sample(g, box, T, A, sve, scd, scf, sph, ntab, u, wlds ? swl + s * BB_WCAP : nullptr,
+        an_sample<LIN>(g, box, T, A, sve, scd, scf, sph, ntab, u, wlds ? swl + s * BB_WCAP : nullptr,
                   wtab + (size_t)s * BB_ZCAP, cs.x, cs.y);
     });
     __syncthreads();
@@ -744,17 +769,21 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
                                          const int* naz, const int4* chunks, int nchunks, const double2* az,
                                          int TX, int TY, double* sub, double* wtab, short4* bbox, short4* sbox,
                                          const double* cd2i, const double* vedges, const double* ptab, int ntab,
-                                         float* out, int accumulate, hipStream_t st) {
+                                         int lin_axis, float* out, int accumulate, hipStream_t st) {
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
     hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, naz, sub, wtab, 0);
     hipLaunchKernelGGL(k_an_bbox, dim3((nchunks * BB_CS + 255) / 256, B), dim3(256), 0, st, *g, box, chunks, nchunks,
                        az, sub, nsub, bbox, sbox);
     const int ntiles = ((bx + TX - 1) / TX) * ((by + TY - 1) / TY);
     const size_t lds = bb_analytic_lds_bytes(g->nz, g->nlines, TX, TY, ntab);
+    // a linear velocity axis with one channel factor: the channel edges and factors computed, not read
+    const bool lin = lin_axis != 0;
     static bool attr = false;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_an_tile, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)(160 * 1024 - bb_analytic_lds_static()));
+        const int dyn = (int)(160 * 1024 - bb_analytic_lds_static());
+        hipError_t e = hipFuncSetAttribute((const void*)k_an_tile<true>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_an_tile<false>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
         if (e != hipSuccess) return (int)e;
         attr = true;
     }
@@ -762,8 +791,12 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
         hipError_t e = hipEventRecord(t_tile_ev[0], st);
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL(k_an_tile, dim3(ntiles, B), dim3(BB_TILE_THREADS), lds, st, *g, box, TX, TY, chunks, nchunks,
-                       az, sub, wtab, nsub, bbox, sbox, cd2i, vedges, ptab, ntab, out, accumulate);
+    if (lin)
+        hipLaunchKernelGGL(k_an_tile<true>, dim3(ntiles, B), dim3(BB_TILE_THREADS), lds, st, *g, box, TX, TY, chunks,
+                           nchunks, az, sub, wtab, nsub, bbox, sbox, cd2i, vedges, ptab, ntab, out, accumulate);
+    else
+        hipLaunchKernelGGL(k_an_tile<false>, dim3(ntiles, B), dim3(BB_TILE_THREADS), lds, st, *g, box, TX, TY, chunks,
+                           nchunks, az, sub, wtab, nsub, bbox, sbox, cd2i, vedges, ptab, ntab, out, accumulate);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && t_tile_ev[0] && t_tile_ev[1]) e = hipEventRecord(t_tile_ev[1], st);
     return (int)e;
@@ -781,7 +814,7 @@ extern "C" int bb_analytic_chunk() { return BB_CS; }
 // workgroup per (vector, tile of TX x TY pixels) holds the tile's nw maps in LDS as 64-bit fixed point (a
 // contribution x adds round(x / qunit): exact sums, independent of the order of the atomics) and writes them
 // once: every count is written by exactly one workgroup.
-__global__ void __launch_bounds__(BB_TILE_THREADS)
+__global__ void __launch_bounds__(BB_RM_THREADS)
 k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks, int nchunks,
           const double2* __restrict__ az, const double* __restrict__ suball, const double* __restrict__ wtaball,
           int nsub, const short4* __restrict__ bball, const short4* __restrict__ sball, const int2* __restrict__ widx,
@@ -830,12 +863,13 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
         const double x = u.R * cs.x, yb = u.R * cs.y * u.cinc, xs = x * u.spa, xc = x * u.cpa;
         const Track tr = utrack(u, g, xs, xc, yb);
         int ka, kb;
-        if (!tile_zrange(u, tr, T, ka, kb)) return;
-        int64_t c0, c1;
+        ka = 0;                 // (the whole column: cheaper here than narrowing it to the tile)
+        kb = u.nzz - 1;
+        int c0, c1;
         upix_fast(u, g, tr, xs, xc, yb, uz(u, ka), c0, c1);
         double ws = uw_(u, nullptr, wt, ka, g.ltype);
         for (int kz = ka + 1; kz <= kb + 1; kz++) {
-            int64_t h0 = 0, h1 = 0;
+            int h0 = 0, h1 = 0;
             bool same = false;
             if (kz <= kb) {
                 upix_fast(u, g, tr, xs, xc, yb, uz(u, kz), h0, h1);
@@ -889,7 +923,7 @@ extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* s
         attr = true;
     }
     const int ntiles = ((bx + TX - 1) / TX) * ((by + TY - 1) / TY);
-    hipLaunchKernelGGL(k_rm_tile, dim3(ntiles, B), dim3(BB_TILE_THREADS), bb_rm_tile_lds(nw, TX, TY), st, *g, box, TX,
+    hipLaunchKernelGGL(k_rm_tile, dim3(ntiles, B), dim3(BB_RM_THREADS), bb_rm_tile_lds(nw, TX, TY), st, *g, box, TX,
                        TY, chunks, nchunks, az, sub, wtab, nsub, bbox, sbox, (const int2*)widx, (const double2*)wval,
                        nmap, map0, nw, 1.0 / qunit, counts);
     return (int)hipGetLastError();
