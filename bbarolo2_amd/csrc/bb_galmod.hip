@@ -474,72 +474,18 @@ __device__ __forceinline__ bool tile_zrange(const Sub& u, const Track& tr, const
     return true;
 }
 
-// Deposit the runs of one azimuth sample that fall in the tile (galmod.py:389-431 for one sample). LIN: a linear
-// velocity axis (AXTYP 4) and one channel factor for all channels: the channel edges are computed as the host
-// computes them, ((k - 1/2) + 1 - crpix3) cdelt3 + crval3 in FP64 (the same operations, the same values), and
-// the channel factor is a constant, instead of being read from LDS.
-template <bool LIN>
-__device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, const TileGeo& T, const Acc& A,
-                                          const double* sve, const double* scd, const float* scf, const float* sph,
-                                          int ntab,
-                                          const Sub& u, const float* wl, const double* __restrict__ wt, double caz,
-                                          double saz) {
-    if (u.zdep) {
-        an_sample_lag(g, box, T, A, sve, scd, sph, ntab, u, wl, wt, caz, saz);
-        return;
-    }
+// The line profile of one sample (galmod.py:411-431), shared by its runs in the tile: runs r with rs[r] >= 0 (the
+// tile slot) and rw[r] (the vertical weight). COMPACT: the runs are rs[0..nr) and the loop over r stops at the
+// first r that no lane of the wavefront has; otherwise rs is indexed by the run's last vertical sample and empty
+// entries are skipped. LIN: a linear velocity axis (AXTYP 4) with one channel factor for all channels: the
+// channel edges are computed as the host computes them, ((k - 1/2) + 1 - crpix3) cdelt3 + crval3 in FP64 (the
+// same operations, the same values), and the channel factor is a constant, instead of being read from LDS.
+template <bool LIN, bool COMPACT>
+__device__ __forceinline__ void an_lines(const bb_grid& g, const Acc& A, const double* sve, const float* scf,
+                                         const float* sph, int ntab, const Sub& u, double vplane, double afl,
+                                         const int* rs, const float* rw, int nr) {
     const int nz = g.nz;
-    const double wside = caz >= 0.0 ? u.wrec : u.wapp;
-    const double x = u.R * caz, yb = u.R * saz * u.cinc, xs = x * u.spa, xc = x * u.cpa;
-    const double vplane = u.vsys + (u.vrot * caz + u.vrad * saz) * u.sinc + u.vvert * u.cinc;
-    const double afl = u.flux * wside;
-    const Track tr = utrack(u, g, xs, xc, yb);
-    int ka, kb;
-    if (!tile_zrange(u, tr, T, ka, kb)) return;
-    int rslot[BB_RMAX];
-    float rws[BB_RMAX];
-    int nr = 0;
-    int c0, c1;
-    upix_fast(u, g, tr, xs, xc, yb, uz(u, ka), c0, c1);
-    double ws = uw_(u, wl, wt, ka, g.ltype);
-    for (int kz = ka + 1; kz <= kb + 1; kz++) {
-        int h0 = 0, h1 = 0;
-        bool same = false;
-        if (kz <= kb) {
-            upix_fast(u, g, tr, xs, xc, yb, uz(u, kz), h0, h1);
-            same = h0 == c0 && h1 == c1;
-        }
-        if (same) {
-            ws += uw_(u, wl, wt, kz, g.ltype);
-            continue;
-        }
-        const int sl = T.slot(c0, c1);
-        if (sl >= 0) {
-            if (nr < BB_RMAX) {
-#pragma unroll
-                for (int r = 0; r < BB_RMAX; r++)
-                    if (r == nr) {
-                        rslot[r] = sl;
-                        rws[r] = (float)ws;
-                    }
-                nr++;
-            } else {
-                for (int nl = 0; nl < g.nlines; nl++) {
-                    int64_t kc = bb_lround(bb_velgrid(vplane, g.axtyp, g.velsys, g.lfreq0[nl], g.crval3, g.cdelt3,
-                                                      g.crpix3));
-                    acc_line(A.tile, A.inv_q, A.nzp, sl, g.relint[nl] * afl * ws, vplane, u.sig, sve + (size_t)nl * (nz + 1), scd, sph,
-                             ntab, kc, u.K, nz);
-                }
-            }
-        }
-        if (kz <= kb) {
-            c0 = h0;
-            c1 = h1;
-            ws = uw_(u, wl, wt, kz, g.ltype);
-        }
-    }
-    // one line profile per emission line, shared by the runs kept (galmod.py:411-431)
-    for (int nl = 0; nl < g.nlines && nr > 0; nl++) {
+    for (int nl = 0; nl < g.nlines; nl++) {
         const double* ve = sve + (size_t)nl * (nz + 1);
         const float a = (float)(g.relint[nl] * afl);
         const int64_t kc = bb_lround(bb_velgrid(vplane, g.axtyp, g.velsys, g.lfreq0[nl], g.crval3, g.cdelt3, g.crpix3));
@@ -548,13 +494,6 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
         int64_t k1 = kc + u.K + 1;
         if (k1 > nz) k1 = nz;
         const float inv = (float)(1.0 / u.sig);
-        float aw[BB_RMAX];
-        int rb[BB_RMAX];
-#pragma unroll
-        for (int r = 0; r < BB_RMAX; r++) {
-            aw[r] = a * rws[r] * A.inv_q;
-            rb[r] = rslot[r] * A.nzp;
-        }
         auto edge = [&](int e) -> double {
             return LIN ? ((double)e + 0.5 - g.crpix3) * g.cdelt3 + g.crval3 : ve[e];
         };
@@ -580,19 +519,118 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
             pp = pe[BB_CB - 1];
 #pragma unroll
             for (int r = 0; r < BB_RMAX; r++) {
-                const bool on = r < nr;
-                if (__ballot(on) == 0) break;      // (no active lane has a run r: the wavefront leaves the loop)
+                const bool on = COMPACT ? r < nr : rs[r] >= 0;
+                if (__ballot(on) == 0) {
+                    if (COMPACT) break;      // (no active lane has a run r: the wavefront leaves the loop)
+                    continue;
+                }
                 if (on) {
                     // (channels past the window have prof 0 and add 0: no branch per channel; the tile has BB_CB
                     // words of padding)
-                    unsigned long long* t = A.tile + rb[r] + kb;
+                    const float aw = a * rw[r] * A.inv_q;
+                    unsigned long long* t = A.tile + rs[r] * A.nzp + kb;
 #pragma unroll
                     for (int j = 0; j < BB_CB; j++)
-                        atomicAdd(t + j, (unsigned long long)(unsigned int)fmaf(aw[r], prof[j], 0.5f));
+                        atomicAdd(t + j, (unsigned long long)(unsigned int)fmaf(aw, prof[j], 0.5f));
                 }
             }
         }
     }
+}
+
+// Deposit the runs of one azimuth sample that fall in the tile (galmod.py:389-431 for one sample).
+template <bool LIN>
+__device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, const TileGeo& T, const Acc& A,
+                                          const double* sve, const double* scd, const float* scf, const float* sph,
+                                          int ntab,
+                                          const Sub& u, const float* wl, const double* __restrict__ wt, double caz,
+                                          double saz) {
+    if (u.zdep) {
+        an_sample_lag(g, box, T, A, sve, scd, sph, ntab, u, wl, wt, caz, saz);
+        return;
+    }
+    const int nz = g.nz;
+    const double wside = caz >= 0.0 ? u.wrec : u.wapp;
+    const double x = u.R * caz, yb = u.R * saz * u.cinc, xs = x * u.spa, xc = x * u.cpa;
+    const double vplane = u.vsys + (u.vrot * caz + u.vrad * saz) * u.sinc + u.vvert * u.cinc;
+    const double afl = u.flux * wside;
+    const Track tr = utrack(u, g, xs, xc, yb);
+    int ka, kb;
+    if (!tile_zrange(u, tr, T, ka, kb)) return;
+    int rs[BB_RMAX];
+    float rw[BB_RMAX];
+    if (u.nzz <= BB_RMAX) {
+        // a short column: the runs are stored at the index of their last vertical sample (a register index
+        // known at compile time in the unrolled loop), so no run is moved; rs = -1 where no run ends
+        int c0 = 0, c1 = 0;
+        double ws = 0.0;
+#pragma unroll
+        for (int kz = 0; kz <= BB_RMAX; kz++) {
+            const bool in = kz >= ka && kz <= kb;
+            int h0 = 0, h1 = 0;
+            if (in) upix_fast(u, g, tr, xs, xc, yb, uz(u, kz), h0, h1);
+            const bool same = in && kz > ka && h0 == c0 && h1 == c1;
+            if (kz > 0) {
+                const bool close = kz - 1 >= ka && kz - 1 <= kb && !same;     // the run open at kz - 1 ends there
+                rs[kz - 1] = close ? T.slot(c0, c1) : -1;
+                rw[kz - 1] = (float)ws;
+            }
+            if (in) {
+                const double w = uw_(u, wl, wt, kz, g.ltype);
+                if (same) {
+                    ws += w;
+                } else {
+                    c0 = h0;
+                    c1 = h1;
+                    ws = w;
+                }
+            }
+        }
+        an_lines<LIN, false>(g, A, sve, scf, sph, ntab, u, vplane, afl, rs, rw, 0);
+        return;
+    }
+    // a long column: the runs in the tile are compacted into rs[0..nr); beyond BB_RMAX they are deposited one by one
+    int nr = 0;
+    int c0, c1;
+    upix_fast(u, g, tr, xs, xc, yb, uz(u, ka), c0, c1);
+    double ws = uw_(u, wl, wt, ka, g.ltype);
+    for (int kz = ka + 1; kz <= kb + 1; kz++) {
+        int h0 = 0, h1 = 0;
+        bool same = false;
+        if (kz <= kb) {
+            upix_fast(u, g, tr, xs, xc, yb, uz(u, kz), h0, h1);
+            same = h0 == c0 && h1 == c1;
+        }
+        if (same) {
+            ws += uw_(u, wl, wt, kz, g.ltype);
+            continue;
+        }
+        const int sl = T.slot(c0, c1);
+        if (sl >= 0) {
+            if (nr < BB_RMAX) {
+#pragma unroll
+                for (int r = 0; r < BB_RMAX; r++)
+                    if (r == nr) {
+                        rs[r] = sl;
+                        rw[r] = (float)ws;
+                    }
+                nr++;
+            } else {
+                for (int nl = 0; nl < g.nlines; nl++) {
+                    int64_t kc = bb_lround(bb_velgrid(vplane, g.axtyp, g.velsys, g.lfreq0[nl], g.crval3, g.cdelt3,
+                                                      g.crpix3));
+                    acc_line(A.tile, A.inv_q, A.nzp, sl, g.relint[nl] * afl * ws, vplane, u.sig,
+                             sve + (size_t)nl * (nz + 1), scd, sph, ntab, kc, u.K, nz);
+                }
+            }
+        }
+        if (kz <= kb) {
+            c0 = h0;
+            c1 = h1;
+            ws = uw_(u, wl, wt, kz, g.ltype);
+        }
+    }
+    if (nr > 0) an_lines<LIN, true>(g, A, sve, scf, sph, ntab, u, vplane, afl, rs, rw, nr);
 }
 
 // The samples of a batch vector that meet the tile [tx0, tx1] x [ty0, ty1] (box-relative): the chunks whose
