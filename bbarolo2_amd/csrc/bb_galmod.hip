@@ -555,13 +555,14 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
     const double vplane = u.vsys + (u.vrot * caz + u.vrad * saz) * u.sinc + u.vvert * u.cinc;
     const double afl = u.flux * wside;
     const Track tr = utrack(u, g, xs, xc, yb);
-    int ka, kb;
-    if (!tile_zrange(u, tr, T, ka, kb)) return;
     int rs[BB_RMAX];
     float rw[BB_RMAX];
     if (u.nzz <= BB_RMAX) {
         // a short column: the runs are stored at the index of their last vertical sample (a register index
-        // known at compile time in the unrolled loop), so no run is moved; rs = -1 where no run ends
+        // known at compile time in the unrolled loop), so no run is moved; rs = -1 where no run ends. The whole
+        // column is walked: the unrolled loop costs the same for any sub-range, and its runs outside the tile
+        // are not deposited.
+        const int ka = 0, kb = u.nzz - 1;
         int c0 = 0, c1 = 0;
         double ws = 0.0;
 #pragma unroll
@@ -590,6 +591,8 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
         return;
     }
     // a long column: the runs in the tile are compacted into rs[0..nr); beyond BB_RMAX they are deposited one by one
+    int ka, kb;
+    if (!tile_zrange(u, tr, T, ka, kb)) return;
     int nr = 0;
     int c0, c1;
     upix_fast(u, g, tr, xs, xc, yb, uz(u, ka), c0, c1);
