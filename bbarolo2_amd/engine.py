@@ -261,6 +261,30 @@ class DiskModel:
         _lib.check(L.bb_conv2d_same(h, dep.data_ptr(), res.data_ptr(), B, by, bx, sp))
         return res
 
+    def orient_tile(self, sr, nzq=None):
+        """Orient the rectangular pixel tile of the deposits (bb_model_set_tile) along the vertical columns of the
+        tables sr (B, 16, nsub): a sample's column runs along the projected minor axis, (sin i cos p / cdelt0,
+        -sin i sin p / cdelt1) in pixels (p = PA + CROTA2), and a tile that is long in that direction is crossed by
+        fewer columns, so fewer samples are gathered by two tiles. Returns the (TX, TY) set."""
+        sr = np.asarray(sr, np.float64).reshape(-1, 16, np.shape(sr)[-1])
+        st = self.setup
+        L = _lib.lib()
+        h = self.handle(nzq_for(sr.reshape(16, -1)) if nzq is None else nzq)
+        tx, ty = C.c_int(), C.c_int()
+        _lib.check(L.bb_model_get_tile(h, C.byref(tx), C.byref(ty)))
+        lo, hi = sorted((tx.value, ty.value))
+        if lo == hi or sr.shape[2] == 0:
+            return tx.value, ty.value
+        sinc = np.abs(np.sin(sr[:, 9]))
+        p = sr[:, 10] + st.crota2
+        w = np.abs(sr[:, 5]) + 1e-300
+        dx = float(np.sum(w * sinc * np.abs(np.cos(p))) / abs(st.cdelt[0]))
+        dy = float(np.sum(w * sinc * np.abs(np.sin(p))) / abs(st.cdelt[1]))
+        want = (lo, hi) if dy > dx else (hi, lo)
+        if want != (tx.value, ty.value):
+            _lib.check(L.bb_model_set_tile(h, want[0], want[1]))
+        return want
+
     def array_batch(self, sr, blo, bhi, convolve=True, out=None, work=None, stream=None, d_sr=None):
         """ANALYTIC models of B sub-ring tables sr (B, 16, nsub) sharing their radii, in the box [blo, bhi):
         a (B, by, bx, nz) float32 device tensor (yxz), convolved with the beam if convolve. d_sr: the tables
@@ -281,6 +305,7 @@ class DiskModel:
         if nsub:
             if d_sr is None:
                 d_sr = torch.from_numpy(sr).to(dev, non_blocking=False)
+            self.orient_tile(sr, nzq)
             p = self.azplan(sr[0, 0], nzq)
             L = _lib.lib()
             wb = int(L.bb_analytic_work_size(p, B))

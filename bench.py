@@ -196,6 +196,7 @@ def main():
     L = _lib.lib()
     nzq = 1 if not np.any(sr[:, 6] > 0) else 1024
     h = dm.handle(nzq)
+    dm.orient_tile(sr, nzq)            # (as DiskModel.array_batch does for every batch)
     plan = dm.azplan(sr[0, 0], nzq)
     wb = int(L.bb_analytic_work_size(plan, B))
     ws = torch.empty(wb, dtype=torch.uint8, device="cuda")
