@@ -404,8 +404,8 @@ extern "C" int bb_ring_maps_deposit(const bb_model* m, const bb_azplan* p, bb_bo
     if (nsub > 0 && work_bytes < bb_ring_maps_work_size(p, B))
         return fail(BB_ERR_ARG, "bb_ring_maps_deposit: work holds %lld bytes, needs %lld", (long long)work_bytes,
                     (long long)bb_ring_maps_work_size(p, B));
-    // the largest pixel tile whose nw maps fit the LDS of a CU
-    static const int cand[][2] = {{16, 16}, {16, 8}, {8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
+    // the largest pixel tile whose nw maps fit the LDS of a CU (a larger tile is crossed by fewer vertical columns)
+    static const int cand[][2] = {{20, 20}, {16, 16}, {16, 8}, {8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
     int TX = 0, TY = 0;
     for (auto& c : cand)
         if (bb_rm_tile_lds(nw, c[0], c[1]) + bb_rm_tile_static() <= BB_LDS_MAX) {
