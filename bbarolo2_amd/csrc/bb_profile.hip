@@ -152,8 +152,11 @@ __global__ void k_gram_reduce(const double* __restrict__ partial, int n, int nbl
 
 // ---------------------------------------------------------------------------------------------------------
 // Non-negative least squares on the normal equations, one wavefront (64 lanes) per system. M (n x n, row
-// major, symmetric positive definite) in global memory, h (n) in LDS; the result in x (LDS).
+// major, symmetric positive definite) and h (n) in LDS; the result in x (LDS). Lane l owns the entries l and
+// l + 64 of every vector; the choices of the active set (the entering index, the step to the boundary) are
+// wavefront reductions that pick the same index as a scan in index order (ties to the lowest index).
 struct NnLds {
+    double* M;      // n x n system
     double* C;      // n x n Cholesky factor of the passive system
     double* z;      // n: passive solution
     double* x;      // n: current solution
@@ -161,33 +164,58 @@ struct NnLds {
     double* h;      // n
     int* P;         // n: passive flags
     int* ban;       // n: excluded from the next choice
-    int* sh;        // scratch ints: [0] chosen index, [1] flag
-    double* shd;    // scratch doubles
 };
 
-// z = M_PP^-1 h_P (0 outside P) by Cholesky; false if a pivot is not positive.
-__device__ bool nn_passive_solve(const double* __restrict__ M, const NnLds& L, int n, int lane) {
-    double* C = L.C;
-    for (int e = lane; e < n * n; e += 64) {
-        const int i = e / n, j = e % n;
-        C[e] = (L.P[i] && L.P[j]) ? M[e] : (i == j ? 1.0 : 0.0);
+// (value, index) of the lane pair with the larger value, ties to the lower index; index -1 loses to any index.
+__device__ __forceinline__ void nn_argmax(double& v, int& i) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off, 64);
+        const int oi = __shfl_xor(i, off, 64);
+        if (oi >= 0 && (i < 0 || ov > v || (ov == v && oi < i))) {
+            v = ov;
+            i = oi;
+        }
     }
+}
+
+__device__ __forceinline__ void nn_argmin(double& v, int& i) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off, 64);
+        const int oi = __shfl_xor(i, off, 64);
+        if (oi >= 0 && (i < 0 || ov < v || (ov == v && oi < i))) {
+            v = ov;
+            i = oi;
+        }
+    }
+}
+
+__device__ __forceinline__ double nn_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// z = M_PP^-1 h_P (0 outside P) by Cholesky; false if a pivot is not positive. The factor is built column by
+// column (left-looking): lane i forms C[i][k] = (M[i][k] - sum_p<k C[i][p] C[k][p]) / C[k][k] as a dot product
+// of reads only, the same operations in the same order as the column-by-column update.
+__device__ bool nn_passive_solve(const NnLds& L, int n, int lane) {
+    double* C = L.C;
+    const double* M = L.M;
     for (int i = lane; i < n; i += 64) L.z[i] = L.P[i] ? L.h[i] : 0.0;
-    __syncthreads();
     bool ok = true;
     for (int k = 0; k < n; k++) {
-        const double d2 = C[k * n + k];
+        const bool pk = L.P[k];
+        const double* Ck = C + (size_t)k * n;
+        double d2 = pk ? M[(size_t)k * n + k] : 1.0;
+        for (int p = 0; p < k; p++) d2 -= Ck[p] * Ck[p];
         if (!(d2 > 0.0)) ok = false;
         const double d = sqrt(d2 > 0.0 ? d2 : 1.0);
-        __syncthreads();
-        for (int i = k + 1 + lane; i < n; i += 64) C[i * n + k] /= d;
-        if (lane == 0) C[k * n + k] = d;
-        __syncthreads();
-        const int r = n - k - 1;
-        for (int e = lane; e < r * r; e += 64) {
-            const int i = k + 1 + e / r, j = k + 1 + e % r;
-            if (j <= i) C[i * n + j] -= C[i * n + k] * C[j * n + k];
+        for (int i = k + 1 + lane; i < n; i += 64) {
+            double* Ci = C + (size_t)i * n;
+            double s = (pk && L.P[i]) ? M[(size_t)i * n + k] : 0.0;
+            for (int p = 0; p < k; p++) s -= Ci[p] * Ck[p];
+            Ci[k] = s / d;
         }
+        if (lane == 0) C[(size_t)k * n + k] = d;
         __syncthreads();
     }
     // forward: C y = z
@@ -212,7 +240,7 @@ __device__ bool nn_passive_solve(const double* __restrict__ M, const NnLds& L, i
 // Lawson-Hanson active set: returns false if the iteration limits were reached or a pivot failed.
 // warm (optional, n): a guess of the positive set. If the passive solution on it is feasible the active set
 // starts there (a few iterations when the guess is the solution of a nearby problem), else from x = 0.
-__device__ bool nn_solve(const double* __restrict__ M, const NnLds& L, int n, int lane, const double* warm = nullptr) {
+__device__ bool nn_solve(const NnLds& L, int n, int lane, const double* warm = nullptr) {
     for (int i = lane; i < n; i += 64) {
         L.x[i] = 0.0;
         L.P[i] = warm ? (warm[i] > 0.0) : 0;
@@ -220,15 +248,11 @@ __device__ bool nn_solve(const double* __restrict__ M, const NnLds& L, int n, in
     }
     __syncthreads();
     if (warm) {
-        bool ok = nn_passive_solve(M, L, n, lane);
-        if (lane == 0) {
-            int feas = ok;
-            for (int i = 0; i < n && feas; i++)
-                if (L.P[i] && !(L.z[i] > 0.0)) feas = 0;
-            L.sh[1] = feas;
-        }
-        __syncthreads();
-        const int feas = L.sh[1];
+        const bool ok = nn_passive_solve(L, n, lane);
+        int bad = 0;
+        for (int i = lane; i < n; i += 64)
+            if (L.P[i] && !(L.z[i] > 0.0)) bad = 1;
+        const bool feas = ok && __ballot(bad) == 0;
         for (int i = lane; i < n; i += 64) {
             if (feas) L.x[i] = L.z[i];
             else L.P[i] = 0;
@@ -239,79 +263,78 @@ __device__ bool nn_solve(const double* __restrict__ M, const NnLds& L, int n, in
     int it = 0;
     for (; it < maxit; it++) {
         // gradient h - M x and the size of its round-off
+        double amax = 0.0;
         for (int i = lane; i < n; i += 64) {
             double s = L.h[i], a = fabs(L.h[i]);
-            const double* Mi = M + (size_t)i * n;
+            const double* Mi = L.M + (size_t)i * n;
             for (int j = 0; j < n; j++) {
                 const double t = Mi[j] * L.x[j];
                 s -= t;
                 a += fabs(t);
             }
             L.g[i] = s;
-            L.shd[1 + i] = a;
+            amax = fmax(amax, a);
         }
+        amax = nn_max(amax);
+        // the entering index: the largest gradient above its round-off among the free, unbanned indices
+        double gv = 0.0;
+        int best = -1;
+        const double gb = 64.0 * (double)n * 2.220446049250313e-16 * amax;
+        for (int i = lane; i < n; i += 64)
+            if (!L.P[i] && !L.ban[i] && L.g[i] > gb && (best < 0 || L.g[i] > gv)) {
+                gv = L.g[i];
+                best = i;
+            }
+        nn_argmax(gv, best);
+        if (best < 0) break;
+        const int j = best;
+        if (lane == (j & 63)) L.P[j] = 1;
         __syncthreads();
-        if (lane == 0) {
-            double amax = 0.0;
-            for (int i = 0; i < n; i++) amax = fmax(amax, L.shd[1 + i]);
-            double gb = 64.0 * (double)n * 2.220446049250313e-16 * amax;
-            int best = -1;
-            for (int i = 0; i < n; i++)
-                if (!L.P[i] && !L.ban[i] && L.g[i] > gb) {
-                    gb = L.g[i];
-                    best = i;
-                }
-            L.sh[0] = best;
-            if (best >= 0) L.P[best] = 1;
-        }
-        __syncthreads();
-        const int j = L.sh[0];
-        if (j < 0) break;
         for (int inner = 0;; inner++) {
             if (inner >= maxit) return false;
-            if (!nn_passive_solve(M, L, n, lane)) return false;
-            if (lane == 0) {
-                int state;          // 0: feasible, 1: the new index cannot enter, 2: step to the boundary
-                if (inner == 0 && !(L.z[j] > 0.0)) {
+            if (!nn_passive_solve(L, n, lane)) return false;
+            if (inner == 0 && !(L.z[j] > 0.0)) {
+                // the new index cannot enter
+                if (lane == (j & 63)) {
                     L.P[j] = 0;
                     L.ban[j] = 1;
-                    state = 1;
-                } else {
-                    double alpha = 2.0;
-                    int imin = -1;
-                    for (int i = 0; i < n; i++)
-                        if (L.P[i] && !(L.z[i] > 0.0)) {
-                            const double a = L.x[i] / (L.x[i] - L.z[i]);
-                            if (imin < 0 || a < alpha) {
-                                alpha = a;
-                                imin = i;
-                            }
-                        }
-                    if (imin < 0) {
-                        for (int i = 0; i < n; i++) {
-                            L.x[i] = L.z[i];
-                            L.ban[i] = 0;
-                        }
-                        state = 0;
-                    } else {
-                        for (int i = 0; i < n; i++) {
-                            L.ban[i] = 0;
-                            if (!L.P[i]) continue;
-                            const double xi = L.x[i] + alpha * (L.z[i] - L.x[i]);
-                            if (i == imin || !(xi > 0.0)) {
-                                L.x[i] = 0.0;
-                                L.P[i] = 0;
-                            } else {
-                                L.x[i] = xi;
-                            }
-                        }
-                        state = 2;
+                }
+                __syncthreads();
+                break;
+            }
+            // the step to the boundary: the first passive index with z <= 0 and the smallest x / (x - z)
+            double av = 2.0;
+            int imin = -1;
+            for (int i = lane; i < n; i += 64)
+                if (L.P[i] && !(L.z[i] > 0.0)) {
+                    const double a = L.x[i] / (L.x[i] - L.z[i]);
+                    if (imin < 0 || a < av) {
+                        av = a;
+                        imin = i;
                     }
                 }
-                L.sh[1] = state;
+            nn_argmin(av, imin);
+            if (imin < 0) {
+                for (int i = lane; i < n; i += 64) {
+                    L.x[i] = L.z[i];
+                    L.ban[i] = 0;
+                }
+                __syncthreads();
+                break;
+            }
+            const double alpha = av;
+            for (int i = lane; i < n; i += 64) {
+                L.ban[i] = 0;
+                if (!L.P[i]) continue;
+                const double xi = L.x[i] + alpha * (L.z[i] - L.x[i]);
+                if (i == imin || !(xi > 0.0)) {
+                    L.x[i] = 0.0;
+                    L.P[i] = 0;
+                } else {
+                    L.x[i] = xi;
+                }
             }
             __syncthreads();
-            if (L.sh[1] != 2) break;
         }
     }
     return it < maxit;
@@ -345,32 +368,32 @@ k_nnls(ProfConst pc, const double* __restrict__ gram, const double* __restrict__
        const int* __restrict__ near, double* __restrict__ Mall, double* __restrict__ xout) {
     extern __shared__ double nn_lds[];
     const int n = pc.n, m = n + 1, lane = threadIdx.x, b = blockIdx.x;
+    (void)Mall;
     NnLds L;
-    L.C = nn_lds;
+    L.M = nn_lds;
+    L.C = L.M + n * n;
     L.z = L.C + n * n;
     L.x = L.z + n;
     L.g = L.x + n;
     L.h = L.g + n;
-    L.shd = L.h + n;                 // 1 + n doubles
-    double* w2 = L.shd + 1 + n;      // n doubles
+    double* w2 = L.h + n;            // n doubles
     double* x0 = w2 + n;             // n doubles
     L.P = (int*)(x0 + n);
     L.ban = L.P + n;
-    L.sh = L.ban + n;
     const double* G = gram + (size_t)b * m * m;
-    double* M = Mall + (size_t)b * n * n;
+    double* M = L.M;
     for (int i = lane; i < n; i += 64) L.h[i] = G[i * m + n];
     __syncthreads();
     bool ok = true;
     if (!(pc.smooth > 0.0) || n < 3) {
         nn_build(M, G, n, lcoef, lcol, nullptr, 0.0, 0, lane);
-        ok = nn_solve(M, L, n, lane);
+        ok = nn_solve(L, n, lane);
     } else {
         double tr = 0.0;
         for (int i = 0; i < n; i++) tr += G[i * m + i];
         const double srel = sqrt(0.1 * tr / fmax(pc.norm, 1e-300));
         nn_build(M, G, n, lcoef, lcol, nullptr, srel * srel, n - 2, lane);
-        ok = nn_solve(M, L, n, lane);
+        ok = nn_solve(L, n, lane);
         for (int i = lane; i < n; i += 64) x0[i] = L.x[i];
         __syncthreads();
         double xm = 0.0;
@@ -400,7 +423,7 @@ k_nnls(ProfConst pc, const double* __restrict__ gram, const double* __restrict__
             }
             __syncthreads();
             nn_build(M, G, n, lcoef, lcol, w2, 0.0, n - 2, lane);
-            ok = nn_solve(M, L, n, lane, x0);
+            ok = nn_solve(L, n, lane, x0);
         }
     }
     for (int i = lane; i < n; i += 64) xout[(size_t)b * n + i] = ok ? L.x[i] : __builtin_nan("");
@@ -444,7 +467,7 @@ extern "C" int bb_launch_rm_finish(const unsigned long long* counts, double* map
 
 extern "C" int bb_gram_blocks(int npix) { return (npix + 2047) / 2048; }
 extern "C" size_t bb_gram_entries(int n) { return (size_t)(n + 1) * (n + 2) / 2; }
-extern "C" size_t bb_nnls_lds(int n) { return (size_t)(n * n + 8 * n + 1) * sizeof(double) + 3 * n * sizeof(int) + 16; }
+extern "C" size_t bb_nnls_lds(int n) { return (size_t)(2 * n * n + 6 * n) * sizeof(double) + 2 * n * sizeof(int) + 16; }
 extern "C" int bb_nnls_max() { return BB_NN_MAX; }
 
 extern "C" int bb_launch_profile_solve(int n, int nr, int nok, double smooth, double corr, double floor_, double norm,
