@@ -145,6 +145,7 @@ class GpuLikelihood:
         self.nvox_box = int(np.prod(dbox.shape))
         self.plan = SubringPlan(dm.setup, self.rings0, dm.ringmode, final=True)
         self._bufs = {}
+        self._streams = None
         # DENS not fitted: the deconvolved profile of the initial rings, and of every parameter vector when the
         # geometry is free (fitter.py:606-615)
         self.useNorm = "dens" not in free.index
@@ -173,8 +174,9 @@ class GpuLikelihood:
             self.dens0 = dens
             self.rings0.dens = dens.copy()
 
-    def _buffers(self, B):
-        b = self._bufs.get(B)
+    def _buffers(self, B, slot=0):
+        """Device buffers of a chunk of B vectors; slot: an independent set (chunks in flight on two streams)."""
+        b = self._bufs.get((B, slot))
         if b is None:
             torch = self.torch
             by, bx, nz = self.bhi[1] - self.blo[1], self.bhi[0] - self.blo[0], self.dm.setup.nsubs
@@ -185,8 +187,14 @@ class GpuLikelihood:
                      work=torch.empty(wl, dtype=torch.float64, device=self.dev),
                      ss=torch.empty(B, dtype=torch.float64, device=self.dev),
                      lg=torch.empty(B, dtype=torch.float64, device=self.dev), wl=wl)
-            self._bufs = {B: b}
+            self._bufs = {k: v for k, v in self._bufs.items() if k[0] == B}
+            self._bufs[(B, slot)] = b
         return b
+
+    def _stream_pair(self):
+        if self._streams is None:
+            self._streams = (self.torch.cuda.Stream(), self.torch.cuda.Stream())     # (default priority)
+        return self._streams
 
     def tables(self, theta):
         """(sr (B, 16, nsub), values) of the rows of theta: the sub-ring tables of the model (density row 1 when
@@ -199,16 +207,22 @@ class GpuLikelihood:
             sr[:, 5, :] = 1.
         return sr, vals
 
-    def residuals(self, sr, vals=None):
-        """(S_box, lg) per table: device launches for one chunk of sub-ring tables (B, 16, nsub); with a
-        per-vector profile, vals are the ring values of the free keys (for the annulus)."""
-        torch = self.torch
+    def _device_part(self, sr, d_sr, vals, slot, sr_ext=None, d_sr_ext=None, after_maps=None):
+        """Launch one chunk on the current stream with the buffers of slot: (ss, lg, dens) device tensors.
+        after_maps: an event recorded once the chunk's ring maps are launched (or, without a profile, at once)."""
         B = sr.shape[0]
-        bufs = self._buffers(B)
-        d_sr = torch.from_numpy(np.ascontiguousarray(sr)).to(self.dev)
+        bufs = self._buffers(B, slot)
         dens = None
         if self.update_prof:
-            dens = self.profile.run(sr, d_sr, vals if vals is not None else {})
+            prof = self.profile
+            if sr_ext is None:
+                sr_ext = prof.ext_tables(vals if vals is not None else {}, B)
+            maps = prof.maps(sr, d_sr, sr_ext, None, d_sr_ext, slot)
+            if after_maps is not None:
+                after_maps.record()
+            dens = prof.set_dens(prof.solve(maps, None, slot), d_sr, None, slot)
+        elif after_maps is not None:
+            after_maps.record()
         if self.dm.setup.galmodtype == "MC":
             tab = sr
             if dens is not None:
@@ -221,14 +235,67 @@ class GpuLikelihood:
         else:
             mod = self.dm.array_batch(sr, self.blo, self.bhi, convolve=True, out=bufs["mod"], work=bufs["dep"],
                                       d_sr=d_sr)
-        st = _stream_ptr(torch)
         s2 = self.sigma ** 2
         _lib.check(_lib.lib().bb_residual_gauss(mod.data_ptr(), self.dvalid.data_ptr(), B, self.nvox_box, s2,
                                                 self.eps ** 2, bufs["work"].data_ptr(), bufs["wl"],
-                                                bufs["ss"].data_ptr(), bufs["lg"].data_ptr(), st))
-        ss, lg = bufs["ss"].cpu().numpy().copy(), bufs["lg"].cpu().numpy().copy()
-        if dens is not None:
-            self.last_dens = dens.cpu().numpy().copy()
+                                                bufs["ss"].data_ptr(), bufs["lg"].data_ptr(),
+                                                _stream_ptr(self.torch)))
+        return bufs["ss"], bufs["lg"], dens
+
+    def device_batch(self, parts):
+        """Launch the chunks parts (dicts: sr, d_sr, vals, optionally sr_ext, d_sr_ext: host / device sub-ring
+        tables) on two streams in turn, so that the latency-bound kernels of one chunk (the profile's least
+        squares) run beside the deposit of the other; the current stream waits for all of them. Returns the
+        (ss, lg, dens) device tensors of every chunk (valid once the current stream reaches this point)."""
+        torch = self.torch
+        cur = torch.cuda.current_stream()
+        streams = self._stream_pair() if len(parts) > 1 else (cur,)
+        out = []
+        prev = None
+        for i, p in enumerate(parts):
+            s = streams[i % len(streams)]
+            if s is not cur:
+                s.wait_stream(cur)
+            if prev is not None:
+                # (the ring maps of this chunk start when those of the previous one are launched: its least
+                # squares, latency-bound, then run beside them)
+                s.wait_event(prev)
+            ev = torch.cuda.Event() if len(parts) > 1 else None
+            with torch.cuda.stream(s):
+                out.append(self._device_part(p["sr"], p["d_sr"], p.get("vals"), i % 2, p.get("sr_ext"),
+                                             p.get("d_sr_ext"), ev))
+            prev = ev
+        for s in streams:
+            if s is not cur:
+                cur.wait_stream(s)
+        return out
+
+    def split_parts(self, sr, vals):
+        """The chunk (sr, vals) as two halves with their device tables (one part below 64 vectors)."""
+        torch = self.torch
+        B = sr.shape[0]
+        cuts = [0, B] if B < 64 else [0, B // 2, B]
+        parts = []
+        for a, b in zip(cuts[:-1], cuts[1:]):
+            p = dict(sr=sr[a:b], d_sr=torch.from_numpy(np.ascontiguousarray(sr[a:b])).to(self.dev), vals=vals)
+            if self.update_prof:
+                # (the values of the free keys are per vector, (B, nr))
+                v = {k: np.asarray(x)[a:b] for k, x in (vals or {}).items()}
+                p["vals"] = v
+                p["sr_ext"] = self.profile.ext_tables(v, b - a)
+                p["d_sr_ext"] = torch.from_numpy(np.ascontiguousarray(p["sr_ext"])).to(self.dev)
+            parts.append(p)
+        return parts
+
+    def residuals(self, sr, vals=None):
+        """(S_box, lg) per table: device launches for one chunk of sub-ring tables (B, 16, nsub), in two halves on
+        two streams; with a per-vector profile, vals are the ring values of the free keys (for the annulus)."""
+        res = self.device_batch(self.split_parts(sr, vals))
+        self.torch.cuda.current_stream().synchronize()
+        ss = np.concatenate([r[0].cpu().numpy() for r in res])
+        lg = np.concatenate([r[1].cpu().numpy() for r in res])
+        if res[0][2] is not None:
+            self.last_dens = np.concatenate([r[2].cpu().numpy() for r in res])
             if not np.all(np.isfinite(self.last_dens)):
                 raise RuntimeError("the density profile did not converge (non-negative least squares)")
         return ss, lg

@@ -163,8 +163,9 @@ class DensityProfile:
         """The values of the annulus per parameter vector: those of the last ring (dict key -> (B, 1))."""
         return {k: np.asarray(v)[..., -1:] for k, v in values.items() if k in ANNULUS_KEYS}
 
-    def _buffers(self, B, nsub_e):
-        b = self._bufs.get(B)
+    def _buffers(self, B, nsub_e, slot=0):
+        """Device buffers of a batch of B vectors; slot: an independent set (batches in flight on two streams)."""
+        b = self._bufs.get((B, slot))
         if b is None:
             torch = self.torch
             L = _lib.lib()
@@ -175,7 +176,8 @@ class DensityProfile:
                      dens=torch.empty((B, self.nr), dtype=torch.float32, device=self.dev),
                      work=torch.empty(int(L.bb_profile_work_size(self.handle, B)), dtype=torch.uint8,
                                       device=self.dev), P=P)
-            self._bufs = {B: b}
+            self._bufs = {k: v for k, v in self._bufs.items() if k[0] == B}
+            self._bufs[(B, slot)] = b
         return b
 
     def ext_tables(self, values, B):
@@ -185,14 +187,14 @@ class DensityProfile:
             sr_ext = np.ascontiguousarray(np.broadcast_to(sr_ext, (B,) + sr_ext.shape[1:]))
         return sr_ext
 
-    def maps(self, sr, d_sr, sr_ext, stream=None, d_sr_ext=None):
+    def maps(self, sr, d_sr, sr_ext, stream=None, d_sr_ext=None, slot=0):
         """Device ring maps (B, nr+1, by, bx) float64 of B parameter vectors: sr / d_sr the host / device sub-ring
         tables of the model (their density row is not read), sr_ext (d_sr_ext) the host (device) tables of the
         annulus."""
         torch = self.torch
         L = _lib.lib()
         B = sr.shape[0]
-        b = self._buffers(B, sr_ext.shape[2])
+        b = self._buffers(B, sr_ext.shape[2], slot)
         st = _stream_ptr(torch, stream)
         box = _lib.bb_box(self.blo[0], self.blo[1], self.bhi[0], self.bhi[1])
         for tab, dtab, widx, wval, map0, nw in ((sr, d_sr, self.d_widx, self.d_wval, 0, self.nr),
@@ -213,30 +215,31 @@ class DensityProfile:
                                          int(self.dm.cfield is not None), b["maps"].data_ptr(), st))
         return b["maps"]
 
-    def solve(self, maps, stream=None):
+    def solve(self, maps, stream=None, slot=0):
         """x (B, nr+1) float64 device tensor: the profile of the rings and the annulus from the maps."""
         torch = self.torch
         B = maps.shape[0]
-        b = self._buffers(B, 0)
+        b = self._buffers(B, 0, slot)
         st = _stream_ptr(torch, stream)
         _lib.check(_lib.lib().bb_profile_solve(self.handle, maps.data_ptr(), B, b["work"].data_ptr(),
                                                b["work"].numel(), b["x"].data_ptr(), st))
         return b["x"]
 
-    def set_dens(self, x, d_sr, stream=None):
+    def set_dens(self, x, d_sr, stream=None, slot=0):
         """float32 densities of the solutions x into row 5 of the device tables d_sr (B, 16, nsub) of the model;
         returns them (B, nr) (device)."""
         B = x.shape[0]
-        b = self._buffers(B, 0)
+        b = self._buffers(B, 0, slot)
         _lib.check(_lib.lib().bb_profile_set_dens(x.data_ptr(), B, self.n, self.nr, self.d_widx.data_ptr(),
                                                   self.d_wval.data_ptr(), d_sr.shape[2], d_sr.data_ptr(),
                                                   b["dens"].data_ptr(), _stream_ptr(self.torch, stream)))
         return b["dens"]
 
-    def run(self, sr, d_sr, values, stream=None):
+    def run(self, sr, d_sr, values, stream=None, slot=0, d_sr_ext=None, sr_ext=None):
         """The profile of B parameter vectors written into row 5 of the device tables d_sr (B, 16, nsub) of the
         model; returns the float32 densities (B, nr) (device)."""
-        sr_ext = self.ext_tables(values, sr.shape[0])
-        maps = self.maps(sr, d_sr, sr_ext, stream)
-        x = self.solve(maps, stream)
-        return self.set_dens(x, d_sr, stream)
+        if sr_ext is None:
+            sr_ext = self.ext_tables(values, sr.shape[0])
+        maps = self.maps(sr, d_sr, sr_ext, stream, d_sr_ext, slot)
+        x = self.solve(maps, stream, slot)
+        return self.set_dens(x, d_sr, stream, slot)
