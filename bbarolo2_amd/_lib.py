@@ -73,6 +73,14 @@ _SIGS = {
     "bb_profile_work_size": (C.c_int64, [_P, C.c_int]),
     "bb_profile_solve": (C.c_int, [_P, _P, C.c_int, _P, C.c_int64, _P, _P]),
     "bb_profile_set_dens": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P]),
+    "bb_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_P)]),
+    "bb_mlp_destroy": (C.c_int, [_P]),
+    "bb_mlp_param_count": (C.c_int64, [_P]),
+    "bb_mlp_set_params": (C.c_int, [_P, _P]),
+    "bb_mlp_get_params": (C.c_int, [_P, _P]),
+    "bb_mlp_forward": (C.c_int, [_P, _P, C.c_int, _P, _P]),
+    "bb_mlp_grad": (C.c_int, [_P, _P, _P, C.c_int, _P, _P, _P]),
+    "bb_mlp_train": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_float, _P, _P]),
 }
 
 _lib = None

@@ -649,3 +649,232 @@ extern "C" int bb_norm3dfit(const float* model, int B, int by, int bx, int nz, c
                             (const int*)ilo, (const int*)ihi, ky, thr, work, value, nblank, ntot, (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_norm3dfit launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
+
+// ---------------------------------------------------------------------------------------------------------
+// The likelihood surrogate (bb_mlp.hip): a din -> H (SiLU) x nh -> 1 network with FP32 master parameters, their
+// BF16 copies, Adam moments and the activations of up to nmax rows, all on the device that created it.
+extern "C" int bb_launch_mlp_gemm(int epi, const unsigned short* A, const unsigned short* B, int M, int N, int K,
+                                  const float* bias, float* Z, unsigned short* Ab, unsigned short* AbT, float* G,
+                                  const float* Zp, int nvalid, hipStream_t st);
+extern "C" int bb_launch_mlp_in(const float* x, int n, int din, int np, int dp, unsigned short* X, unsigned short* XT,
+                                hipStream_t st);
+extern "C" int bb_launch_mlp_out(const unsigned short* a, const unsigned short* w, const float* b, int np, int H,
+                                 float* y, hipStream_t st);
+extern "C" int bb_launch_mlp_out_bwd(const float* y, const float* t, int n, int np, int H, const unsigned short* w,
+                                     const float* Z, unsigned short* D, unsigned short* DT, hipStream_t st);
+extern "C" int bb_launch_mlp_out_grad(const float* y, const float* t, int n, const unsigned short* a, int H, float* gw,
+                                      float* gb, float* loss, hipStream_t st);
+extern "C" int bb_launch_mlp_bias_grad(const unsigned short* DT, int np, int H, float* g, hipStream_t st);
+extern "C" int bb_launch_mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                                  float eps, float c1, float c2, hipStream_t st);
+extern "C" int bb_launch_mlp_cast(const float* W, int rows, int cols, unsigned short* Wb, unsigned short* WbT,
+                                  hipStream_t st);
+
+#define BB_MLP_DP 32          // padded input width (one MFMA K step)
+
+struct bb_mlp {
+    int device;
+    int din, H, nh, nmax;     // nmax: rows, a multiple of 32
+    int64_t np_;              // parameters
+    float* P;                 // FP32 master parameters: per hidden layer W (H x K) then b (H); then w (H), b (1)
+    float* Gr;                // gradient
+    float* Mo;                // Adam moments
+    float* Ve;
+    unsigned short* Wb;       // BF16 weights: per layer (H x K) and its transpose (K x H); then w (H)
+    unsigned short *X, *XT;   // inputs (nmax x DP), (DP x nmax)
+    float* Z;                 // pre-activations, per layer (nmax x H)
+    unsigned short *A, *AT;   // activations, per layer (nmax x H), (H x nmax)
+    unsigned short *D, *DT;   // layer gradients, two (nmax x H) + (H x nmax) sets
+    float *y, *t;             // outputs and targets (nmax)
+    int64_t adam_t;           // Adam steps taken
+};
+
+static int64_t mlp_K(const bb_mlp* n, int l) { return l == 0 ? BB_MLP_DP : n->H; }
+static int64_t mlp_woff(const bb_mlp* n, int l) {       // offset of W_l in P (b_l follows it)
+    int64_t o = 0;
+    for (int i = 0; i < l; i++) o += (int64_t)n->H * mlp_K(n, i) + n->H;
+    return o;
+}
+static int64_t mlp_bwoff(const bb_mlp* n, int l) {      // offset of W_l (then its transpose) in Wb
+    int64_t o = 0;
+    for (int i = 0; i < l; i++) o += 2 * (int64_t)n->H * mlp_K(n, i);
+    return o;
+}
+
+extern "C" int bb_mlp_destroy(bb_mlp* n) {
+    if (!n) return 0;
+    void* ptrs[] = {n->P, n->Gr, n->Mo, n->Ve, n->Wb, n->X, n->XT, n->Z, n->A, n->AT, n->D, n->DT, n->y, n->t};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete n;
+    return 0;
+}
+
+extern "C" int bb_mlp_create(int din, int hidden, int nhidden, int nmax, bb_mlp** out) {
+    if (!out) return fail(BB_ERR_ARG, "bb_mlp_create: out is NULL");
+    *out = nullptr;
+    if (din < 1 || din > BB_MLP_DP) return fail(BB_ERR_ARG, "bb_mlp_create: din = %d (1..%d)", din, BB_MLP_DP);
+    if (hidden < 32 || hidden > 512 || hidden % 32) return fail(BB_ERR_ARG, "bb_mlp_create: hidden = %d", hidden);
+    if (nhidden < 1 || nhidden > 16) return fail(BB_ERR_ARG, "bb_mlp_create: nhidden = %d", nhidden);
+    if (nmax < 1 || nmax > (1 << 22)) return fail(BB_ERR_ARG, "bb_mlp_create: nmax = %d", nmax);
+    bb_mlp* n = new bb_mlp();
+    memset(n, 0, sizeof *n);
+    (void)hipGetDevice(&n->device);
+    n->din = din;
+    n->H = hidden;
+    n->nh = nhidden;
+    n->nmax = (nmax + 31) / 32 * 32;
+    n->np_ = mlp_woff(n, nhidden) + hidden + 1;
+    const int64_t nw = mlp_bwoff(n, nhidden) + hidden;
+    const int64_t R = n->nmax, H = hidden;
+    int rc = 0;
+    auto alloc = [&](void** p, size_t bytes, const char* what) {
+        if (!rc) rc = hip_check(hipMalloc(p, bytes), what);
+        if (!rc) rc = hip_check(hipMemset(*p, 0, bytes), what);
+    };
+    alloc((void**)&n->P, n->np_ * sizeof(float), "bb_mlp_create");
+    alloc((void**)&n->Gr, n->np_ * sizeof(float), "bb_mlp_create");
+    alloc((void**)&n->Mo, n->np_ * sizeof(float), "bb_mlp_create");
+    alloc((void**)&n->Ve, n->np_ * sizeof(float), "bb_mlp_create");
+    alloc((void**)&n->Wb, nw * 2, "bb_mlp_create");
+    alloc((void**)&n->X, R * BB_MLP_DP * 2, "bb_mlp_create");
+    alloc((void**)&n->XT, R * BB_MLP_DP * 2, "bb_mlp_create");
+    alloc((void**)&n->Z, R * H * nhidden * sizeof(float), "bb_mlp_create");
+    alloc((void**)&n->A, R * H * nhidden * 2, "bb_mlp_create");
+    alloc((void**)&n->AT, R * H * nhidden * 2, "bb_mlp_create");
+    alloc((void**)&n->D, R * H * 2 * 2, "bb_mlp_create");
+    alloc((void**)&n->DT, R * H * 2 * 2, "bb_mlp_create");
+    alloc((void**)&n->y, R * sizeof(float), "bb_mlp_create");
+    alloc((void**)&n->t, R * sizeof(float), "bb_mlp_create");
+    if (rc) {
+        bb_mlp_destroy(n);
+        return rc;
+    }
+    *out = n;
+    return 0;
+}
+
+extern "C" int64_t bb_mlp_param_count(const bb_mlp* n) { return n ? n->np_ : -1; }
+
+static int mlp_cast(bb_mlp* n, hipStream_t st) {
+    int rc = 0;
+    for (int l = 0; l < n->nh && !rc; l++) {
+        const int K = (int)mlp_K(n, l);
+        unsigned short* wb = n->Wb + mlp_bwoff(n, l);
+        rc = bb_launch_mlp_cast(n->P + mlp_woff(n, l), n->H, K, wb, wb + (int64_t)n->H * K, st);
+    }
+    if (!rc) rc = bb_launch_mlp_cast(n->P + mlp_woff(n, n->nh), 1, n->H, n->Wb + mlp_bwoff(n, n->nh), nullptr, st);
+    return rc;
+}
+
+extern "C" int bb_mlp_set_params(bb_mlp* n, const float* host) {
+    if (!n || !host) return fail(BB_ERR_ARG, "bb_mlp_set_params: NULL argument");
+    int rc = hip_check(hipMemcpy(n->P, host, n->np_ * sizeof(float), hipMemcpyHostToDevice), "bb_mlp_set_params");
+    if (!rc) rc = hip_check(hipMemset(n->Mo, 0, n->np_ * sizeof(float)), "bb_mlp_set_params");
+    if (!rc) rc = hip_check(hipMemset(n->Ve, 0, n->np_ * sizeof(float)), "bb_mlp_set_params");
+    n->adam_t = 0;
+    if (!rc) rc = hip_check((hipError_t)mlp_cast(n, nullptr), "bb_mlp_set_params");
+    if (!rc) rc = hip_check(hipDeviceSynchronize(), "bb_mlp_set_params");
+    return rc;
+}
+
+extern "C" int bb_mlp_get_params(const bb_mlp* n, float* host) {
+    if (!n || !host) return fail(BB_ERR_ARG, "bb_mlp_get_params: NULL argument");
+    return hip_check(hipMemcpy(host, n->P, n->np_ * sizeof(float), hipMemcpyDeviceToHost), "bb_mlp_get_params");
+}
+
+// forward of rows [0, n) of x (device, n x din): activations kept for a backward pass; y (device, n) if given
+static int mlp_forward(bb_mlp* n, const float* x, int rows, hipStream_t st) {
+    const int R = (rows + 31) / 32 * 32, H = n->H;
+    int rc = bb_launch_mlp_in(x, rows, n->din, R, BB_MLP_DP, n->X, n->XT, st);
+    for (int l = 0; l < n->nh && !rc; l++) {
+        const unsigned short* in = l == 0 ? n->X : n->A + (int64_t)(l - 1) * n->nmax * H;
+        const int K = (int)mlp_K(n, l);
+        rc = bb_launch_mlp_gemm(0, in, n->Wb + mlp_bwoff(n, l), R, H, K, n->P + mlp_woff(n, l) + (int64_t)H * K,
+                                n->Z + (int64_t)l * n->nmax * H, n->A + (int64_t)l * n->nmax * H,
+                                n->AT + (int64_t)l * n->nmax * H, nullptr, nullptr, rows, st);
+    }
+    if (!rc)
+        rc = bb_launch_mlp_out(n->A + (int64_t)(n->nh - 1) * n->nmax * H, n->Wb + mlp_bwoff(n, n->nh),
+                               n->P + mlp_woff(n, n->nh) + H, R, H, n->y, st);
+    return rc;
+}
+
+// gradient of 0.5 mean (y - t)^2 over rows [0, n) into n->Gr (after mlp_forward of the same rows); loss (device)
+static int mlp_backward(bb_mlp* n, const float* t, int rows, float* loss, hipStream_t st) {
+    const int R = (rows + 31) / 32 * 32, H = n->H, L = n->nh;
+    const int64_t lay = (int64_t)n->nmax * H;
+    float* gout = n->Gr + mlp_woff(n, L);
+    int rc = bb_launch_mlp_out_grad(n->y, t, rows, n->A + (L - 1) * lay, H, gout, gout + H, loss, st);
+    if (!rc)
+        rc = bb_launch_mlp_out_bwd(n->y, t, rows, R, H, n->Wb + mlp_bwoff(n, L), n->Z + (L - 1) * lay, n->D, n->DT,
+                                   st);
+    int cur = 0;
+    for (int l = L - 1; l >= 0 && !rc; l--) {
+        const int K = (int)mlp_K(n, l);
+        unsigned short* D = n->D + cur * lay;
+        unsigned short* DT = n->DT + cur * lay;
+        const unsigned short* inT = l == 0 ? n->XT : n->AT + (l - 1) * lay;
+        float* gw = n->Gr + mlp_woff(n, l);
+        // dW (H x K) = D^T (H x R) . in (R x K): rows of D^T and of in^T, contiguous along the R samples
+        rc = bb_launch_mlp_gemm(1, DT, inT, H, K, R, nullptr, nullptr, nullptr, nullptr, gw, nullptr, rows, st);
+        if (!rc) rc = bb_launch_mlp_bias_grad(DT, R, H, gw + (int64_t)H * K, st);
+        if (!rc && l > 0) {
+            // the gradient of layer l - 1: (D (R x H) . W (H x K)) * silu'(Z_{l-1}), W^T rows contiguous along H
+            const unsigned short* WT = n->Wb + mlp_bwoff(n, l) + (int64_t)H * K;
+            rc = bb_launch_mlp_gemm(2, D, WT, R, K, H, nullptr, nullptr, n->D + (1 - cur) * lay,
+                                    n->DT + (1 - cur) * lay, nullptr, n->Z + (l - 1) * lay, rows, st);
+            cur = 1 - cur;
+        }
+    }
+    return rc;
+}
+
+static int mlp_check_rows(const bb_mlp* n, const void* x, int rows, const char* what) {
+    if (!n || !x) return fail(BB_ERR_ARG, "%s: NULL argument", what);
+    if (rows < 1 || rows > n->nmax) return fail(BB_ERR_ARG, "%s: %d rows (1..%d)", what, rows, n->nmax);
+    int dev = -1;
+    (void)hipGetDevice(&dev);
+    if (dev != n->device) return fail(BB_ERR_ARG, "%s: the network lives on device %d, not %d", what, n->device, dev);
+    return 0;
+}
+
+extern "C" int bb_mlp_forward(bb_mlp* n, const float* x, int rows, float* y, void* stream) {
+    int rc = mlp_check_rows(n, x, rows, "bb_mlp_forward");
+    if (rc) return rc;
+    if (!y) return fail(BB_ERR_ARG, "bb_mlp_forward: y is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    rc = mlp_forward(n, x, rows, st);
+    if (!rc) rc = (int)hipMemcpyAsync(y, n->y, (size_t)rows * sizeof(float), hipMemcpyDeviceToDevice, st);
+    return rc ? fail(BB_ERR_HIP, "bb_mlp_forward: %s", hipGetErrorString((hipError_t)rc)) : 0;
+}
+
+extern "C" int bb_mlp_grad(bb_mlp* n, const float* x, const float* t, int rows, float* grad, float* loss, void* stream) {
+    int rc = mlp_check_rows(n, x, rows, "bb_mlp_grad");
+    if (rc) return rc;
+    if (!t || !grad) return fail(BB_ERR_ARG, "bb_mlp_grad: NULL argument");
+    hipStream_t st = (hipStream_t)stream;
+    rc = mlp_forward(n, x, rows, st);
+    if (!rc) rc = mlp_backward(n, t, rows, loss, st);
+    if (!rc) rc = (int)hipMemcpyAsync(grad, n->Gr, n->np_ * sizeof(float), hipMemcpyDeviceToDevice, st);
+    return rc ? fail(BB_ERR_HIP, "bb_mlp_grad: %s", hipGetErrorString((hipError_t)rc)) : 0;
+}
+
+extern "C" int bb_mlp_train(bb_mlp* n, const float* x, const float* t, int rows, int steps, float lr, float* loss,
+                            void* stream) {
+    int rc = mlp_check_rows(n, x, rows, "bb_mlp_train");
+    if (rc) return rc;
+    if (!t) return fail(BB_ERR_ARG, "bb_mlp_train: t is NULL");
+    if (steps < 0 || !(lr > 0.f) || !isfinite(lr)) return fail(BB_ERR_ARG, "bb_mlp_train: steps %d, lr %g", steps, lr);
+    hipStream_t st = (hipStream_t)stream;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int s = 0; s < steps && !rc; s++) {
+        rc = mlp_forward(n, x, rows, st);
+        if (!rc) rc = mlp_backward(n, t, rows, loss ? loss + s : nullptr, st);
+        n->adam_t++;
+        const float c1 = 1.f - powf(b1, (float)n->adam_t), c2 = 1.f - powf(b2, (float)n->adam_t);
+        if (!rc) rc = bb_launch_mlp_adam(n->P, n->Gr, n->Mo, n->Ve, n->np_, lr, b1, b2, eps, c1, c2, st);
+        if (!rc) rc = mlp_cast(n, st);
+    }
+    return rc ? fail(BB_ERR_HIP, "bb_mlp_train: %s", hipGetErrorString((hipError_t)rc)) : 0;
+}

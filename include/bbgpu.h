@@ -188,6 +188,34 @@ int bb_profile_solve(const bb_profile* p, const double* maps, int B, void* work,
 int bb_profile_set_dens(const double* x, int B, int n, int nr, const int32_t* widx, const double* wval, int nsub,
                         double* sr, float* dens, void* stream);
 
+/* The likelihood surrogate: a network din -> hidden (SiLU) x nhidden -> 1 with BF16 weights and activations and
+ * FP32 accumulation on the matrix cores (v_mfma_f32_16x16x32_bf16), FP32 master parameters and full-batch Adam
+ * (beta 0.9 / 0.999, eps 1e-8), trained on the device on the (parameters, log-likelihood) pairs of a sampler. It
+ * stands where the regression network of nautilus's neural bounds stands in the reference
+ * (bbarolo/bayesian/nautilus/mlp.py:17-222, fitted in neural.py:60-127 and used by bounds/neural.py): a learned
+ * model of the likelihood that decides which proposals to evaluate. The reference's network is a scikit-learn-like
+ * NumPy MLP on the CPU; this one has no counterpart at the C level. din <= 32, hidden a multiple of 32 (<= 512),
+ * nmax: the most rows of any call. Parameters (bb_mlp_param_count floats): per hidden layer l the weights
+ * (hidden x K_l, K_0 = 32 with the inputs zero-padded, K_l = hidden) then the biases (hidden); then the output
+ * weights (hidden) and bias (1). x (rows, din), t, y (rows) and grad are DEVICE float32 arrays; the network's own
+ * buffers live on the device current at bb_mlp_create, which must be current at every call. */
+typedef struct bb_mlp bb_mlp;
+int bb_mlp_create(int din, int hidden, int nhidden, int nmax, bb_mlp** out);
+int bb_mlp_destroy(bb_mlp* net);
+int64_t bb_mlp_param_count(const bb_mlp* net);
+/* Host float32 parameters in / out; set also resets the Adam moments and step count. */
+int bb_mlp_set_params(bb_mlp* net, const float* host_params);
+int bb_mlp_get_params(const bb_mlp* net, float* host_params);
+/* y = net(x) for rows rows. */
+int bb_mlp_forward(bb_mlp* net, const float* x, int rows, float* y, void* stream);
+/* grad = d/dparams of 0.5 mean (net(x) - t)^2 through the BF16 copies of the parameters; loss (device, 1 float)
+ * may be NULL. */
+int bb_mlp_grad(bb_mlp* net, const float* x, const float* t, int rows, float* grad, float* loss, void* stream);
+/* steps full-batch Adam steps of learning rate lr on (x, t); loss (device, steps floats: the loss before each
+ * step) may be NULL. Deterministic (fixed-order reductions, no atomics). */
+int bb_mlp_train(bb_mlp* net, const float* x, const float* t, int rows, int steps, float lr, float* loss,
+                 void* stream);
+
 #ifdef __cplusplus
 }
 #endif
