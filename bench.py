@@ -1,11 +1,13 @@
-"""Benchmark: likelihood evaluations per second of the 160x160x96 GLOBAL configuration on one GPU.
+"""Benchmark: likelihood evaluations per second of the 160x160x96 GLOBAL configuration on one GPU, and the wall time
+of a Bayesian fit of the 64x64x40 configuration with and without the surrogate screen.
 
 One step = one batch of B parameter vectors (free VROT, VDISP per ring, INC, PA: 82-D) evaluated by the
 model-cube likelihood, inputs resident on the device: the deconvolved density profile of every vector (ring maps,
 non-negative least squares), then ANALYTIC GALMOD -> beam convolution -> masked Gaussian residual, as
-GpuLikelihood.device_batch runs it (two halves of the batch on two streams). Prints one JSON line (see --help).
+GpuLikelihood.device_batch runs it (two halves of the batch on two streams). The synthetic cube is built on the
+GPU (ANALYTIC GALMOD of the truth rings, beam, noise 3% of the peak). Prints one JSON line (see --help).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-bayes]
 """
 import argparse
 import ctypes as C
@@ -27,34 +29,15 @@ TRAFFIC_SHAPE = (256, 160, 160, 96)
 TRAFFIC_TILE_BYTES = (4298626.4 * 2.0 + 4653887.0 * 1.0) * 1024   # KiB x scale -> bytes
 
 
-def workload(n_theta, seed=1):
-    """The configuration-2 cube (truth built with the oracle on the host, noise 3% of the peak, seed 7), its
-    likelihood inputs and n_theta parameter vectors drawn from the default priors (default_rng(seed))."""
-    import oracle
-    from bbarolo2_amd.synth import config, default_priors, add_noise, likelihood_inputs, theta_of
-    from bbarolo2_amd.galmod import GalmodSetup, model_rings, nzq_for, PHI_TAB, model_box, zstep_for
-    from bbarolo2_amd.smooth import beam_kernel
-    from bbarolo2_amd.likelihood import FreePars, float32_rings
-    h, truth, par, noise_frac, noise_seed, spec, desc = config(2)
-    st = GalmodSetup(h, par)
-    kern, _ = beam_kernel(h)
-    st.zstep = zstep_for(h, st, True)
-    r32 = float32_rings(truth)
-    sr, _ = st.subrings(model_rings(r32, "CENTRED", st.radsep))
-    nx, ny = h.DimAx(0), h.DimAx(1)
-    mod = oracle.conv_same(oracle.galmod_analytic(st, sr, (0, 0), (nx, ny), nzq_for(sr), PHI_TAB), kern)
-    model_zyx = mod.transpose(2, 0, 1)
-    rms = noise_frac * model_zyx.max()
-    data = add_noise(model_zyx, rms, kern, noise_seed)
-    box = model_box(h, r32, "CENTRED", final=True)
-    likemask, region, sigma, likefactor = likelihood_inputs(h, model_zyx, data, rms, box)
-    free = FreePars(spec, truth.nr)
-    lo, hi = default_priors(truth, free)
-    u = np.random.default_rng(seed).random((n_theta, free.ndim))
-    theta = lo + u * (hi - lo)
-    return dict(h=h, truth=truth, par=par, data=data, likemask=likemask, region=region, sigma=sigma,
-                likefactor=likefactor, free=free, theta=theta, box=box, kern=kern, desc=desc,
-                theta_true=theta_of(truth, free))
+def workload(n_theta, B, seed=1):
+    """The 160x160x96 configuration on the GPU (truth cube: ANALYTIC GALMOD of the truth rings on the device, beam;
+    noise 3% of the peak, seed 7), its likelihood (GpuLikelihood, chunk B) and n_theta parameter vectors drawn from
+    the default priors (default_rng(seed))."""
+    from bbarolo2_amd.bayes import synthetic_fit
+    fit = synthetic_fit(2, chunk=B)
+    u = np.random.default_rng(seed).random((n_theta, fit["free"].ndim))
+    fit["theta"] = fit["lo"] + u * (fit["hi"] - fit["lo"])
+    return fit
 
 
 _W = {}
@@ -115,12 +98,13 @@ def _cpu_eval(i):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(wl, per_worker=2):
-    """The oracle timed on the host cores: P workers (fork pool, one evaluation per core as the reference's
-    sampler pools), each doing 1 untimed + per_worker timed evaluations; evals/s = P / median time."""
+def cpu_baseline(wl, per_worker=3):
+    """The oracle timed on the host cores: P worker processes (one evaluation per core, as the reference's sampler
+    pools; started by spawn, since this process holds the GPU), each doing 1 untimed + per_worker timed
+    evaluations; evals/s = P / median time."""
     import multiprocessing as mp
     P = max(1, min(16, os.cpu_count() or 1))
-    ctx = mp.get_context("fork")
+    ctx = mp.get_context("spawn")
     small = {k: wl[k] for k in ("h", "truth", "par", "data", "likemask", "region", "free", "theta", "box", "kern",
                                 "sigma", "likefactor")}
     with ctx.Pool(P, initializer=_cpu_worker_init, initargs=(small,)) as pool:
@@ -156,6 +140,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-bayes", action="store_true", help="skip the Bayesian fits (plain and surrogate-screened)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device buffers 5 times (the reference kernel of the HBM counter passes)")
     a = ap.parse_args()
@@ -167,20 +152,16 @@ def main():
         copy_calibration()
         return
     B = a.batch
-    wl = workload(max(200, B))
+    import torch
+    from bbarolo2_amd import _lib
+    from bbarolo2_amd.engine import DiskModel, _stream_ptr
+    torch.cuda.set_device(0)
+    wl = workload(max(200, B), B)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(wl)
         print(f"[bench] cpu baseline {cpu['value']:.2f} evals/s on {cpu['cores']} cores", file=sys.stderr, flush=True)
-
-    import torch
-    from bbarolo2_amd import _lib
-    from bbarolo2_amd.engine import DiskModel, _stream_ptr
-    from bbarolo2_amd.likelihood import GpuLikelihood
-    torch.cuda.set_device(0)
-    dm = DiskModel(wl["h"], wl["par"], galmodtype="ANALYTIC")
-    like = GpuLikelihood(dm, wl["data"], wl["truth"], wl["free"], likemask=wl["likemask"], region=wl["region"],
-                         likebox=wl["box"], sigma=wl["sigma"], likefactor=wl["likefactor"], chunk=B)
+    dm, like = wl["dm"], wl["like"]
     if like.profile is None or not like.update_prof:
         raise RuntimeError("the workload fits the geometry without DENS: the profile must be per vector")
     theta = np.resize(wl["theta"], (B, wl["free"].ndim))
@@ -291,7 +272,7 @@ def main():
     if (B, by, bx, nz) == TRAFFIC_SHAPE:
         traffic = TRAFFIC_TILE_BYTES
     out = {
-        "metric": "likelihood evals/s (160x160x96 cube)", "value": value, "unit": "evals/s", "n_gpus": 1,
+        "metric": "likelihood evals/s (160\u00b2\u00d796 cube); Bayes fit wall-time, surrogate vs plain", "value": value, "unit": "evals/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 (f64 geometry and sums)", "data": "synthetic",
         "config": {"workload": "GLOBAL likelihood, 160x160x96, 8\" pixels, 5.2 km/s, beam 30\", 40 rings, "
@@ -310,7 +291,35 @@ def main():
     }
     if cpu is not None:
         out["gpu_over_cpu"] = value / cpu["value"]
+    if not a.no_bayes:
+        out["bayes_fit"] = bayes_fits()
     print(json.dumps(out), flush=True)
+
+
+def bayes_fits(n_live=500, walks=25, seed=3):
+    """The second half of the metric: wall time of the Bayesian fit of the 64x64x40 cube (6 rings, free VROT VDISP
+    INC: 13-D), the nested sampler with constrained random walks, plain and with its walks screened by the BF16
+    surrogate trained on the device; and how far the screened posterior is from the plain one."""
+    from bbarolo2_amd import bayes
+    fit = bayes.synthetic_fit(5, chunk=n_live)
+    res = {}
+    for scr in (False, True):
+        t0 = time.perf_counter()
+        s = bayes.nested_fit(fit, n_live=n_live, n_batch=n_live, seed=seed, surrogate=scr, bound="rwalk",
+                             walks=walks)
+        res[scr] = (time.perf_counter() - t0, s)
+        print(f"[bench] bayes fit surrogate={scr}: {res[scr][0]:.1f} s, {s.n_like} likelihood evaluations",
+              file=sys.stderr, flush=True)
+    (tp, sp), (ts, ss) = res[False], res[True]
+    shift, width = bayes.compare(sp, ss)
+    return {"workload": "64x64x40 cube, 20 km/s channels, beam 4x3 px at PA 30, 6 rings, ANALYTIC, free VROT VDISP "
+                        "INC (13-D), DENS from the deconvolved profile of every vector; nested sampling, "
+                        f"n_live = n_batch = {n_live}, random walks of {walks} steps, f_live 0.01",
+            "plain_s": round(tp, 3), "surrogate_s": round(ts, 3), "speedup": round(tp / ts, 3),
+            "plain_n_like": sp.n_like, "surrogate_n_like": ss.n_like, "surrogate_screened": ss.n_screened,
+            "surrogate_train_predict_s": round(ss.t_surrogate, 3), "plain_like_s": round(sp.t_like, 3),
+            "surrogate_like_s": round(ss.t_like, 3), "log_z": [sp.log_z, ss.log_z],
+            "median_shift_sigma_max": shift, "ci68_width_change_max": width}
 
 
 if __name__ == "__main__":
