@@ -655,7 +655,8 @@ extern "C" int bb_norm3dfit(const float* model, int B, int by, int bx, int nz, c
 // BF16 copies, Adam moments and the activations of up to nmax rows, all on the device that created it.
 extern "C" int bb_launch_mlp_gemm(int epi, const unsigned short* A, const unsigned short* B, int M, int N, int K,
                                   const float* bias, float* Z, unsigned short* Ab, unsigned short* AbT, float* G,
-                                  const float* Zp, int nvalid, hipStream_t st);
+                                  const float* Zp, int nvalid, float* part, hipStream_t st);
+extern "C" int bb_mlp_splits(int K);
 extern "C" int bb_launch_mlp_in(const float* x, int n, int din, int np, int dp, unsigned short* X, unsigned short* XT,
                                 hipStream_t st);
 extern "C" int bb_launch_mlp_out(const unsigned short* a, const unsigned short* w, const float* b, int np, int H,
@@ -666,7 +667,9 @@ extern "C" int bb_launch_mlp_out_grad(const float* y, const float* t, int n, con
                                       float* gb, float* loss, hipStream_t st);
 extern "C" int bb_launch_mlp_bias_grad(const unsigned short* DT, int np, int H, float* g, hipStream_t st);
 extern "C" int bb_launch_mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
-                                  float eps, float c1, float c2, hipStream_t st);
+                                  float eps, const float* cc, hipStream_t st);
+extern "C" int bb_launch_mlp_tick(int* count, float* cc, const float* loss_slot, float* hist, int hist_cap, float b1,
+                                  float b2, hipStream_t st);
 extern "C" int bb_launch_mlp_cast(const float* W, int rows, int cols, unsigned short* Wb, unsigned short* WbT,
                                   hipStream_t st);
 
@@ -686,8 +689,18 @@ struct bb_mlp {
     unsigned short *A, *AT;   // activations, per layer (nmax x H), (H x nmax)
     unsigned short *D, *DT;   // layer gradients, two (nmax x H) + (H x nmax) sets
     float *y, *t;             // outputs and targets (nmax)
-    int64_t adam_t;           // Adam steps taken
+    int* cnt;                 // device: Adam steps taken, losses recorded by the current bb_mlp_train call
+    float* cc;                // device: Adam bias corrections of the current step
+    float* loss_slot;         // device: loss of the current step
+    float* hist;              // device: losses of the steps of the current bb_mlp_train call
+    float* part;              // device: split-K partial weight gradients (bb_mlp_splits(nmax) x H x max K)
+    hipStream_t cs;           // private stream of the graph capture
+    hipGraphExec_t gx;        // one captured training step (forward, backward, Adam, BF16 copies) for gx_rows rows
+    float gx_lr;
+    int gx_rows;
 };
+
+#define BB_MLP_HIST 4096      // most steps of one bb_mlp_train call
 
 static int64_t mlp_K(const bb_mlp* n, int l) { return l == 0 ? BB_MLP_DP : n->H; }
 static int64_t mlp_woff(const bb_mlp* n, int l) {       // offset of W_l in P (b_l follows it)
@@ -703,7 +716,10 @@ static int64_t mlp_bwoff(const bb_mlp* n, int l) {      // offset of W_l (then i
 
 extern "C" int bb_mlp_destroy(bb_mlp* n) {
     if (!n) return 0;
-    void* ptrs[] = {n->P, n->Gr, n->Mo, n->Ve, n->Wb, n->X, n->XT, n->Z, n->A, n->AT, n->D, n->DT, n->y, n->t};
+    if (n->gx) (void)hipGraphExecDestroy(n->gx);
+    if (n->cs) (void)hipStreamDestroy(n->cs);
+    void* ptrs[] = {n->P, n->Gr, n->Mo, n->Ve, n->Wb, n->X, n->XT, n->Z, n->A, n->AT, n->D, n->DT, n->y, n->t,
+                    n->cnt, n->cc, n->loss_slot, n->hist, n->part};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete n;
@@ -746,6 +762,13 @@ extern "C" int bb_mlp_create(int din, int hidden, int nhidden, int nmax, bb_mlp*
     alloc((void**)&n->DT, R * H * 2 * 2, "bb_mlp_create");
     alloc((void**)&n->y, R * sizeof(float), "bb_mlp_create");
     alloc((void**)&n->t, R * sizeof(float), "bb_mlp_create");
+    alloc((void**)&n->cnt, 2 * sizeof(int), "bb_mlp_create");
+    alloc((void**)&n->cc, 2 * sizeof(float), "bb_mlp_create");
+    alloc((void**)&n->loss_slot, sizeof(float), "bb_mlp_create");
+    alloc((void**)&n->hist, BB_MLP_HIST * sizeof(float), "bb_mlp_create");
+    alloc((void**)&n->part, (size_t)bb_mlp_splits((int)R) * H * (H > BB_MLP_DP ? H : BB_MLP_DP) * sizeof(float),
+          "bb_mlp_create");
+    if (!rc) rc = hip_check(hipStreamCreateWithFlags(&n->cs, hipStreamNonBlocking), "bb_mlp_create");
     if (rc) {
         bb_mlp_destroy(n);
         return rc;
@@ -772,7 +795,7 @@ extern "C" int bb_mlp_set_params(bb_mlp* n, const float* host) {
     int rc = hip_check(hipMemcpy(n->P, host, n->np_ * sizeof(float), hipMemcpyHostToDevice), "bb_mlp_set_params");
     if (!rc) rc = hip_check(hipMemset(n->Mo, 0, n->np_ * sizeof(float)), "bb_mlp_set_params");
     if (!rc) rc = hip_check(hipMemset(n->Ve, 0, n->np_ * sizeof(float)), "bb_mlp_set_params");
-    n->adam_t = 0;
+    if (!rc) rc = hip_check(hipMemset(n->cnt, 0, 2 * sizeof(int)), "bb_mlp_set_params");
     if (!rc) rc = hip_check((hipError_t)mlp_cast(n, nullptr), "bb_mlp_set_params");
     if (!rc) rc = hip_check(hipDeviceSynchronize(), "bb_mlp_set_params");
     return rc;
@@ -783,20 +806,28 @@ extern "C" int bb_mlp_get_params(const bb_mlp* n, float* host) {
     return hip_check(hipMemcpy(host, n->P, n->np_ * sizeof(float), hipMemcpyDeviceToHost), "bb_mlp_get_params");
 }
 
-// forward of rows [0, n) of x (device, n x din): activations kept for a backward pass; y (device, n) if given
-static int mlp_forward(bb_mlp* n, const float* x, int rows, hipStream_t st) {
+// the layers of the forward pass of rows [0, rows) of n->X (activations kept for a backward pass; outputs in n->y)
+static int mlp_layers(bb_mlp* n, int rows, hipStream_t st) {
     const int R = (rows + 31) / 32 * 32, H = n->H;
-    int rc = bb_launch_mlp_in(x, rows, n->din, R, BB_MLP_DP, n->X, n->XT, st);
+    int rc = 0;
     for (int l = 0; l < n->nh && !rc; l++) {
         const unsigned short* in = l == 0 ? n->X : n->A + (int64_t)(l - 1) * n->nmax * H;
         const int K = (int)mlp_K(n, l);
         rc = bb_launch_mlp_gemm(0, in, n->Wb + mlp_bwoff(n, l), R, H, K, n->P + mlp_woff(n, l) + (int64_t)H * K,
                                 n->Z + (int64_t)l * n->nmax * H, n->A + (int64_t)l * n->nmax * H,
-                                n->AT + (int64_t)l * n->nmax * H, nullptr, nullptr, rows, st);
+                                n->AT + (int64_t)l * n->nmax * H, nullptr, nullptr, rows, nullptr, st);
     }
     if (!rc)
         rc = bb_launch_mlp_out(n->A + (int64_t)(n->nh - 1) * n->nmax * H, n->Wb + mlp_bwoff(n, n->nh),
                                n->P + mlp_woff(n, n->nh) + H, R, H, n->y, st);
+    return rc;
+}
+
+// forward of rows [0, rows) of x (device, rows x din)
+static int mlp_forward(bb_mlp* n, const float* x, int rows, hipStream_t st) {
+    const int R = (rows + 31) / 32 * 32;
+    int rc = bb_launch_mlp_in(x, rows, n->din, R, BB_MLP_DP, n->X, n->XT, st);
+    if (!rc) rc = mlp_layers(n, rows, st);
     return rc;
 }
 
@@ -817,13 +848,14 @@ static int mlp_backward(bb_mlp* n, const float* t, int rows, float* loss, hipStr
         const unsigned short* inT = l == 0 ? n->XT : n->AT + (l - 1) * lay;
         float* gw = n->Gr + mlp_woff(n, l);
         // dW (H x K) = D^T (H x R) . in (R x K): rows of D^T and of in^T, contiguous along the R samples
-        rc = bb_launch_mlp_gemm(1, DT, inT, H, K, R, nullptr, nullptr, nullptr, nullptr, gw, nullptr, rows, st);
+        rc = bb_launch_mlp_gemm(1, DT, inT, H, K, R, nullptr, nullptr, nullptr, nullptr, gw, nullptr, rows, n->part,
+                                st);
         if (!rc) rc = bb_launch_mlp_bias_grad(DT, R, H, gw + (int64_t)H * K, st);
         if (!rc && l > 0) {
             // the gradient of layer l - 1: (D (R x H) . W (H x K)) * silu'(Z_{l-1}), W^T rows contiguous along H
             const unsigned short* WT = n->Wb + mlp_bwoff(n, l) + (int64_t)H * K;
             rc = bb_launch_mlp_gemm(2, D, WT, R, K, H, nullptr, nullptr, n->D + (1 - cur) * lay,
-                                    n->DT + (1 - cur) * lay, nullptr, n->Z + (l - 1) * lay, rows, st);
+                                    n->DT + (1 - cur) * lay, nullptr, n->Z + (l - 1) * lay, rows, nullptr, st);
             cur = 1 - cur;
         }
     }
@@ -860,21 +892,56 @@ extern "C" int bb_mlp_grad(bb_mlp* n, const float* x, const float* t, int rows, 
     return rc ? fail(BB_ERR_HIP, "bb_mlp_grad: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
 
+// One training step on the rows of n->X and n->t: forward, backward, Adam (constants from the device counter),
+// BF16 copies. The same launches every step, so it is captured once into a graph and replayed.
+static int mlp_train_step(bb_mlp* n, int rows, float lr, hipStream_t st) {
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    int rc = mlp_layers(n, rows, st);
+    if (!rc) rc = mlp_backward(n, n->t, rows, n->loss_slot, st);
+    if (!rc) rc = bb_launch_mlp_tick(n->cnt, n->cc, n->loss_slot, n->hist, BB_MLP_HIST, b1, b2, st);
+    if (!rc) rc = bb_launch_mlp_adam(n->P, n->Gr, n->Mo, n->Ve, n->np_, lr, b1, b2, eps, n->cc, st);
+    if (!rc) rc = mlp_cast(n, st);
+    return rc;
+}
+
 extern "C" int bb_mlp_train(bb_mlp* n, const float* x, const float* t, int rows, int steps, float lr, float* loss,
                             void* stream) {
     int rc = mlp_check_rows(n, x, rows, "bb_mlp_train");
     if (rc) return rc;
     if (!t) return fail(BB_ERR_ARG, "bb_mlp_train: t is NULL");
-    if (steps < 0 || !(lr > 0.f) || !isfinite(lr)) return fail(BB_ERR_ARG, "bb_mlp_train: steps %d, lr %g", steps, lr);
+    if (steps < 0 || steps > BB_MLP_HIST || !(lr > 0.f) || !isfinite(lr))
+        return fail(BB_ERR_ARG, "bb_mlp_train: steps %d (0..%d), lr %g", steps, BB_MLP_HIST, lr);
+    if (steps == 0) return 0;
     hipStream_t st = (hipStream_t)stream;
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    for (int s = 0; s < steps && !rc; s++) {
-        rc = mlp_forward(n, x, rows, st);
-        if (!rc) rc = mlp_backward(n, t, rows, loss ? loss + s : nullptr, st);
-        n->adam_t++;
-        const float c1 = 1.f - powf(b1, (float)n->adam_t), c2 = 1.f - powf(b2, (float)n->adam_t);
-        if (!rc) rc = bb_launch_mlp_adam(n->P, n->Gr, n->Mo, n->Ve, n->np_, lr, b1, b2, eps, c1, c2, st);
-        if (!rc) rc = mlp_cast(n, st);
+    const int R = (rows + 31) / 32 * 32;
+    // the inputs and targets into the network's own buffers (the captured step reads only those)
+    rc = bb_launch_mlp_in(x, rows, n->din, R, BB_MLP_DP, n->X, n->XT, st);
+    if (!rc) rc = (int)hipMemcpyAsync(n->t, t, (size_t)rows * sizeof(float), hipMemcpyDeviceToDevice, st);
+    if (!rc) rc = (int)hipMemsetAsync(n->cnt + 1, 0, sizeof(int), st);
+    if (rc) return fail(BB_ERR_HIP, "bb_mlp_train: %s", hipGetErrorString((hipError_t)rc));
+    if (!n->gx || n->gx_rows != rows || n->gx_lr != lr) {
+        // capture one step on the private stream
+        if (n->gx) {
+            (void)hipGraphExecDestroy(n->gx);
+            n->gx = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(n->cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            const int r2 = mlp_train_step(n, rows, lr, n->cs);
+            e = hipStreamEndCapture(n->cs, &g);
+            if (e == hipSuccess && r2) e = (hipError_t)r2;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&n->gx, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            n->gx = nullptr;
+            return fail(BB_ERR_HIP, "bb_mlp_train: graph capture: %s", hipGetErrorString(e));
+        }
+        n->gx_rows = rows;
+        n->gx_lr = lr;
     }
+    for (int s = 0; s < steps && !rc; s++) rc = (int)hipGraphLaunch(n->gx, st);
+    if (!rc && loss) rc = (int)hipMemcpyAsync(loss, n->hist, (size_t)steps * sizeof(float), hipMemcpyDeviceToDevice, st);
     return rc ? fail(BB_ERR_HIP, "bb_mlp_train: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }

@@ -37,13 +37,15 @@ enum { MLP_FWD = 0, MLP_GRAD = 1, MLP_BWD = 2 };
 
 // C = A . B^T (M, N multiples of 16, K of 32) with the epilogue EPI:
 //   MLP_FWD  z = C + bias[col]: Z (fp32, M x N), a = silu(z) as BF16 in Ab (M x N) and AbT (N x M);
-//   MLP_GRAD G (fp32, M x N) = C;
+//   MLP_GRAD the partial product over K slice blockIdx.y ([y kchunk, (y + 1) kchunk)) into G + y M N (fp32; the
+//            weight gradients have K = the training rows: split so that the small outputs fill the chip);
 //   MLP_BWD  d = C * silu'(Zp) for rows < nvalid (0 beyond) as BF16 in Ab (M x N) and AbT (N x M).
 template <int EPI>
 __global__ void __launch_bounds__(256)
 k_mlp_gemm(const unsigned short* __restrict__ A, const unsigned short* __restrict__ B, int M, int N, int K,
            const float* __restrict__ bias, float* __restrict__ Z, unsigned short* __restrict__ Ab,
-           unsigned short* __restrict__ AbT, float* __restrict__ G, const float* __restrict__ Zp, int nvalid) {
+           unsigned short* __restrict__ AbT, float* __restrict__ G, const float* __restrict__ Zp, int nvalid,
+           int kchunk) {
     const int lane = threadIdx.x & 63;
     const int tiles_n = N >> 4;
     const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -51,8 +53,9 @@ k_mlp_gemm(const unsigned short* __restrict__ A, const unsigned short* __restric
     const int m0 = (tile / tiles_n) << 4, n0 = (tile % tiles_n) << 4;
     const unsigned short* a = A + (size_t)(m0 + (lane & 15)) * K + 8 * (lane >> 4);
     const unsigned short* b = B + (size_t)(n0 + (lane & 15)) * K + 8 * (lane >> 4);
+    const int kb = blockIdx.y * kchunk, ke = kb + kchunk < K ? kb + kchunk : K;
     bb_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k = 0; k < K; k += 32) {
+    for (int k = kb; k < ke; k += 32) {
         const bb_bf16x8 av = *(const bb_bf16x8*)(a + k);
         const bb_bf16x8 bv = *(const bb_bf16x8*)(b + k);
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
@@ -69,7 +72,7 @@ k_mlp_gemm(const unsigned short* __restrict__ A, const unsigned short* __restric
             Ab[o] = h;
             AbT[(size_t)col * M + row] = h;
         } else if (EPI == MLP_GRAD) {
-            G[o] = acc[j];
+            G[(size_t)blockIdx.y * M * N + o] = acc[j];
         } else {
             const float d = row < nvalid ? acc[j] * mlp_dsilu(Zp[o]) : 0.f;
             const unsigned short h = mlp_bf(d);
@@ -77,6 +80,15 @@ k_mlp_gemm(const unsigned short* __restrict__ A, const unsigned short* __restric
             AbT[(size_t)col * M + row] = h;
         }
     }
+}
+
+// G[e] = sum over the splits s (in order) of part[s][e].
+__global__ void k_mlp_sum_parts(const float* __restrict__ part, int nsplit, int64_t n, float* __restrict__ G) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    float s = 0.f;
+    for (int k = 0; k < nsplit; k++) s += part[(size_t)k * n + e];
+    G[e] = s;
 }
 
 // Inputs (n, din) fp32 -> BF16 rows X (np, dp) and columns XT (dp, np), zero beyond n and din.
@@ -166,12 +178,28 @@ k_mlp_bias_grad(const unsigned short* __restrict__ DT, int np, float* __restrict
     if (threadIdx.x == 0) g[o] = s;
 }
 
-// Adam (bias-corrected) on the FP32 master parameters.
+// The step counter of Adam (count[0]) and its bias corrections cc = (1 - b1^t, 1 - b2^t); the loss of the step
+// (loss_slot) appended to hist at count[1]. One thread; runs between the backward pass and k_mlp_adam, so that a
+// captured training step replays with the right constants.
+__global__ void k_mlp_tick(int* __restrict__ count, float* __restrict__ cc, const float* __restrict__ loss_slot,
+                           float* __restrict__ hist, int hist_cap, float b1, float b2) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int t = count[0] + 1;
+    count[0] = t;
+    cc[0] = 1.f - powf(b1, (float)t);
+    cc[1] = 1.f - powf(b2, (float)t);
+    const int k = count[1];
+    if (k < hist_cap) hist[k] = loss_slot[0];
+    count[1] = k + 1;
+}
+
+// Adam (bias-corrected by cc) on the FP32 master parameters.
 __global__ void k_mlp_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
-                           float* __restrict__ v, int64_t np_, float lr, float b1, float b2, float eps, float c1,
-                           float c2) {
+                           float* __restrict__ v, int64_t np_, float lr, float b1, float b2, float eps,
+                           const float* __restrict__ cc) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np_) return;
+    const float c1 = cc[0], c2 = cc[1];
     const float gi = g[i];
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
@@ -195,17 +223,33 @@ __global__ void k_mlp_cast(const float* __restrict__ W, int rows, int cols, unsi
 // Launchers (host). Shapes are checked by the C-ABI before these run.
 static inline int mlp_blocks(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
+// Splits of K for the weight gradients: slices of 512 rows (at least), at most BB_MLP_SPLITS of them.
+#define BB_MLP_SPLITS 64
+extern "C" int bb_mlp_splits(int K) {
+    int ns = (K + 511) / 512;
+    return ns < 1 ? 1 : (ns > BB_MLP_SPLITS ? BB_MLP_SPLITS : ns);
+}
+
+// part: for MLP_GRAD, bb_mlp_splits(K) x M x N floats of partial sums (G receives their sum).
 extern "C" int bb_launch_mlp_gemm(int epi, const unsigned short* A, const unsigned short* B, int M, int N, int K,
                                   const float* bias, float* Z, unsigned short* Ab, unsigned short* AbT, float* G,
-                                  const float* Zp, int nvalid, hipStream_t st) {
+                                  const float* Zp, int nvalid, float* part, hipStream_t st) {
     const int tiles = (M / 16) * (N / 16);
-    const dim3 grid(mlp_blocks(tiles, 4)), block(256);
-    if (epi == MLP_FWD)
-        hipLaunchKernelGGL(k_mlp_gemm<MLP_FWD>, grid, block, 0, st, A, B, M, N, K, bias, Z, Ab, AbT, G, Zp, nvalid);
-    else if (epi == MLP_GRAD)
-        hipLaunchKernelGGL(k_mlp_gemm<MLP_GRAD>, grid, block, 0, st, A, B, M, N, K, bias, Z, Ab, AbT, G, Zp, nvalid);
-    else
-        hipLaunchKernelGGL(k_mlp_gemm<MLP_BWD>, grid, block, 0, st, A, B, M, N, K, bias, Z, Ab, AbT, G, Zp, nvalid);
+    const dim3 block(256);
+    if (epi == MLP_FWD) {
+        hipLaunchKernelGGL(k_mlp_gemm<MLP_FWD>, dim3(mlp_blocks(tiles, 4)), block, 0, st, A, B, M, N, K, bias, Z, Ab,
+                           AbT, G, Zp, nvalid, K);
+    } else if (epi == MLP_GRAD) {
+        const int ns = bb_mlp_splits(K);
+        const int kchunk = ((K + ns - 1) / ns + 31) / 32 * 32;
+        hipLaunchKernelGGL(k_mlp_gemm<MLP_GRAD>, dim3(mlp_blocks(tiles, 4), ns), block, 0, st, A, B, M, N, K, bias, Z,
+                           Ab, AbT, part, Zp, nvalid, kchunk);
+        hipLaunchKernelGGL(k_mlp_sum_parts, dim3(mlp_blocks((int64_t)M * N, 256)), block, 0, st, part, ns,
+                           (int64_t)M * N, G);
+    } else {
+        hipLaunchKernelGGL(k_mlp_gemm<MLP_BWD>, dim3(mlp_blocks(tiles, 4)), block, 0, st, A, B, M, N, K, bias, Z, Ab,
+                           AbT, G, Zp, nvalid, K);
+    }
     return (int)hipGetLastError();
 }
 
@@ -240,8 +284,14 @@ extern "C" int bb_launch_mlp_bias_grad(const unsigned short* DT, int np, int H, 
 }
 
 extern "C" int bb_launch_mlp_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
-                                  float eps, float c1, float c2, hipStream_t st) {
-    hipLaunchKernelGGL(k_mlp_adam, dim3(mlp_blocks(n, 256)), dim3(256), 0, st, p, g, m, v, n, lr, b1, b2, eps, c1, c2);
+                                  float eps, const float* cc, hipStream_t st) {
+    hipLaunchKernelGGL(k_mlp_adam, dim3(mlp_blocks(n, 256)), dim3(256), 0, st, p, g, m, v, n, lr, b1, b2, eps, cc);
+    return (int)hipGetLastError();
+}
+
+extern "C" int bb_launch_mlp_tick(int* count, float* cc, const float* loss_slot, float* hist, int hist_cap, float b1,
+                                  float b2, hipStream_t st) {
+    hipLaunchKernelGGL(k_mlp_tick, dim3(1), dim3(64), 0, st, count, cc, loss_slot, hist, hist_cap, b1, b2);
     return (int)hipGetLastError();
 }
 

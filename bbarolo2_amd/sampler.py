@@ -20,8 +20,9 @@ enlarge_per_dim per dimension, within the unit cube) or from constrained random 
 (bound="rwalk": `walks` Gaussian steps with the live points' covariance, all chains stepping together, one
 likelihood call per step), which follow curved degeneracies (VROT x sin INC) that a single ellipsoid bounds poorly.
 
-Screening (surrogate=...): the surrogate (fit(x, y), predict(x) on unit-cube points) is trained every
-retrain_every evaluations on every exactly evaluated proposal, its targets clipped one live-set range below the
+Screening (surrogate=...): the surrogate (fit(x, y), predict(x) on unit-cube points) is trained once at least
+retrain_every evaluations and a factor retrain_growth more evaluations than at its last training have been made,
+on every exactly evaluated proposal, its targets clipped one live-set range below the
 threshold. Its margin is screen_sigma x its RMS error on the newest evaluations it was not trained on; while that
 error is larger than half the live set's range of log-likelihood, or the training set is small, nothing is
 screened (the plain sampler). With the ellipsoid, proposals it predicts below threshold - margin are not
@@ -94,7 +95,7 @@ class Sampler:
 
     def __init__(self, prior, likelihood, n_dim, n_live=2000, n_batch=None, enlarge_per_dim=1.1, n_update=None,
                  seed=None, surrogate=None, screen_sigma=3.0, train_steps=300, retrain_every=None, train_max=16384,
-                 bound="ellipsoid", walks=25):
+                 bound="ellipsoid", walks=25, retrain_growth=1.0):
         self.prior, self.likelihood = prior, likelihood
         self.n_dim, self.n_live = int(n_dim), int(n_live)
         self.n_batch = int(n_batch) if n_batch else max(100, self.n_live // 4)
@@ -106,6 +107,7 @@ class Sampler:
         self.train_steps = int(train_steps)
         self.retrain_every = int(retrain_every) if retrain_every else 10 * self.n_batch
         self.train_max = int(train_max)
+        self.retrain_growth = float(retrain_growth)
         if bound not in ("ellipsoid", "rwalk"):
             raise ValueError(f"bound {bound!r}: 'ellipsoid' or 'rwalk'")
         self.bound, self.walks, self.scale = bound, int(walks), 1.0
@@ -204,7 +206,8 @@ class Sampler:
                 fin = live_l[np.isfinite(live_l)]
                 lrange = float(fin.max() - lmin) if fin.size and np.isfinite(lmin) else np.inf
                 if (self.surrogate is not None and np.isfinite(lmin) and
-                        self.n_like - last_train >= self.retrain_every):
+                        self.n_like - last_train >= self.retrain_every and
+                        self.n_like >= self.retrain_growth * last_train):
                     self._train(lmin, lrange)
                     last_train = self.n_like
                     self._walk_pass = 1.0        # (a new network gets its chance)

@@ -115,7 +115,7 @@ class MLPSurrogate:
                                            float(lr), ls.data_ptr(), _stream_ptr(self.torch)))
         return ls[:steps].cpu().numpy()
 
-    def fit(self, x, y, steps=400, lr=2e-3, restart=False):
+    def fit(self, x, y, steps=400, lr=3e-3, restart=False):
         """Train on unit-cube points x and values y (standardised: mean 0, deviation 1); returns the last loss."""
         y = np.asarray(y, np.float64).ravel()
         if restart:

@@ -45,7 +45,7 @@ def test_forward_matches_oracle(n):
     assert np.max(err) <= 1e-2 and np.mean(err <= 1e-4) >= 0.99, (np.max(err), np.mean(err <= 1e-4))
 
 
-@pytest.mark.parametrize("n", [5, 300, 1000])
+@pytest.mark.parametrize("n", [5, 300, 1000, 4000])
 def test_gradient_matches_oracle(n):
     net = _net(seed=4)
     p = net.get_params()
@@ -87,8 +87,8 @@ def test_training_fits_a_smooth_function():
     xv, tv = _data(1000, seed=9)
     net = _net(seed=6)
     loss0 = net.grad(x, (t - t.mean()) / t.std())[1]
-    last = net.fit(x, t, steps=600, lr=2e-3)
-    assert last < 0.02 * loss0
+    last = net.fit(x, t, steps=1500, lr=3e-3)
+    assert last < 0.01 * loss0
     pred = net.predict(xv)
     rms = np.sqrt(np.mean((pred - tv) ** 2))
-    assert rms < 0.15 * tv.std(), (rms, tv.std())
+    assert rms < 0.1 * tv.std(), (rms, tv.std())
