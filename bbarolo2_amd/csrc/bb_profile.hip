@@ -155,15 +155,23 @@ __global__ void k_gram_reduce(const double* __restrict__ partial, int n, int nbl
 // major, symmetric positive definite) and h (n) in LDS; the result in x (LDS). Lane l owns the entries l and
 // l + 64 of every vector; the choices of the active set (the entering index, the step to the boundary) are
 // wavefront reductions that pick the same index as a scan in index order (ties to the lowest index).
+//
+// The passive system M_PP is kept as a Cholesky factor L (L L^T = M_PP) in the order the indices entered: an
+// entering index appends one row (a forward substitution), so an iteration of Lawson-Hanson costs O(k^2) instead of
+// a factorisation; an index that leaves rebuilds the factor (rare). The substitutions run column by column: the
+// pivot of step i is read from its lane (readlane) and every lane updates its own entries with one FMA, so a step
+// is one short dependency chain, not a dot product per lane.
 struct NnLds {
     double* M;      // n x n system
-    double* C;      // n x n Cholesky factor of the passive system
+    double* L;      // n x n: row r of the Cholesky factor of the passive system, in entering order
+    double* invd;   // n: 1 / L[r][r]
     double* z;      // n: passive solution
     double* x;      // n: current solution
     double* g;      // n: gradient h - M x
     double* h;      // n
     int* P;         // n: passive flags
     int* ban;       // n: excluded from the next choice
+    int* ord;       // n: the index of factor row r
 };
 
 // (value, index) of the lane pair with the larger value, ties to the lower index; index -1 loses to any index.
@@ -194,69 +202,107 @@ __device__ __forceinline__ double nn_max(double v) {
     return v;
 }
 
-// z = M_PP^-1 h_P (0 outside P) by Cholesky; false if a pivot is not positive. The factor is built column by
-// column (left-looking): lane i forms C[i][k] = (M[i][k] - sum_p<k C[i][p] C[k][p]) / C[k][k] as a dot product
-// of reads only, the same operations in the same order as the column-by-column update.
-__device__ bool nn_passive_solve(const NnLds& L, int n, int lane) {
-    double* C = L.C;
-    const double* M = L.M;
-    for (int i = lane; i < n; i += 64) L.z[i] = L.P[i] ? L.h[i] : 0.0;
-    bool ok = true;
-    for (int k = 0; k < n; k++) {
-        const bool pk = L.P[k];
-        const double* Ck = C + (size_t)k * n;
-        double d2 = pk ? M[(size_t)k * n + k] : 1.0;
-        for (int p = 0; p < k; p++) d2 -= Ck[p] * Ck[p];
-        if (!(d2 > 0.0)) ok = false;
-        const double d = sqrt(d2 > 0.0 ? d2 : 1.0);
-        for (int i = k + 1 + lane; i < n; i += 64) {
-            double* Ci = C + (size_t)i * n;
-            double s = (pk && L.P[i]) ? M[(size_t)i * n + k] : 0.0;
-            for (int p = 0; p < k; p++) s -= Ci[p] * Ck[p];
-            Ci[k] = s / d;
-        }
-        if (lane == 0) C[(size_t)k * n + k] = d;
-        __syncthreads();
+__device__ __forceinline__ double nn_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// the value of entry i of a vector held as (v0: entries lane, v1: entries lane + 64), from its lane
+__device__ __forceinline__ double nn_get(double v0, double v1, int i) {
+    const double v = i < 64 ? v0 : v1;
+    const int l = i & 63;
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l), hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
+// Forward substitution L y = b over rows [0, k): b in (b0, b1) (entry r in lane r & 63), y returned in place.
+__device__ __forceinline__ void nn_forward(const NnLds& S, int n, int k, int lane, double& b0, double& b1) {
+    for (int i = 0; i < k; i++) {
+        const double yi = nn_get(b0, b1, i) * S.invd[i];
+        if (lane > i && lane < k) b0 -= S.L[(size_t)lane * n + i] * yi;
+        if (lane + 64 > i && lane + 64 < k) b1 -= S.L[(size_t)(lane + 64) * n + i] * yi;
+        if (lane == i) b0 = yi;
+        if (lane + 64 == i) b1 = yi;
     }
-    // forward: C y = z
-    for (int k = 0; k < n; k++) {
-        const double yk = L.z[k] / C[k * n + k];
-        __syncthreads();
-        for (int i = k + 1 + lane; i < n; i += 64) L.z[i] -= C[i * n + k] * yk;
-        if (lane == 0) L.z[k] = yk;
-        __syncthreads();
+}
+
+// Backward substitution L^T z = c over rows [0, k), in place.
+__device__ __forceinline__ void nn_backward(const NnLds& S, int n, int k, int lane, double& c0, double& c1) {
+    for (int i = k - 1; i >= 0; i--) {
+        const double zi = nn_get(c0, c1, i) * S.invd[i];
+        const double* Li = S.L + (size_t)i * n;
+        if (lane < i) c0 -= Li[lane] * zi;
+        if (lane + 64 < i) c1 -= Li[lane + 64] * zi;
+        if (lane == i) c0 = zi;
+        if (lane + 64 == i) c1 = zi;
     }
-    // backward: C^T z = y
-    for (int k = n - 1; k >= 0; k--) {
-        const double zk = L.z[k] / C[k * n + k];
-        __syncthreads();
-        for (int i = lane; i < k; i += 64) L.z[i] -= C[k * n + i] * zk;
-        if (lane == 0) L.z[k] = zk;
-        __syncthreads();
+}
+
+// Append index j as row k of the factor; false (nothing changed) if its pivot is not positive.
+__device__ bool nn_append(const NnLds& S, int n, int& k, int lane, int j) {
+    double b0 = 0.0, b1 = 0.0;
+    if (lane < k) b0 = S.M[(size_t)S.ord[lane] * n + j];
+    if (lane + 64 < k) b1 = S.M[(size_t)S.ord[lane + 64] * n + j];
+    nn_forward(S, n, k, lane, b0, b1);
+    const double d2 = S.M[(size_t)j * n + j] - nn_sum((lane < k ? b0 * b0 : 0.0) + (lane + 64 < k ? b1 * b1 : 0.0));
+    if (!(d2 > 0.0)) return false;
+    const double d = sqrt(d2);
+    double* Lk = S.L + (size_t)k * n;
+    if (lane < k) Lk[lane] = b0;
+    if (lane + 64 < k) Lk[lane + 64] = b1;
+    if (lane == 0) {
+        Lk[k] = d;
+        S.invd[k] = 1.0 / d;
+        S.ord[k] = j;
     }
-    return ok;
+    __syncthreads();
+    k++;
+    return true;
+}
+
+// The factor of the passive set from scratch, in index order; false if a pivot is not positive.
+__device__ bool nn_rebuild(const NnLds& S, int n, int& k, int lane) {
+    k = 0;
+    for (int j = 0; j < n; j++)
+        if (S.P[j] && !nn_append(S, n, k, lane, j)) return false;
+    return true;
+}
+
+// z = M_PP^-1 h_P (0 outside P) from the factor of its k rows.
+__device__ void nn_factor_solve(const NnLds& S, int n, int k, int lane) {
+    double c0 = lane < k ? S.h[S.ord[lane]] : 0.0, c1 = lane + 64 < k ? S.h[S.ord[lane + 64]] : 0.0;
+    nn_forward(S, n, k, lane, c0, c1);
+    nn_backward(S, n, k, lane, c0, c1);
+    for (int i = lane; i < n; i += 64) S.z[i] = 0.0;
+    __syncthreads();
+    if (lane < k) S.z[S.ord[lane]] = c0;
+    if (lane + 64 < k) S.z[S.ord[lane + 64]] = c1;
+    __syncthreads();
 }
 
 // Lawson-Hanson active set: returns false if the iteration limits were reached or a pivot failed.
 // warm (optional, n): a guess of the positive set. If the passive solution on it is feasible the active set
 // starts there (a few iterations when the guess is the solution of a nearby problem), else from x = 0.
-__device__ bool nn_solve(const NnLds& L, int n, int lane, const double* warm = nullptr) {
+__device__ bool nn_solve(const NnLds& S, int n, int lane, const double* warm = nullptr) {
     for (int i = lane; i < n; i += 64) {
-        L.x[i] = 0.0;
-        L.P[i] = warm ? (warm[i] > 0.0) : 0;
-        L.ban[i] = 0;
+        S.x[i] = 0.0;
+        S.P[i] = warm ? (warm[i] > 0.0) : 0;
+        S.ban[i] = 0;
     }
     __syncthreads();
+    int k = 0;
     if (warm) {
-        const bool ok = nn_passive_solve(L, n, lane);
+        bool ok = nn_rebuild(S, n, k, lane);
+        if (ok) nn_factor_solve(S, n, k, lane);
         int bad = 0;
         for (int i = lane; i < n; i += 64)
-            if (L.P[i] && !(L.z[i] > 0.0)) bad = 1;
+            if (S.P[i] && !(S.z[i] > 0.0)) bad = 1;
         const bool feas = ok && __ballot(bad) == 0;
         for (int i = lane; i < n; i += 64) {
-            if (feas) L.x[i] = L.z[i];
-            else L.P[i] = 0;
+            if (feas) S.x[i] = S.z[i];
+            else S.P[i] = 0;
         }
+        if (!feas) k = 0;
         __syncthreads();
     }
     const int maxit = 3 * n + 10;
@@ -265,14 +311,14 @@ __device__ bool nn_solve(const NnLds& L, int n, int lane, const double* warm = n
         // gradient h - M x and the size of its round-off
         double amax = 0.0;
         for (int i = lane; i < n; i += 64) {
-            double s = L.h[i], a = fabs(L.h[i]);
-            const double* Mi = L.M + (size_t)i * n;
+            double s = S.h[i], a = fabs(S.h[i]);
+            const double* Mi = S.M + (size_t)i * n;
             for (int j = 0; j < n; j++) {
-                const double t = Mi[j] * L.x[j];
+                const double t = Mi[j] * S.x[j];
                 s -= t;
                 a += fabs(t);
             }
-            L.g[i] = s;
+            S.g[i] = s;
             amax = fmax(amax, a);
         }
         amax = nn_max(amax);
@@ -281,23 +327,25 @@ __device__ bool nn_solve(const NnLds& L, int n, int lane, const double* warm = n
         int best = -1;
         const double gb = 64.0 * (double)n * 2.220446049250313e-16 * amax;
         for (int i = lane; i < n; i += 64)
-            if (!L.P[i] && !L.ban[i] && L.g[i] > gb && (best < 0 || L.g[i] > gv)) {
-                gv = L.g[i];
+            if (!S.P[i] && !S.ban[i] && S.g[i] > gb && (best < 0 || S.g[i] > gv)) {
+                gv = S.g[i];
                 best = i;
             }
         nn_argmax(gv, best);
         if (best < 0) break;
         const int j = best;
-        if (lane == (j & 63)) L.P[j] = 1;
+        if (!nn_append(S, n, k, lane, j)) return false;
+        if (lane == (j & 63)) S.P[j] = 1;
         __syncthreads();
         for (int inner = 0;; inner++) {
             if (inner >= maxit) return false;
-            if (!nn_passive_solve(L, n, lane)) return false;
-            if (inner == 0 && !(L.z[j] > 0.0)) {
-                // the new index cannot enter
+            nn_factor_solve(S, n, k, lane);
+            if (inner == 0 && !(S.z[j] > 0.0)) {
+                // the new index cannot enter (it is the last row of the factor)
+                k--;
                 if (lane == (j & 63)) {
-                    L.P[j] = 0;
-                    L.ban[j] = 1;
+                    S.P[j] = 0;
+                    S.ban[j] = 1;
                 }
                 __syncthreads();
                 break;
@@ -306,8 +354,8 @@ __device__ bool nn_solve(const NnLds& L, int n, int lane, const double* warm = n
             double av = 2.0;
             int imin = -1;
             for (int i = lane; i < n; i += 64)
-                if (L.P[i] && !(L.z[i] > 0.0)) {
-                    const double a = L.x[i] / (L.x[i] - L.z[i]);
+                if (S.P[i] && !(S.z[i] > 0.0)) {
+                    const double a = S.x[i] / (S.x[i] - S.z[i]);
                     if (imin < 0 || a < av) {
                         av = a;
                         imin = i;
@@ -316,25 +364,27 @@ __device__ bool nn_solve(const NnLds& L, int n, int lane, const double* warm = n
             nn_argmin(av, imin);
             if (imin < 0) {
                 for (int i = lane; i < n; i += 64) {
-                    L.x[i] = L.z[i];
-                    L.ban[i] = 0;
+                    S.x[i] = S.z[i];
+                    S.ban[i] = 0;
                 }
                 __syncthreads();
                 break;
             }
             const double alpha = av;
             for (int i = lane; i < n; i += 64) {
-                L.ban[i] = 0;
-                if (!L.P[i]) continue;
-                const double xi = L.x[i] + alpha * (L.z[i] - L.x[i]);
+                S.ban[i] = 0;
+                if (!S.P[i]) continue;
+                const double xi = S.x[i] + alpha * (S.z[i] - S.x[i]);
                 if (i == imin || !(xi > 0.0)) {
-                    L.x[i] = 0.0;
-                    L.P[i] = 0;
+                    S.x[i] = 0.0;
+                    S.P[i] = 0;
                 } else {
-                    L.x[i] = xi;
+                    S.x[i] = xi;
                 }
             }
             __syncthreads();
+            // indices left the passive set: the factor of the rest
+            if (!nn_rebuild(S, n, k, lane)) return false;
         }
     }
     return it < maxit;
@@ -371,8 +421,9 @@ k_nnls(ProfConst pc, const double* __restrict__ gram, const double* __restrict__
     (void)Mall;
     NnLds L;
     L.M = nn_lds;
-    L.C = L.M + n * n;
-    L.z = L.C + n * n;
+    L.L = L.M + n * n;
+    L.invd = L.L + n * n;
+    L.z = L.invd + n;
     L.x = L.z + n;
     L.g = L.x + n;
     L.h = L.g + n;
@@ -380,6 +431,7 @@ k_nnls(ProfConst pc, const double* __restrict__ gram, const double* __restrict__
     double* x0 = w2 + n;             // n doubles
     L.P = (int*)(x0 + n);
     L.ban = L.P + n;
+    L.ord = L.ban + n;
     const double* G = gram + (size_t)b * m * m;
     double* M = L.M;
     for (int i = lane; i < n; i += 64) L.h[i] = G[i * m + n];
@@ -467,7 +519,7 @@ extern "C" int bb_launch_rm_finish(const unsigned long long* counts, double* map
 
 extern "C" int bb_gram_blocks(int npix) { return (npix + 2047) / 2048; }
 extern "C" size_t bb_gram_entries(int n) { return (size_t)(n + 1) * (n + 2) / 2; }
-extern "C" size_t bb_nnls_lds(int n) { return (size_t)(2 * n * n + 6 * n) * sizeof(double) + 2 * n * sizeof(int) + 16; }
+extern "C" size_t bb_nnls_lds(int n) { return (size_t)(2 * n * n + 7 * n) * sizeof(double) + 3 * n * sizeof(int) + 16; }
 extern "C" int bb_nnls_max() { return BB_NN_MAX; }
 
 extern "C" int bb_launch_profile_solve(int n, int nr, int nok, double smooth, double corr, double floor_, double norm,

@@ -317,6 +317,16 @@ class DiskModel:
             dep.zero_()
         if not convolve or self.cfield is None:
             return dep
+        return self.convolve_batch(dep, out=out, stream=stream, nzq=nzq)
+
+    def convolve_batch(self, dep, out=None, stream=None, nzq=None):
+        """The beam convolution of the device models dep (B, by, bx, nz) (galmod.py:1264-1283, final=True):
+        a new tensor, or out."""
+        torch = _torch()
+        if self.cfield is None:
+            return dep
+        B, by, bx, _ = dep.shape
+        h = self.handle(ZSAMPLES_MAX if nzq is None else nzq)
         res = torch.empty_like(dep) if out is None else out
-        _lib.check(_lib.lib().bb_conv2d_same(h, dep.data_ptr(), res.data_ptr(), B, by, bx, st))
+        _lib.check(_lib.lib().bb_conv2d_same(h, dep.data_ptr(), res.data_ptr(), B, by, bx, _stream_ptr(torch, stream)))
         return res

@@ -69,13 +69,14 @@ def _solve_gpu(A, b, radii, smooth, corr=1.):
         L.bb_profile_destroy(h)
 
 
-# (seed, shape of A, radii, smooth, tolerance). Coinciding radii put a curvature row of weight ~1/(1e-6 extent)^2
-# in the system: the normal equations square its conditioning, and the profile keeps 1e-5 (the rtol with which the
-# reference's own test compares that limit, tests/test_bbarolo.py:2954-2956).
+# (seed, shape of A, radii, smooth, tolerance). Coinciding radii (two rings, or the annulus on a ring's radius) put a
+# curvature row of weight ~1/(1e-6 extent)^2 in the system: the normal equations square its conditioning, and the
+# profile keeps 1e-5 (the rtol with which the reference's own test compares that limit,
+# tests/test_bbarolo.py:2954-2956).
 SYSTEMS = {
     "annulus_beyond": (1, (80, 6), [1., 2., 3., 4., 5., 6.5], 0.3, 1e-7),
     "annulus_inside": (1, (80, 6), [1., 2., 3., 4., 5., 4.5], 0.3, 1e-7),
-    "annulus_on_ring": (1, (80, 6), [1., 2., 3., 4., 5., 4.], 0.3, 1e-7),
+    "annulus_on_ring": (1, (80, 6), [1., 2., 3., 4., 5., 4.], 0.3, 1e-5),
     "coinciding": (1, (50, 4), [1., 2., 3., 3.], 0.3, 1e-5),
     "no_prior": (4, (60, 7), list(np.arange(7.)), 0., 1e-7),
     "forty_rings": (9, (400, 41), list(np.arange(40) * 30. + 15.) + [1230.], 0.3, 1e-7),
