@@ -13,7 +13,8 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
                                          const int* naz, const int4* chunks, int nchunks, const double2* az,
                                          int TX, int TY, double* sub, double* wtab, short4* bbox, short4* sbox,
                                          const double* cd2i, const double* vedges, const double* ptab, int ntab,
-                                         int lin_axis, float* out, int accumulate, hipStream_t st);
+                                         int lin_axis, float* out, int accumulate, int* bin, int bin_cap,
+                                         hipStream_t st);
 extern "C" void bb_set_tile_events(void* ev_begin, void* ev_end);
 extern "C" size_t bb_analytic_sub_bytes();
 extern "C" size_t bb_analytic_wt_bytes();
@@ -27,8 +28,8 @@ extern "C" int bb_residual_blocks(int64_t nvox);
 extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* sr, int nsub, int B, const int* naz,
                                    const int4* chunks, int nchunks, const double2* az, int TX, int TY, double* sub,
                                    double* wtab, short4* bbox, short4* sbox, const int* widx, const double* wval,
-                                   int nmap, int map0, int nw, double qunit, unsigned long long* counts,
-                                   hipStream_t st);
+                                   int nmap, int map0, int nw, double qunit, unsigned long long* counts, int* bin,
+                                   int bin_cap, hipStream_t st);
 extern "C" size_t bb_rm_tile_lds(int nw, int TX, int TY);
 extern "C" int bb_norm_blocks(int by, int bx);
 extern "C" int bb_launch_norm(int B, int by, int bx, int nz, int ny, int nx, int64_t blo0, int64_t blo1, double wpow,
@@ -298,11 +299,23 @@ extern "C" int bb_azplan_create(const bb_model* m, const double* R0, int nsub, b
     return 0;
 }
 
+// The chunk lists of the pixel tiles (bin_chunks): up to BB_BIN_PER_CHUNK tiles per chunk on average, and at most
+// nchunks tiles (beyond: the tiles test every chunk box).
+#define BB_BIN_PER_CHUNK 8
+static int64_t bin_cap(const bb_azplan* p) { return (int64_t)p->nchunks * BB_BIN_PER_CHUNK; }
+static size_t bin_ints(const bb_azplan* p) { return (size_t)3 * (p->nchunks + 1) + 1 + (size_t)bin_cap(p); }
+
 extern "C" int64_t bb_analytic_work_size(const bb_azplan* p, int B) {
     if (!p || B < 0) return -1;
     size_t per = (size_t)p->nsub * (bb_analytic_sub_bytes() + bb_analytic_wt_bytes()) +
-                 (size_t)p->nchunks * (1 + bb_analytic_chunk()) * sizeof(short4);
+                 (size_t)p->nchunks * (1 + bb_analytic_chunk()) * sizeof(short4) + bin_ints(p) * sizeof(int);
     return (int64_t)(per * B + 256);
+}
+
+// the bin region of the work buffer after the sample boxes, or NULL when the box has more tiles than chunks
+static int* work_bin(const bb_azplan* p, int B, short4* sbox, long long ntiles) {
+    if (ntiles > p->nchunks || bin_cap(p) > INT32_MAX) return nullptr;
+    return (int*)(sbox + (size_t)B * p->nchunks * bb_analytic_chunk());
 }
 
 static int check_box(const bb_model* m, bb_box box) {
@@ -339,7 +352,7 @@ extern "C" int bb_galmod_analytic(const bb_model* m, const bb_azplan* p, bb_box 
     short4* sbox = (short4*)w;
     rc = bb_launch_galmod_analytic(&m->g, box, sr, nsub, B, p->d_naz, p->d_chunks, p->nchunks, p->d_az, m->TX, m->TY,
                                    sub, wtab, bbox, sbox, m->d_cd2i, m->d_vedges, m->d_ptab, m->ntab, m->lin, out,
-                                   accumulate, (hipStream_t)stream);
+                                   accumulate, work_bin(p, B, sbox, ntiles), (int)bin_cap(p), (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_galmod_analytic launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
 
@@ -430,9 +443,10 @@ extern "C" int bb_ring_maps_deposit(const bb_model* m, const bb_azplan* p, bb_bo
     short4* bbox = (short4*)w;
     w += (size_t)B * p->nchunks * sizeof(short4);
     short4* sbox = (short4*)w;
+    const long long ntiles = (long long)((bx + TX - 1) / TX) * ((by + TY - 1) / TY);
     rc = bb_launch_ring_maps(&m->g, box, sr, nsub, B, p->d_naz, p->d_chunks, p->nchunks, p->d_az, TX, TY, sub, wtab,
                              bbox, sbox, (const int*)widx, wval, nmap, map0, nw, qunit, (unsigned long long*)counts,
-                             (hipStream_t)stream);
+                             work_bin(p, B, sbox, ntiles), (int)bin_cap(p), (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_ring_maps_deposit launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
 

@@ -222,6 +222,105 @@ k_an_bbox(bb_grid g, bb_box box, const int4* __restrict__ chunks, int nchunks, c
 }
 
 // ---------------------------------------------------------------------------------------------------------
+// The chunks of every pixel tile (a counting sort of the chunk boxes by the tiles they meet), so that a tile
+// reads its own list instead of testing every chunk box of its vector. Per vector: cnt (ntiles) counts, then offs
+// (ntiles + 1) exclusive offsets with offs[ntiles] the total, list (cap) the chunk indices. A vector whose total
+// exceeds cap gets offs[ntiles] = -1: its tiles test all the chunks.
+__global__ void __launch_bounds__(256)
+k_tile_zero(int* __restrict__ cnt, int* __restrict__ cur, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        cnt[i] = 0;
+        cur[i] = 0;
+    }
+}
+
+template <bool FILL>
+__global__ void __launch_bounds__(256)
+k_tile_bin(const short4* __restrict__ bball, int nchunks, int TX, int TY, int ntx, int ntiles, int* __restrict__ cnt,
+           const int* __restrict__ offs, int* __restrict__ cur, int* __restrict__ list, int cap) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+    if (c >= nchunks) return;
+    const short4 r = bball[(size_t)b * nchunks + c];
+    if (r.x > r.y) return;                               // (empty box)
+    const int ntyv = ntiles / ntx;
+    const int qx0 = r.x / TX, qx1 = r.y / TX, qy0 = r.z / TY, qy1 = r.w / TY;
+    if (FILL && offs[(size_t)b * (ntiles + 1) + ntiles] < 0) return;
+    for (int qy = qy0; qy <= qy1 && qy < ntyv; qy++)
+        for (int qx = qx0; qx <= qx1 && qx < ntx; qx++) {
+            const int t = qy * ntx + qx;
+            if (!FILL) {
+                atomicAdd(&cnt[(size_t)b * ntiles + t], 1);
+            } else {
+                const int pos = offs[(size_t)b * (ntiles + 1) + t] + atomicAdd(&cur[(size_t)b * ntiles + t], 1);
+                list[(size_t)b * cap + pos] = c;
+            }
+        }
+}
+
+// offs = exclusive scan of cnt per vector (one block per vector), offs[ntiles] = total or -1 beyond cap.
+__global__ void __launch_bounds__(256)
+k_tile_scan(const int* __restrict__ cnt, int ntiles, int cap, int* __restrict__ offs) {
+    __shared__ int part[256];
+    const int b = blockIdx.x;
+    const int* cb = cnt + (size_t)b * ntiles;
+    int* ob = offs + (size_t)b * (ntiles + 1);
+    const int per = (ntiles + 255) / 256, t0 = threadIdx.x * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
+    int s = 0;
+    for (int t = t0; t < t1; t++) s += cb[t];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int i = 0; i < 256; i++) {
+            const int v = part[i];
+            part[i] = acc;
+            acc += v;
+        }
+        ob[ntiles] = acc <= cap ? acc : -1;
+    }
+    __syncthreads();
+    int acc = part[threadIdx.x];
+    for (int t = t0; t < t1; t++) {
+        ob[t] = acc;
+        acc += cb[t];
+    }
+}
+
+// the chunk lists of the tiles of TX x TY pixels of the box (bin: ntiles x 3 + 1 ints and cap list entries per
+// vector, laid out as cnt | cur | offs | list)
+static int bin_chunks(bb_box box, int B, const short4* bbox, int nchunks, int TX, int TY, int* bin, int cap,
+                      hipStream_t st) {
+    const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
+    const int ntx = (bx + TX - 1) / TX, ntiles = ntx * ((by + TY - 1) / TY);
+    int* cnt = bin;
+    int* cur = cnt + (size_t)B * ntiles;
+    int* offs = cur + (size_t)B * ntiles;
+    int* list = offs + (size_t)B * (ntiles + 1);
+    const int nz = B * ntiles;
+    hipLaunchKernelGGL(k_tile_zero, dim3((nz + 255) / 256), dim3(256), 0, st, cnt, cur, nz);
+    hipLaunchKernelGGL(k_tile_bin<false>, dim3((nchunks + 255) / 256, B), dim3(256), 0, st, bbox, nchunks, TX, TY, ntx,
+                       ntiles, cnt, offs, cur, list, cap);
+    hipLaunchKernelGGL(k_tile_scan, dim3(B), dim3(256), 0, st, cnt, ntiles, cap, offs);
+    hipLaunchKernelGGL(k_tile_bin<true>, dim3((nchunks + 255) / 256, B), dim3(256), 0, st, bbox, nchunks, TX, TY, ntx,
+                       ntiles, cnt, offs, cur, list, cap);
+    return (int)hipGetLastError();
+}
+
+// The bin's layout in a kernel: the list of the tile of vector b, [lo, hi) (tlist NULL: no bin, or it overflowed).
+__device__ __forceinline__ void tile_list(const int* bin, int B, int ntiles, int cap, int b, int tile,
+                                          const int*& tlist, int& lo, int& hi) {
+    tlist = nullptr;
+    lo = hi = 0;
+    if (!bin) return;
+    const int* offs = bin + (size_t)2 * B * ntiles + (size_t)b * (ntiles + 1);
+    if (offs[ntiles] < 0) return;
+    tlist = bin + (size_t)2 * B * ntiles + (size_t)B * (ntiles + 1) + (size_t)b * cap;
+    lo = offs[tile];
+    hi = offs[tile + 1];
+}
+
+// ---------------------------------------------------------------------------------------------------------
 // The deposit of one tile. Per-sub-ring constants are held in registers (Sub), the vertical weights of the
 // chunk being processed, the channel edges and the channel factors in LDS; the line profile is built from
 // the Phi table (galmod.py:183-214) and added to the LDS spectra of the runs kept.
@@ -645,20 +744,45 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
 // it (it synchronises).
 template <class Dep>
 __device__ __forceinline__ void tile_samples(int nchunks, const short4* __restrict__ bb, const short4* __restrict__ sb,
-                                             int tx0, int tx1, int ty0, int ty1, int* clist, int* slist, int& nlist,
-                                             int& nsamp, bool active, Dep&& dep) {
+                                             const int* __restrict__ tlist, int tlo, int thi, int tx0, int tx1,
+                                             int ty0, int ty1, int* clist, int* slist, int& nlist, int& nsamp,
+                                             bool active, Dep&& dep) {
+    // the candidate chunks: entries [tlo, thi) of the tile's chunk list, or all the chunks (tlist NULL)
+    const int lo = tlist ? tlo : 0, hi = tlist ? thi : nchunks;
     const int lane = threadIdx.x % BB_CS, grp = threadIdx.x / BB_CS;
-    for (int base = 0; base < nchunks && active; base += blockDim.x) {
+    for (int base = lo; base < hi && active; base += blockDim.x) {
         // 1. the chunks whose bounding box meets the tile
-        const int c = base + threadIdx.x;
-        if (c < nchunks) {
+        if (base + (int)threadIdx.x < hi) {
+            const int c = tlist ? tlist[base + threadIdx.x] : base + threadIdx.x;
             const short4 r = bb[c];
             if (r.x <= r.y && r.x <= tx1 && r.y >= tx0 && r.z <= ty1 && r.w >= ty0) clist[atomicAdd(&nlist, 1)] = c;
         }
         __syncthreads();
         const int n = nlist;
         __syncthreads();                            // (every thread has read n before the list changes)
-        if (!(n > BB_LIST_CAP - (int)blockDim.x || base + (int)blockDim.x >= nchunks)) continue;
+        if (!(n > BB_LIST_CAP - (int)blockDim.x || base + (int)blockDim.x >= hi)) continue;
+        if (tlist) {
+            // the tile's list is filled in no particular order: sort the chunks (bitonic, in LDS), so that the
+            // samples come back in the order of their sub-rings and azimuths, as a scan of all the chunks gives them
+            int np2 = 1;
+            while (np2 < n) np2 <<= 1;
+            for (int i = n + (int)threadIdx.x; i < np2; i += blockDim.x) clist[i] = 0x7fffffff;
+            __syncthreads();
+            for (int k = 2; k <= np2; k <<= 1)
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    for (int i = threadIdx.x; i < np2; i += blockDim.x) {
+                        const int ixj = i ^ j;
+                        if (ixj > i) {
+                            const int a = clist[i], bv = clist[ixj];
+                            if ((a > bv) == ((i & k) == 0)) {
+                                clist[i] = bv;
+                                clist[ixj] = a;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+        }
         const int ngrp = (int)blockDim.x / BB_CS;
         for (int j0 = 0; j0 < n; j0 += ngrp) {
             // 2. their samples whose vertical segment meets the tile (one lane per azimuth)
@@ -695,7 +819,7 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
           const double2* __restrict__ az, const double* __restrict__ suball, const double* __restrict__ wtaball,
           int nsub, const short4* __restrict__ bball, const short4* __restrict__ sball,
           const double* __restrict__ cd2i, const double* __restrict__ vedges, const double* __restrict__ ptab,
-          int ntab, float* __restrict__ outall, int accumulate) {
+          int ntab, float* __restrict__ outall, int accumulate, const int* __restrict__ bin, int bin_cap) {
     // dynamic LDS: [vedges (nlines, nz+1) f64][cd2i (nz) f64][cd2i (nz) f32][Phi (ntab) f32][weights f32][tile u64]
     extern __shared__ double dyn[];
     __shared__ int clist[BB_LIST_CAP];
@@ -764,7 +888,10 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     T.ay0 = box.blo1 + ty0;
     T.ay1 = box.blo1 + (ty1 < by - 1 ? ty1 : by - 1);
     T.TX = TX;
-    tile_samples(nchunks, bb, sb, tx0, tx1, ty0, ty1, clist, slist, nlist, nsamp, amax > 0., [&](int e) {
+    const int* tl;
+    int tlo, thi;
+    tile_list(bin, gridDim.y, gridDim.x, bin_cap, b, blockIdx.x, tl, tlo, thi);
+    tile_samples(nchunks, bb, sb, tl, tlo, thi, tx0, tx1, ty0, ty1, clist, slist, nlist, nsamp, amax > 0., [&](int e) {
         const int4 ch = chunks[e >> 5];
         const int s = ch.x >> 1;
         Sub u;
@@ -810,12 +937,17 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
                                          const int* naz, const int4* chunks, int nchunks, const double2* az,
                                          int TX, int TY, double* sub, double* wtab, short4* bbox, short4* sbox,
                                          const double* cd2i, const double* vedges, const double* ptab, int ntab,
-                                         int lin_axis, float* out, int accumulate, hipStream_t st) {
+                                         int lin_axis, float* out, int accumulate, int* bin, int bin_cap,
+                                         hipStream_t st) {
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
     hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, naz, sub, wtab, 0);
     hipLaunchKernelGGL(k_an_bbox, dim3((nchunks * BB_CS + 255) / 256, B), dim3(256), 0, st, *g, box, chunks, nchunks,
                        az, sub, nsub, bbox, sbox);
     const int ntiles = ((bx + TX - 1) / TX) * ((by + TY - 1) / TY);
+    if (bin) {
+        const int e = bin_chunks(box, B, bbox, nchunks, TX, TY, bin, bin_cap, st);
+        if (e) return e;
+    }
     const size_t lds = bb_analytic_lds_bytes(g->nz, g->nlines, TX, TY, ntab);
     // a linear velocity axis with one channel factor: the channel edges and factors computed, not read
     const bool lin = lin_axis != 0;
@@ -834,10 +966,12 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
     }
     if (lin)
         hipLaunchKernelGGL(k_an_tile<true>, dim3(ntiles, B), dim3(BB_TILE_THREADS), lds, st, *g, box, TX, TY, chunks,
-                           nchunks, az, sub, wtab, nsub, bbox, sbox, cd2i, vedges, ptab, ntab, out, accumulate);
+                           nchunks, az, sub, wtab, nsub, bbox, sbox, cd2i, vedges, ptab, ntab, out, accumulate, bin,
+                           bin_cap);
     else
         hipLaunchKernelGGL(k_an_tile<false>, dim3(ntiles, B), dim3(BB_TILE_THREADS), lds, st, *g, box, TX, TY, chunks,
-                           nchunks, az, sub, wtab, nsub, bbox, sbox, cd2i, vedges, ptab, ntab, out, accumulate);
+                           nchunks, az, sub, wtab, nsub, bbox, sbox, cd2i, vedges, ptab, ntab, out, accumulate, bin,
+                           bin_cap);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && t_tile_ev[0] && t_tile_ev[1]) e = hipEventRecord(t_tile_ev[1], st);
     return (int)e;
@@ -860,7 +994,7 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
           const double2* __restrict__ az, const double* __restrict__ suball, const double* __restrict__ wtaball,
           int nsub, const short4* __restrict__ bball, const short4* __restrict__ sball, const int2* __restrict__ widx,
           const double2* __restrict__ wval, int nmap, int map0, int nw, double inv_q,
-          unsigned long long* __restrict__ counts) {
+          unsigned long long* __restrict__ counts, const int* __restrict__ bin, int bin_cap) {
     extern __shared__ double dyn[];
     __shared__ int clist[BB_LIST_CAP];
     __shared__ int slist[BB_SAMP_CAP];
@@ -886,8 +1020,11 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     T.ay0 = box.blo1 + ty0;
     T.ay1 = box.blo1 + (ty1 < by - 1 ? ty1 : by - 1);
     T.TX = TX;
-    tile_samples(nchunks, bball + (size_t)b * nchunks, sball + (size_t)b * nchunks * BB_CS, tx0, tx1, ty0, ty1, clist,
-                 slist, nlist, nsamp, true, [&](int e) {
+    const int* tl;
+    int tlo, thi;
+    tile_list(bin, gridDim.y, gridDim.x, bin_cap, b, blockIdx.x, tl, tlo, thi);
+    tile_samples(nchunks, bball + (size_t)b * nchunks, sball + (size_t)b * nchunks * BB_CS, tl, tlo, thi, tx0, tx1, ty0,
+                 ty1, clist, slist, nlist, nsamp, true, [&](int e) {
         const int4 ch = chunks[e >> 5];
         const int s = ch.x >> 1;
         int2 wi = widx[s];
@@ -950,12 +1087,16 @@ extern "C" size_t bb_rm_tile_static() { return (BB_LIST_CAP + BB_SAMP_CAP) * siz
 extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* sr, int nsub, int B, const int* naz,
                                    const int4* chunks, int nchunks, const double2* az, int TX, int TY, double* sub,
                                    double* wtab, short4* bbox, short4* sbox, const int* widx, const double* wval,
-                                   int nmap, int map0, int nw, double qunit, unsigned long long* counts,
-                                   hipStream_t st) {
+                                   int nmap, int map0, int nw, double qunit, unsigned long long* counts, int* bin,
+                                   int bin_cap, hipStream_t st) {
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
     hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, naz, sub, wtab, 1);
     hipLaunchKernelGGL(k_an_bbox, dim3((nchunks * BB_CS + 255) / 256, B), dim3(256), 0, st, *g, box, chunks, nchunks,
                        az, sub, nsub, bbox, sbox);
+    if (bin) {
+        const int e = bin_chunks(box, B, bbox, nchunks, TX, TY, bin, bin_cap, st);
+        if (e) return e;
+    }
     static bool attr = false;
     if (!attr) {
         hipError_t e = hipFuncSetAttribute((const void*)k_rm_tile, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -966,6 +1107,6 @@ extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* s
     const int ntiles = ((bx + TX - 1) / TX) * ((by + TY - 1) / TY);
     hipLaunchKernelGGL(k_rm_tile, dim3(ntiles, B), dim3(BB_RM_THREADS), bb_rm_tile_lds(nw, TX, TY), st, *g, box, TX,
                        TY, chunks, nchunks, az, sub, wtab, nsub, bbox, sbox, (const int2*)widx, (const double2*)wval,
-                       nmap, map0, nw, 1.0 / qunit, counts);
+                       nmap, map0, nw, 1.0 / qunit, counts, bin, bin_cap);
     return (int)hipGetLastError();
 }
