@@ -56,6 +56,17 @@ def config(n):
         par = GalmodPar(GALMODTYPE="ANALYTIC")
         return (h, r, par, 0.05, 5, [("vrot", "ring"), ("vdisp", "ring"), ("inc", "single")],
                 "64x64x40 BAYESIAN 13-D surrogate")
+    if n == "ngc2403":
+        # the geometry of the reference's own likelihood benchmark (performance/bench.py:46-49: the NGC 2403 cube,
+        # 150x150x62, 16" pixels, 5.15 km/s channels, 59" beam; rings 60:700:120, free VROT VDISP_SINGLE INC_SINGLE
+        # PHI_SINGLE), with a synthetic truth in place of the observed data
+        h = emptyheader([150, 150, 62], [-16 / 3600., 16 / 3600., 5.15], beam=59 / 3600., obj="NGC2403LIKE")
+        R = np.arange(60., 700., 120.)
+        r = Rings().setRings(radii=R, xpos=77., ypos=77., vsys=0., vrot=100., vdisp=10., z0=10., dens=1., inc=60.,
+                             phi=123.7)
+        par = GalmodPar(GALMODTYPE="ANALYTIC")
+        return (h, r, par, 0.05, 1, [("vrot", "ring"), ("vdisp", "single"), ("inc", "single"), ("phi", "single")],
+                "150x150x62 NGC 2403-like, the reference's likelihood benchmark")
     raise ValueError(f"no configuration {n}")
 
 

@@ -86,7 +86,7 @@ def _cpu_eval(i):
     blo, bhi = w["box"]
     sl = (slice(blo[1], bhi[1]), slice(blo[0], bhi[0]))
     dens = op.deconvolved(st, w["kern"], 1.0, rr, (blo, bhi), w["ymap"][sl], w["smap"][sl], w["okmap"][sl],
-                          beam_arcsec=30., corr=1. / w["likefactor"], fft=True)
+                          beam_arcsec=abs(w["h"].Bmaj()) * 3600., corr=1. / w["likefactor"], fft=True)
     vals["dens"] = dens.astype(np.float32).astype(np.float64)[None]
     sr, _ = plan.tables(vals)
     m = oracle.galmod_analytic(st, sr[0], blo, bhi, nzq_for(sr[0]), PHI_TAB)
@@ -119,6 +119,32 @@ def cpu_baseline(wl, per_worker=3):
                        f"of the 160x160x96 workload on {P} worker processes, 1 untimed each; median "
                        f"{med * 1e3:.0f} ms/eval/core, wall {wall:.1f} s; single core {1 / med:.2f} evals/s",
                 single_core=1 / med)
+
+
+# The reference's published time of one likelihood call on this geometry, ANALYTIC, INC and PA free, one BLAS
+# thread (CHANGES.md:1276-1278, docs/tasks/3dfit/likelihood.md:586-587)
+REF_NGC2403_MS = 22.0
+
+
+def cpu_calibration(n_eval=6, seed=1):
+    """The oracle on the geometry of the reference's own likelihood benchmark (performance/bench.py:46-49), one
+    worker process, one core: ms per call beside the reference's published 22 ms, so that the CPU baseline can be
+    read against the reference's own speed."""
+    import multiprocessing as mp
+    from bbarolo2_amd.bayes import synthetic_fit
+    fit = synthetic_fit("ngc2403", chunk=16)
+    u = np.random.default_rng(seed).random((n_eval, fit["free"].ndim))
+    fit["theta"] = fit["lo"] + u * (fit["hi"] - fit["lo"])
+    small = {k: fit[k] for k in ("h", "truth", "par", "data", "likemask", "region", "free", "theta", "box", "kern",
+                                 "sigma", "likefactor")}
+    with mp.get_context("spawn").Pool(1, initializer=_cpu_worker_init, initargs=(small,)) as pool:
+        pool.map(_cpu_eval, [0], chunksize=1)
+        times = pool.map(_cpu_eval, range(n_eval), chunksize=n_eval)
+    ms = 1e3 * float(np.median(times))
+    return {"geometry": "150x150x62, 16\" pixels, 5.15 km/s, beam 59\", 6 rings (60:700:120), ANALYTIC, free VROT "
+                        "VDISP_SINGLE INC_SINGLE PHI_SINGLE, synthetic truth and noise",
+            "oracle_ms_per_call": round(ms, 2), "reference_published_ms_per_call": REF_NGC2403_MS,
+            "oracle_over_reference": round(ms / REF_NGC2403_MS, 3), "evaluations": n_eval}
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):
@@ -161,6 +187,8 @@ def main():
     if not a.no_cpu:
         cpu = cpu_baseline(wl)
         print(f"[bench] cpu baseline {cpu['value']:.2f} evals/s on {cpu['cores']} cores", file=sys.stderr, flush=True)
+        cpu["calibration"] = cpu_calibration()
+        print(f"[bench] cpu calibration {cpu['calibration']}", file=sys.stderr, flush=True)
     dm, like = wl["dm"], wl["like"]
     if like.profile is None or not like.update_prof:
         raise RuntimeError("the workload fits the geometry without DENS: the profile must be per vector")
@@ -286,7 +314,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_an_tile", "kernel_ms": round(tile_ms, 4), "achieved": ach / 1e9,
                      "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": ach / PEAK_HBM, "traffic": traffic,
                      "traffic_unit": "bytes per launch", "algorithmic_bytes": 4 * nvox * B},
-        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample",
+                                                                      "calibration")},
         "loglike_sample": [float(x) for x in ll],
     }
     if cpu is not None:
