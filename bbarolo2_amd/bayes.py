@@ -17,7 +17,7 @@ from .surrogate import MLPSurrogate
 from .synth import config, default_priors, add_noise, likelihood_inputs, theta_of
 
 
-def synthetic_fit(n, chunk=1024):
+def synthetic_fit(n, chunk=1024, graphs=False):
     """The likelihood of configuration n on the current GPU: its truth cube (ANALYTIC GALMOD of the truth rings on
     the GPU, convolved with the beam), noise, likelihood inputs, and the uniform priors of the free parameters.
     Returns a dict: like (GpuLikelihood), dm (DiskModel), free, lo, hi, theta_true, data, h, truth, par, box, sigma,
@@ -37,7 +37,7 @@ def synthetic_fit(n, chunk=1024):
     likemask, region, sigma, likefactor = likelihood_inputs(h, model_zyx, data, rms, box)
     free = FreePars(spec, truth.nr)
     like = GpuLikelihood(dm, data, truth, free, likemask=likemask, region=region, likebox=box, sigma=sigma,
-                         likefactor=likefactor, chunk=chunk)
+                         likefactor=likefactor, chunk=chunk, graphs=graphs)
     lo, hi = default_priors(truth, free)
     return dict(like=like, free=free, lo=lo, hi=hi, theta_true=theta_of(truth, free), data=data, h=h, truth=truth,
                 par=par, box=box, sigma=sigma, desc=desc, dm=dm, likemask=likemask, region=region,

@@ -94,7 +94,7 @@ class GpuLikelihood:
 
     def __init__(self, dm, data, rings0, free, likemask=None, region=None, likebox=None, sigma=1.0,
                  likefactor=1.0, fracerr=None, maskmode=2, chunk=256, densprof_smooth=SMOOTH_DEFAULT,
-                 likefactor2d=None, beam_arcsec=None):
+                 likefactor2d=None, beam_arcsec=None, graphs=False):
         torch = _torch()
         self.torch = torch
         self.dm = dm
@@ -146,6 +146,10 @@ class GpuLikelihood:
         self.plan = SubringPlan(dm.setup, self.rings0, dm.ringmode, final=True)
         self._bufs = {}
         self._streams = None
+        # graphs: a batch of a given size runs as one captured HIP graph (ANALYTIC models), replayed with the
+        # tables of every new batch copied into its input buffers
+        self.use_graphs = bool(graphs) and dm.setup.galmodtype != "MC"
+        self._graphs = {}
         # DENS not fitted: the deconvolved profile of the initial rings, and of every parameter vector when the
         # geometry is free (fitter.py:606-615)
         self.useNorm = "dens" not in free.index
@@ -287,10 +291,41 @@ class GpuLikelihood:
             parts.append(p)
         return parts
 
+    def _graph_batch(self, sr, vals):
+        """The launches of device_batch for the tables sr (and vals), as a HIP graph per batch size: captured on the
+        first batch of that size (after one eager run that allocates the buffers), replayed afterwards with the new
+        tables copied into the graph's input buffers. The ring orientation of the deposit tiles is that of the first
+        batch (it changes the speed, not the result). Returns device_batch's output tensors."""
+        torch = self.torch
+        B = sr.shape[0]
+        parts = self.split_parts(sr, vals)
+        g = self._graphs.get(B)
+        if g is None:
+            self.device_batch(parts)               # (eager: allocates the buffers, sets the kernel attributes)
+            torch.cuda.synchronize()
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                out = self.device_batch(parts)
+            torch.cuda.synchronize()
+            g = (graph, parts, out)
+            if len(self._graphs) >= 4:                # (a few batch sizes: a sampler's batches vary in size)
+                self._graphs.pop(next(iter(self._graphs)))
+            self._graphs[B] = g
+        graph, static, out = g
+        for p, q in zip(static, parts):
+            p["d_sr"].copy_(q["d_sr"])
+            if "d_sr_ext" in p:
+                p["d_sr_ext"].copy_(q["d_sr_ext"])
+        graph.replay()
+        return out
+
     def residuals(self, sr, vals=None):
-        """(S_box, lg) per table: device launches for one chunk of sub-ring tables (B, 16, nsub), in two halves on
+        """(S_box, lg) per table: device launches for one chunk of sub-ring tables (B, 16, nsub), in parts on
         two streams; with a per-vector profile, vals are the ring values of the free keys (for the annulus)."""
-        res = self.device_batch(self.split_parts(sr, vals))
+        if self.use_graphs:
+            res = self._graph_batch(sr, vals)
+        else:
+            res = self.device_batch(self.split_parts(sr, vals))
         self.torch.cuda.current_stream().synchronize()
         ss = np.concatenate([r[0].cpu().numpy() for r in res])
         lg = np.concatenate([r[1].cpu().numpy() for r in res])
