@@ -394,6 +394,39 @@ extern "C" int bb_residual_gauss(const float* model, const float* dvalid, int B,
     return rc ? fail(BB_ERR_HIP, "bb_residual_gauss launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
 
+extern "C" int64_t bb_conv_resid_blocks(int by, int bx, int nz);
+extern "C" int bb_launch_conv_resid(const float* in, const float* dvalid, int B, int by, int bx, int nz,
+                                    const float* kern, int ky, int kx, float scale, double sigma2, double eps2,
+                                    double* partial, hipStream_t st);
+extern "C" int bb_launch_residual_final(const double* partial, int B, int nblk, double* out_ss, double* out_lg,
+                                        hipStream_t st);
+
+extern "C" int64_t bb_conv_residual_work_size(const bb_model* m, int B, int by, int bx) {
+    if (!m || B < 0 || by < 1 || bx < 1) return -1;
+    return (int64_t)B * bb_conv_resid_blocks(by, bx, m->g.nz) * 2;
+}
+
+extern "C" int bb_conv_residual_gauss(const bb_model* m, const float* dep, const float* dvalid, int B, int by, int bx,
+                                      double sigma2, double eps2, double* work, int64_t work_len, double* out_ss,
+                                      double* out_lg, void* stream) {
+    if (!m) return fail(BB_ERR_ARG, "bb_conv_residual_gauss: model is NULL");
+    if (!m->d_kern) return fail(BB_ERR_ARG, "bb_conv_residual_gauss: the model has no beam kernel");
+    if (B < 0 || by < 1 || bx < 1 || B > 65535) return fail(BB_ERR_ARG, "bb_conv_residual_gauss: bad sizes");
+    if (B == 0) return 0;
+    if (!dep || !dvalid || !work || !out_ss) return fail(BB_ERR_ARG, "bb_conv_residual_gauss: NULL array");
+    if (eps2 > 0 && (!out_lg || !(sigma2 > 0)))
+        return fail(BB_ERR_ARG, "bb_conv_residual_gauss: fracerr needs sigma2 > 0 and out_lg");
+    const int64_t need = bb_conv_residual_work_size(m, B, by, bx);
+    if (work_len < need)
+        return fail(BB_ERR_ARG, "bb_conv_residual_gauss: work holds %lld doubles, needs %lld", (long long)work_len,
+                    (long long)need);
+    hipStream_t st = (hipStream_t)stream;
+    int rc = bb_launch_conv_resid(dep, dvalid, B, by, bx, m->g.nz, m->d_kern, m->ky, m->kx, (float)m->scale, sigma2,
+                                  eps2, work, st);
+    if (!rc) rc = bb_launch_residual_final(work, B, (int)(need / (2 * B)), out_ss, out_lg, st);
+    return rc ? fail(BB_ERR_HIP, "bb_conv_residual_gauss launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
+}
+
 // ---------------------------------------------------------------------------------------------------------
 // Ring maps and the deconvolved density profile.
 extern "C" int64_t bb_ring_maps_work_size(const bb_azplan* p, int B) { return bb_analytic_work_size(p, B); }

@@ -76,6 +76,111 @@ k_conv2d_same(const float* __restrict__ inall, float* __restrict__ outall, int b
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// The convolution fused with the masked Gaussian residual (FFTConvolver then _sumsq_residuals /
+// _residuals_fracerr, bbarolo/smooth.py:145-165, bbarolo/bayesian/likelihood.py:22-77): every convolved voxel is
+// compared at once with the data (dvalid, yxz, NaN where not summed) and never written. The terms are summed in
+// FP64 per thread, then per block in a fixed order into partial (2 doubles per block: the residual sum and, for
+// FRAC, the log-variance sum); k_residual_final reduces the blocks of every vector in a fixed order.
+__device__ __forceinline__ double cv_wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+template <int KX, bool FRAC>
+__global__ void __launch_bounds__(BB_CV_THREADS)
+k_conv_resid(const float* __restrict__ inall, const float* __restrict__ dvalid, int by, int bx, int nz, int ZL,
+             const float* __restrict__ kern, int ky, float scale, float sigma2f, float eps2f,
+             double* __restrict__ partial) {
+    constexpr int W = BB_CV_RX + KX - 1;
+    constexpr int hx = KX / 2;
+    __shared__ double red[2][BB_CV_THREADS / 64];
+    const int hy = ky / 2;
+    const int b = blockIdx.z;
+    const int zl = threadIdx.x % ZL, grp = threadIdx.x / ZL, ngrp = blockDim.x / ZL;
+    const int z = blockIdx.y * ZL + zl;
+    const int nbx = (bx + BB_CV_RX - 1) / BB_CV_RX, nby = (by + BB_CV_RY - 1) / BB_CV_RY;
+    const int blk = blockIdx.x * ngrp + grp;
+    double s = 0., lg = 0.;
+    if (z < nz && blk < nbx * nby) {
+        const int y0 = (blk / nbx) * BB_CV_RY, x0 = (blk % nbx) * BB_CV_RX;
+        const float* in = inall + (size_t)b * by * bx * nz + z;
+        float acc[BB_CV_RY][BB_CV_RX];
+#pragma unroll
+        for (int r = 0; r < BB_CV_RY; r++)
+#pragma unroll
+            for (int c = 0; c < BB_CV_RX; c++) acc[r][c] = 0.f;
+        const int iylo = y0 - hy, iyhi = y0 + BB_CV_RY - 1 + hy;
+        for (int iy = iylo; iy <= iyhi; iy++) {
+            if (iy < 0 || iy >= by) continue;
+            float row[W];
+            const float* ir = in + (size_t)iy * bx * nz;
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                const int ix = x0 - hx + w;
+                row[w] = (ix >= 0 && ix < bx) ? ir[(size_t)ix * nz] : 0.f;
+            }
+#pragma unroll
+            for (int r = 0; r < BB_CV_RY; r++) {
+                const int j = y0 + r + hy - iy;
+                if (j < 0 || j >= ky) continue;
+                const float* kr = kern + j * KX;
+#pragma unroll
+                for (int i = 0; i < KX; i++) {
+                    const float kv = kr[i];
+#pragma unroll
+                    for (int c = 0; c < BB_CV_RX; c++) acc[r][c] = fmaf(kv, row[c + KX - 1 - i], acc[r][c]);
+                }
+            }
+        }
+        const float* dv = dvalid + z;
+#pragma unroll
+        for (int r = 0; r < BB_CV_RY; r++) {
+            const int y = y0 + r;
+            if (y >= by) break;
+#pragma unroll
+            for (int c = 0; c < BB_CV_RX; c++) {
+                const int x = x0 + c;
+                if (x < bx) {
+                    const float d = dv[((size_t)y * bx + x) * nz];
+                    if (d == d) {
+                        const float v = acc[r][c];
+                        const float mm = v < 1e-12f ? 0.f : v * scale;
+                        const float rr = d - mm;
+                        if (!FRAC) {
+                            s += (double)(rr * rr);
+                        } else if (mm != 0.f) {
+                            const float var = fmaf(eps2f * mm, mm, sigma2f);
+                            s += (double)(rr * rr / var);
+                            lg += (double)logf(var / sigma2f);
+                        } else {
+                            s += (double)(rr * rr / sigma2f);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    s = cv_wave_sum(s);
+    if (FRAC) lg = cv_wave_sum(lg);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][w] = s;
+        red[1][w] = lg;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0., c = 0.;
+        for (int k = 0; k < BB_CV_THREADS / 64; k++) {
+            a += red[0][k];
+            c += red[1][k];
+        }
+        const size_t o = ((size_t)b * gridDim.y * gridDim.x + (size_t)blockIdx.y * gridDim.x + blockIdx.x) * 2;
+        partial[o] = a;
+        partial[o + 1] = c;
+    }
+}
+
 // Channel lanes per group: the width in {64, 32, 16, 8} that leaves the fewest idle lanes.
 static int pick_zl(int nz) {
     int best = 64, waste = 1 << 30;
@@ -100,6 +205,44 @@ static void launch(const float* in, float* out, int B, int by, int bx, int nz, c
 }
 
 extern "C" int bb_conv_max_kx() { return 25; }
+
+// Blocks per vector of k_conv_resid (2 partial doubles each).
+extern "C" int64_t bb_conv_resid_blocks(int by, int bx, int nz) {
+    const int ZL = pick_zl(nz);
+    const int ngrp = BB_CV_THREADS / ZL;
+    const int64_t nblk = (int64_t)((bx + BB_CV_RX - 1) / BB_CV_RX) * ((by + BB_CV_RY - 1) / BB_CV_RY);
+    return ((nblk + ngrp - 1) / ngrp) * ((nz + ZL - 1) / ZL);
+}
+
+template <int KX>
+static void launch_cr(const float* in, const float* dvalid, int B, int by, int bx, int nz, const float* kern, int ky,
+                      float scale, float sigma2, float eps2, double* partial, hipStream_t st) {
+    const int ZL = pick_zl(nz);
+    const int ngrp = BB_CV_THREADS / ZL;
+    const int nblk = ((bx + BB_CV_RX - 1) / BB_CV_RX) * ((by + BB_CV_RY - 1) / BB_CV_RY);
+    dim3 grid((nblk + ngrp - 1) / ngrp, (nz + ZL - 1) / ZL, B);
+    if (eps2 > 0.f)
+        hipLaunchKernelGGL((k_conv_resid<KX, true>), grid, dim3(BB_CV_THREADS), 0, st, in, dvalid, by, bx, nz, ZL, kern,
+                           ky, scale, sigma2, eps2, partial);
+    else
+        hipLaunchKernelGGL((k_conv_resid<KX, false>), grid, dim3(BB_CV_THREADS), 0, st, in, dvalid, by, bx, nz, ZL,
+                           kern, ky, scale, 1.f, 0.f, partial);
+}
+
+// The fused convolution and residual: partial (B x bb_conv_resid_blocks x 2 doubles).
+extern "C" int bb_launch_conv_resid(const float* in, const float* dvalid, int B, int by, int bx, int nz,
+                                    const float* kern, int ky, int kx, float scale, double sigma2, double eps2,
+                                    double* partial, hipStream_t st) {
+    switch (kx) {
+#define BB_CASE(K) \
+    case K: launch_cr<K>(in, dvalid, B, by, bx, nz, kern, ky, scale, (float)sigma2, (float)eps2, partial, st); break;
+        BB_CASE(1) BB_CASE(3) BB_CASE(5) BB_CASE(7) BB_CASE(9) BB_CASE(11) BB_CASE(13) BB_CASE(15) BB_CASE(17)
+        BB_CASE(19) BB_CASE(21) BB_CASE(23) BB_CASE(25)
+#undef BB_CASE
+        default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
 
 // kern: (ky, kx) float32 taps on the device, kx odd <= 25.
 extern "C" int bb_launch_conv2d_same(const float* in, float* out, int B, int by, int bx, int nz, const float* kern,

@@ -96,6 +96,12 @@ k_residual_final(const double* __restrict__ partial, int nblk, double* __restric
     }
 }
 
+extern "C" int bb_launch_residual_final(const double* partial, int B, int nblk, double* out_ss, double* out_lg,
+                                        hipStream_t st) {
+    hipLaunchKernelGGL(k_residual_final, dim3(B), dim3(BB_RS_THREADS), 0, st, partial, nblk, out_ss, out_lg);
+    return (int)hipGetLastError();
+}
+
 extern "C" int bb_residual_blocks(int64_t nvox) {
     const int64_t per = (int64_t)BB_RS_THREADS * BB_RS_PER_THREAD;
     return (int)((nvox + per - 1) / per);

@@ -23,7 +23,7 @@ import numpy as np
 
 from . import _lib
 from .engine import _torch, _stream_ptr
-from .galmod import SubringPlan
+from .galmod import SubringPlan, nzq_for
 from .profile import DensityProfile, SMOOTH_DEFAULT
 from .rings import Rings
 
@@ -186,11 +186,15 @@ class GpuLikelihood:
             by, bx, nz = self.bhi[1] - self.blo[1], self.bhi[0] - self.blo[0], self.dm.setup.nsubs
             L = _lib.lib()
             wl = int(L.bb_residual_work_size(B, self.nvox_box))
+            cwl = int(L.bb_conv_residual_work_size(self.dm.handle(), B, by, bx)) if self.dm.cfield is not None else 1
             b = dict(dep=torch.empty((B, by, bx, nz), dtype=torch.float32, device=self.dev),
-                     mod=torch.empty((B, by, bx, nz), dtype=torch.float32, device=self.dev),
                      work=torch.empty(wl, dtype=torch.float64, device=self.dev),
+                     cwork=torch.empty(max(cwl, 1), dtype=torch.float64, device=self.dev), cwl=cwl,
                      ss=torch.empty(B, dtype=torch.float64, device=self.dev),
                      lg=torch.empty(B, dtype=torch.float64, device=self.dev), wl=wl)
+            if self.dm.setup.galmodtype == "MC":
+                # (the MC model is convolved into its own buffer, then compared)
+                b["mod"] = torch.empty((B, by, bx, nz), dtype=torch.float32, device=self.dev)
             self._bufs = {k: v for k, v in self._bufs.items() if k[0] == B}
             self._bufs[(B, slot)] = b
         return b
@@ -237,8 +241,17 @@ class GpuLikelihood:
             mod = self.dm.array_batch_mc(sr, self.blo, self.bhi, convolve=True, out=bufs["mod"], work=bufs["dep"],
                                          d_sr=d_sr, max_clouds=int(self.dm.mc_clouds(tab).max()))
         else:
-            mod = self.dm.array_batch(sr, self.blo, self.bhi, convolve=True, out=bufs["mod"], work=bufs["dep"],
-                                      d_sr=d_sr)
+            dep = self.dm.array_batch(sr, self.blo, self.bhi, convolve=False, work=bufs["dep"], d_sr=d_sr)
+            if self.dm.cfield is not None:
+                # the beam convolution and the residual in one pass: the convolved models are never stored
+                by, bx = self.bhi[1] - self.blo[1], self.bhi[0] - self.blo[0]
+                h = self.dm.handle(nzq_for(sr.reshape(16, -1)))
+                _lib.check(_lib.lib().bb_conv_residual_gauss(h, dep.data_ptr(), self.dvalid.data_ptr(), B, by, bx,
+                                                             self.sigma ** 2, self.eps ** 2, bufs["cwork"].data_ptr(),
+                                                             bufs["cwl"], bufs["ss"].data_ptr(),
+                                                             bufs["lg"].data_ptr(), _stream_ptr(self.torch)))
+                return bufs["ss"], bufs["lg"], dens
+            mod = dep
         s2 = self.sigma ** 2
         _lib.check(_lib.lib().bb_residual_gauss(mod.data_ptr(), self.dvalid.data_ptr(), B, self.nvox_box, s2,
                                                 self.eps ** 2, bufs["work"].data_ptr(), bufs["wl"],

@@ -213,7 +213,7 @@ def main():
     s2 = like.sigma ** 2
     stream = torch.cuda.current_stream()
     sp = _stream_ptr(torch)
-    names = ["ring_maps", "profile_solve", "galmod_analytic", "conv2d_same", "residual"]
+    names = ["ring_maps", "profile_solve", "galmod_analytic", "conv_residual"]
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
     stage = np.zeros(len(names))
     # the deposit kernel alone (k_an_tile), timed by events that the library records around its launch
@@ -235,12 +235,10 @@ def main():
         _lib.check(L.bb_galmod_analytic(h, plan, box, d_sr.data_ptr(), sr.shape[2], B, ws.data_ptr(), wb,
                                         bufs["dep"].data_ptr(), 0, sp))
         mark(3)
-        _lib.check(L.bb_conv2d_same(h, bufs["dep"].data_ptr(), bufs["mod"].data_ptr(), B, by, bx, sp))
+        _lib.check(L.bb_conv_residual_gauss(h, bufs["dep"].data_ptr(), like.dvalid.data_ptr(), B, by, bx, s2, 0.0,
+                                            bufs["cwork"].data_ptr(), bufs["cwl"], bufs["ss"].data_ptr(),
+                                            bufs["lg"].data_ptr(), sp))
         mark(4)
-        _lib.check(L.bb_residual_gauss(bufs["mod"].data_ptr(), like.dvalid.data_ptr(), B, nvox, s2, 0.0,
-                                       bufs["work"].data_ptr(), bufs["wl"], bufs["ss"].data_ptr(),
-                                       bufs["lg"].data_ptr(), sp))
-        mark(5)
 
     # the timed steps: the batch as GpuLikelihood evaluates it, two halves on two streams (device_batch)
     parts = like.split_parts(sr, vals)
@@ -292,7 +290,7 @@ def main():
     # algorithmic HBM bytes per evaluation: the cube written once (deposit), read and written (convolution), read
     # with the data (residual); the maps: counts written and read, FP64 maps written (finish) and read (Gram)
     bytes_per_eval = {"ring_maps": 8 * nmap * 3, "profile_solve": 8 * nmap, "galmod_analytic": 4 * nvox,
-                      "conv2d_same": 8 * nvox, "residual": 8 * nvox}
+                      "conv_residual": 8 * nvox}
     # the dominant kernel, k_an_tile: it writes the model cube once (4 B per voxel); everything else it touches
     # (sub-ring records, boxes, LDS tables) is small and re-read from cache
     ach = 4 * nvox * B / (tile_ms * 1e-3)

@@ -99,6 +99,16 @@ int64_t bb_residual_work_size(int B, int64_t nvox);
 int bb_residual_gauss(const float* model, const float* dvalid, int B, int64_t nvox, double sigma2, double eps2,
                       double* work, int64_t work_len, double* out_ss, double* out_lg, void* stream);
 
+/* The beam convolution fused with the masked Gaussian residual: FFTConvolver (bbarolo/smooth.py:145-165, with the
+ * 1e-12 cut and the flux scale of DiskModel.convolve, galmod.py:1273-1283) followed by _sumsq_residuals /
+ * _residuals_fracerr (bbarolo/bayesian/likelihood.py:22-77), the convolved model compared with the data voxel by
+ * voxel and never stored. dep (B, by, bx, nz) float32 yxz: the unconvolved models (device); dvalid as in
+ * bb_residual_gauss. work: bb_conv_residual_work_size(m, B, by, bx) float64 elements. */
+int64_t bb_conv_residual_work_size(const bb_model* m, int B, int by, int bx);
+int bb_conv_residual_gauss(const bb_model* m, const float* dep, const float* dvalid, int B, int by, int bx,
+                           double sigma2, double eps2, double* work, int64_t work_len, double* out_ss, double* out_lg,
+                           void* stream);
+
 /* Monte-Carlo GALMOD (_galmod_mc, bbarolo/galmod.py:92-171) of B sub-ring tables sr (B, 16, nsub) with a
  * counter-based stream: the draws of cloud ic of sub-ring s are Philox4x32-10 of counters (ic, block) under the
  * key (seed, s) (the reference's xoroshiro128+ / ziggurat stream is sequential), so the model of a box is the crop

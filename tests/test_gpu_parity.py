@@ -149,3 +149,45 @@ def test_batch_rows_independent(small_header):
     for b in range(3):
         one = dm.array_batch(sr[b:b + 1], blo, bhi, convolve=True)[0].cpu().numpy()
         assert np.abs(got[b] - one).max() <= 1e-6 * np.abs(one).max()
+
+
+def test_conv_residual_fused(small_header):
+    """bb_conv_residual_gauss (the convolution and the residual in one pass) against bb_conv2d_same followed by
+    bb_residual_gauss (each pinned to the oracle above): the same FP32 terms summed in FP64 in another order, so
+    within 1e-12 relative; both variants of the residual, blanks, and voxels under the 1e-12 cut."""
+    import torch
+    from bbarolo2_amd import _lib
+    from bbarolo2_amd.engine import DiskModel
+    L = _lib.lib()
+    dm = DiskModel(small_header, galmodtype="ANALYTIC")
+    h = dm.handle()
+    rng = np.random.default_rng(11)
+    B, by, bx, nz = 3, 37, 41, small_header.DimAx(2)
+    dep = rng.random((B, by, bx, nz)).astype(np.float32)
+    dep[rng.random(dep.shape) < 0.5] = 0.            # (zeros around which the convolution falls under 1e-12)
+    dv = (rng.random((by, bx, nz)) + 0.1).astype(np.float32)
+    dv[rng.random(dv.shape) < 0.1] = np.nan
+    ddep, ddv = torch.from_numpy(dep).cuda(), torch.from_numpy(dv).cuda()
+    mod = torch.empty_like(ddep)
+    _lib.check(L.bb_conv2d_same(h, ddep.data_ptr(), mod.data_ptr(), B, by, bx, None))
+    nvox = by * bx * nz
+    for eps2 in (0.0, 0.04):
+        s2 = 0.3
+        wl = int(L.bb_residual_work_size(B, nvox))
+        work = torch.empty(wl, dtype=torch.float64, device="cuda")
+        ss0, lg0 = (torch.empty(B, dtype=torch.float64, device="cuda") for _ in range(2))
+        _lib.check(L.bb_residual_gauss(mod.data_ptr(), ddv.data_ptr(), B, nvox, s2, eps2, work.data_ptr(), wl,
+                                       ss0.data_ptr(), lg0.data_ptr(), None))
+        cwl = int(L.bb_conv_residual_work_size(h, B, by, bx))
+        cwork = torch.empty(cwl, dtype=torch.float64, device="cuda")
+        ss1, lg1 = (torch.empty(B, dtype=torch.float64, device="cuda") for _ in range(2))
+        _lib.check(L.bb_conv_residual_gauss(h, ddep.data_ptr(), ddv.data_ptr(), B, by, bx, s2, eps2,
+                                            cwork.data_ptr(), cwl, ss1.data_ptr(), lg1.data_ptr(), None))
+        a, b = ss0.cpu().numpy(), ss1.cpu().numpy()
+        assert np.all(np.abs(a - b) <= 1e-12 * np.abs(a)), (eps2, a, b)
+        if eps2 > 0:
+            a, b = lg0.cpu().numpy(), lg1.cpu().numpy()
+            assert np.all(np.abs(a - b) <= 1e-12 * np.abs(a)), (a, b)
+    # too small a work buffer is rejected before any launch
+    assert L.bb_conv_residual_gauss(h, ddep.data_ptr(), ddv.data_ptr(), B, by, bx, 0.3, 0.0, cwork.data_ptr(), 1,
+                                    ss1.data_ptr(), None, None) != 0
