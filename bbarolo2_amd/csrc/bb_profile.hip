@@ -72,6 +72,64 @@ k_rm_finish(const unsigned long long* __restrict__ counts, double* __restrict__ 
     out[(size_t)y * bx + x] = acc * unit * scale;
 }
 
+// The same convolution with the kernel width a template parameter (odd, <= 25): one wavefront per 16 x 16 output
+// tile, every lane computes BB_RMF_RX consecutive outputs of one row; it holds each input row segment that feeds
+// them in registers and adds it through every kernel row, so an input value is read from LDS once per kernel row
+// instead of once per tap (the taps are wave-uniform LDS reads).
+#define BB_RMF_RX 4
+template <int KX>
+__global__ void __launch_bounds__(64)
+k_rm_finish_t(const unsigned long long* __restrict__ counts, double* __restrict__ maps, int by, int bx,
+              const double* __restrict__ kern, int ky, double unit, double scale) {
+    extern __shared__ double rm_lds[];
+    constexpr int hx = KX / 2, WR = BB_RMF_RX + KX - 1, W = BB_RM_T + KX - 1;
+    const int p = blockIdx.z;
+    const int hy = ky / 2, H = BB_RM_T + ky - 1;
+    const int x0 = blockIdx.x * BB_RM_T, y0 = blockIdx.y * BB_RM_T;
+    const unsigned long long* in = counts + (size_t)p * by * bx;
+    double* out = maps + (size_t)p * by * bx;
+    double* tile = rm_lds;
+    double* tk = rm_lds + W * H;
+    const int iy0 = y0 + hy - (ky - 1), ix0 = x0 + hx - (KX - 1);
+    int any = 0;
+    for (int e = threadIdx.x; e < W * H; e += 64) {
+        const int r = e / W, c = e % W;
+        const int iy = iy0 + r, ix = ix0 + c;
+        const unsigned long long v = (iy >= 0 && iy < by && ix >= 0 && ix < bx) ? in[(size_t)iy * bx + ix] : 0ull;
+        any |= v != 0ull;
+        tile[e] = (double)v;
+    }
+    for (int e = threadIdx.x; e < ky * KX; e += 64) tk[e] = kern[e];
+    const int nonzero = __syncthreads_or(any);
+    const int ty = threadIdx.x / (BB_RM_T / BB_RMF_RX), tx = (threadIdx.x % (BB_RM_T / BB_RMF_RX)) * BB_RMF_RX;
+    const int y = y0 + ty;
+    if (y >= by) return;
+    double acc[BB_RMF_RX];
+#pragma unroll
+    for (int c = 0; c < BB_RMF_RX; c++) acc[c] = 0.0;
+    if (nonzero) {
+        // out(y, x) = sum_{j,i} K[j][i] tile[ty + ky - 1 - j][tx + c + KX - 1 - i]
+        for (int j = 0; j < ky; j++) {
+            const double* row = tile + (ty + ky - 1 - j) * W + tx;
+            double rv[WR];
+#pragma unroll
+            for (int w = 0; w < WR; w++) rv[w] = row[w];
+            const double* kr = tk + j * KX;
+#pragma unroll
+            for (int i = 0; i < KX; i++) {
+                const double kv = kr[i];
+#pragma unroll
+                for (int c = 0; c < BB_RMF_RX; c++) acc[c] = fma(kv, rv[c + KX - 1 - i], acc[c]);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < BB_RMF_RX; c++) {
+        const int x = x0 + tx + c;
+        if (x < bx) out[(size_t)y * bx + x] = acc[c] * unit * scale;
+    }
+}
+
 // ---------------------------------------------------------------------------------------------------------
 // Entry e of the upper triangle (i <= j) of an m x m matrix, row by row.
 __device__ __forceinline__ void tri_ij(int e, int m, int& i, int& j) {
@@ -89,6 +147,7 @@ __global__ void __launch_bounds__(BB_GR_THREADS)
 k_gram(const double* __restrict__ maps, const double* __restrict__ wpix, const double* __restrict__ ypix, int n,
        int npix, int ppb, double* __restrict__ partial) {
     extern __shared__ double gr_lds[];
+    __shared__ int act[BB_NN_MAX + 1];          // columns with a non-zero value among the staged pixels
     const int m = n + 1, ne = m * (m + 1) / 2;
     const int b = blockIdx.y, blk = blockIdx.x, nblk = gridDim.x;
     double* st = gr_lds;                        // [BB_GR_PIX][m + 1] (odd stride)
@@ -108,6 +167,8 @@ k_gram(const double* __restrict__ maps, const double* __restrict__ wpix, const d
         for (int pa = p0; pa < p1; pa += BB_GR_PIX) {
             const int np = min(BB_GR_PIX, p1 - pa);
             __syncthreads();
+            for (int c = threadIdx.x; c < m; c += BB_GR_THREADS) act[c] = 0;
+            __syncthreads();
             for (int e = threadIdx.x; e < np * m; e += BB_GR_THREADS) {
                 const int c = e / np, q = e % np;     // column c, pixel q: coalesced reads of one map
                 const int pix = pa + q;
@@ -116,11 +177,14 @@ k_gram(const double* __restrict__ maps, const double* __restrict__ wpix, const d
                 if (c < n) v = w != 0.0 ? maps[((size_t)b * n + c) * npix + pix] * w : 0.0;
                 else v = ypix[pix];
                 st[q * sm + c] = v;
+                if (v != 0.0) act[c] = 1;
             }
             __syncthreads();
+            // (a ring's map is non-zero only near the ring: the products of columns that are zero on all the
+            // staged pixels add nothing and are skipped)
 #pragma unroll
             for (int k = 0; k < EPT; k++) {
-                if (ei[k] < 0) continue;
+                if (ei[k] < 0 || !act[ei[k]] || !act[ej[k]]) continue;
                 const double* a = st + ei[k];
                 const double* c = st + ej[k];
                 double s = acc[k];
@@ -506,6 +570,24 @@ extern "C" int bb_launch_rm_finish(const unsigned long long* counts, double* map
                                    const double* kern, int ky, int kx, double unit, double scale, int convolve,
                                    hipStream_t st) {
     dim3 grid((bx + BB_RM_T - 1) / BB_RM_T, (by + BB_RM_T - 1) / BB_RM_T, P);
+    if (convolve && kx <= 25 && (kx & 1)) {
+        const size_t lt = bb_rm_finish_lds(ky, kx);
+        switch (kx) {
+#define BB_CASE(K)                                                                                                 \
+    case K:                                                                                                        \
+        if (lt > 64 * 1024) {                                                                                      \
+            hipError_t e = hipFuncSetAttribute((const void*)k_rm_finish_t<K>,                                     \
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);            \
+            if (e != hipSuccess) return (int)e;                                                                    \
+        }                                                                                                          \
+        hipLaunchKernelGGL(k_rm_finish_t<K>, grid, dim3(64), lt, st, counts, maps, by, bx, kern, ky, unit, scale); \
+        break;
+            BB_CASE(1) BB_CASE(3) BB_CASE(5) BB_CASE(7) BB_CASE(9) BB_CASE(11) BB_CASE(13) BB_CASE(15) BB_CASE(17)
+            BB_CASE(19) BB_CASE(21) BB_CASE(23) BB_CASE(25)
+#undef BB_CASE
+        }
+        return (int)hipGetLastError();
+    }
     const size_t lds = convolve ? bb_rm_finish_lds(ky, kx) : 0;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)k_rm_finish, hipFuncAttributeMaxDynamicSharedMemorySize,
