@@ -28,6 +28,7 @@
 #define BB_CB 8       // channels of a line profile computed together
 #define BB_TILE_THREADS 768     // 12 wavefronts: 3 per SIMD at the LDS of one workgroup per CU
 #define BB_RM_THREADS 512      // ring maps workgroup (1024 threads, 4 wavefronts per SIMD, measured no faster)
+#define BB_RM_WCAP 8           // vertical weights per sub-ring kept in LDS by k_rm_tile (a column of <= 16 samples)
 
 enum {
     F_R0 = 0, F_R1, F_XPOS, F_YPOS, F_SPA, F_CPA, F_SINC, F_CINC, F_VSYS, F_VROT, F_VRAD, F_VVERT, F_SIG, F_K,
@@ -994,7 +995,7 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
           const double2* __restrict__ az, const double* __restrict__ suball, const double* __restrict__ wtaball,
           int nsub, const short4* __restrict__ bball, const short4* __restrict__ sball, const int2* __restrict__ widx,
           const double2* __restrict__ wval, int nmap, int map0, int nw, double inv_q,
-          unsigned long long* __restrict__ counts, const int* __restrict__ bin, int bin_cap) {
+          unsigned long long* __restrict__ counts, const int* __restrict__ bin, int bin_cap, int wlds) {
     extern __shared__ double dyn[];
     __shared__ int clist[BB_LIST_CAP];
     __shared__ int slist[BB_SAMP_CAP];
@@ -1006,6 +1007,8 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     const int tx0 = (blockIdx.x % ntx) * TX, ty0 = (blockIdx.x / ntx) * TY;
     const int tx1 = tx0 + TX - 1, ty1 = ty0 + TY - 1;
     unsigned long long* tile = (unsigned long long*)dyn;
+    // (wlds: the first BB_RM_WCAP vertical weights of every sub-ring of the vector, FP64, after the tile)
+    double* swd = wlds ? (double*)(tile + (size_t)TX * TY * nwp) : nullptr;
     for (int i = threadIdx.x; i < TX * TY * nwp; i += blockDim.x) tile[i] = 0ull;
     if (threadIdx.x == 0) {
         nlist = 0;
@@ -1014,6 +1017,10 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     __syncthreads();
     const double* sub = suball + (size_t)b * nsub * BB_NSF;
     const double* wtab = wtaball + (size_t)b * nsub * BB_ZCAP;
+    if (wlds)
+        for (int i = threadIdx.x; i < nsub * BB_RM_WCAP; i += blockDim.x)
+            swd[i] = wtab[(size_t)(i / BB_RM_WCAP) * BB_ZCAP + i % BB_RM_WCAP];
+    __syncthreads();
     TileGeo T;
     T.ax0 = box.blo0 + tx0;
     T.ax1 = box.blo0 + (tx1 < bx - 1 ? tx1 : bx - 1);
@@ -1034,6 +1041,12 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
         Sub u;
         load_sub(u, sub + (size_t)s * BB_NSF, ch.x & 1);
         const double* wt = wtab + (size_t)s * BB_ZCAP;
+        const double* wd = swd ? swd + (size_t)s * BB_RM_WCAP : nullptr;
+        auto wgt = [&](int kz) -> double {        // the weight of vertical sample kz, FP64
+            if (u.nzz <= 1) return 1.0;
+            const int j = zbin_of(kz, u.half);
+            return (wd && j < BB_RM_WCAP) ? wd[j] : uw_(u, nullptr, wt, kz, g.ltype);
+        };
         const double2 cs = az[ch.w + (e & 31)];
         const double wside = cs.x >= 0.0 ? u.wrec : u.wapp;
         if (wside == 0.0) return;
@@ -1045,7 +1058,7 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
         kb = u.nzz - 1;
         int c0, c1;
         upix_fast(u, g, tr, xs, xc, yb, uz(u, ka), c0, c1);
-        double ws = uw_(u, nullptr, wt, ka, g.ltype);
+        double ws = wgt(ka);
         for (int kz = ka + 1; kz <= kb + 1; kz++) {
             int h0 = 0, h1 = 0;
             bool same = false;
@@ -1054,7 +1067,7 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
                 same = h0 == c0 && h1 == c1;
             }
             if (same) {
-                ws += uw_(u, nullptr, wt, kz, g.ltype);
+                ws += wgt(kz);
                 continue;
             }
             const int sl = T.slot(c0, c1);
@@ -1066,7 +1079,7 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
             if (kz <= kb) {
                 c0 = h0;
                 c1 = h1;
-                ws = uw_(u, nullptr, wt, kz, g.ltype);
+                ws = wgt(kz);
             }
         }
     });
@@ -1105,8 +1118,11 @@ extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* s
         attr = true;
     }
     const int ntiles = ((bx + TX - 1) / TX) * ((by + TY - 1) / TY);
-    hipLaunchKernelGGL(k_rm_tile, dim3(ntiles, B), dim3(BB_RM_THREADS), bb_rm_tile_lds(nw, TX, TY), st, *g, box, TX,
-                       TY, chunks, nchunks, az, sub, wtab, nsub, bbox, sbox, (const int2*)widx, (const double2*)wval,
-                       nmap, map0, nw, 1.0 / qunit, counts, bin, bin_cap);
+    // the sub-rings' vertical weights in LDS beside the tile when they fit
+    const size_t lt = bb_rm_tile_lds(nw, TX, TY), lw = (size_t)nsub * BB_RM_WCAP * sizeof(double);
+    const int wl = lt + lw + bb_rm_tile_static() <= 160 * 1024 ? 1 : 0;
+    hipLaunchKernelGGL(k_rm_tile, dim3(ntiles, B), dim3(BB_RM_THREADS), lt + (wl ? lw : 0), st, *g, box, TX, TY, chunks,
+                       nchunks, az, sub, wtab, nsub, bbox, sbox, (const int2*)widx, (const double2*)wval, nmap, map0,
+                       nw, 1.0 / qunit, counts, bin, bin_cap, wl);
     return (int)hipGetLastError();
 }
