@@ -69,3 +69,16 @@ __device__ __forceinline__ double bb_layer_extent(int ltype) {
 __device__ __forceinline__ double bb_layer_core(int ltype) {
     return ltype == 1 ? 3.717 : ltype == 2 ? 4.147 : ltype == 3 ? 6.908 : ltype == 4 ? 31.61 : 1e30;
 }
+
+// The (tile, vector) of a workgroup of a (tiles, vectors) grid. The dispatcher deals workgroups to the 8 XCDs
+// in turn by linear id, and each XCD has its own L2; renumbered, XCD x takes the x-th eighth of the (vector,
+// tile) pairs in order, so the tiles of one vector run on one XCD and read its chunk tables, boxes and sub-ring
+// records through one L2 instead of eight. The ids past the last multiple of 8 keep their place.
+#define BB_NXCD 8
+__device__ __forceinline__ void xcd_tile(int& tile, int& b) {
+    const int lin = blockIdx.y * gridDim.x + blockIdx.x;
+    const int per = (gridDim.x * gridDim.y) / BB_NXCD;
+    const int r = lin < per * BB_NXCD ? (lin % BB_NXCD) * per + lin / BB_NXCD : lin;
+    b = r / gridDim.x;
+    tile = r - b * gridDim.x;
+}

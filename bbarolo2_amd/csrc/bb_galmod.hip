@@ -827,11 +827,12 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     __shared__ int slist[BB_SAMP_CAP];
     __shared__ int nlist, nsamp;
     __shared__ double red[BB_TILE_THREADS / 64];
-    const int b = blockIdx.y;
+    int b, tid;
+    xcd_tile(tid, b);
     const int nz = g.nz, nl_ = g.nlines, nzp = bb_tile_stride(nz);
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
     const int ntx = (bx + TX - 1) / TX;
-    const int tx0 = (blockIdx.x % ntx) * TX, ty0 = (blockIdx.x / ntx) * TY;
+    const int tx0 = (tid % ntx) * TX, ty0 = (tid / ntx) * TY;
     const int tx1 = tx0 + TX - 1, ty1 = ty0 + TY - 1;
     const bool wlds = nsub * BB_WCAP <= BB_WLDS_MAX;
     double* sve = dyn;
@@ -891,7 +892,7 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     T.TX = TX;
     const int* tl;
     int tlo, thi;
-    tile_list(bin, gridDim.y, gridDim.x, bin_cap, b, blockIdx.x, tl, tlo, thi);
+    tile_list(bin, gridDim.y, gridDim.x, bin_cap, b, tid, tl, tlo, thi);
     tile_samples(nchunks, bb, sb, tl, tlo, thi, tx0, tx1, ty0, ty1, clist, slist, nlist, nsamp, amax > 0., [&](int e) {
         const int4 ch = chunks[e >> 5];
         const int s = ch.x >> 1;
@@ -1000,11 +1001,12 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     __shared__ int clist[BB_LIST_CAP];
     __shared__ int slist[BB_SAMP_CAP];
     __shared__ int nlist, nsamp;
-    const int b = blockIdx.y;
+    int b, tid;
+    xcd_tile(tid, b);
     const int nwp = nw | 1;
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
     const int ntx = (bx + TX - 1) / TX;
-    const int tx0 = (blockIdx.x % ntx) * TX, ty0 = (blockIdx.x / ntx) * TY;
+    const int tx0 = (tid % ntx) * TX, ty0 = (tid / ntx) * TY;
     const int tx1 = tx0 + TX - 1, ty1 = ty0 + TY - 1;
     unsigned long long* tile = (unsigned long long*)dyn;
     // (wlds: the first BB_RM_WCAP vertical weights of every sub-ring of the vector, FP64, after the tile)
@@ -1029,7 +1031,7 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     T.TX = TX;
     const int* tl;
     int tlo, thi;
-    tile_list(bin, gridDim.y, gridDim.x, bin_cap, b, blockIdx.x, tl, tlo, thi);
+    tile_list(bin, gridDim.y, gridDim.x, bin_cap, b, tid, tl, tlo, thi);
     tile_samples(nchunks, bball + (size_t)b * nchunks, sball + (size_t)b * nchunks * BB_CS, tl, tlo, thi, tx0, tx1, ty0,
                  ty1, clist, slist, nlist, nsamp, true, [&](int e) {
         const int4 ch = chunks[e >> 5];

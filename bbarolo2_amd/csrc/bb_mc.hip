@@ -18,6 +18,7 @@
 #include <stdint.h>
 
 #include "../../include/bbgpu.h"
+#include "bb_device.h"
 
 #define BB_MC_THREADS 512
 #define BB_PI_MC 3.141592653589793
@@ -362,7 +363,8 @@ k_mc_tile(bb_grid g, McGeo G, int bx, int by, uint32_t seed, int ntiles, const i
           const double* __restrict__ cd2i, int nvmax, float* __restrict__ outall,
           unsigned long long* __restrict__ countsall, int accumulate) {
     extern __shared__ unsigned long long mtile[];
-    const int b = blockIdx.y, t = blockIdx.x;
+    int b, t;
+    xcd_tile(t, b);
     const int nz = g.nz, nzp = nz | 1;
     const int TX = G.TX, TY = G.TY;
     const int tx0 = (t % G.ntx) * TX, ty0 = (t / G.ntx) * TY;
