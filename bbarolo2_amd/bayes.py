@@ -22,7 +22,13 @@ def synthetic_fit(n, chunk=1024, graphs=False):
     the GPU, convolved with the beam), noise, likelihood inputs, and the uniform priors of the free parameters.
     Returns a dict: like (GpuLikelihood), dm (DiskModel), free, lo, hi, theta_true, data, h, truth, par, box, sigma,
     likemask, region, likefactor, kern, desc."""
-    h, truth, par, noise_frac, noise_seed, spec, desc = config(n)
+    return fit_of(config(n), chunk, graphs)
+
+
+def fit_of(cfg, chunk=1024, graphs=False):
+    """synthetic_fit for a configuration tuple as synth.config returns it (header, truth, GalmodPar, noise
+    fraction, noise seed, free spec, description)."""
+    h, truth, par, noise_frac, noise_seed, spec, desc = cfg
     dm = DiskModel(h, par, galmodtype=par.GALMODTYPE if par.GALMODTYPE else "ANALYTIC")
     st = dm.setup
     r32 = float32_rings(truth)

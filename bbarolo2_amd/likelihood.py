@@ -281,17 +281,23 @@ class GpuLikelihood:
             with torch.cuda.stream(s):
                 out.append(self._device_part(p["sr"], p["d_sr"], p.get("vals"), i % 2, p.get("sr_ext"),
                                              p.get("d_sr_ext"), ev))
+            if s is not cur:
+                # (the tables are read on s: their memory must not return to the caller's stream before that)
+                for k in ("d_sr", "d_sr_ext"):
+                    if k in p:
+                        p[k].record_stream(s)
             prev = ev
         for s in streams:
             if s is not cur:
                 cur.wait_stream(s)
         return out
 
-    def split_parts(self, sr, vals):
-        """The chunk (sr, vals) as two halves with their device tables (one part below 64 vectors)."""
+    def split_parts(self, sr, vals, halves=True):
+        """The chunk (sr, vals) as two halves with their device tables (one part below 64 vectors or without
+        halves)."""
         torch = self.torch
         B = sr.shape[0]
-        cuts = [0, B] if B < 64 else [0, B // 2, B]
+        cuts = [0, B] if B < 64 or not halves else [0, B // 2, B]
         parts = []
         for a, b in zip(cuts[:-1], cuts[1:]):
             p = dict(sr=sr[a:b], d_sr=torch.from_numpy(np.ascontiguousarray(sr[a:b])).to(self.dev), vals=vals)
@@ -340,6 +346,18 @@ class GpuLikelihood:
         else:
             res = self.device_batch(self.split_parts(sr, vals))
         self.torch.cuda.current_stream().synchronize()
+        return self.collect(res)
+
+    def launch(self, theta, halves=False):
+        """Launch the evaluation of the rows of theta (at most chunk rows, no negative positive-only parameter) on
+        the current stream and return at once: the device tensors for collect(), valid once that stream reaches
+        this point. halves: as residuals, two halves on two streams."""
+        sr, vals = self.tables(np.atleast_2d(np.asarray(theta, float)))
+        return self.device_batch(self.split_parts(sr, vals, halves))
+
+    def collect(self, res):
+        """(S_box, lg) of the device tensors of launch (or device_batch), as numpy arrays; the density profiles of
+        the chunk are kept in last_dens."""
         ss = np.concatenate([r[0].cpu().numpy() for r in res])
         lg = np.concatenate([r[1].cpu().numpy() for r in res])
         if res[0][2] is not None:
@@ -355,13 +373,16 @@ class GpuLikelihood:
         out = np.full(theta.shape[0], -np.inf)
         ok = ~self.free.negative(theta)
         idx = np.flatnonzero(ok)
-        s2 = self.sigma ** 2
         for c in range(0, len(idx), self.chunk):
             ii = idx[c:c + self.chunk]
-            ss, lg = self.residuals(*self.tables(theta[ii]))
-            if self.eps > 0:
-                nll = 0.5 * (ss + self.ss_out / s2) + 0.5 * self.nvox * math.log(2 * math.pi * s2) + 0.5 * lg
-            else:
-                nll = 0.5 * (ss + self.ss_out) / s2 + 0.5 * self.nvox * math.log(2 * math.pi * s2)
-            out[ii] = -self.likefactor * nll
+            out[ii] = self.finish(*self.residuals(*self.tables(theta[ii])))
         return out
+
+    def finish(self, ss, lg):
+        """The log-likelihoods of the residual sums (S_box, lg) of residuals (likelihood.py:567-619)."""
+        s2 = self.sigma ** 2
+        if self.eps > 0:
+            nll = 0.5 * (ss + self.ss_out / s2) + 0.5 * self.nvox * math.log(2 * math.pi * s2) + 0.5 * lg
+        else:
+            nll = 0.5 * (ss + self.ss_out) / s2 + 0.5 * self.nvox * math.log(2 * math.pi * s2)
+        return -self.likefactor * nll

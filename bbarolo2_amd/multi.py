@@ -1,0 +1,154 @@
+"""Many galaxies of uneven cube sizes evaluated together on one GPU.
+
+Each galaxy keeps its own likelihood (GpuLikelihood: its cube, model box, density profile and device buffers).
+A step evaluates one batch of parameter vectors for every galaxy: largest cube first, each batch's host tables are
+built and copied to the device, and the batch is launched on the next of a few streams, so that the host work of
+one galaxy overlaps the device work of the ones before it and the deposits of small galaxies fill the CUs that
+the large ones leave idle; one synchronisation ends the step. The reference fits galaxies one after the other, each with its own process pool (bbarolo/galfit.py,
+bbarolo/bayesian/fitter.py); it has no batch over galaxies.
+"""
+
+import time
+
+import numpy as np
+
+from .bayes import fit_of
+from .synth import multi_galaxy_configs
+from .tuner import tune_tiles
+
+
+def build_fits(cfgs, chunk=64):
+    """synthetic fits (bayes.fit_of dicts) of configuration tuples, on the current GPU; returns (fits, seconds of
+    set-up per galaxy)."""
+    fits, secs = [], []
+    for cfg in cfgs:
+        t = time.perf_counter()
+        fits.append(fit_of(cfg, chunk=chunk))
+        secs.append(time.perf_counter() - t)
+    return fits, np.array(secs)
+
+
+def draw_thetas(fits, B, seed=1):
+    """B parameter vectors per galaxy from its uniform priors (default_rng(seed))."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for f in fits:
+        u = rng.random((B, len(f["lo"])))
+        out.append(f["lo"] + u * (f["hi"] - f["lo"]))
+    return out
+
+
+class GalaxyBatch:
+    """The likelihoods likes (GpuLikelihood, one per galaxy) evaluated together: loglike(thetas) takes one array
+    of parameter vectors per galaxy (each at most that likelihood's chunk rows) and returns their log-likelihoods
+    (-inf where a positive-only parameter is negative), as the galaxies' own loglike would."""
+
+    def __init__(self, likes, n_streams=4):
+        self.likes = list(likes)
+        torch = self.likes[0].torch
+        self.torch = torch
+        self.streams = [torch.cuda.Stream() for _ in range(n_streams)]       # (default priority)
+        size = [l.dvalid.numel() for l in self.likes]
+        self.order = np.argsort(size, kind="stable")[::-1]                  # largest model box first
+
+    def tune(self, thetas, reps=2):
+        """Pick the deposit tile of every galaxy for batches like thetas (tuner.tune_tiles); returns the shapes."""
+        out = []
+        for like, th in zip(self.likes, thetas):
+            th = np.atleast_2d(th)[~like.free.negative(th)]
+            out.append(tune_tiles(like, th, reps=reps)[0] if len(th) else None)
+        return out
+
+    def launch(self, thetas, events=False):
+        """Launch the step without waiting: (pending, end events or None). Galaxy by galaxy, largest first, the
+        host tables are built and copied to the device on the current stream, and the batch is launched on the
+        next of the streams once the copy is there: the host work of a galaxy overlaps the device work of the
+        ones before it."""
+        torch = self.torch
+        for like, th in zip(self.likes, thetas):
+            if np.atleast_2d(th).shape[0] > like.chunk:
+                raise ValueError(f"{np.atleast_2d(th).shape[0]} vectors for a likelihood of chunk {like.chunk}")
+        cur = torch.cuda.current_stream()
+        pending = [None] * len(self.likes)
+        ends = [None] * len(self.likes) if events else None
+        for j, g in enumerate(self.order):
+            like = self.likes[g]
+            th = np.atleast_2d(np.asarray(thetas[g], float))
+            ok = np.flatnonzero(~like.free.negative(th))
+            parts = None
+            if len(ok):
+                sr, vals = like.tables(th[ok])
+                parts = like.split_parts(sr, vals, halves=False)
+            s = self.streams[j % len(self.streams)]
+            s.wait_stream(cur)
+            with torch.cuda.stream(s):
+                res = like.device_batch(parts) if parts is not None else None
+                if events:
+                    ends[g] = torch.cuda.Event(enable_timing=True)
+                    ends[g].record()
+            # (the tables were allocated on the current stream and are read on s: without this the allocator could
+            # hand their memory to the next galaxy's tables before s has read them)
+            for p in parts or ():
+                for k in ("d_sr", "d_sr_ext"):
+                    if k in p:
+                        p[k].record_stream(s)
+            pending[g] = (th.shape[0], ok, res, parts)
+        for s in self.streams:
+            cur.wait_stream(s)
+        return pending, ends
+
+    def collect(self, pending):
+        """The log-likelihoods of a launched step (waits for it)."""
+        self.torch.cuda.current_stream().synchronize()
+        out = []
+        for like, (n, ok, res, _) in zip(self.likes, pending):
+            v = np.full(n, -np.inf)
+            if res is not None:
+                v[ok] = like.finish(*like.collect(res))
+            out.append(v)
+        return out
+
+    def loglike(self, thetas):
+        return self.collect(self.launch(thetas)[0])
+
+    def timed_step(self, thetas):
+        """One step with HIP events: (log-likelihoods, ms of the whole step, ms from the step's start to the end of
+        each galaxy's batch, ms of host time to launch it)."""
+        torch = self.torch
+        e0 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        t0 = time.perf_counter()
+        pending, ends = self.launch(thetas, events=True)
+        host = (time.perf_counter() - t0) * 1e3
+        e1 = torch.cuda.Event(enable_timing=True)
+        e1.record()
+        out = self.collect(pending)
+        return out, e0.elapsed_time(e1), np.array([e0.elapsed_time(e) for e in ends]), host
+
+    def alone_ms(self, thetas, reps=1):
+        """ms of each galaxy's batch run by itself on the idle GPU (after one untimed run)."""
+        torch = self.torch
+        out = []
+        for like, th in zip(self.likes, thetas):
+            th = np.atleast_2d(th)[~like.free.negative(th)]
+            if not len(th):
+                out.append(np.nan)
+                continue
+            sr, vals = like.tables(th)
+            parts = like.split_parts(sr, vals, halves=False)
+            like.device_batch(parts)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                like.device_batch(parts)
+            e1.record()
+            e1.synchronize()
+            out.append(e0.elapsed_time(e1) / reps)
+        return np.array(out)
+
+
+def multi_galaxy(n=64, seed=11, chunk=64):
+    """The uneven multi-galaxy batch (synth.multi_galaxy_configs(n, seed)) on the current GPU: (GalaxyBatch, fits,
+    set-up seconds per galaxy)."""
+    fits, secs = build_fits(multi_galaxy_configs(n, seed), chunk)
+    return GalaxyBatch([f["like"] for f in fits]), fits, secs

@@ -70,6 +70,45 @@ def config(n):
     raise ValueError(f"no configuration {n}")
 
 
+MULTI_PIX = 10.      # arcsec: the pixel of every cube of the uneven multi-galaxy batch
+
+
+def multi_galaxy_configs(n=64, seed=11):
+    """The uneven multi-galaxy batch: n galaxies drawn with default_rng(seed), each as config(n) returns one
+    (header, truth Rings, GalmodPar, noise fraction, noise seed, free spec, description). Square cubes of side
+    48-256 pixels (10") and 32-128 channels; 5-40 rings (at most one per pixel of the disk radius, 0.4 of the
+    side: GALMOD needs rings wider than its 0.75-pixel sub-rings); inclination 30-80 deg, position angle 0-360 deg;
+    arctan rotation curve of vflat 50-300 km/s turning over at 0.15 of the disk radius; dispersion 5-30 km/s;
+    exponential density of scale length half the disk radius; 10" thick layer; beam of 3-6 pixels FWHM; noise
+    2-10% of the peak. The channel width puts +-(vflat sin i + 4 vdisp) in 80% of the channels. Every galaxy is
+    fitted GLOBAL with free VROT VDISP per ring and one INC and PA."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        side = int(rng.integers(48, 257))
+        nz = int(rng.integers(32, 129))
+        nr = int(rng.integers(5, 41))
+        inc, pa = float(rng.uniform(30, 80)), float(rng.uniform(0, 360))
+        vflat, vdisp = float(rng.uniform(50, 300)), float(rng.uniform(5, 30))
+        beam_pix, noise = float(rng.uniform(3, 6)), float(rng.uniform(0.02, 0.10))
+        rmax = 0.4 * side * MULTI_PIX
+        nr = min(nr, int(rmax / MULTI_PIX))
+        width = rmax / nr
+        R = (np.arange(nr) + 0.5) * width
+        vext = vflat * np.sin(np.radians(inc)) + 4 * vdisp
+        dv = 2 * vext / (0.8 * nz)
+        h = emptyheader([side, side, nz], [-MULTI_PIX / 3600., MULTI_PIX / 3600., dv],
+                        beam=beam_pix * MULTI_PIX / 3600., obj=f"MULTI{i:02d}")
+        r = Rings().setRings(radii=R, xpos=side / 2., ypos=side / 2., vsys=0.,
+                             vrot=vflat * (2 / np.pi) * np.arctan(R / (0.15 * rmax)), vdisp=vdisp, z0=10.,
+                             dens=np.exp(-R / (0.5 * rmax)), inc=inc, phi=pa)
+        par = GalmodPar(GALMODTYPE="ANALYTIC")
+        out.append((h, r, par, noise, 1000 + i, [("vrot", "ring"), ("vdisp", "ring"), ("inc", "single"),
+                                                 ("phi", "single")],
+                    f"galaxy {i}: {side}x{side}x{nz}, {nr} rings, beam {beam_pix:.1f} px"))
+    return out
+
+
 def theta_of(rings, free):
     """The parameter vector of the rings for the free parameters (FreePars)."""
     th = np.zeros(free.ndim)

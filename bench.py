@@ -1,5 +1,6 @@
-"""Benchmark: likelihood evaluations per second of the 160x160x96 GLOBAL configuration on one GPU, and the wall time
-of a Bayesian fit of the 64x64x40 configuration with and without the surrogate screen.
+"""Benchmark: likelihood evaluations per second of the 160x160x96 GLOBAL configuration on one GPU, the wall time
+of a Bayesian fit of the 64x64x40 configuration with and without the surrogate screen, and the evaluations per
+second of an uneven batch of 64 galaxies (tile shapes picked by the tuner; per-galaxy batch times p50 / p99).
 
 One step = one batch of B parameter vectors (free VROT, VDISP per ring, INC, PA: 82-D) evaluated by the
 model-cube likelihood, inputs resident on the device: the deconvolved density profile of every vector (ring maps,
@@ -7,7 +8,7 @@ non-negative least squares), then ANALYTIC GALMOD -> beam convolution -> masked 
 GpuLikelihood.device_batch runs it (two halves of the batch on two streams). The synthetic cube is built on the
 GPU (ANALYTIC GALMOD of the truth rings, beam, noise 3% of the peak). Prints one JSON line (see --help).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-bayes]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-bayes] [--no-multi]
 """
 import argparse
 import ctypes as C
@@ -167,6 +168,7 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-bayes", action="store_true", help="skip the Bayesian fits (plain and surrogate-screened)")
+    ap.add_argument("--no-multi", action="store_true", help="skip the uneven multi-galaxy batch")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device buffers 5 times (the reference kernel of the HBM counter passes)")
     a = ap.parse_args()
@@ -320,7 +322,56 @@ def main():
         out["gpu_over_cpu"] = value / cpu["value"]
     if not a.no_bayes:
         out["bayes_fit"] = bayes_fits()
+    if not a.no_multi:
+        out["multi_galaxy"] = multi_galaxy_bench()
     print(json.dumps(out), flush=True)
+
+
+def multi_galaxy_bench(n=64, B=32, steps=3, seed=11):
+    """The uneven multi-galaxy batch: n synthetic galaxies (cube sides 48-256 pixels, 32-128 channels, 5-40 rings,
+    default_rng(seed)), each with its GLOBAL likelihood (free VROT VDISP per ring, INC, PA; DENS from the profile of
+    every vector); the deposit tile of each picked by the tuner; then steps of B vectors per galaxy, all galaxies
+    launched together on four streams. Reports evals/s over all galaxies and the per-galaxy batch times."""
+    from bbarolo2_amd.multi import multi_galaxy, draw_thetas
+    t0 = time.perf_counter()
+    gb, fits, secs = multi_galaxy(n, seed, chunk=B)
+    t_setup = time.perf_counter() - t0
+    print(f"[bench] multi-galaxy set-up: {t_setup:.1f} s", file=sys.stderr, flush=True)
+    thetas = draw_thetas(fits, B, seed=1)
+    t0 = time.perf_counter()
+    tiles = gb.tune(thetas, reps=1)
+    t_tune = time.perf_counter() - t0
+    print(f"[bench] multi-galaxy tile tuning: {t_tune:.1f} s", file=sys.stderr, flush=True)
+    gb.loglike(thetas)
+    step_ms, done, host = [], [], []
+    for _ in range(steps):
+        ll, ms, d, hm = gb.timed_step(thetas)
+        if not all(np.all(np.isfinite(v)) for v in ll):
+            raise RuntimeError("multi-galaxy batch: a log-likelihood is not finite")
+        step_ms.append(ms)
+        done.append(d)
+        host.append(hm)
+    alone = gb.alone_ms(thetas)
+    ms = float(np.median(step_ms))
+    d = np.median(np.array(done), axis=0)
+    vox = [f["h"].DimAx(0) * f["h"].DimAx(1) * f["h"].DimAx(2) for f in fits]
+    shapes = {}
+    for t in tiles:
+        shapes[f"{t[0]}x{t[1]}"] = shapes.get(f"{t[0]}x{t[1]}", 0) + 1
+    print(f"[bench] multi-galaxy step {ms:.1f} ms ({n * B / ms * 1e3:.0f} evals/s), sum of alone "
+          f"{alone.sum():.1f} ms", file=sys.stderr, flush=True)
+    return {"workload": f"{n} galaxies (seed {seed}), cube sides {min(f['h'].DimAx(0) for f in fits)}-"
+                        f"{max(f['h'].DimAx(0) for f in fits)} px, {sum(vox) / 1e6:.1f} M voxels in all; GLOBAL "
+                        f"likelihood, {B} vectors per galaxy per step, four streams",
+            "evals_per_s": n * B / ms * 1e3, "ms_per_step": round(ms, 3), "steps": steps,
+            "host_launch_ms": round(float(np.median(host)), 3),
+            "per_galaxy_ms_in_step": {"p50": round(float(np.percentile(d, 50)), 3),
+                                      "p99": round(float(np.percentile(d, 99)), 3)},
+            "per_galaxy_ms_alone": {"p50": round(float(np.percentile(alone, 50)), 3),
+                                    "p99": round(float(np.percentile(alone, 99)), 3),
+                                    "sum": round(float(alone.sum()), 3)},
+            "setup_s": round(t_setup, 2), "setup_s_per_galaxy_p50": round(float(np.median(secs)), 3),
+            "tune_s": round(t_tune, 2), "tiles": shapes}
 
 
 def bayes_fits(n_live=500, walks=25, seed=3):
