@@ -292,21 +292,27 @@ class GpuLikelihood:
                 cur.wait_stream(s)
         return out
 
-    def split_parts(self, sr, vals, halves=True):
+    def split_parts(self, sr, vals, halves=True, pinned=False):
         """The chunk (sr, vals) as two halves with their device tables (one part below 64 vectors or without
-        halves)."""
+        halves). pinned: the tables go through page-locked host buffers and are copied asynchronously on the
+        current stream (the host does not wait; use the parts on that stream)."""
         torch = self.torch
         B = sr.shape[0]
         cuts = [0, B] if B < 64 or not halves else [0, B // 2, B]
+
+        def up(a):
+            t = torch.from_numpy(np.ascontiguousarray(a))
+            return t.pin_memory().to(self.dev, non_blocking=True) if pinned else t.to(self.dev)
+
         parts = []
         for a, b in zip(cuts[:-1], cuts[1:]):
-            p = dict(sr=sr[a:b], d_sr=torch.from_numpy(np.ascontiguousarray(sr[a:b])).to(self.dev), vals=vals)
+            p = dict(sr=sr[a:b], d_sr=up(sr[a:b]), vals=vals)
             if self.update_prof:
                 # (the values of the free keys are per vector, (B, nr))
                 v = {k: np.asarray(x)[a:b] for k, x in (vals or {}).items()}
                 p["vals"] = v
                 p["sr_ext"] = self.profile.ext_tables(v, b - a)
-                p["d_sr_ext"] = torch.from_numpy(np.ascontiguousarray(p["sr_ext"])).to(self.dev)
+                p["d_sr_ext"] = up(p["sr_ext"])
             parts.append(p)
         return parts
 

@@ -2,9 +2,9 @@
 
 Each galaxy keeps its own likelihood (GpuLikelihood: its cube, model box, density profile and device buffers).
 A step evaluates one batch of parameter vectors for every galaxy: largest cube first, each batch's host tables are
-built and copied to the device, and the batch is launched on the next of a few streams, so that the host work of
-one galaxy overlaps the device work of the ones before it and the deposits of small galaxies fill the CUs that
-the large ones leave idle; one synchronisation ends the step. The reference fits galaxies one after the other, each with its own process pool (bbarolo/galfit.py,
+built, copied to the device asynchronously from page-locked buffers and the batch launched, on the next of a few
+streams, so that the host work of one galaxy overlaps the device work of the ones before it and the deposits of
+small galaxies fill the CUs that the large ones leave idle; one synchronisation ends the step. The reference fits galaxies one after the other, each with its own process pool (bbarolo/galfit.py,
 bbarolo/bayesian/fitter.py); it has no batch over galaxies.
 """
 
@@ -61,37 +61,32 @@ class GalaxyBatch:
 
     def launch(self, thetas, events=False):
         """Launch the step without waiting: (pending, end events or None). Galaxy by galaxy, largest first, the
-        host tables are built and copied to the device on the current stream, and the batch is launched on the
-        next of the streams once the copy is there: the host work of a galaxy overlaps the device work of the
-        ones before it."""
+        host tables are built, copied and launched on the next of the streams: the host work of a galaxy overlaps
+        the device work of the ones before it. The pending record keeps the tables until collect()."""
         torch = self.torch
         for like, th in zip(self.likes, thetas):
             if np.atleast_2d(th).shape[0] > like.chunk:
                 raise ValueError(f"{np.atleast_2d(th).shape[0]} vectors for a likelihood of chunk {like.chunk}")
         cur = torch.cuda.current_stream()
+        for s in self.streams:
+            s.wait_stream(cur)
         pending = [None] * len(self.likes)
         ends = [None] * len(self.likes) if events else None
         for j, g in enumerate(self.order):
             like = self.likes[g]
             th = np.atleast_2d(np.asarray(thetas[g], float))
             ok = np.flatnonzero(~like.free.negative(th))
-            parts = None
-            if len(ok):
-                sr, vals = like.tables(th[ok])
-                parts = like.split_parts(sr, vals, halves=False)
             s = self.streams[j % len(self.streams)]
-            s.wait_stream(cur)
+            parts = res = None
             with torch.cuda.stream(s):
-                res = like.device_batch(parts) if parts is not None else None
+                if len(ok):
+                    # (the tables are copied on s from page-locked buffers, without a host wait, and read there)
+                    sr, vals = like.tables(th[ok])
+                    parts = like.split_parts(sr, vals, halves=False, pinned=True)
+                    res = like.device_batch(parts)
                 if events:
                     ends[g] = torch.cuda.Event(enable_timing=True)
                     ends[g].record()
-            # (the tables were allocated on the current stream and are read on s: without this the allocator could
-            # hand their memory to the next galaxy's tables before s has read them)
-            for p in parts or ():
-                for k in ("d_sr", "d_sr_ext"):
-                    if k in p:
-                        p[k].record_stream(s)
             pending[g] = (th.shape[0], ok, res, parts)
         for s in self.streams:
             cur.wait_stream(s)
