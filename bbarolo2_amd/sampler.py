@@ -95,7 +95,7 @@ class Sampler:
 
     def __init__(self, prior, likelihood, n_dim, n_live=2000, n_batch=None, enlarge_per_dim=1.1, n_update=None,
                  seed=None, surrogate=None, screen_sigma=3.0, train_steps=300, retrain_every=None, train_max=16384,
-                 bound="ellipsoid", walks=25, retrain_growth=1.0):
+                 bound="ellipsoid", walks=25, retrain_growth=1.0, walk_min_pass=0.2, low_pass="exact"):
         self.prior, self.likelihood = prior, likelihood
         self.n_dim, self.n_live = int(n_dim), int(n_live)
         self.n_batch = int(n_batch) if n_batch else max(100, self.n_live // 4)
@@ -113,6 +113,13 @@ class Sampler:
         self.bound, self.walks, self.scale = bound, int(walks), 1.0
         self._last_keep = None
         self._walk_pass = 1.0        # fraction of the last surrogate walks' end points above the threshold
+        self.walk_min_pass = float(walk_min_pass)
+        if low_pass not in ("exact", "retrain"):
+            raise ValueError(f"low_pass {low_pass!r}: 'exact' or 'retrain'")
+        self.low_pass = low_pass
+        self._retrain_now = False    # (low_pass "retrain": a low pass rate asks for a training at the next batch)
+        self._low_streak = 0         # surrogate walks in a row with a low pass rate
+        self.n_train = 0             # trainings of the surrogate
         self.n_walk_judged = 0       # end points of surrogate walks judged by the exact likelihood
         self.n_walk_passed = 0
         self.n_like = 0            # exact likelihood evaluations
@@ -159,6 +166,7 @@ class Sampler:
             # the most recent points, which are the ones near the threshold
             U, Y = U[-self.train_max:], Y[-self.train_max:]
         self.surrogate.fit(U, Y, steps=self.train_steps)
+        self.n_train += 1
         self._trained_upto = sum(len(a) for a in self._eval_l)
         self.t_surrogate += time.perf_counter() - t0
 
@@ -205,12 +213,16 @@ class Sampler:
                 pool_u, pool_l = pool_u[pool_ok], pool_l[pool_ok]
                 fin = live_l[np.isfinite(live_l)]
                 lrange = float(fin.max() - lmin) if fin.size and np.isfinite(lmin) else np.inf
+                due = (self.n_like - last_train >= self.retrain_every and
+                       self.n_like >= self.retrain_growth * last_train)
                 if (self.surrogate is not None and np.isfinite(lmin) and
-                        self.n_like - last_train >= self.retrain_every and
-                        self.n_like >= self.retrain_growth * last_train):
+                        (due or (self._retrain_now and self.n_like > last_train))):
                     self._train(lmin, lrange)
                     last_train = self.n_like
                     self._walk_pass = 1.0        # (a new network gets its chance)
+                    self._retrain_now = False
+                    if due:
+                        self._low_streak = 0
                 if since_bound >= self.n_update or bound is None:
                     bound = Ellipsoid(live_u, self.enlarge) if self.bound == "ellipsoid" else np.linalg.cholesky(
                         np.cov(live_u, rowvar=False) + 1e-12 * np.eye(self.n_dim))
@@ -272,14 +284,17 @@ class Sampler:
         Plain: the constraint is the exact likelihood > lmin, one likelihood call per step for all chains. Screened
         (the surrogate is ready): the chains walk on the surrogate's region {prediction >= lmin - margin} with no
         likelihood call, then their end points are judged by the exact likelihood (one call) and those at or below
-        lmin are dropped; while fewer than 20% of the end points pass, the walks go back to the exact likelihood."""
+        lmin are dropped; while fewer than walk_min_pass of the end points pass, the walks go back to the exact
+        likelihood until the next training (low_pass "exact"), or the network is trained again at once on the
+        points just judged and the walks go back to the exact likelihood only if the next pass rate is low too
+        (low_pass "retrain")."""
         above = np.flatnonzero(live_l > lmin)
         if above.size == 0:
             above = np.arange(len(live_l))
         start = self.rng.choice(above, self.n_batch)
         x, lx = live_u[start].copy(), live_l[start].copy()
         moved = np.zeros(self.n_batch, bool)
-        screened = self._screen_ready(lrange) and self._walk_pass > 0.2
+        screened = self._screen_ready(lrange) and self._walk_pass > self.walk_min_pass
         margin = self.screen_sigma * self.screen_err if screened else 0.0
         t_sur = 0.0
         for _ in range(self.walks):
@@ -313,6 +328,15 @@ class Sampler:
         lm = self._evaluate(xm)
         good = lm > lmin
         self._walk_pass = float(good.mean())
+        if self._walk_pass > self.walk_min_pass:
+            self._low_streak = 0
+        else:
+            self._low_streak += 1
+            if self.low_pass == "retrain" and self._low_streak == 1:
+                # train again on the points just judged and keep walking on the surrogate; a second low pass rate
+                # in a row (the new network no better) sends the walks back to the exact likelihood
+                self._retrain_now = True
+                self._walk_pass = 1.0
         self.n_walk_judged += len(xm)
         self.n_walk_passed += int(good.sum())
         return xm[good], lm[good]

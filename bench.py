@@ -383,8 +383,9 @@ def bayes_fits(n_live=500, walks=25, seed=3):
     res = {}
     for scr in (False, True):
         t0 = time.perf_counter()
+        kw = dict(low_pass="retrain", walk_min_pass=0.1, train_max=8192) if scr else {}
         s = bayes.nested_fit(fit, n_live=n_live, n_batch=n_live, seed=seed, surrogate=scr, bound="rwalk",
-                             walks=walks)
+                             walks=walks, **kw)
         res[scr] = (time.perf_counter() - t0, s)
         print(f"[bench] bayes fit surrogate={scr}: {res[scr][0]:.1f} s, {s.n_like} likelihood evaluations",
               file=sys.stderr, flush=True)
@@ -392,10 +393,14 @@ def bayes_fits(n_live=500, walks=25, seed=3):
     shift, width = bayes.compare(sp, ss)
     return {"workload": "64x64x40 cube, 20 km/s channels, beam 4x3 px at PA 30, 6 rings, ANALYTIC, free VROT VDISP "
                         "INC (13-D), DENS from the deconvolved profile of every vector; nested sampling, "
-                        f"n_live = n_batch = {n_live}, random walks of {walks} steps, f_live 0.01",
+                        f"n_live = n_batch = {n_live}, random walks of {walks} steps, f_live 0.01; screened: walks on "
+                        "the surrogate's region, end points judged exactly, retrained at once when under 10% pass "
+                        "(most recent 8192 points)",
             "plain_s": round(tp, 3), "surrogate_s": round(ts, 3), "speedup": round(tp / ts, 3),
             "plain_n_like": sp.n_like, "surrogate_n_like": ss.n_like, "surrogate_screened": ss.n_screened,
-            "surrogate_train_predict_s": round(ss.t_surrogate, 3), "plain_like_s": round(sp.t_like, 3),
+            "surrogate_train_predict_s": round(ss.t_surrogate, 3), "surrogate_trainings": ss.n_train,
+            "walk_end_points_judged": ss.n_walk_judged, "walk_end_points_passed": ss.n_walk_passed,
+            "plain_like_s": round(sp.t_like, 3),
             "surrogate_like_s": round(ss.t_like, 3), "log_z": [sp.log_z, ss.log_z],
             "median_shift_sigma_max": shift, "ci68_width_change_max": width}
 

@@ -118,3 +118,37 @@ def test_surrogate_walks_keep_the_posterior_with_fewer_evaluations():
         width = qp[2] - qp[0]
         assert abs(qs[1] - qp[1]) < 0.1 * width
         assert abs((qs[2] - qs[0]) / width - 1) < 0.1
+
+
+class Improving(NoisyExact):
+    """A stand-in surrogate that is poor (noise 4 nats, biased down by 3) for its first three trainings, then good."""
+
+    def predict(self, u):
+        if self.calls <= 3:
+            return gauss_logl(u) - 3.0 + self.rng.normal(0, 4.0, len(u))
+        return super().predict(u)
+
+
+def test_low_pass_retrain_keeps_walking_on_the_surrogate():
+    import pytest
+    d = 8
+    with pytest.raises(ValueError):
+        Sampler(identity, gauss_logl, d, low_pass="never")
+    plain = Sampler(identity, gauss_logl, d, n_live=400, n_batch=200, seed=7, bound="rwalk", walks=20).run(f_live=0.01)
+    runs = {}
+    for mode in ("exact", "retrain"):
+        runs[mode] = Sampler(identity, gauss_logl, d, n_live=400, n_batch=200, seed=7, bound="rwalk", walks=20,
+                             surrogate=Improving(2), retrain_every=1000, walk_min_pass=0.3,
+                             low_pass=mode).run(f_live=0.01)
+    ex, rt = runs["exact"], runs["retrain"]
+    # a low pass rate trains the network again at once instead of a batch of exact walks
+    assert rt.n_train > ex.n_train and rt.n_like < ex.n_like, (rt.n_train, ex.n_train, rt.n_like, ex.n_like)
+    assert abs(rt.log_z - plain.log_z) < 0.5
+    pp, lwp, _ = plain.posterior()
+    ps, lws, _ = rt.posterior()
+    for k in range(d):
+        qp = weighted_quantiles(pp[:, k], np.exp(lwp), [0.16, 0.5, 0.84])
+        qs = weighted_quantiles(ps[:, k], np.exp(lws), [0.16, 0.5, 0.84])
+        width = qp[2] - qp[0]
+        assert abs(qs[1] - qp[1]) < 0.1 * width
+        assert abs((qs[2] - qs[0]) / width - 1) < 0.1
