@@ -59,6 +59,8 @@ _SIGS = {
     "bb_residual_gauss": (C.c_int, [_P, _P, C.c_int, C.c_int64, C.c_double, C.c_double, _P, C.c_int64, _P, _P,
                                     _P]),
     "bb_conv_residual_work_size": (C.c_int64, [_P, C.c_int, C.c_int, C.c_int]),
+    "bb_dilate_mask": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _P, _P, _P]),
+    "bb_dilate_rmax": (C.c_int, []),
     "bb_conv_residual_gauss": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _P, C.c_int64,
                                          _P, _P, _P]),
     "bb_mc_work_size": (C.c_int64, [_P, bb_box, C.c_int, C.c_int, C.c_int64]),

@@ -10,7 +10,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbbgpu.so")
-SOURCES = ["bb_capi.hip", "bb_galmod.hip", "bb_conv.hip", "bb_resid.hip", "bb_profile.hip", "bb_mc.hip", "bb_norm.hip", "bb_mlp.hip"]
+SOURCES = ["bb_capi.hip", "bb_galmod.hip", "bb_conv.hip", "bb_resid.hip", "bb_profile.hip", "bb_mc.hip", "bb_norm.hip", "bb_mlp.hip",
+           "bb_setup.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

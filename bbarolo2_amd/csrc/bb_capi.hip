@@ -61,6 +61,8 @@ extern "C" int bb_launch_profile_solve(int n, int nr, int nok, double smooth, do
                                        double* partial, double* gram, double* Mall, double* x, hipStream_t st);
 extern "C" int bb_launch_set_dens(const double* x, int B, int n, int nr, const int* widx, const double* wval, int nsub,
                                   double* sr, float* dens, hipStream_t st);
+extern "C" int bb_launch_dilate(const uint8_t* mask, int nz, int ny, int nx, double radius, int nchan, uint8_t* tmp,
+                                uint8_t* out, hipStream_t st);
 extern "C" int bb_launch_residual(const float* model, const float* dvalid, int B, int64_t nvox, double sigma2,
                                   double eps2, double* partial, double* out_ss, double* out_lg, hipStream_t st);
 
@@ -991,4 +993,22 @@ extern "C" int bb_mlp_train(bb_mlp* n, const float* x, const float* t, int rows,
     for (int s = 0; s < steps && !rc; s++) rc = (int)hipGraphLaunch(n->gx, st);
     if (!rc && loss) rc = (int)hipMemcpyAsync(loss, n->hist, (size_t)steps * sizeof(float), hipMemcpyDeviceToDevice, st);
     return rc ? fail(BB_ERR_HIP, "bb_mlp_train: %s", hipGetErrorString((hipError_t)rc)) : 0;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Per-fit set-up: the likelihood mask (dilate_mask, bbarolo/bayesian/noise.py:18-30).
+extern "C" int bb_dilate_mask(const uint8_t* mask, int nz, int ny, int nx, double radius, int nchan, uint8_t* tmp,
+                              uint8_t* out, void* stream) {
+    if (nz < 0 || ny < 0 || nx < 0 || nz > 65535 || (ny + 31) / 32 > 65535 || (nx + 31) / 32 > 65535)
+        return fail(BB_ERR_ARG, "bb_dilate_mask: bad sizes %d x %d x %d", nz, ny, nx);
+    if (!(radius == radius) || radius > 1e9 || nchan < 0)
+        return fail(BB_ERR_ARG, "bb_dilate_mask: bad radius %g or nchan %d", radius, nchan);
+    if (radius >= 1.0 && (int)floor(radius) > bb_dilate_rmax())
+        return fail(BB_ERR_ARG, "bb_dilate_mask: radius %g pixels exceeds %d", radius, bb_dilate_rmax());
+    if ((int64_t)nz * ny * nx == 0) return 0;
+    if (!mask || !out) return fail(BB_ERR_ARG, "bb_dilate_mask: NULL array");
+    if (radius >= 1.0 && nchan > 0 && !tmp) return fail(BB_ERR_ARG, "bb_dilate_mask: tmp is needed");
+    if (out == mask || (tmp && (out == tmp || tmp == mask))) return fail(BB_ERR_ARG, "bb_dilate_mask: arrays alias");
+    int rc = bb_launch_dilate(mask, nz, ny, nx, radius, nchan, tmp, out, (hipStream_t)stream);
+    return rc ? fail(BB_ERR_HIP, "bb_dilate_mask launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }

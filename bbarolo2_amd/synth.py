@@ -7,8 +7,8 @@ default_rng(seed) normal noise, convolved with the beam and rescaled to the rms.
 Likelihood inputs (the once-per-fit setup of LikelihoodMixin, bbarolo/bayesian/likelihood.py:83-440), with
 these simplifications, stated once here:
   * the emission mask is the truth model above 2 noise rms (the reference's default is MASK SEARCH);
-  * the likelihood mask dilates it by one beam FWHM in each channel and +-3 channels (dilate_mask,
-    bayesian/noise.py:18-30, the default LIKEOPTS dilation (1, 3));
+  * the likelihood mask dilates it by one beam FWHM in each channel and +-3 channels (noise.dilate_mask on the
+    GPU, bayesian/noise.py:18-30, the default LIKEOPTS dilation (1, 3));
   * sigma is 1.4826 x the MAD of the data outside the emission mask (likelihood.py:107-114);
   * likefactor is 1 / beam area in pixels (LIKEOPTS noisecorr='beam');
   * the region is the box of the model of the initial rings (DiskModel.box, final=True).
@@ -21,6 +21,7 @@ import numpy as np
 from .header import emptyheader, beam_area_pix
 from .rings import Rings
 from .galmod import GalmodPar
+from .noise import dilate_mask
 
 
 def config(n):
@@ -154,25 +155,11 @@ def add_noise(model_zyx, rms, kern, seed):
     return (model_zyx + noise).astype(np.float32)
 
 
-def dilate_mask(mask, radius, nchan):
-    """A disk of the given radius (pixels) in each channel, then +-nchan channels (bayesian/noise.py:18-30)."""
-    from scipy.ndimage import distance_transform_edt, maximum_filter1d
-    m = np.asarray(mask, bool)
-    out = m.copy()
-    if radius > 0:
-        for k in np.flatnonzero(m.any(axis=(1, 2))):
-            out[k] = distance_transform_edt(~m[k]) <= radius
-    if nchan > 0:
-        out = maximum_filter1d(out.view(np.uint8), size=2 * int(nchan) + 1, axis=0, mode="constant",
-                               cval=0).astype(bool)
-    return out
-
-
 def likelihood_inputs(h, truth_zyx, data, noise_rms, box):
     """(likemask, region, sigma, likefactor) for a synthetic cube (see the module docstring)."""
     mask = truth_zyx > 2. * noise_rms
     bpix = abs(h.Bmaj() / h.Cdelt(0)) if h.Bmaj() else 1.
-    likemask = dilate_mask(mask, bpix, 3)
+    likemask = dilate_mask(mask, bpix, 3)          # (on the GPU)
     out = data[~mask & (data != 0)]
     sigma = float(1.4826 * np.median(np.abs(out - np.median(out)))) if out.size else float(noise_rms)
     nz, ny, nx = data.shape

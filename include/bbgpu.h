@@ -226,6 +226,16 @@ int bb_mlp_grad(bb_mlp* net, const float* x, const float* t, int rows, float* gr
 int bb_mlp_train(bb_mlp* net, const float* x, const float* t, int rows, int steps, float lr, float* loss,
                  void* stream);
 
+/* ---- per-fit set-up: the likelihood mask -----------------------------------------------------------------------
+ * Replaces dilate_mask (bbarolo/bayesian/noise.py:18-30; called by _setup_likemask, likelihood.py:160-175, with the
+ * radius nbeam x beam FWHM in pixels): in every channel, the pixels within a Euclidean distance radius of a mask
+ * pixel of that channel, then the maximum over +-nchan channels (zeros outside the cube). mask, tmp and out are
+ * DEVICE (nz, ny, nx) uint8 arrays (mask: nonzero = in the mask; out: 0 / 1); tmp is needed when radius >= 1 and
+ * nchan > 0 (else it may be NULL); out must not alias mask or tmp. floor(radius) <= bb_dilate_rmax(). */
+int bb_dilate_mask(const uint8_t* mask, int nz, int ny, int nx, double radius, int nchan, uint8_t* tmp,
+                   uint8_t* out, void* stream);
+int bb_dilate_rmax(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,3 +78,18 @@ def test_argument_errors_of_the_profile_mc_and_ring_entry_points():
     # deposit events: both or neither
     assert L.bb_trace_deposit(v, None) != 0
     assert L.bb_trace_deposit(None, None) == 0
+
+
+def test_argument_errors_of_the_mask_dilation():
+    L = _lib.lib()
+    p = C.c_void_p(64)
+    assert L.bb_dilate_rmax() >= 32
+    assert L.bb_dilate_mask(p, 70000, 8, 8, 2.0, 1, p, C.c_void_p(128), None) != 0 and b"sizes" in L.bb_last_error()
+    assert L.bb_dilate_mask(p, 4, 8, 8, float("nan"), 1, p, C.c_void_p(128), None) != 0
+    assert L.bb_dilate_mask(p, 4, 8, 8, 2.0, -1, p, C.c_void_p(128), None) != 0
+    rc = L.bb_dilate_mask(p, 4, 8, 8, L.bb_dilate_rmax() + 1.5, 1, C.c_void_p(96), C.c_void_p(128), None)
+    assert rc != 0 and b"exceeds" in L.bb_last_error()
+    assert L.bb_dilate_mask(p, 4, 8, 8, 2.0, 1, None, C.c_void_p(128), None) != 0 and b"tmp" in L.bb_last_error()
+    assert L.bb_dilate_mask(p, 4, 8, 8, 2.0, 1, C.c_void_p(96), p, None) != 0 and b"alias" in L.bb_last_error()
+    assert L.bb_dilate_mask(None, 4, 8, 8, 2.0, 1, C.c_void_p(96), C.c_void_p(128), None) != 0
+    assert L.bb_dilate_mask(None, 0, 8, 8, 2.0, 1, None, None, None) == 0          # (an empty cube: nothing to do)
