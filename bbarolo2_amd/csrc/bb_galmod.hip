@@ -574,13 +574,12 @@ __device__ __forceinline__ bool tile_zrange(const Sub& u, const Track& tr, const
     return true;
 }
 
-// The line profile of one sample (galmod.py:411-431), shared by its runs in the tile: runs r with rs[r] >= 0 (the
-// tile slot) and rw[r] (the vertical weight). COMPACT: the runs are rs[0..nr) and the loop over r stops at the
-// first r that no lane of the wavefront has; otherwise rs is indexed by the run's last vertical sample and empty
-// entries are skipped. LIN: a linear velocity axis (AXTYP 4) with one channel factor for all channels: the
+// The line profile of one sample (galmod.py:411-431), shared by its runs in the tile: the runs rs[0..nr) (tile
+// slots) with their vertical weights rw[0..nr); the loop over r stops at the first r that no lane of the
+// wavefront has. LIN: a linear velocity axis (AXTYP 4) with one channel factor for all channels: the
 // channel edges are computed as the host computes them, ((k - 1/2) + 1 - crpix3) cdelt3 + crval3 in FP64 (the
 // same operations, the same values), and the channel factor is a constant, instead of being read from LDS.
-template <bool LIN, bool COMPACT>
+template <bool LIN>
 __device__ __forceinline__ void an_lines(const bb_grid& g, const Acc& A, const double* sve, const float* scf,
                                          const float* sph, int ntab, const Sub& u, double vplane, double afl,
                                          const int* rs, const float* rw, int nr) {
@@ -619,11 +618,8 @@ __device__ __forceinline__ void an_lines(const bb_grid& g, const Acc& A, const d
             pp = pe[BB_CB - 1];
 #pragma unroll
             for (int r = 0; r < BB_RMAX; r++) {
-                const bool on = COMPACT ? r < nr : rs[r] >= 0;
-                if (__ballot(on) == 0) {
-                    if (COMPACT) break;      // (no active lane has a run r: the wavefront leaves the loop)
-                    continue;
-                }
+                const bool on = r < nr;
+                if (__ballot(on) == 0) break;      // (no lane has a run r: the wavefront leaves the loop)
                 if (on) {
                     // (channels past the window have prof 0 and add 0: no branch per channel; the tile has BB_CB
                     // words of padding)
@@ -658,13 +654,14 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
     int rs[BB_RMAX];
     float rw[BB_RMAX];
     if (u.nzz <= BB_RMAX) {
-        // a short column: the runs are stored at the index of their last vertical sample (a register index
-        // known at compile time in the unrolled loop), so no run is moved; rs = -1 where no run ends. The whole
-        // column is walked: the unrolled loop costs the same for any sub-range, and its runs outside the tile
-        // are not deposited.
+        // a short column: the whole column is walked (the unrolled loop costs the same for any sub-range) and
+        // each run that ends inside the tile is appended to rs[0..nr) (a select over the unrolled slots, no
+        // register indexing), so the line loop runs over as many slots as the wavefront's longest list (runs
+        // stored at the index of their last vertical sample instead left 3 of 4 lane slots idle: 4.7% slower)
         const int ka = 0, kb = u.nzz - 1;
         int c0 = 0, c1 = 0;
         double ws = 0.0;
+        int nr = 0;
 #pragma unroll
         for (int kz = 0; kz <= BB_RMAX; kz++) {
             const bool in = kz >= ka && kz <= kb;
@@ -673,8 +670,16 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
             const bool same = in && kz > ka && h0 == c0 && h1 == c1;
             if (kz > 0) {
                 const bool close = kz - 1 >= ka && kz - 1 <= kb && !same;     // the run open at kz - 1 ends there
-                rs[kz - 1] = close ? T.slot(c0, c1) : -1;
-                rw[kz - 1] = (float)ws;
+                const int sl = close ? T.slot(c0, c1) : -1;
+                if (sl >= 0) {
+#pragma unroll
+                    for (int r = 0; r < BB_RMAX; r++)
+                        if (r == nr) {
+                            rs[r] = sl;
+                            rw[r] = (float)ws;
+                        }
+                    nr++;
+                }
             }
             if (in) {
                 const double w = uw_(u, wl, wt, kz, g.ltype);
@@ -687,7 +692,7 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
                 }
             }
         }
-        an_lines<LIN, false>(g, A, sve, scf, sph, ntab, u, vplane, afl, rs, rw, 0);
+        if (nr > 0) an_lines<LIN>(g, A, sve, scf, sph, ntab, u, vplane, afl, rs, rw, nr);
         return;
     }
     // a long column: the runs in the tile are compacted into rs[0..nr); beyond BB_RMAX they are deposited one by one
@@ -733,7 +738,7 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
             ws = uw_(u, wl, wt, kz, g.ltype);
         }
     }
-    if (nr > 0) an_lines<LIN, true>(g, A, sve, scf, sph, ntab, u, vplane, afl, rs, rw, nr);
+    if (nr > 0) an_lines<LIN>(g, A, sve, scf, sph, ntab, u, vplane, afl, rs, rw, nr);
 }
 
 // The samples of a batch vector that meet the tile [tx0, tx1] x [ty0, ty1] (box-relative): the chunks whose
