@@ -26,7 +26,8 @@ from .noise import dilate_mask
 
 def config(n):
     """(header, truth Rings, GalmodPar, noise fraction of the peak, noise seed, free spec, description) of
-    configuration n (1, 2, 3, 5)."""
+    configuration n (1: CLASSIC 128x128x64, 2: GLOBAL 160x160x96, 3: 16-D Bayesian 128x128x64, 5: 13-D Bayesian
+    64x64x40; "ngc2403": the geometry of the reference's likelihood benchmark)."""
     if n == 1:
         h = emptyheader([128, 128, 64], [-10 / 3600., 10 / 3600., 5.], beam=30 / 3600., obj="CFG1")
         R = np.arange(15., 346., 30.)
