@@ -31,6 +31,8 @@ extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* s
                                    int nmap, int map0, int nw, double qunit, unsigned long long* counts, int* bin,
                                    int bin_cap, hipStream_t st);
 extern "C" size_t bb_rm_tile_lds(int nw, int TX, int TY);
+extern "C" int bb_launch_zero_planes(unsigned long long* counts, int B, int nmap, int map0, int nw, int64_t plane,
+                                     hipStream_t st);
 extern "C" int bb_norm_blocks(int by, int bx);
 extern "C" int bb_launch_norm(int B, int by, int bx, int nz, int ny, int nx, int64_t blo0, int64_t blo1, double wpow,
                               int side, int ftype, double data_noise, double bweight, int normtype, int foot,
@@ -464,11 +466,8 @@ extern "C" int bb_ring_maps_deposit(const bb_model* m, const bb_azplan* p, bb_bo
     if (!TX) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: %d maps do not fit the LDS tile", nw);
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
     if (nsub == 0) {
-        const size_t plane = (size_t)by * bx;
-        for (int b = 0; b < B; b++)
-            rc = hip_check(hipMemsetAsync(counts + ((size_t)b * nmap + map0) * plane, 0, plane * nw * sizeof(uint64_t),
-                                          (hipStream_t)stream), "bb_ring_maps_deposit");
-        return rc;
+        rc = bb_launch_zero_planes((unsigned long long*)counts, B, nmap, map0, nw, (int64_t)by * bx, (hipStream_t)stream);
+        return rc ? fail(BB_ERR_HIP, "bb_ring_maps_deposit: %s", hipGetErrorString((hipError_t)rc)) : 0;
     }
     char* w = (char*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
     double* sub = (double*)w;

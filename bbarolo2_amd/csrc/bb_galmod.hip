@@ -1098,6 +1098,22 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     }
 }
 
+// Zero planes [map0, map0 + nw) of every vector's counts (B, nmap, plane): a kernel rather than a memset, so that the
+// launch can be captured in a HIP graph.
+__global__ void k_zero_planes(unsigned long long* __restrict__ counts, int nmap, int map0, int nw, int64_t plane) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < plane * nw) counts[((int64_t)blockIdx.y * nmap + map0) * plane + i] = 0ull;
+}
+
+extern "C" int bb_launch_zero_planes(unsigned long long* counts, int B, int nmap, int map0, int nw, int64_t plane,
+                                     hipStream_t st) {
+    const int64_t n = plane * nw;
+    if (n == 0 || B == 0) return 0;
+    hipLaunchKernelGGL(k_zero_planes, dim3((unsigned)((n + 255) / 256), B), dim3(256), 0, st, counts, nmap, map0, nw,
+                       plane);
+    return (int)hipGetLastError();
+}
+
 extern "C" size_t bb_rm_tile_lds(int nw, int TX, int TY) {
     return (size_t)TX * TY * (nw | 1) * sizeof(unsigned long long);
 }

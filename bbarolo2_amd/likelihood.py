@@ -199,6 +199,10 @@ class GpuLikelihood:
             self._bufs[(B, slot)] = b
         return b
 
+    def buffers_in_use(self):
+        """References to the device buffers of the current batch size (this likelihood's and its profile's)."""
+        return (dict(self._bufs), dict(self.profile._bufs) if self.profile is not None else None)
+
     def _stream_pair(self):
         if self._streams is None:
             self._streams = (self.torch.cuda.Stream(), self.torch.cuda.Stream())     # (default priority)
@@ -332,11 +336,13 @@ class GpuLikelihood:
             with torch.cuda.graph(graph):
                 out = self.device_batch(parts)
             torch.cuda.synchronize()
-            g = (graph, parts, out)
+            # (the graph writes into the buffers of this batch size: hold them, since a batch of another size
+            # replaces them in _bufs)
+            g = (graph, parts, out, self.buffers_in_use())
             if len(self._graphs) >= 4:                # (a few batch sizes: a sampler's batches vary in size)
                 self._graphs.pop(next(iter(self._graphs)))
             self._graphs[B] = g
-        graph, static, out = g
+        graph, static, out, _ = g
         for p, q in zip(static, parts):
             p["d_sr"].copy_(q["d_sr"])
             if "d_sr_ext" in p:

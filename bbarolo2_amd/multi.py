@@ -43,13 +43,16 @@ class GalaxyBatch:
     of parameter vectors per galaxy (each at most that likelihood's chunk rows) and returns their log-likelihoods
     (-inf where a positive-only parameter is negative), as the galaxies' own loglike would."""
 
-    def __init__(self, likes, n_streams=4):
+    def __init__(self, likes, n_streams=4, graphs=False):
         self.likes = list(likes)
         torch = self.likes[0].torch
         self.torch = torch
         self.streams = [torch.cuda.Stream() for _ in range(n_streams)]       # (default priority)
         size = [l.dvalid.numel() for l in self.likes]
         self.order = np.argsort(size, kind="stable")[::-1]                  # largest model box first
+        self.graphs = bool(graphs)
+        self._graph = None          # (key: rows per galaxy, CUDAGraph, static parts, outputs)
+        self.n_replays = 0
 
     def tune(self, thetas, reps=2):
         """Pick the deposit tile of every galaxy for batches like thetas (tuner.tune_tiles); returns the shapes."""
@@ -92,6 +95,64 @@ class GalaxyBatch:
             cur.wait_stream(s)
         return pending, ends
 
+    def _launch_parts(self, parts):
+        """The device work of prepared parts (one list per galaxy, or None) on the streams, largest galaxy first;
+        returns each galaxy's output tensors."""
+        torch = self.torch
+        cur = torch.cuda.current_stream()
+        for s in self.streams:
+            s.wait_stream(cur)
+        res = [None] * len(self.likes)
+        for j, g in enumerate(self.order):
+            if parts[g] is None:
+                continue
+            with torch.cuda.stream(self.streams[j % len(self.streams)]):
+                res[g] = self.likes[g].device_batch(parts[g])
+        for s in self.streams:
+            cur.wait_stream(s)
+        return res
+
+    def graph_step(self, thetas):
+        """The step as one captured HIP graph: the first step with these row counts runs eagerly (allocating the
+        buffers), then its launches on all the streams are captured once; every later step copies its tables into
+        the graph's input buffers and replays it. Steps with a row whose positive-only parameter is negative (their
+        batches shrink) run eagerly. Returns the log-likelihoods, as loglike."""
+        torch = self.torch
+        ths = [np.atleast_2d(np.asarray(t, float)) for t in thetas]
+        if any(np.any(l.free.negative(t)) for l, t in zip(self.likes, ths)):
+            return self.loglike(thetas)
+        key = tuple(t.shape[0] for t in ths)
+        parts = []
+        for like, th in zip(self.likes, ths):
+            if th.shape[0] > like.chunk:
+                raise ValueError(f"{th.shape[0]} vectors for a likelihood of chunk {like.chunk}")
+            sr, vals = like.tables(th)
+            parts.append(like.split_parts(sr, vals, halves=False) if th.shape[0] else None)
+        if self._graph is None or self._graph[0] != key:
+            self._graph = None
+            self._launch_parts(parts)           # (eager: allocates every buffer, sets the kernel attributes)
+            torch.cuda.synchronize()
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                out = self._launch_parts(parts)
+            torch.cuda.synchronize()
+            # (the graph writes into the likelihoods' buffers of these batch sizes: hold them, since a batch of
+            # another size replaces them)
+            self._graph = (key, graph, parts, out, [l.buffers_in_use() for l in self.likes])
+        else:
+            _, graph, static, out, _ = self._graph
+            for ps, pn in zip(static, parts):
+                for a, b in zip(ps or (), pn or ()):
+                    a["d_sr"].copy_(b["d_sr"])
+                    if "d_sr_ext" in a:
+                        a["d_sr_ext"].copy_(b["d_sr_ext"])
+        self._graph[1].replay()
+        self.n_replays += 1
+        torch.cuda.current_stream().synchronize()
+        res = self._graph[3]
+        return [like.finish(*like.collect(r)) if r is not None else np.zeros(0)
+                for like, r in zip(self.likes, res)]
+
     def collect(self, pending):
         """The log-likelihoods of a launched step (waits for it)."""
         self.torch.cuda.current_stream().synchronize()
@@ -105,6 +166,16 @@ class GalaxyBatch:
 
     def loglike(self, thetas):
         return self.collect(self.launch(thetas)[0])
+
+    def timed_graph_step(self, thetas):
+        """graph_step between HIP events: (log-likelihoods, ms of the step from the table copies to the end)."""
+        torch = self.torch
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        out = self.graph_step(thetas)
+        e1.record()
+        e1.synchronize()
+        return out, e0.elapsed_time(e1)
 
     def timed_step(self, thetas):
         """One step with HIP events: (log-likelihoods, ms of the whole step, ms from the step's start to the end of

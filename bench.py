@@ -351,6 +351,13 @@ def multi_galaxy_bench(n=64, B=32, steps=3, seed=11):
         step_ms.append(ms)
         done.append(d)
         host.append(hm)
+    # the same step as one captured HIP graph (captured on its first step, replayed with new tables)
+    from bbarolo2_amd.multi import GalaxyBatch
+    gg = GalaxyBatch(gb.likes, n_streams=len(gb.streams), graphs=True)
+    ref = gg.graph_step(thetas)
+    if not all(np.array_equal(a, b) for a, b in zip(ref, ll)):
+        raise RuntimeError("multi-galaxy batch: the graph replay differs from the eager step")
+    graph_ms = float(np.median([gg.timed_graph_step(thetas)[1] for _ in range(steps)]))
     alone = gb.alone_ms(thetas)
     ms = float(np.median(step_ms))
     d = np.median(np.array(done), axis=0)
@@ -358,13 +365,14 @@ def multi_galaxy_bench(n=64, B=32, steps=3, seed=11):
     shapes = {}
     for t in tiles:
         shapes[f"{t[0]}x{t[1]}"] = shapes.get(f"{t[0]}x{t[1]}", 0) + 1
-    print(f"[bench] multi-galaxy step {ms:.1f} ms ({n * B / ms * 1e3:.0f} evals/s), sum of alone "
-          f"{alone.sum():.1f} ms", file=sys.stderr, flush=True)
+    print(f"[bench] multi-galaxy step {ms:.1f} ms ({n * B / ms * 1e3:.0f} evals/s), as a graph {graph_ms:.1f} ms, "
+          f"sum of alone {alone.sum():.1f} ms", file=sys.stderr, flush=True)
     return {"workload": f"{n} galaxies (seed {seed}), cube sides {min(f['h'].DimAx(0) for f in fits)}-"
                         f"{max(f['h'].DimAx(0) for f in fits)} px, {sum(vox) / 1e6:.1f} M voxels in all; GLOBAL "
-                        f"likelihood, {B} vectors per galaxy per step, four streams",
+                        f"likelihood, {B} vectors per galaxy per step, four streams; also as one captured HIP graph",
             "evals_per_s": n * B / ms * 1e3, "ms_per_step": round(ms, 3), "steps": steps,
             "host_launch_ms": round(float(np.median(host)), 3),
+            "graph_ms_per_step": round(graph_ms, 3), "graph_evals_per_s": n * B / graph_ms * 1e3,
             "per_galaxy_ms_in_step": {"p50": round(float(np.percentile(d, 50)), 3),
                                       "p99": round(float(np.percentile(d, 99)), 3)},
             "per_galaxy_ms_alone": {"p50": round(float(np.percentile(alone, 50)), 3),

@@ -77,3 +77,27 @@ def test_galaxy_batch_matches_each_galaxy():
     assert tiles[3] is None and all(t is not None for i, t in enumerate(tiles) if i != 3)
     again = gb.loglike(thetas)
     assert all(np.array_equal(a, b) for a, b in zip(again, got))
+
+
+def test_galaxy_batch_graph_replays_match_eager():
+    import torch
+    from bbarolo2_amd.multi import build_fits, draw_thetas, GalaxyBatch
+    torch.cuda.set_device(0)
+    fits, _ = build_fits(_small(3), chunk=16)
+    gb = GalaxyBatch([f["like"] for f in fits], n_streams=4, graphs=True)
+    for rep in range(4):
+        thetas = draw_thetas(fits, 16, seed=10 + rep)
+        got = gb.graph_step(thetas)
+        # eager work in between: every galaxy's own loglike, with another batch size, then the full size
+        for f, th in zip(fits, thetas):
+            f["like"].loglike(th[:5])
+        want = [f["like"].loglike(th) for f, th in zip(fits, thetas)]
+        for g in range(3):
+            print(f"replay {rep} galaxy {g}: max diff {np.max(np.abs(got[g] - want[g])):.3e}", flush=True)
+            assert np.array_equal(got[g], want[g]), (rep, g)
+        torch.ones(1 << 20, device="cuda").sum().item()
+    assert gb.n_replays == 4 and gb._graph is not None
+    # a row with a negative positive-only parameter: the step runs eagerly
+    thetas[0][2, fits[0]["free"].index["vdisp"][0]] = -1.0
+    got = gb.graph_step(thetas)
+    assert np.isneginf(got[0][2]) and gb.n_replays == 4
