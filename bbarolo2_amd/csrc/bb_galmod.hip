@@ -491,7 +491,8 @@ __device__ __forceinline__ double uw_(const Sub& u, const float* wl, const doubl
 }
 
 #define BB_LIST_CAP 1024     // chunks meeting the tile, gathered before their samples are tested
-#define BB_SAMP_CAP 2048     // samples meeting the tile, gathered before they are deposited
+#define BB_AN_SAMP_CAP 4096  // samples meeting the tile, gathered before they are deposited (ANALYTIC deposit)
+#define BB_RM_SAMP_CAP 3072  // the same for the ring maps (a longer list is slower there: 24.9 ms against 23.5)
 #define BB_WLDS_MAX 4096     // vertical weights (float) kept in LDS for all the sub-rings of a vector
 
 // Odd stride of the pixel spectra in the LDS tile: consecutive pixels start in different banks.
@@ -747,8 +748,8 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
 // one sub-ring next to each other; lane l of wavefront w takes entry l * W + w of each window of blockDim
 // entries (W wavefronts), so the lanes of one LDS atomic are W azimuths apart (different pixels: same-address
 // atomics of a wavefront serialise) on neighbouring sub-rings (similar work). Every thread of the block calls
-// it (it synchronises).
-template <class Dep>
+// it (it synchronises). SCAP: the entries of slist.
+template <int SCAP, class Dep>
 __device__ __forceinline__ void tile_samples(int nchunks, const short4* __restrict__ bb, const short4* __restrict__ sb,
                                              const int* __restrict__ tlist, int tlo, int thi, int tx0, int tx1,
                                              int ty0, int ty1, int* clist, int* slist, int& nlist, int& nsamp,
@@ -802,7 +803,7 @@ __device__ __forceinline__ void tile_samples(int nchunks, const short4* __restri
             __syncthreads();
             const int ns = nsamp;
             __syncthreads();
-            if (ns > BB_SAMP_CAP - (int)blockDim.x || j0 + ngrp >= n) {
+            if (ns > SCAP - (int)blockDim.x || j0 + ngrp >= n) {
                 // 3. the samples gathered, one per thread
                 const int W = blockDim.x >> 6;
                 for (int base2 = 0; base2 < ns; base2 += blockDim.x) {
@@ -829,7 +830,7 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     // dynamic LDS: [vedges (nlines, nz+1) f64][cd2i (nz) f64][cd2i (nz) f32][Phi (ntab) f32][weights f32][tile u64]
     extern __shared__ double dyn[];
     __shared__ int clist[BB_LIST_CAP];
-    __shared__ int slist[BB_SAMP_CAP];
+    __shared__ int slist[BB_AN_SAMP_CAP];
     __shared__ int nlist, nsamp;
     __shared__ double red[BB_TILE_THREADS / 64];
     int b, tid;
@@ -898,7 +899,8 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     const int* tl;
     int tlo, thi;
     tile_list(bin, gridDim.y, gridDim.x, bin_cap, b, tid, tl, tlo, thi);
-    tile_samples(nchunks, bb, sb, tl, tlo, thi, tx0, tx1, ty0, ty1, clist, slist, nlist, nsamp, amax > 0., [&](int e) {
+    tile_samples<BB_AN_SAMP_CAP>(nchunks, bb, sb, tl, tlo, thi, tx0, tx1, ty0, ty1, clist, slist, nlist, nsamp, amax > 0.,
+                                 [&](int e) {
         const int4 ch = chunks[e >> 5];
         const int s = ch.x >> 1;
         Sub u;
@@ -928,7 +930,7 @@ extern "C" size_t bb_analytic_lds_bytes(int nz, int nlines, int TX, int TY, int 
 }
 
 extern "C" size_t bb_analytic_lds_static() {
-    return (BB_LIST_CAP + BB_SAMP_CAP) * sizeof(int) + (BB_TILE_THREADS / 64) * sizeof(double) + 64;
+    return (BB_LIST_CAP + BB_AN_SAMP_CAP) * sizeof(int) + (BB_TILE_THREADS / 64) * sizeof(double) + 64;
 }
 
 // Events recorded around every k_an_tile launch of this host thread (measurement; null: none).
@@ -1004,7 +1006,7 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
           unsigned long long* __restrict__ counts, const int* __restrict__ bin, int bin_cap, int wlds) {
     extern __shared__ double dyn[];
     __shared__ int clist[BB_LIST_CAP];
-    __shared__ int slist[BB_SAMP_CAP];
+    __shared__ int slist[BB_RM_SAMP_CAP];
     __shared__ int nlist, nsamp;
     int b, tid;
     xcd_tile(tid, b);
@@ -1037,7 +1039,7 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     const int* tl;
     int tlo, thi;
     tile_list(bin, gridDim.y, gridDim.x, bin_cap, b, tid, tl, tlo, thi);
-    tile_samples(nchunks, bball + (size_t)b * nchunks, sball + (size_t)b * nchunks * BB_CS, tl, tlo, thi, tx0, tx1, ty0,
+    tile_samples<BB_RM_SAMP_CAP>(nchunks, bball + (size_t)b * nchunks, sball + (size_t)b * nchunks * BB_CS, tl, tlo, thi, tx0, tx1, ty0,
                  ty1, clist, slist, nlist, nsamp, true, [&](int e) {
         const int4 ch = chunks[e >> 5];
         const int s = ch.x >> 1;
@@ -1118,7 +1120,7 @@ extern "C" size_t bb_rm_tile_lds(int nw, int TX, int TY) {
     return (size_t)TX * TY * (nw | 1) * sizeof(unsigned long long);
 }
 
-extern "C" size_t bb_rm_tile_static() { return (BB_LIST_CAP + BB_SAMP_CAP) * sizeof(int) + 64; }
+extern "C" size_t bb_rm_tile_static() { return (BB_LIST_CAP + BB_RM_SAMP_CAP) * sizeof(int) + 64; }
 
 extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* sr, int nsub, int B, const int* naz,
                                    const int4* chunks, int nchunks, const double2* az, int TX, int TY, double* sub,
