@@ -478,22 +478,19 @@ __device__ __noinline__ double uw_cold(int j, int nin, double step, double zlim,
     return bb_layer_cdf(ltype, hi) - bb_layer_cdf(ltype, lo);
 }
 
-// weight of vertical sample kz: the LDS copy of the sub-ring's bins (wl, BB_WCAP of them) when there is
-// one, else the global table (BB_ZCAP bins), else the layer CDF
-#define BB_WCAP 16
-__device__ __forceinline__ double uw_(const Sub& u, const float* wl, const double* __restrict__ wt, int kz,
-                                      int ltype) {
+// weight of vertical sample kz: the sub-ring's table (BB_ZCAP bins, FP64), else the layer CDF
+__device__ __forceinline__ double uw_(const Sub& u, const double* __restrict__ wt, int kz, int ltype) {
     if (u.nzz <= 1) return 1.0;
     const int j = zbin_of(kz, u.half);
-    if (wl && j < BB_WCAP) return (double)wl[j];
     if (j < BB_ZCAP) return wt[j];
     return uw_cold(j, u.nin, u.step, u.zlim, ltype);
 }
 
 #define BB_LIST_CAP 1024     // chunks meeting the tile, gathered before their samples are tested
-#define BB_AN_SAMP_CAP 4096  // samples meeting the tile, gathered before they are deposited (ANALYTIC deposit)
+#define BB_AN_SAMP_CAP 8192  // samples meeting the tile, gathered before they are deposited (ANALYTIC deposit; the
+                             // LDS it takes is worth more here than float copies of the vertical weights: 66.1 ms
+                             // against 68.0 with 4096 samples and the weights in LDS)
 #define BB_RM_SAMP_CAP 3072  // the same for the ring maps (a longer list is slower there: 24.9 ms against 23.5)
-#define BB_WLDS_MAX 4096     // vertical weights (float) kept in LDS for all the sub-rings of a vector
 
 // Odd stride of the pixel spectra in the LDS tile: consecutive pixels start in different banks.
 __device__ __host__ __forceinline__ int bb_tile_stride(int nz) { return nz | 1; }
@@ -511,7 +508,7 @@ struct TileGeo {
 // A height-dependent velocity (galmod.py:432-465): every vertical pair has its own line.
 __device__ __forceinline__ void an_sample_lag(const bb_grid& g, const bb_box& box, const TileGeo& T, const Acc& A,
                                            const double* sve, const double* scd, const float* sph, int ntab,
-                                           const Sub& u, const float* wl, const double* __restrict__ wt, double caz,
+                                           const Sub& u, const double* __restrict__ wt, double caz,
                                            double saz) {
     const int nz = g.nz;
     const double wside = caz >= 0.0 ? u.wrec : u.wapp;
@@ -535,7 +532,7 @@ __device__ __forceinline__ void an_sample_lag(const bb_grid& g, const bb_box& bo
             v = u.vsys + (bb_lagged(u.vrot, u.dvdz, az_ - u.zcyl) * caz + u.vrad * saz) * u.sinc + u.vvert * u.cinc;
         else
             v = vplane;
-        const double w1 = uw_(u, wl, wt, kz, g.ltype), w2 = uw_(u, wl, wt, kz2, g.ltype);
+        const double w1 = uw_(u, wt, kz, g.ltype), w2 = uw_(u, wt, kz2, g.ltype);
         for (int nl = 0; nl < g.nlines; nl++) {
             const double* ve = sve + (size_t)nl * (nz + 1);
             const double a = g.relint[nl] * afl;
@@ -640,10 +637,10 @@ template <bool LIN>
 __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, const TileGeo& T, const Acc& A,
                                           const double* sve, const double* scd, const float* scf, const float* sph,
                                           int ntab,
-                                          const Sub& u, const float* wl, const double* __restrict__ wt, double caz,
+                                          const Sub& u, const double* __restrict__ wt, double caz,
                                           double saz) {
     if (u.zdep) {
-        an_sample_lag(g, box, T, A, sve, scd, sph, ntab, u, wl, wt, caz, saz);
+        an_sample_lag(g, box, T, A, sve, scd, sph, ntab, u, wt, caz, saz);
         return;
     }
     const int nz = g.nz;
@@ -683,7 +680,7 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
                 }
             }
             if (in) {
-                const double w = uw_(u, wl, wt, kz, g.ltype);
+                const double w = uw_(u, wt, kz, g.ltype);
                 if (same) {
                     ws += w;
                 } else {
@@ -702,7 +699,7 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
     int nr = 0;
     int c0, c1;
     upix_fast(u, g, tr, xs, xc, yb, uz(u, ka), c0, c1);
-    double ws = uw_(u, wl, wt, ka, g.ltype);
+    double ws = uw_(u, wt, ka, g.ltype);
     for (int kz = ka + 1; kz <= kb + 1; kz++) {
         int h0 = 0, h1 = 0;
         bool same = false;
@@ -711,7 +708,7 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
             same = h0 == c0 && h1 == c1;
         }
         if (same) {
-            ws += uw_(u, wl, wt, kz, g.ltype);
+            ws += uw_(u, wt, kz, g.ltype);
             continue;
         }
         const int sl = T.slot(c0, c1);
@@ -736,7 +733,7 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
         if (kz <= kb) {
             c0 = h0;
             c1 = h1;
-            ws = uw_(u, wl, wt, kz, g.ltype);
+            ws = uw_(u, wt, kz, g.ltype);
         }
     }
     if (nr > 0) an_lines<LIN>(g, A, sve, scf, sph, ntab, u, vplane, afl, rs, rw, nr);
@@ -827,7 +824,7 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
           int nsub, const short4* __restrict__ bball, const short4* __restrict__ sball,
           const double* __restrict__ cd2i, const double* __restrict__ vedges, const double* __restrict__ ptab,
           int ntab, float* __restrict__ outall, int accumulate, const int* __restrict__ bin, int bin_cap) {
-    // dynamic LDS: [vedges (nlines, nz+1) f64][cd2i (nz) f64][cd2i (nz) f32][Phi (ntab) f32][weights f32][tile u64]
+    // dynamic LDS: [vedges (nlines, nz+1) f64][cd2i (nz) f64][cd2i (nz) f32][Phi (ntab) f32][tile u64]
     extern __shared__ double dyn[];
     __shared__ int clist[BB_LIST_CAP];
     __shared__ int slist[BB_AN_SAMP_CAP];
@@ -840,13 +837,11 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     const int ntx = (bx + TX - 1) / TX;
     const int tx0 = (tid % ntx) * TX, ty0 = (tid / ntx) * TY;
     const int tx1 = tx0 + TX - 1, ty1 = ty0 + TY - 1;
-    const bool wlds = nsub * BB_WCAP <= BB_WLDS_MAX;
     double* sve = dyn;
     double* scd = dyn + (size_t)nl_ * (nz + 1);
     float* scf = (float*)(scd + nz);
     float* sph = scf + ((nz + 1) & ~1);
-    float* swl = sph + ((ntab + 1) & ~1);
-    unsigned long long* tile = (unsigned long long*)(swl + (wlds ? BB_WLDS_MAX : 0));
+    unsigned long long* tile = (unsigned long long*)(sph + ((ntab + 1) & ~1));
     const int ntile = TX * TY * nzp;
     const double* sub = suball + (size_t)b * nsub * BB_NSF;
     const double* wtab = wtaball + (size_t)b * nsub * BB_ZCAP;
@@ -857,9 +852,6 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
         scf[i] = (float)cd2i[i];
     }
     for (int i = threadIdx.x; i < ntab; i += blockDim.x) sph[i] = (float)ptab[i];
-    if (wlds)
-        for (int i = threadIdx.x; i < nsub * BB_WCAP; i += blockDim.x)
-            swl[i] = (float)wtab[(size_t)(i / BB_WCAP) * BB_ZCAP + i % BB_WCAP];
     // the fixed-point scale of this vector: amax bounds any single contribution
     double am = 0.;
     for (int s = threadIdx.x; s < nsub; s += blockDim.x) {
@@ -906,7 +898,7 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
         Sub u;
         load_sub(u, sub + (size_t)s * BB_NSF, ch.x & 1);
         const double2 cs = az[ch.w + (e & 31)];
-        an_sample<LIN>(g, box, T, A, sve, scd, scf, sph, ntab, u, wlds ? swl + s * BB_WCAP : nullptr,
+        an_sample<LIN>(g, box, T, A, sve, scd, scf, sph, ntab, u,
                   wtab + (size_t)s * BB_ZCAP, cs.x, cs.y);
     });
     __syncthreads();
@@ -922,11 +914,10 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     }
 }
 
-// LDS bytes of the dynamic part of k_an_tile: channel edges and factors (f64), Phi table and vertical weights
-// (f32), the tile (u64).
+// LDS bytes of the dynamic part of k_an_tile: channel edges and factors (f64), Phi table (f32), the tile (u64).
 extern "C" size_t bb_analytic_lds_bytes(int nz, int nlines, int TX, int TY, int ntab) {
     return ((size_t)nlines * (nz + 1) + nz) * sizeof(double) + (size_t)(((nz + 1) & ~1) + ((ntab + 1) & ~1)) * sizeof(float) +
-           (size_t)BB_WLDS_MAX * sizeof(float) + ((size_t)TX * TY * bb_tile_stride(nz) + BB_CB) * sizeof(unsigned long long);
+           ((size_t)TX * TY * bb_tile_stride(nz) + BB_CB) * sizeof(unsigned long long);
 }
 
 extern "C" size_t bb_analytic_lds_static() {
@@ -1054,7 +1045,7 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
         auto wgt = [&](int kz) -> double {        // the weight of vertical sample kz, FP64
             if (u.nzz <= 1) return 1.0;
             const int j = zbin_of(kz, u.half);
-            return (wd && j < BB_RM_WCAP) ? wd[j] : uw_(u, nullptr, wt, kz, g.ltype);
+            return (wd && j < BB_RM_WCAP) ? wd[j] : uw_(u, wt, kz, g.ltype);
         };
         const double2 cs = az[ch.w + (e & 31)];
         const double wside = cs.x >= 0.0 ? u.wrec : u.wapp;
