@@ -27,7 +27,8 @@
 #define BB_RMAX 16    // runs of a sample inside a tile held in registers (beyond: deposited one by one)
 #define BB_CB 8       // channels of a line profile computed together
 #define BB_TILE_THREADS 768     // 12 wavefronts: 3 per SIMD at the LDS of one workgroup per CU
-#define BB_RM_THREADS 512      // ring maps workgroup (1024 threads, 4 wavefronts per SIMD, measured no faster)
+#define BB_RM_THREADS 768      // ring maps workgroup: 3 wavefronts per SIMD (maps of 256 vectors 21.1 ms against
+                               // 23.4 with 512 threads; 1024 measured no faster)
 #define BB_RM_WCAP 8           // vertical weights per sub-ring kept in LDS by k_rm_tile (a column of <= 16 samples)
 
 enum {
