@@ -27,7 +27,7 @@ PEAK_HBM = 8.0e12          # MI355X HBM3E, bytes/s
 # HBM bytes per launch of k_an_tile at batch 256 on the 160x160x96 box, from two rocprofv3 counter passes
 # (FETCH_SIZE, WRITE_SIZE) scaled by the same counters of a plain 1 GiB copy (bench.py --copy-calibration)
 TRAFFIC_SHAPE = (256, 160, 160, 96)
-TRAFFIC_TILE_BYTES = (1837822.9 * 2.0 + 4709236.5 * 1.0) * 1024   # KiB x scale -> bytes
+TRAFFIC_TILE_BYTES = (1837977.6 * 2.0 + 4709621.0 * 1.0) * 1024   # KiB x scale -> bytes
 
 
 def workload(n_theta, B, seed=1):
