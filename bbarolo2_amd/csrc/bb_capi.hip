@@ -81,6 +81,7 @@ static int fail(int code, const char* fmt, ...) {
 #define BB_ERR_ARG 1
 #define BB_ERR_HIP 2
 #define BB_LDS_MAX (160 * 1024)   // LDS of a gfx950 CU, all usable by one workgroup
+#define BB_BIN_PER_CHUNK 8        // most entries per chunk of the tiles' chunk lists (the work buffers hold them)
 
 static int hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) return fail(BB_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
@@ -99,6 +100,7 @@ struct bb_model {
     double* d_kern64;
     double scale;
     int TX, TY;        // pixel tile of the ANALYTIC deposit (its spectra fit the LDS budget)
+    int bin_per;       // capacity of the tiles' chunk lists, in entries per chunk (0: no lists, every tile scans)
     int lin;           // linear velocity axis whose edges are exactly ((k - 1/2) + 1 - crpix3) cdelt3 + crval3 and
                        // one channel factor cd2i for all channels: the deposit computes them instead of reading them
 };
@@ -178,6 +180,7 @@ extern "C" int bb_model_create(const bb_grid* g, const double* cd2i, const doubl
     m->g = *g;
     m->ntab = ntab;
     m->scale = scale;
+    m->bin_per = BB_BIN_PER_CHUNK;
     (void)hipGetDevice(&m->device);
     rc = upload(&m->d_cd2i, cd2i, (size_t)g->nz, "upload cd2i");
     if (!rc) rc = upload(&m->d_vedges, vedges, (size_t)g->nlines * (g->nz + 1), "upload vedges");
@@ -227,6 +230,13 @@ extern "C" int bb_model_set_tile(bb_model* m, int TX, int TY) {
         return fail(BB_ERR_ARG, "bb_model_set_tile: a %dx%d tile of %d channels exceeds the LDS", TX, TY, m->g.nz);
     m->TX = TX;
     m->TY = TY;
+    return 0;
+}
+
+extern "C" int bb_model_set_bin(bb_model* m, int per_chunk) {
+    if (!m || per_chunk < 0 || per_chunk > BB_BIN_PER_CHUNK)
+        return fail(BB_ERR_ARG, "bb_model_set_bin: %d entries per chunk (0..%d)", per_chunk, BB_BIN_PER_CHUNK);
+    m->bin_per = per_chunk;
     return 0;
 }
 
@@ -303,9 +313,9 @@ extern "C" int bb_azplan_create(const bb_model* m, const double* R0, int nsub, b
     return 0;
 }
 
-// The chunk lists of the pixel tiles (bin_chunks): up to BB_BIN_PER_CHUNK tiles per chunk on average, and at most
+// The chunk lists of the pixel tiles (bin_chunks): up to BB_BIN_PER_CHUNK tiles per chunk on average (the work
+// buffers are sized for that; a model may use fewer, bb_model_set_bin), and at most
 // nchunks tiles (beyond: the tiles test every chunk box).
-#define BB_BIN_PER_CHUNK 8
 static int64_t bin_cap(const bb_azplan* p) { return (int64_t)p->nchunks * BB_BIN_PER_CHUNK; }
 static size_t bin_ints(const bb_azplan* p) { return (size_t)3 * (p->nchunks + 1) + 1 + (size_t)bin_cap(p); }
 
@@ -317,8 +327,8 @@ extern "C" int64_t bb_analytic_work_size(const bb_azplan* p, int B) {
 }
 
 // the bin region of the work buffer after the sample boxes, or NULL when the box has more tiles than chunks
-static int* work_bin(const bb_azplan* p, int B, short4* sbox, long long ntiles) {
-    if (ntiles > p->nchunks || bin_cap(p) > INT32_MAX) return nullptr;
+static int* work_bin(const bb_model* m, const bb_azplan* p, int B, short4* sbox, long long ntiles) {
+    if (m->bin_per <= 0 || ntiles > p->nchunks || bin_cap(p) > INT32_MAX) return nullptr;
     return (int*)(sbox + (size_t)B * p->nchunks * bb_analytic_chunk());
 }
 
@@ -356,7 +366,7 @@ extern "C" int bb_galmod_analytic(const bb_model* m, const bb_azplan* p, bb_box 
     short4* sbox = (short4*)w;
     rc = bb_launch_galmod_analytic(&m->g, box, sr, nsub, B, p->d_naz, p->d_chunks, p->nchunks, p->d_az, m->TX, m->TY,
                                    sub, wtab, bbox, sbox, m->d_cd2i, m->d_vedges, m->d_ptab, m->ntab, m->lin, out,
-                                   accumulate, work_bin(p, B, sbox, ntiles), (int)bin_cap(p), (hipStream_t)stream);
+                                   accumulate, work_bin(m, p, B, sbox, ntiles), (int)(p->nchunks * (int64_t)m->bin_per), (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_galmod_analytic launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
 
@@ -480,7 +490,7 @@ extern "C" int bb_ring_maps_deposit(const bb_model* m, const bb_azplan* p, bb_bo
     const long long ntiles = (long long)((bx + TX - 1) / TX) * ((by + TY - 1) / TY);
     rc = bb_launch_ring_maps(&m->g, box, sr, nsub, B, p->d_naz, p->d_chunks, p->nchunks, p->d_az, TX, TY, sub, wtab,
                              bbox, sbox, (const int*)widx, wval, nmap, map0, nw, qunit, (unsigned long long*)counts,
-                             work_bin(p, B, sbox, ntiles), (int)bin_cap(p), (hipStream_t)stream);
+                             work_bin(m, p, B, sbox, ntiles), (int)(p->nchunks * (int64_t)m->bin_per), (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_ring_maps_deposit launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
 

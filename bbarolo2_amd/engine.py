@@ -63,6 +63,14 @@ class DiskModel:
         self._handle = None
         self._plans = None
         self._device = device
+        self._bin = None
+
+    def set_bin(self, per_chunk):
+        """The capacity of the deposits' per-tile chunk lists (bb_model_set_bin): 0..8 entries per chunk, 0 for
+        none (every tile tests every chunk box). The models do not depend on it."""
+        self._bin = int(per_chunk)
+        for h in (self._handle or {}).values():
+            _lib.check(_lib.lib().bb_model_set_bin(h, self._bin))
 
     # ------------------------------------------------------------------ device objects
     def grid(self, nzq):
@@ -105,6 +113,8 @@ class DiskModel:
             _lib.check(L.bb_model_create(C.byref(g), cd.ctypes.data, ve.ctypes.data, pt.ctypes.data, len(pt), kp,
                                          ky, kx, float(self.scalefactor), C.byref(out)))
             h = out
+            if self._bin is not None:
+                _lib.check(L.bb_model_set_bin(h, int(self._bin)))
             self._handle[key] = h
         return h
 

@@ -62,6 +62,11 @@ int bb_model_destroy(bb_model* m);
  * set_tile lets a tuner choose another (rejected if it does not fit). No counterpart in the reference. */
 int bb_model_set_tile(bb_model* m, int TX, int TY);
 int bb_model_get_tile(const bb_model* m, int* TX, int* TY);
+/* The capacity of the per-tile chunk lists of the deposits (a counting sort of the chunks' pixel boxes by tile),
+ * in entries per chunk, 0..8 (default 8). A vector whose chunk boxes meet more tiles than that has no lists and its
+ * tiles test every chunk box; 0 turns the lists off. The results do not depend on it (exact integer sums). No
+ * counterpart in the reference. */
+int bb_model_set_bin(bb_model* m, int per_chunk);
 
 /* The azimuth plan of a set of sub-ring radii R0 (arcmin, HOST array, nsub): the ANALYTIC samples of every
  * sub-ring (naz = max(int(3 circumference / pixel + 0.5), 16), two radial samples, galmod.py:339-345) and the
