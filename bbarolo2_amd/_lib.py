@@ -35,6 +35,30 @@ class bb_profile_desc(C.Structure):
                 ("dmed", C.c_double)]
 
 
+NKEY = 18      # ring keys of the likelihood context (bbgpu.h BB_K_*)
+
+
+class bb_ring_plan(C.Structure):
+    _fields_ = [("nmr", C.c_int32), ("nsub", C.c_int32), ("thick", C.c_int32), ("from_fit", C.c_uint32),
+                ("ridx", C.c_void_p), ("rad", C.c_void_p), ("base", C.c_void_p), ("idx", C.c_void_p),
+                ("dr", C.c_void_p), ("cst", C.c_void_p), ("rads", C.c_void_p), ("densw", C.c_void_p),
+                ("side", C.c_void_p)]
+
+
+class bb_like_desc(C.Structure):
+    _fields_ = [("nr", C.c_int32), ("ndim", C.c_int32), ("chunk", C.c_int32), ("mc", C.c_int32), ("nvpar", C.c_int32),
+                ("seed", C.c_uint32), ("box_free", C.c_int32), ("halves", C.c_int32), ("likebox", bb_box),
+                ("fixbox", bb_box), ("sigma", C.c_double), ("likefactor", C.c_double), ("eps", C.c_double),
+                ("ext", C.c_double), ("pixas0", C.c_double), ("pixas1", C.c_double), ("nx", C.c_int32),
+                ("ny", C.c_int32), ("mc_cap", C.c_int64)]
+
+
+class bb_like_profile(C.Structure):
+    _fields_ = [("d", bb_profile_desc), ("wpix", C.c_void_p), ("ypix", C.c_void_p), ("lcoef", C.c_void_p),
+                ("lcol", C.c_void_p), ("near", C.c_void_p), ("widx", C.c_void_p), ("wval", C.c_void_p),
+                ("widx_e", C.c_void_p), ("wval_e", C.c_void_p), ("qunit", C.c_double), ("unit", C.c_double)]
+
+
 class bb_norm_par(C.Structure):
     _fields_ = [("side", C.c_int32), ("ftype", C.c_int32), ("normtype", C.c_int32), ("foot", C.c_int32),
                 ("wpow", C.c_double), ("data_noise", C.c_double), ("bweight", C.c_double), ("hmax", C.c_double)]
@@ -79,6 +103,18 @@ _SIGS = {
     "bb_profile_work_size": (C.c_int64, [_P, C.c_int]),
     "bb_profile_solve": (C.c_int, [_P, _P, C.c_int, _P, C.c_int64, _P, _P]),
     "bb_profile_set_dens": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P, _P, _P]),
+    "bb_ctx_create": (C.c_int, [_P, C.POINTER(bb_like_desc), _P, _P, _P, _P, _P, _P, C.POINTER(bb_ring_plan),
+                                C.POINTER(bb_ring_plan), C.POINTER(bb_like_profile), C.POINTER(_P)]),
+    "bb_ctx_destroy": (C.c_int, [_P]),
+    "bb_loglike_launch": (C.c_int, [_P, _P, C.c_int, _P]),
+    "bb_loglike_collect": (C.c_int, [_P, _P, _P]),
+    "bb_loglike_batch": (C.c_int, [_P, _P, C.c_int, _P, _P, _P]),
+    "bb_ctx_tables": (C.c_int, [_P, _P, C.c_int, _P, _P]),
+    "bb_ctx_info": (C.c_int, [_P, _P, _P, _P, _P, _P]),
+    "bb_ctx_set_halves": (C.c_int, [_P, C.c_int]),
+    "bb_ctx_set_graphs": (C.c_int, [_P, C.c_int]),
+    "bb_ctx_graph_replays": (C.c_int64, [_P]),
+    "bb_ctx_trace": (C.c_int, [_P, _P]),
     "bb_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_P)]),
     "bb_mlp_destroy": (C.c_int, [_P]),
     "bb_mlp_param_count": (C.c_int64, [_P]),

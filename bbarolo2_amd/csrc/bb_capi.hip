@@ -8,13 +8,17 @@
 #include <vector>
 
 #include "../../include/bbgpu.h"
+#include "bb_internal.h"
+
+#define fail bb_fail
+#define hip_check bb_hip_check
 
 extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const double* sr, int nsub, int B,
                                          const int* naz, const int4* chunks, int nchunks, const double2* az,
                                          int TX, int TY, double* sub, double* wtab, short4* bbox, short4* sbox,
                                          const double* cd2i, const double* vedges, const double* ptab, int ntab,
                                          int lin_axis, float* out, int accumulate, int* bin, int bin_cap,
-                                         hipStream_t st);
+                                         const int4* vbox, hipStream_t st);
 extern "C" void bb_set_tile_events(void* ev_begin, void* ev_end);
 extern "C" size_t bb_analytic_sub_bytes();
 extern "C" size_t bb_analytic_wt_bytes();
@@ -41,13 +45,14 @@ extern "C" int bb_launch_norm(int B, int by, int bx, int nz, int ny, int nx, int
                               const int* ihi, int ky, const double* thr, double* work, double* val, int64_t* nb,
                               int64_t* nt, hipStream_t st);
 extern "C" size_t bb_mc_sub_bytes();
+extern "C" int bb_launch_zero_u32(void* p, int64_t n, hipStream_t st);
 extern "C" size_t bb_mc_rec_bytes();
 extern "C" size_t bb_mc_lds_bytes(int nz, int TX, int TY);
 extern "C" int bb_launch_galmod_mc(const bb_grid* g, bb_box box, const double* sr, int nsub, int B, int nvpar,
                                    uint32_t seed, int TX, int TY, int64_t cap, void* subs, double* qv, float* invq,
                                    int64_t* total, int* tcount, int64_t* toff, int* tcursor, void* recs, int* overflow,
                                    const double* cd2i, float* out, unsigned long long* counts, int accumulate,
-                                   hipStream_t st);
+                                   const int4* vbox, hipStream_t st);
 extern "C" size_t bb_rm_tile_static();
 extern "C" size_t bb_rm_finish_lds(int ky, int kx);
 extern "C" int bb_launch_rm_finish(const unsigned long long* counts, double* maps, int P, int by, int bx,
@@ -66,11 +71,12 @@ extern "C" int bb_launch_set_dens(const double* x, int B, int n, int nr, const i
 extern "C" int bb_launch_dilate(const uint8_t* mask, int nz, int ny, int nx, double radius, int nchan, uint8_t* tmp,
                                 uint8_t* out, hipStream_t st);
 extern "C" int bb_launch_residual(const float* model, const float* dvalid, int B, int64_t nvox, double sigma2,
-                                  double eps2, double* partial, double* out_ss, double* out_lg, hipStream_t st);
+                                  double eps2, double* partial, double* out_ss, double* out_lg, const int4* vbox,
+                                  const float2* sv, int frac, int bx, int nz, hipStream_t st);
 
 static thread_local char g_err[512] = "";
 
-static int fail(int code, const char* fmt, ...) {
+int bb_fail(int code, const char* fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
     vsnprintf(g_err, sizeof g_err, fmt, ap);
@@ -78,45 +84,10 @@ static int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define BB_ERR_ARG 1
-#define BB_ERR_HIP 2
-#define BB_LDS_MAX (160 * 1024)   // LDS of a gfx950 CU, all usable by one workgroup
-#define BB_BIN_PER_CHUNK 8        // most entries per chunk of the tiles' chunk lists (the work buffers hold them)
-
-static int hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) return fail(BB_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+int bb_hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) return bb_fail(BB_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
     return 0;
 }
-
-struct bb_model {
-    bb_grid g;
-    int device;
-    int ntab;
-    double* d_cd2i;
-    double* d_vedges;
-    double* d_ptab;
-    int ky, kx;
-    float* d_kern;
-    double* d_kern64;
-    double scale;
-    int TX, TY;        // pixel tile of the ANALYTIC deposit (its spectra fit the LDS budget)
-    int bin_per;       // capacity of the tiles' chunk lists, in entries per chunk (0: no lists, every tile scans)
-    int lin;           // linear velocity axis whose edges are exactly ((k - 1/2) + 1 - crpix3) cdelt3 + crval3 and
-                       // one channel factor cd2i for all channels: the deposit computes them instead of reading them
-};
-
-// The azimuth plan of a set of sub-ring radii (ANALYTIC samples): naz per sub-ring, the chunks of BB_CS
-// azimuths of every (sub-ring, radial sample) and the cosine / sine of every azimuth, built on the host with
-// the reference's recurrence (galmod.py:372-388) so that the device rounds pixels as the reference does.
-struct bb_azplan {
-    int device;
-    int nsub;
-    int nchunks;
-    std::vector<double> R0;
-    int* d_naz;
-    int4* d_chunks;
-    double2* d_az;
-};
 
 extern "C" const char* bb_last_error(void) { return g_err; }
 
@@ -141,14 +112,6 @@ static int check_grid(const bb_grid* g) {
     return 0;
 }
 
-template <class T>
-static int upload(T** dst, const T* src, size_t n, const char* what) {
-    *dst = nullptr;
-    if (n == 0) return 0;
-    int rc = hip_check(hipMalloc((void**)dst, n * sizeof(T)), what);
-    if (rc) return rc;
-    return hip_check(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice), what);
-}
 
 extern "C" int bb_model_destroy(bb_model* m) {
     if (!m) return 0;
@@ -341,6 +304,11 @@ static int check_box(const bb_model* m, bb_box box) {
 
 extern "C" int bb_galmod_analytic(const bb_model* m, const bb_azplan* p, bb_box box, const double* sr, int nsub,
                                   int B, void* work, int64_t work_bytes, float* out, int accumulate, void* stream) {
+    return bb_galmod_analytic_vb(m, p, box, nullptr, sr, nsub, B, work, work_bytes, out, accumulate, stream);
+}
+
+int bb_galmod_analytic_vb(const bb_model* m, const bb_azplan* p, bb_box box, const int4* vbox, const double* sr,
+                          int nsub, int B, void* work, int64_t work_bytes, float* out, int accumulate, void* stream) {
     if (!m || !p) return fail(BB_ERR_ARG, "bb_galmod_analytic: model or plan is NULL");
     int rc = check_box(m, box);
     if (rc) return rc;
@@ -366,7 +334,8 @@ extern "C" int bb_galmod_analytic(const bb_model* m, const bb_azplan* p, bb_box 
     short4* sbox = (short4*)w;
     rc = bb_launch_galmod_analytic(&m->g, box, sr, nsub, B, p->d_naz, p->d_chunks, p->nchunks, p->d_az, m->TX, m->TY,
                                    sub, wtab, bbox, sbox, m->d_cd2i, m->d_vedges, m->d_ptab, m->ntab, m->lin, out,
-                                   accumulate, work_bin(m, p, B, sbox, ntiles), (int)(p->nchunks * (int64_t)m->bin_per), (hipStream_t)stream);
+                                   accumulate, work_bin(m, p, B, sbox, ntiles), (int)(p->nchunks * (int64_t)m->bin_per), vbox,
+                                   (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_galmod_analytic launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
 
@@ -404,14 +373,15 @@ extern "C" int bb_residual_gauss(const float* model, const float* dvalid, int B,
     if (work_len < bb_residual_work_size(B, nvox))
         return fail(BB_ERR_ARG, "bb_residual_gauss: work holds %lld doubles, needs %lld", (long long)work_len,
                     (long long)bb_residual_work_size(B, nvox));
-    int rc = bb_launch_residual(model, dvalid, B, nvox, sigma2, eps2, work, out_ss, out_lg, (hipStream_t)stream);
+    int rc = bb_launch_residual(model, dvalid, B, nvox, sigma2, eps2, work, out_ss, out_lg, nullptr, nullptr, 0, 1, 1,
+                                (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_residual_gauss launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
 
 extern "C" int64_t bb_conv_resid_blocks(int by, int bx, int nz);
 extern "C" int bb_launch_conv_resid(const float* in, const float* dvalid, int B, int by, int bx, int nz,
                                     const float* kern, int ky, int kx, float scale, double sigma2, double eps2,
-                                    double* partial, hipStream_t st);
+                                    double* partial, const int4* vbox, const float2* sv, int frac, hipStream_t st);
 extern "C" int bb_launch_residual_final(const double* partial, int B, int nblk, double* out_ss, double* out_lg,
                                         hipStream_t st);
 
@@ -436,7 +406,7 @@ extern "C" int bb_conv_residual_gauss(const bb_model* m, const float* dep, const
                     (long long)need);
     hipStream_t st = (hipStream_t)stream;
     int rc = bb_launch_conv_resid(dep, dvalid, B, by, bx, m->g.nz, m->d_kern, m->ky, m->kx, (float)m->scale, sigma2,
-                                  eps2, work, st);
+                                  eps2, work, nullptr, nullptr, 0, st);
     if (!rc) rc = bb_launch_residual_final(work, B, (int)(need / (2 * B)), out_ss, out_lg, st);
     return rc ? fail(BB_ERR_HIP, "bb_conv_residual_gauss launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
@@ -507,16 +477,6 @@ extern "C" int bb_ring_maps_finish(const bb_model* m, const uint64_t* counts, in
                                  m->scale, convolve, (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_ring_maps_finish launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
-
-struct bb_profile {
-    bb_profile_desc d;
-    int device;
-    double* d_wpix;
-    double* d_ypix;
-    double* d_lcoef;
-    int* d_lcol;
-    int* d_near;
-};
 
 extern "C" int bb_profile_destroy(bb_profile* p) {
     if (!p) return 0;
@@ -624,6 +584,13 @@ extern "C" int64_t bb_mc_work_size(const bb_model* m, bb_box box, int nsub, int 
 extern "C" int bb_galmod_mc(const bb_model* m, bb_box box, const double* sr, int nsub, int B, int nvpar, uint32_t seed,
                             int64_t max_clouds, void* work, int64_t work_bytes, float* out, uint64_t* counts,
                             int* overflow, int accumulate, void* stream) {
+    return bb_galmod_mc_vb(m, box, nullptr, sr, nsub, B, nvpar, seed, max_clouds, work, work_bytes, out, counts,
+                           overflow, accumulate, stream);
+}
+
+int bb_galmod_mc_vb(const bb_model* m, bb_box box, const int4* vbox, const double* sr, int nsub, int B, int nvpar,
+                    uint32_t seed, int64_t max_clouds, void* work, int64_t work_bytes, float* out, uint64_t* counts,
+                    int* overflow, int accumulate, void* stream) {
     if (!m) return fail(BB_ERR_ARG, "bb_galmod_mc: model is NULL");
     int rc = check_box(m, box);
     if (rc) return rc;
@@ -657,18 +624,19 @@ extern "C" int bb_galmod_mc(const bb_model* m, bb_box box, const double* sr, int
     w += mc_align(sizeof(int));
     void* recs = w;
     hipStream_t st = (hipStream_t)stream;
-    rc = hip_check(hipMemsetAsync(tcount, 0, (size_t)B * nt * sizeof(int), st), "bb_galmod_mc");
-    if (!rc) rc = hip_check(hipMemsetAsync(overflow, 0, sizeof(int), st), "bb_galmod_mc");
+    rc = hip_check((hipError_t)bb_launch_zero_u32(tcount, (int64_t)B * nt, st), "bb_galmod_mc");
+    if (!rc) rc = hip_check((hipError_t)bb_launch_zero_u32(overflow, 1, st), "bb_galmod_mc");
     if (rc) return rc;
     if (nsub == 0) {
         const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
-        if (!accumulate) rc = hip_check(hipMemsetAsync(out, 0, (size_t)B * by * bx * m->g.nz * sizeof(float), st), "bb_galmod_mc");
-        if (!rc && counts) rc = hip_check(hipMemsetAsync(counts, 0, (size_t)B * by * bx * m->g.nz * 8, st), "bb_galmod_mc");
+        const int64_t nv = (int64_t)B * by * bx * m->g.nz;
+        if (!accumulate) rc = hip_check((hipError_t)bb_launch_zero_u32(out, nv, st), "bb_galmod_mc");
+        if (!rc && counts) rc = hip_check((hipError_t)bb_launch_zero_u32(counts, 2 * nv, st), "bb_galmod_mc");
         return rc;
     }
     rc = bb_launch_galmod_mc(&m->g, box, sr, nsub, B, nvpar, seed, m->TX, m->TY, max_clouds, subs, qv, invq, total,
                              tcount, toff, tcursor, recs, overflow, m->d_cd2i, out, (unsigned long long*)counts,
-                             accumulate, st);
+                             accumulate, vbox, st);
     return rc ? fail(BB_ERR_HIP, "bb_galmod_mc launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
 

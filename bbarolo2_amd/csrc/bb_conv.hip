@@ -91,12 +91,26 @@ template <int KX, bool FRAC>
 __global__ void __launch_bounds__(BB_CV_THREADS)
 k_conv_resid(const float* __restrict__ inall, const float* __restrict__ dvalid, int by, int bx, int nz, int ZL,
              const float* __restrict__ kern, int ky, float scale, float sigma2f, float eps2f,
-             double* __restrict__ partial) {
+             double* __restrict__ partial, const int4* __restrict__ vbox, const float2* __restrict__ sv) {
     constexpr int W = BB_CV_RX + KX - 1;
     constexpr int hx = KX / 2;
     __shared__ double red[2][BB_CV_THREADS / 64];
     const int hy = ky / 2;
     const int b = blockIdx.z;
+    // the voxels summed: the vector's own model box (box-relative [x, z) x [y, w)), where the reference's model
+    // of that box lives; the rest of the region enters through the outside sum of the caller
+    int rx0 = 0, ry0 = 0, rx1 = bx, ry1 = by;
+    if (vbox) {
+        const int4 v = vbox[b];
+        rx0 = v.x;
+        ry0 = v.y;
+        rx1 = v.z;
+        ry1 = v.w;
+    }
+    if (sv) {
+        sigma2f = sv[b].x;
+        eps2f = sv[b].y;
+    }
     const int zl = threadIdx.x % ZL, grp = threadIdx.x / ZL, ngrp = blockDim.x / ZL;
     const int z = blockIdx.y * ZL + zl;
     const int nbx = (bx + BB_CV_RX - 1) / BB_CV_RX, nby = (by + BB_CV_RY - 1) / BB_CV_RY;
@@ -138,10 +152,11 @@ k_conv_resid(const float* __restrict__ inall, const float* __restrict__ dvalid, 
         for (int r = 0; r < BB_CV_RY; r++) {
             const int y = y0 + r;
             if (y >= by) break;
+            if (y < ry0 || y >= ry1) continue;
 #pragma unroll
             for (int c = 0; c < BB_CV_RX; c++) {
                 const int x = x0 + c;
-                if (x < bx) {
+                if (x >= rx0 && x < rx1) {
                     const float d = dv[((size_t)y * bx + x) * nz];
                     if (d == d) {
                         const float v = acc[r][c];
@@ -216,26 +231,32 @@ extern "C" int64_t bb_conv_resid_blocks(int by, int bx, int nz) {
 
 template <int KX>
 static void launch_cr(const float* in, const float* dvalid, int B, int by, int bx, int nz, const float* kern, int ky,
-                      float scale, float sigma2, float eps2, double* partial, hipStream_t st) {
+                      float scale, float sigma2, float eps2, double* partial, const int4* vbox, const float2* sv,
+                      bool frac, hipStream_t st) {
     const int ZL = pick_zl(nz);
     const int ngrp = BB_CV_THREADS / ZL;
     const int nblk = ((bx + BB_CV_RX - 1) / BB_CV_RX) * ((by + BB_CV_RY - 1) / BB_CV_RY);
     dim3 grid((nblk + ngrp - 1) / ngrp, (nz + ZL - 1) / ZL, B);
-    if (eps2 > 0.f)
+    if (frac)
         hipLaunchKernelGGL((k_conv_resid<KX, true>), grid, dim3(BB_CV_THREADS), 0, st, in, dvalid, by, bx, nz, ZL, kern,
-                           ky, scale, sigma2, eps2, partial);
+                           ky, scale, sigma2, eps2, partial, vbox, sv);
     else
         hipLaunchKernelGGL((k_conv_resid<KX, false>), grid, dim3(BB_CV_THREADS), 0, st, in, dvalid, by, bx, nz, ZL,
-                           kern, ky, scale, 1.f, 0.f, partial);
+                           kern, ky, scale, 1.f, 0.f, partial, vbox, (const float2*)nullptr);
 }
 
-// The fused convolution and residual: partial (B x bb_conv_resid_blocks x 2 doubles).
+// The fused convolution and residual: partial (B x bb_conv_resid_blocks x 2 doubles). vbox (B, box-relative [x, z) x
+// [y, w)) restricts the sums to each vector's own model box (NULL: the whole box); sv (B: sigma2, eps2) replaces the
+// scalars per vector (NULL: the scalars); frac selects the fracerr form (also when eps2 is 0 for every vector).
 extern "C" int bb_launch_conv_resid(const float* in, const float* dvalid, int B, int by, int bx, int nz,
                                     const float* kern, int ky, int kx, float scale, double sigma2, double eps2,
-                                    double* partial, hipStream_t st) {
+                                    double* partial, const int4* vbox, const float2* sv, int frac, hipStream_t st) {
+    const bool fr = frac != 0 || eps2 > 0;
     switch (kx) {
 #define BB_CASE(K) \
-    case K: launch_cr<K>(in, dvalid, B, by, bx, nz, kern, ky, scale, (float)sigma2, (float)eps2, partial, st); break;
+    case K: launch_cr<K>(in, dvalid, B, by, bx, nz, kern, ky, scale, (float)sigma2, (float)eps2, partial, vbox, sv, fr, \
+                         st);                                                                                          \
+        break;
         BB_CASE(1) BB_CASE(3) BB_CASE(5) BB_CASE(7) BB_CASE(9) BB_CASE(11) BB_CASE(13) BB_CASE(15) BB_CASE(17)
         BB_CASE(19) BB_CASE(21) BB_CASE(23) BB_CASE(25)
 #undef BB_CASE

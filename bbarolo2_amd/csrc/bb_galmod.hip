@@ -57,9 +57,17 @@ __device__ __forceinline__ double zq_at(int kz, int half, int nin, double step, 
 // ---------------------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(64)
 k_an_prep(bb_grid g, bb_box box, const double* __restrict__ srall, int nsub, const int* __restrict__ naz_tab,
-          double* __restrict__ suball, double* __restrict__ wtaball, int unit_dens) {
+          double* __restrict__ suball, double* __restrict__ wtaball, int unit_dens, const int4* __restrict__ vbox) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
     if (s >= nsub) return;
+    if (vbox) {
+        // the vector's own model box: the extent of its vertical samples depends on it (galmod.py:295-300)
+        const int4 v = vbox[b];
+        box.blo0 = v.x;
+        box.blo1 = v.y;
+        box.bhi0 = v.z;
+        box.bhi1 = v.w;
+    }
     const double* sr = srall + (size_t)b * 16 * nsub;
     double* q = suball + ((size_t)b * nsub + s) * BB_NSF;
     double* wt = wtaball + ((size_t)b * nsub + s) * BB_ZCAP;
@@ -169,7 +177,8 @@ __device__ __forceinline__ void pix_at(const double* __restrict__ q, const bb_gr
 // ---------------------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_an_bbox(bb_grid g, bb_box box, const int4* __restrict__ chunks, int nchunks, const double2* __restrict__ az,
-          const double* __restrict__ suball, int nsub, short4* __restrict__ bball, short4* __restrict__ sball) {
+          const double* __restrict__ suball, int nsub, short4* __restrict__ bball, short4* __restrict__ sball,
+          const int4* __restrict__ vbox) {
     // one thread per azimuth sample (BB_CS lanes per chunk): the box of its vertical segment, then the box of
     // the chunk by a reduction over its lanes
     const int t = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
@@ -178,6 +187,15 @@ k_an_bbox(bb_grid g, bb_box box, const int4* __restrict__ chunks, int nchunks, c
     const int4 ch = valid ? chunks[c] : make_int4(0, 0, 0, 0);
     const int s = ch.x >> 1, kr = ch.x & 1;
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
+    // the clip: the layout box, or the vector's own model box inside it (box-relative, inclusive)
+    int cx0 = 0, cx1 = bx - 1, cy0 = 0, cy1 = by - 1;
+    if (vbox) {
+        const int4 v = vbox[b];
+        cx0 = max(cx0, v.x - box.blo0);
+        cx1 = min(cx1, v.z - 1 - box.blo0);
+        cy0 = max(cy0, v.y - box.blo1);
+        cy1 = min(cy1, v.w - 1 - box.blo1);
+    }
     int x0 = 1 << 30, x1 = -(1 << 30), y0 = 1 << 30, y1 = -(1 << 30);
     if (valid && lane < ch.z) {
         const double* q = suball + ((size_t)b * nsub + s) * BB_NSF;
@@ -195,10 +213,10 @@ k_an_bbox(bb_grid g, bb_box box, const int4* __restrict__ chunks, int nchunks, c
                 // box-relative, clipped to the box (the pixel of a sample is monotone in its height)
                 int64_t lx = (a0 < e0 ? a0 : e0) - box.blo0, hx = (a0 > e0 ? a0 : e0) - box.blo0;
                 int64_t ly = (a1 < e1 ? a1 : e1) - box.blo1, hy = (a1 > e1 ? a1 : e1) - box.blo1;
-                lx = lx < 0 ? 0 : lx;
-                ly = ly < 0 ? 0 : ly;
-                hx = hx > bx - 1 ? bx - 1 : hx;
-                hy = hy > by - 1 ? by - 1 : hy;
+                lx = lx < cx0 ? cx0 : lx;
+                ly = ly < cy0 ? cy0 : ly;
+                hx = hx > cx1 ? cx1 : hx;
+                hy = hy > cy1 ? cy1 : hy;
                 if (lx <= hx && ly <= hy) {
                     x0 = (int)lx;
                     x1 = (int)hx;
@@ -497,11 +515,12 @@ __device__ __forceinline__ double uw_(const Sub& u, const double* __restrict__ w
 __device__ __host__ __forceinline__ int bb_tile_stride(int nz) { return nz | 1; }
 
 struct TileGeo {
-    int ax0, ax1, ay0, ay1;     // the tile in absolute pixels (inclusive)
+    int ax0, ax1, ay0, ay1;     // the pixels of the tile that receive samples, absolute (inclusive)
+    int ox, oy;                 // the tile's first pixel: slot (0, 0) of its LDS spectra
     int TX;
     __device__ __forceinline__ int slot(int p0, int p1) const {
         if (p0 < ax0 || p0 > ax1 || p1 < ay0 || p1 > ay1) return -1;
-        return (p1 - ay0) * TX + (p0 - ax0);
+        return (p1 - oy) * TX + (p0 - ox);
     }
     __device__ __forceinline__ int slot(int64_t p0, int64_t p1) const { return slot(pix32(p0), pix32(p1)); }
 };
@@ -824,7 +843,8 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
           const double2* __restrict__ az, const double* __restrict__ suball, const double* __restrict__ wtaball,
           int nsub, const short4* __restrict__ bball, const short4* __restrict__ sball,
           const double* __restrict__ cd2i, const double* __restrict__ vedges, const double* __restrict__ ptab,
-          int ntab, float* __restrict__ outall, int accumulate, const int* __restrict__ bin, int bin_cap) {
+          int ntab, float* __restrict__ outall, int accumulate, const int* __restrict__ bin, int bin_cap,
+          const int4* __restrict__ vbox) {
     // dynamic LDS: [vedges (nlines, nz+1) f64][cd2i (nz) f64][cd2i (nz) f32][Phi (ntab) f32][tile u64]
     extern __shared__ double dyn[];
     __shared__ int clist[BB_LIST_CAP];
@@ -884,15 +904,25 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     const short4* bb = bball + (size_t)b * nchunks;
     const short4* sb = sball + (size_t)b * nchunks * BB_CS;
     TileGeo T;
-    T.ax0 = box.blo0 + tx0;
+    T.ax0 = T.ox = box.blo0 + tx0;
     T.ax1 = box.blo0 + (tx1 < bx - 1 ? tx1 : bx - 1);
-    T.ay0 = box.blo1 + ty0;
+    T.ay0 = T.oy = box.blo1 + ty0;
     T.ay1 = box.blo1 + (ty1 < by - 1 ? ty1 : by - 1);
     T.TX = TX;
+    if (vbox) {
+        // only the pixels of the vector's own model box receive samples (the reference's model of that box); the
+        // slots keep the tile's origin
+        const int4 v = vbox[b];
+        T.ax0 = max(T.ax0, v.x);
+        T.ax1 = min(T.ax1, v.z - 1);
+        T.ay0 = max(T.ay0, v.y);
+        T.ay1 = min(T.ay1, v.w - 1);
+    }
+    const bool live = amax > 0. && T.ax0 <= T.ax1 && T.ay0 <= T.ay1;
     const int* tl;
     int tlo, thi;
     tile_list(bin, gridDim.y, gridDim.x, bin_cap, b, tid, tl, tlo, thi);
-    tile_samples<BB_AN_SAMP_CAP>(nchunks, bb, sb, tl, tlo, thi, tx0, tx1, ty0, ty1, clist, slist, nlist, nsamp, amax > 0.,
+    tile_samples<BB_AN_SAMP_CAP>(nchunks, bb, sb, tl, tlo, thi, tx0, tx1, ty0, ty1, clist, slist, nlist, nsamp, live,
                                  [&](int e) {
         const int4 ch = chunks[e >> 5];
         const int s = ch.x >> 1;
@@ -939,11 +969,12 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
                                          int TX, int TY, double* sub, double* wtab, short4* bbox, short4* sbox,
                                          const double* cd2i, const double* vedges, const double* ptab, int ntab,
                                          int lin_axis, float* out, int accumulate, int* bin, int bin_cap,
-                                         hipStream_t st) {
+                                         const int4* vbox, hipStream_t st) {
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
-    hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, naz, sub, wtab, 0);
+    hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, naz, sub, wtab, 0,
+                       vbox);
     hipLaunchKernelGGL(k_an_bbox, dim3((nchunks * BB_CS + 255) / 256, B), dim3(256), 0, st, *g, box, chunks, nchunks,
-                       az, sub, nsub, bbox, sbox);
+                       az, sub, nsub, bbox, sbox, vbox);
     const int ntiles = ((bx + TX - 1) / TX) * ((by + TY - 1) / TY);
     if (bin) {
         const int e = bin_chunks(box, B, bbox, nchunks, TX, TY, bin, bin_cap, st);
@@ -968,11 +999,11 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
     if (lin)
         hipLaunchKernelGGL(k_an_tile<true>, dim3(ntiles, B), dim3(BB_TILE_THREADS), lds, st, *g, box, TX, TY, chunks,
                            nchunks, az, sub, wtab, nsub, bbox, sbox, cd2i, vedges, ptab, ntab, out, accumulate, bin,
-                           bin_cap);
+                           bin_cap, vbox);
     else
         hipLaunchKernelGGL(k_an_tile<false>, dim3(ntiles, B), dim3(BB_TILE_THREADS), lds, st, *g, box, TX, TY, chunks,
                            nchunks, az, sub, wtab, nsub, bbox, sbox, cd2i, vedges, ptab, ntab, out, accumulate, bin,
-                           bin_cap);
+                           bin_cap, vbox);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && t_tile_ev[0] && t_tile_ev[1]) e = hipEventRecord(t_tile_ev[1], st);
     return (int)e;
@@ -1023,9 +1054,9 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
             swd[i] = wtab[(size_t)(i / BB_RM_WCAP) * BB_ZCAP + i % BB_RM_WCAP];
     __syncthreads();
     TileGeo T;
-    T.ax0 = box.blo0 + tx0;
+    T.ax0 = T.ox = box.blo0 + tx0;
     T.ax1 = box.blo0 + (tx1 < bx - 1 ? tx1 : bx - 1);
-    T.ay0 = box.blo1 + ty0;
+    T.ay0 = T.oy = box.blo1 + ty0;
     T.ay1 = box.blo1 + (ty1 < by - 1 ? ty1 : by - 1);
     T.TX = TX;
     const int* tl;
@@ -1120,9 +1151,10 @@ extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* s
                                    int nmap, int map0, int nw, double qunit, unsigned long long* counts, int* bin,
                                    int bin_cap, hipStream_t st) {
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
-    hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, naz, sub, wtab, 1);
+    hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, naz, sub, wtab, 1,
+                       (const int4*)nullptr);
     hipLaunchKernelGGL(k_an_bbox, dim3((nchunks * BB_CS + 255) / 256, B), dim3(256), 0, st, *g, box, chunks, nchunks,
-                       az, sub, nsub, bbox, sbox);
+                       az, sub, nsub, bbox, sbox, (const int4*)nullptr);
     if (bin) {
         const int e = bin_chunks(box, B, bbox, nchunks, TX, TY, bin, bin_cap, st);
         if (e) return e;

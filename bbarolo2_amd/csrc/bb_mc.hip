@@ -288,8 +288,8 @@ struct McGeo {
 };
 
 // the pixel of cloud ic of sub-ring u, and its record; false if it is outside the box or on a zero-weight half
-__device__ __forceinline__ bool mc_cloud(const bb_grid& g, const McGeo& G, const McSub& u, int s, uint32_t seed,
-                                         uint32_t ic, int& tile, McRec& rec) {
+__device__ __forceinline__ bool mc_cloud(const bb_grid& g, const McGeo& G, const int4& vb, const McSub& u, int s,
+                                         uint32_t seed, uint32_t ic, int& tile, McRec& rec) {
     uint32_t x[4];
     philox(ic, 0u, 0u, 0u, seed, (uint32_t)s, x);
     const float R = mc_sqrt(fmaf(u.r_cross, fabsf(2.0f * mc_u01(x[0]) - 1.0f), u.r_in2));
@@ -301,7 +301,7 @@ __device__ __forceinline__ bool mc_cloud(const bb_grid& g, const McGeo& G, const
     const int64_t g0 = mc_lround_f(u.xpos + (xx * u.spa - yy * u.cpa) * G.icd0);
     const int64_t g1 = mc_lround_f(u.ypos + (xx * u.cpa + yy * u.spa) * G.icd1);
     const float wside = caz >= 0.0f ? u.wrec : u.wapp;
-    if (wside == 0.0f || g0 < G.blo0 || g0 >= G.bhi0 || g1 < G.blo1 || g1 >= G.bhi1) return false;
+    if (wside == 0.0f || g0 < vb.x || g0 >= vb.z || g1 < vb.y || g1 >= vb.w) return false;
     const float az_ = fabsf(z);
     float vs;
     if (az_ > u.zcyl) vs = u.vsys + (mc_lag(u.vrot, u.dvdz, az_ - u.zcyl) * caz + u.vrad * saz) * u.sinc;
@@ -325,13 +325,15 @@ __device__ __forceinline__ bool mc_cloud(const bb_grid& g, const McGeo& G, const
 __global__ void __launch_bounds__(256)
 k_mc_clouds(bb_grid g, McGeo G, const McSub* __restrict__ subs, int nsub, uint32_t seed, int ntiles, int pass,
             int* __restrict__ tcount, const int64_t* __restrict__ toff, int* __restrict__ tcursor,
-            McRec* __restrict__ recs, int64_t cap, int* __restrict__ overflow) {
+            McRec* __restrict__ recs, int64_t cap, int* __restrict__ overflow, const int4* __restrict__ vbox) {
     const int s = blockIdx.x, b = blockIdx.y;
     const McSub u = subs[(size_t)b * nsub + s];
+    // the box of the clouds kept: the layout box, or the vector's own model box inside it
+    const int4 vb = vbox ? vbox[b] : make_int4((int)G.blo0, (int)G.blo1, (int)G.bhi0, (int)G.bhi1);
     for (int64_t ic = threadIdx.x; ic < u.nc; ic += blockDim.x) {
         int tile;
         McRec rec;
-        if (!mc_cloud(g, G, u, s, seed, (uint32_t)ic, tile, rec)) continue;
+        if (!mc_cloud(g, G, vb, u, s, seed, (uint32_t)ic, tile, rec)) continue;
         if (pass == 0) {
             atomicAdd(tcount + (size_t)b * ntiles + tile, 1);
         } else {
@@ -425,6 +427,19 @@ k_mc_tile(bb_grid g, McGeo G, int bx, int by, uint32_t seed, int ntiles, const i
 }
 
 // ---------------------------------------------------------------------------------------------------------
+// zero n 32-bit words (a kernel, not a memset: the chain stays capturable into a HIP graph)
+__global__ void k_zero_u32(uint32_t* __restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+extern "C" int bb_launch_zero_u32(void* p, int64_t n, hipStream_t st) {
+    if (n <= 0) return 0;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65535) blocks = 65535;
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)blocks), dim3(256), 0, st, (uint32_t*)p, n);
+    return (int)hipGetLastError();
+}
+
 extern "C" size_t bb_mc_sub_bytes() { return sizeof(McSub); }
 extern "C" size_t bb_mc_rec_bytes() { return sizeof(McRec); }
 extern "C" size_t bb_mc_lds_bytes(int nz, int TX, int TY) { return (size_t)TX * TY * (nz | 1) * sizeof(unsigned long long); }
@@ -433,7 +448,7 @@ extern "C" int bb_launch_galmod_mc(const bb_grid* g, bb_box box, const double* s
                                    uint32_t seed, int TX, int TY, int64_t cap, void* subs, double* qv, float* invq,
                                    int64_t* total, int* tcount, int64_t* toff, int* tcursor, void* recs, int* overflow,
                                    const double* cd2i, float* out, unsigned long long* counts, int accumulate,
-                                   hipStream_t st) {
+                                   const int4* vbox, hipStream_t st) {
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
     McGeo G;
     G.blo0 = box.blo0;
@@ -453,10 +468,10 @@ extern "C" int bb_launch_galmod_mc(const bb_grid* g, bb_box box, const double* s
     hipLaunchKernelGGL(k_mc_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, sr, nsub, nvpar, S);
     hipLaunchKernelGGL(k_mc_scan, dim3(B), dim3(256), 0, st, *g, S, nsub, qv, invq, total);
     hipLaunchKernelGGL(k_mc_clouds, dim3(nsub, B), dim3(256), 0, st, *g, G, S, nsub, seed, ntiles, 0, tcount,
-                       (const int64_t*)nullptr, tcursor, R, cap, overflow);
+                       (const int64_t*)nullptr, tcursor, R, cap, overflow, vbox);
     hipLaunchKernelGGL(k_mc_tscan, dim3(B), dim3(256), 0, st, tcount, ntiles, toff, tcursor);
     hipLaunchKernelGGL(k_mc_clouds, dim3(nsub, B), dim3(256), 0, st, *g, G, S, nsub, seed, ntiles, 1, tcount, toff,
-                       tcursor, R, cap, overflow);
+                       tcursor, R, cap, overflow, vbox);
     static bool attr = false;
     if (!attr) {
         hipError_t e = hipFuncSetAttribute((const void*)k_mc_tile, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -20,10 +20,17 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <bool FRAC>
 __global__ void __launch_bounds__(BB_RS_THREADS)
 k_residual(const float* __restrict__ model, const float* __restrict__ dvalid, int64_t nvox, float sigma2f,
-           float eps2f, double* __restrict__ partial) {
+           float eps2f, double* __restrict__ partial, const int4* __restrict__ vbox, const float2* __restrict__ sv,
+           int bx, int nz) {
     __shared__ double red[2][BB_RS_THREADS / 64];
     const int b = blockIdx.y;
     const float* m = model + (size_t)b * nvox;
+    if (sv) {
+        sigma2f = sv[b].x;
+        eps2f = sv[b].y;
+    }
+    int4 vb = make_int4(0, 0, 0, 0);
+    if (vbox) vb = vbox[b];
     double s = 0., lg = 0.;
     const int64_t base = (int64_t)blockIdx.x * BB_RS_THREADS * BB_RS_PER_THREAD;
 #pragma unroll
@@ -31,6 +38,12 @@ k_residual(const float* __restrict__ model, const float* __restrict__ dvalid, in
         int64_t i = base + (int64_t)k * BB_RS_THREADS + threadIdx.x;
         if (i < nvox) {
             float d = dvalid[i];
+            if (vbox) {
+                // only the voxels of the vector's own model box (box-relative [x, z) x [y, w), yxz layout)
+                const int64_t p = i / nz;
+                const int x = (int)(p % bx), y = (int)(p / bx);
+                if (x < vb.x || x >= vb.z || y < vb.y || y >= vb.w) d = __builtin_nanf("");
+            }
             if (d == d) {
                 float mm = m[i];
                 float r = d - mm;
@@ -107,16 +120,18 @@ extern "C" int bb_residual_blocks(int64_t nvox) {
     return (int)((nvox + per - 1) / per);
 }
 
+// vbox / sv / frac as bb_launch_conv_resid (bx, nz: the box width and the channels of the yxz layout, for vbox).
 extern "C" int bb_launch_residual(const float* model, const float* dvalid, int B, int64_t nvox, double sigma2,
-                                  double eps2, double* partial, double* out_ss, double* out_lg, hipStream_t st) {
+                                  double eps2, double* partial, double* out_ss, double* out_lg, const int4* vbox,
+                                  const float2* sv, int frac, int bx, int nz, hipStream_t st) {
     const int nblk = bb_residual_blocks(nvox);
     dim3 grid(nblk, B);
-    if (eps2 > 0)
+    if (eps2 > 0 || frac)
         hipLaunchKernelGGL(k_residual<true>, grid, dim3(BB_RS_THREADS), 0, st, model, dvalid, nvox, (float)sigma2,
-                           (float)eps2, partial);
+                           (float)eps2, partial, vbox, sv, bx, nz);
     else
         hipLaunchKernelGGL(k_residual<false>, grid, dim3(BB_RS_THREADS), 0, st, model, dvalid, nvox, 1.f, 0.f,
-                           partial);
+                           partial, vbox, (const float2*)nullptr, bx, nz);
     hipLaunchKernelGGL(k_residual_final, dim3(B), dim3(BB_RS_THREADS), 0, st, partial, nblk, out_ss, out_lg);
     return (int)hipGetLastError();
 }

@@ -533,3 +533,33 @@ class SubringPlan:
     def _edges(self):
         p = self.proto
         return getattr(p, "edge", None) is not None or any(e is not None for e in self.setup.side_edges(p))
+
+    def weights(self):
+        """(densw, side): the edge weights of the density of the nsub sub-rings of one layer (None without an edge
+        of the disk) and the weights (2, nsub_total) of the receding and approaching halves, as _finish_subrings
+        applies them (galmod.py:833-851)."""
+        p = self.proto
+        radsep = self.setup.radsep
+        densw = None
+        if getattr(p, "edge", None) is not None and self.nsub:
+            densw = np.ascontiguousarray(edge_weights(self.rads * 60., radsep * 60., float(p.edge),
+                                                      float(p.edgewidth or 0.)))
+        R = np.concatenate([self.rads, self.rads]) if self.thick else self.rads
+        side = np.ones((2, R.size))
+        for k, e in enumerate(self.setup.side_edges(p)[::-1]):          # (receding, approaching)
+            if e is not None and R.size:
+                side[k] = edge_weights(R * 60., radsep * 60., e)
+        return densw, np.ascontiguousarray(side)
+
+
+# the ring keys of the likelihood context, in the order of its ring table (bbgpu.h BB_K_*)
+LIKE_KEYS = ("vrot", "vrad", "vvert", "vdisp", "dens", "z0", "dvdz", "zcyl", "inc", "phi", "xpos", "ypos", "vsys",
+             "fthick", "z0thick", "vdispthick", "sigma", "fracerr")
+
+
+def ring_table(rings):
+    """(len(LIKE_KEYS), nr) float64: the ring values of the keys in the reference's units (sigma and fracerr: 0)."""
+    out = np.zeros((len(LIKE_KEYS), rings.nr))
+    for k, key in enumerate(LIKE_KEYS[:16]):
+        out[k] = np.asarray(getattr(rings, key), float)
+    return out

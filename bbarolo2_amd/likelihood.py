@@ -1,19 +1,22 @@
-"""The Gaussian log-likelihood of Bayesian3DFIT (GLOBAL / BAYESIAN), evaluated for batches of parameter
-vectors on the GPU.
+"""The Gaussian log-likelihood of Bayesian3DFIT (GLOBAL / BAYESIAN), evaluated for batches of parameter vectors on the
+GPU by the library's likelihood context (bbgpu.h bb_ctx_create / bb_loglike_batch).
 
-Mirrors LikelihoodMixin._log_likelihood (bbarolo/bayesian/likelihood.py:567-619) for the default Gaussian
-residuals (_calculate_residuals / _residuals_fracerr, likelihood.py:457-531):
+Mirrors LikelihoodMixin._log_likelihood (bbarolo/bayesian/likelihood.py:567-619) for the default Gaussian residuals
+(_calculate_residuals / _residuals_fracerr, likelihood.py:457-531):
 
     -logL = likefactor * [ 0.5 (S_box + ss_out) / sigma^2 + 0.5 nvox ln(2 pi sigma^2) ]      (+ fracerr terms)
 
-with S_box the masked residual sum inside the model box, ss_out the squared data of the likelihood voxels of
-the region outside the box (a summed-area table, likelihood.py:437-440) and nvox the number of likelihood
-voxels of the region. theta -> rings follows _update_rings (bayesian/fitter.py:1109-1136): the free values
-are stored as float32 in the ring table (bayesian/rings.py:54-61), then used as float64.
+with S_box the masked residual sum inside the vector's model box, ss_out the squared data of the likelihood voxels of
+the region outside that box (a summed-area table, likelihood.py:437-440) and nvox the number of likelihood voxels of
+the region. theta -> rings follows _update_rings (bayesian/fitter.py:1109-1136): the free values are stored as
+float32 in the ring table (bayesian/rings.py:54-61), then used as float64.
 
-The setup that the reference does once per fit (likelihood mask, region, noise) is taken as given: the
-caller passes the likelihood mask, the region and sigma (see synth.likelihood_inputs for the conventions
-of the synthetic configurations).
+This module does the per-fit set-up that the reference does in Python (the sub-ring plan of the fixed radii, the
+constants of the deconvolved profile, the model box) and hands it to the context once; every batch then runs in the
+library from theta (host) to log L (host): the sub-ring tables are built on the device, each vector gets its own model
+box when XPOS, YPOS or Z0 is free, and the profile, deposit, convolution and residual follow without a host round
+trip. The set-up that the reference does once per fit from the data (likelihood mask, region, noise) is taken as
+given: the caller passes the likelihood mask, the region and sigma (see synth.likelihood_inputs).
 """
 
 import ctypes as C
@@ -23,9 +26,9 @@ import numpy as np
 
 from . import _lib
 from .engine import _torch, _stream_ptr
-from .galmod import SubringPlan, nzq_for
-from .profile import DensityProfile, SMOOTH_DEFAULT
-from .rings import Rings
+from .galmod import SubringPlan, LIKE_KEYS, ring_table, outer_edge, seed32, ZSAMPLES_MAX
+from .header import arcsconv
+from .profile import ProfileSetup, SMOOTH_DEFAULT, ANNULUS_KEYS
 
 # free parameters that change the deconvolved density profile (fitter.py:614-615)
 PROFILE_KEYS = ('inc', 'phi', 'xpos', 'ypos', 'radmax', 'z0', 'fthick')
@@ -33,11 +36,17 @@ PROFILE_KEYS = ('inc', 'phi', 'xpos', 'ypos', 'radmax', 'z0', 'fthick')
 # ring parameters that make the likelihood zero when negative (likelihood.py:532-535)
 POSITIVE = ('vrot', 'vdisp', 'inc', 'xpos', 'ypos', 'z0', 'dens', 'zcyl', 'z0thick', 'fthick', 'vdispthick')
 
+# free parameters that move the model box (DiskModel.box: the centre of the last ring and the layer height)
+BOX_KEYS = ('xpos', 'ypos', 'z0', 'z0thick')
+
+# free parameters that are not ring values
+GLOBAL_KEYS = ('sigma', 'fracerr')
+
 
 class FreePars:
-    """The free parameters of a fit: an ordered list of (ring key, 'ring' | 'single'); 'ring' takes nr
-    values of theta (one per ring), 'single' one value for all the rings (the FREE keys VROT and
-    INC_SINGLE of the reference, bayesian/fitter.py:434-473)."""
+    """The free parameters of a fit: an ordered list of (key, 'ring' | 'single'); 'ring' takes nr values of theta
+    (one per ring), 'single' one value for all the rings (the FREE keys VROT and INC_SINGLE of the reference,
+    bayesian/fitter.py:434-473). SIGMA and FRACERR are single values that are not ring values."""
 
     def __init__(self, spec, nr):
         self.spec = [(k.lower(), m) for k, m in spec]
@@ -45,18 +54,21 @@ class FreePars:
         self.index = {}
         i = 0
         for k, m in self.spec:
+            if k in GLOBAL_KEYS and m == "ring":
+                raise ValueError(f"{k.upper()} is one value, not one per ring")
             n = nr if m == "ring" else 1
             self.index[k] = np.arange(i, i + n)
             i += n
         self.ndim = i
 
     def values(self, theta):
-        """dict key -> (B, nr) float64 ring values of the free keys, rounded to float32 as the ring table
-        of the reference stores them."""
+        """dict key -> (B, nr) float64 ring values of the free ring keys, rounded to float32 as the ring table of the
+        reference stores them."""
         theta = np.atleast_2d(np.asarray(theta, float))
-        B = theta.shape[0]
         out = {}
         for k, m in self.spec:
+            if k in GLOBAL_KEYS:
+                continue
             v = theta[:, self.index[k]]
             if m != "ring":
                 v = np.repeat(v, self.nr, axis=1)
@@ -64,13 +76,30 @@ class FreePars:
         return out
 
     def negative(self, theta):
-        """True where a free positive parameter is negative (likelihood.py:537-559)."""
+        """True where the likelihood is zero (likelihood.py:537-559): a free positive-only value below 0, a free FTHICK
+        above 1, SIGMA not above 0."""
         theta = np.atleast_2d(np.asarray(theta, float))
         bad = np.zeros(theta.shape[0], bool)
         for k, _ in self.spec:
+            v = theta[:, self.index[k]]
             if k in POSITIVE:
-                bad |= np.any(theta[:, self.index[k]].astype(np.float32) < 0, axis=1)
+                bad |= np.any(v.astype(np.float32) < 0, axis=1)
+            if k == "fthick":
+                bad |= np.any(v.astype(np.float32) > 1, axis=1)
+            if k == "sigma":
+                bad |= ~(v[:, 0] > 0)
         return bad
+
+    def offsets(self):
+        """(free_off, free_ring) int32 (NKEY,): the offset of each key of the context's ring table in theta (-1: not
+        free) and whether it takes one value per ring."""
+        off = np.full(_lib.NKEY, -1, np.int32)
+        ring = np.zeros(_lib.NKEY, np.int32)
+        for k, m in self.spec:
+            j = LIKE_KEYS.index(k)
+            off[j] = self.index[k][0]
+            ring[j] = 1 if m == "ring" else 0
+        return off, ring
 
 
 def float32_rings(rings):
@@ -84,82 +113,100 @@ def float32_rings(rings):
     return r
 
 
+def _plan_struct(plan, ridx, from_fit, keep):
+    """bb_ring_plan of a SubringPlan (arrays kept alive in keep)."""
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return a.ctypes.data
+    densw, side = plan.weights()
+    p = _lib.bb_ring_plan()
+    p.nmr, p.nsub, p.thick, p.from_fit = len(plan.rad), plan.nsub, int(plan.thick), from_fit
+    p.ridx = arr(ridx, np.int32)
+    p.rad = arr(plan.rad, np.float64)
+    p.base = arr(ring_table(plan.proto), np.float64)
+    p.idx = arr(plan.idx, np.int32)
+    p.dr = arr(plan.dr, np.float64)
+    p.cst = arr(plan.const, np.uint8)
+    p.rads = arr(plan.rads, np.float64)
+    p.densw = None if densw is None else arr(densw, np.float64)
+    p.side = arr(side, np.float64)
+    keep.append(p)
+    return p
+
+
 class GpuLikelihood:
     """Batched log-likelihood of a cube for the free parameters `free` around the rings `rings0`.
 
-    dm: engine.DiskModel of the cube (ANALYTIC or MC). data (nz, ny, nx) float32. likemask (nz, ny, nx) bool
-    (maskmode 2: only those voxels; None: all). region (ny, nx) bool (None: all). likebox ((blo), (bhi)):
-    the likelihood region's box. sigma: noise per voxel. likefactor: 1 / correlation volume.
-    fracerr: fixed fractional model error (None: 0)."""
+    dm: engine.DiskModel of the cube (ANALYTIC or MC). data (nz, ny, nx) float32. likemask (nz, ny, nx) bool (maskmode
+    2: only those voxels; None: all). region (ny, nx) bool (None: the likelihood box). likebox ((blo), (bhi)): the
+    likelihood region's box. sigma: noise per voxel. likefactor: 1 / correlation volume. fracerr: fixed fractional
+    model error (None: 0). chunk: the most vectors of one launch. halves: the two halves of a launch on two streams."""
 
     def __init__(self, dm, data, rings0, free, likemask=None, region=None, likebox=None, sigma=1.0,
                  likefactor=1.0, fracerr=None, maskmode=2, chunk=256, densprof_smooth=SMOOTH_DEFAULT,
-                 likefactor2d=None, beam_arcsec=None, graphs=False):
-        torch = _torch()
-        self.torch = torch
+                 likefactor2d=None, beam_arcsec=None, graphs=False, halves=True):
+        self.torch = _torch()
         self.dm = dm
         self.free = free
-        self.rings0 = float32_rings(rings0)
+        st = dm.setup
+        for k in free.index:
+            if k not in LIKE_KEYS:
+                raise NotImplementedError(f"free {k.upper()} is not supported by this engine")
+        r0 = rings0.copy()
+        if "fthick" in free.index:
+            r0.twolayer = True           # (a free FTHICK needs the thick layer, also where it starts at 0)
+        self.rings0 = float32_rings(r0)
         self.sigma = float(sigma)
         self.likefactor = float(likefactor)
         self.eps = float(fracerr) if fracerr else 0.
         self.chunk = int(chunk)
+        self.use_graphs = bool(graphs)
         nz, ny, nx = data.shape
         if likebox is None:
             likebox = ((0, 0), (nx, ny))
-        Lb, Hb = likebox
+        Lb, Hb = (tuple(int(v) for v in likebox[0]), tuple(int(v) for v in likebox[1]))
+        self.likebox = (Lb, Hb)
         if region is None:
             region = np.zeros((ny, nx), bool)
             region[Lb[1]:Hb[1], Lb[0]:Hb[0]] = True
-        blank = ~np.isfinite(data) | (data == 0)
-        lm = np.ones(data.shape, bool) if likemask is None else np.asarray(likemask, bool)
+        region = np.asarray(region, bool)
         if maskmode == 2:
-            use = lm & ~blank & region[None]
-            dused = np.where(use, data, np.nan)
-        elif maskmode == 1:
-            use = ~blank & region[None]
-            dused = np.where(use, np.where(lm, data, 0.), np.nan)
+            lm = np.ones(data.shape, bool) if likemask is None else np.asarray(likemask, bool)
+        elif maskmode == 0:
+            lm = np.ones(data.shape, bool)
         else:
-            use = ~blank & region[None]
-            dused = np.where(use, data, np.nan)
-        self.nvox = int(use.sum())
-        d2 = np.where(use, np.nan_to_num(dused), 0.).astype(np.float64) ** 2
-        dm2 = d2.sum(axis=0)
-        self._sat = np.zeros((ny + 1, nx + 1))
-        self._sat[1:, 1:] = np.cumsum(np.cumsum(dm2, axis=0), axis=1)
-        # the model box (fixed: the centre, heights and radii of the batch are those of rings0)
-        for k in ("xpos", "ypos", "z0", "z0thick", "radii"):
-            if k in free.index:
-                raise NotImplementedError(f"free {k.upper()} moves the model box; not supported by this engine yet")
+            raise NotImplementedError("maskmode 1 (the data times the mask) is not supported by the likelihood context")
+        data = np.ascontiguousarray(data, np.float32)
+        blank = ~np.isfinite(data) | (data == 0)
+        v = lm & ~blank
+        nvmap = v.sum(axis=0)
+        self.nvox = int(nvmap[region].sum())
+        d2 = np.where(v, data, 0.).astype(np.float64) ** 2
+        dm2 = np.where(region, d2.sum(axis=0), 0.)
+        sat = np.zeros((ny + 1, nx + 1))
+        sat[1:, 1:] = np.cumsum(np.cumsum(dm2, axis=0), axis=1)
+        self._sat = sat
+        # the model box: that of the initial rings (fixed), or one per vector when a free parameter moves it
+        self.box_free = any(k in free.index for k in BOX_KEYS)
         blo, bhi = dm.box(self.rings0, final=True)
         blo = [max(blo[0], Lb[0]), max(blo[1], Lb[1])]
         bhi = [max(min(bhi[0], Hb[0]), blo[0]), max(min(bhi[1], Hb[1]), blo[1])]
-        self.blo, self.bhi = blo, bhi
-        S = self._sat
-        inbox = S[bhi[1], bhi[0]] - S[blo[1], bhi[0]] - S[bhi[1], blo[0]] + S[blo[1], blo[0]]
-        self.ss_out = max(float(S[-1, -1] - inbox), 0.)
-        # the data of the box in the layout of the models (yxz), NaN where not summed
-        dbox = dused[:, blo[1]:bhi[1], blo[0]:bhi[0]].transpose(1, 2, 0)
-        self.dev = torch.device("cuda", torch.cuda.current_device())
-        self.dvalid = torch.from_numpy(np.ascontiguousarray(dbox, np.float32)).to(self.dev)
-        self.nvox_box = int(np.prod(dbox.shape))
-        self.plan = SubringPlan(dm.setup, self.rings0, dm.ringmode, final=True)
-        self._bufs = {}
-        self._streams = None
-        # graphs: a batch of a given size runs as one captured HIP graph (ANALYTIC models), replayed with the
-        # tables of every new batch copied into its input buffers
-        self.use_graphs = bool(graphs) and dm.setup.galmodtype != "MC"
-        self._graphs = {}
+        self.fixbox = (blo, bhi)
+        self.blo, self.bhi = (list(Lb), list(Hb)) if self.box_free else (blo, bhi)
+        self.ss_out = max(float(sat[-1, -1]) - self._boxsum(blo, bhi), 0.)
+        self.plan = SubringPlan(st, self.rings0, dm.ringmode, final=True)
         # DENS not fitted: the deconvolved profile of the initial rings, and of every parameter vector when the
         # geometry is free (fitter.py:606-615)
         self.useNorm = "dens" not in free.index
         self.update_prof = self.useNorm and any(k in free.index for k in PROFILE_KEYS)
+        if self.update_prof and "fthick" in free.index:
+            raise NotImplementedError("a free FTHICK changes the ring weights of the deconvolved profile: fit DENS")
         self.profile = None
         self.dens0 = None
+        self.last_dens = None
         if self.useNorm:
-            v = lm & ~blank
             ymap = np.where(v, data, 0.).astype(np.float64).sum(axis=0)
-            nvmap = v.sum(axis=0)
             smap = self.sigma * np.sqrt(nvmap.astype(np.float64))
             sl = (slice(Lb[1], Hb[1]), slice(Lb[0], Hb[0]))
             ok = region[sl] & (nvmap[sl] > 0)
@@ -167,50 +214,172 @@ class GpuLikelihood:
                 bm = dm.header.Bmaj()
                 beam_arcsec = abs(bm) * 3600. if bm else None
             l2d = self.likefactor if likefactor2d is None else float(likefactor2d)
-            self.profile = DensityProfile(dm, self.rings0, self.plan, (Lb, Hb), ymap[sl], smap[sl], ok,
-                                          smooth=densprof_smooth, corr=1. / l2d, beam_arcsec=beam_arcsec)
-            sr = self.plan.tables({})[0]
-            sr[:, 5, :] = 1.
-            d_sr = torch.from_numpy(np.ascontiguousarray(sr)).to(self.dev)
-            dens = self.profile.run(sr, d_sr, {}).cpu().numpy()[0].astype(np.float64)
+            self.profile = ProfileSetup(dm, self.rings0, self.plan, (Lb, Hb), ymap[sl], smap[sl], ok,
+                                        smooth=densprof_smooth, corr=1. / l2d, beam_arcsec=beam_arcsec)
+        self._data = (data, lm, region)
+        if self.useNorm and not self.update_prof:
+            # the profile of the initial rings, once (a context without free parameters that computes it)
+            ctx0 = self._make_ctx(FreePars([], self.rings0.nr), 1, True, halves=False)
+            ll = np.zeros(1)
+            dens = np.zeros((1, self.rings0.nr), np.float32)
+            th = np.zeros(1)
+            try:
+                _lib.check(_lib.lib().bb_loglike_batch(ctx0, th.ctypes.data, 1, ll.ctypes.data, dens.ctypes.data,
+                                                       _stream_ptr(self.torch)))
+            finally:
+                _lib.lib().bb_ctx_destroy(ctx0)
             if not np.all(np.isfinite(dens)):
                 raise RuntimeError("the density profile of the initial rings did not converge")
-            self.dens0 = dens
-            self.rings0.dens = dens.copy()
+            self.dens0 = dens[0].astype(np.float64)
+            self.rings0.dens = self.dens0.copy()
+        self._ctx = self._make_ctx(free, self.chunk, self.update_prof, halves=halves)
+        del self._data
+        if self.use_graphs:
+            self.set_graphs(True)
 
-    def _buffers(self, B, slot=0):
-        """Device buffers of a chunk of B vectors; slot: an independent set (chunks in flight on two streams)."""
-        b = self._bufs.get((B, slot))
-        if b is None:
-            torch = self.torch
-            by, bx, nz = self.bhi[1] - self.blo[1], self.bhi[0] - self.blo[0], self.dm.setup.nsubs
-            L = _lib.lib()
-            wl = int(L.bb_residual_work_size(B, self.nvox_box))
-            cwl = int(L.bb_conv_residual_work_size(self.dm.handle(), B, by, bx)) if self.dm.cfield is not None else 1
-            b = dict(dep=torch.empty((B, by, bx, nz), dtype=torch.float32, device=self.dev),
-                     work=torch.empty(wl, dtype=torch.float64, device=self.dev),
-                     cwork=torch.empty(max(cwl, 1), dtype=torch.float64, device=self.dev), cwl=cwl,
-                     ss=torch.empty(B, dtype=torch.float64, device=self.dev),
-                     lg=torch.empty(B, dtype=torch.float64, device=self.dev), wl=wl)
-            if self.dm.setup.galmodtype == "MC":
-                # (the MC model is convolved into its own buffer, then compared)
-                b["mod"] = torch.empty((B, by, bx, nz), dtype=torch.float32, device=self.dev)
-            self._bufs = {k: v for k, v in self._bufs.items() if k[0] == B}
-            self._bufs[(B, slot)] = b
-        return b
+    # ------------------------------------------------------------------ set-up
+    def _boxsum(self, blo, bhi):
+        S = self._sat
+        return float(S[bhi[1], bhi[0]] - S[blo[1], bhi[0]] - S[bhi[1], blo[0]] + S[blo[1], blo[0]])
 
-    def buffers_in_use(self):
-        """References to the device buffers of the current batch size (this likelihood's and its profile's)."""
-        return (dict(self._bufs), dict(self.profile._bufs) if self.profile is not None else None)
+    def _nzq(self, free):
+        r = self.rings0
+        z = np.any(np.asarray(r.z0) > 0) or (r.thick() and np.any(r.thick_z0() > 0))
+        return ZSAMPLES_MAX if (z or "z0" in free.index or "z0thick" in free.index) else 1
 
-    def _stream_pair(self):
-        if self._streams is None:
-            self._streams = (self.torch.cuda.Stream(), self.torch.cuda.Stream())     # (default priority)
-        return self._streams
+    def _make_ctx(self, free, chunk, with_profile, halves=True):
+        """A bb_ctx of the fit for the free parameters free (the device buffers of chunk vectors)."""
+        dm, st = self.dm, self.dm.setup
+        data, lm, region = self._data
+        nz, ny, nx = data.shape
+        keep = []
+        nzq = self._nzq(free)
+        h = dm.handle(nzq)
+        self.handle = h
+        sr0 = self.plan.tables({})[0]
+        dm.orient_tile(sr0, nzq)
+        nr = self.rings0.nr
+        main = _plan_struct(self.plan, self.plan.ring_index, 0xFFFF, keep)
+        prof = None
+        ann = None
+        if with_profile:
+            P = self.profile
+            bits = 0
+            for k in ANNULUS_KEYS:
+                bits |= 1 << LIKE_KEYS.index(k)
+            ann = _plan_struct(P.plan_ext, np.full(len(P.plan_ext.rad), nr - 1), bits, keep)
+            prof = _lib.bb_like_profile()
+            prof.d = P.desc()
+            for name in ("wpix", "ypix", "lcoef", "lcol", "near", "widx", "wval", "widx_e", "wval_e"):
+                a = np.ascontiguousarray(getattr(P, name))
+                if name in ("lcol", "near", "widx", "widx_e"):
+                    a = a.astype(np.int32)
+                keep.append(a)
+                if name in ("lcoef", "lcol", "near") and not P.prior():
+                    setattr(prof, name, None)
+                else:
+                    setattr(prof, name, a.ctypes.data)
+            prof.qunit, prof.unit = P.qunit, P.unit
+        d = _lib.bb_like_desc()
+        d.nr, d.ndim, d.chunk = nr, max(free.ndim, 1), int(chunk)
+        d.mc = 1 if st.galmodtype == "MC" else 0
+        d.nvpar = st.nvpar if st.nvpar >= 1 else st.nsubs
+        d.seed = seed32(st.iseed)
+        d.box_free = 1 if any(k in free.index for k in BOX_KEYS) else 0
+        d.halves = 1 if halves else 0
+        (Lb, Hb), (fb, fh) = self.likebox, self.fixbox
+        d.likebox = _lib.bb_box(Lb[0], Lb[1], Hb[0], Hb[1])
+        d.fixbox = _lib.bb_box(fb[0], fb[1], fh[0], fh[1])
+        d.sigma, d.likefactor, d.eps = self.sigma, self.likefactor, self.eps
+        r = self.rings0
+        ext = r.radii[-1] + r.radsep / 2.
+        d.ext = max(ext, outer_edge(r, dm.ringmode))
+        hd = dm.header
+        conv = arcsconv(hd.Cunit(0))
+        d.pixas0, d.pixas1 = abs(hd.Cdelt(0)) * conv, abs(hd.Cdelt(1)) * conv
+        d.nx, d.ny = nx, ny
+        if d.mc:
+            d.mc_cap = max(2 * int(dm.mc_clouds(sr0).max()), 4096)
+        off, ring = free.offsets()
+        rt = np.ascontiguousarray(ring_table(r))
+        lmu = np.ascontiguousarray(lm, np.uint8)
+        rgu = np.ascontiguousarray(region, np.uint8)
+        keep += [rt, off, ring, lmu, rgu, data]
+        out = C.c_void_p()
+        _lib.check(_lib.lib().bb_ctx_create(h, C.byref(d), data.ctypes.data, lmu.ctypes.data, rgu.ctypes.data,
+                                            rt.ctypes.data, off.ctypes.data, ring.ctypes.data, C.byref(main),
+                                            C.byref(ann) if ann is not None else None,
+                                            C.byref(prof) if prof is not None else None, C.byref(out)))
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "_ctx", None):
+                _lib.lib().bb_ctx_destroy(self._ctx)
+                self._ctx = None
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ evaluation
+    def _theta(self, theta):
+        theta = np.ascontiguousarray(np.atleast_2d(np.asarray(theta, float)))
+        if self.free.ndim == 0:
+            theta = np.zeros((theta.shape[0], 1))
+        elif theta.shape[1] != self.free.ndim:
+            raise ValueError(f"theta has {theta.shape[1]} columns, the fit {self.free.ndim}")
+        return theta
+
+    def loglike(self, theta):
+        """log-likelihood of each row of theta (B, ndim) -> (B,) float64 (-inf where the likelihood is zero)."""
+        theta = self._theta(theta)
+        B = theta.shape[0]
+        out = np.empty(B)
+        dens = np.empty((B, self.rings0.nr), np.float32) if self.update_prof else None
+        _lib.check(_lib.lib().bb_loglike_batch(self._ctx, theta.ctypes.data, B, out.ctypes.data,
+                                               None if dens is None else dens.ctypes.data, _stream_ptr(self.torch)))
+        if dens is not None:
+            self.last_dens = dens.astype(np.float64)
+        return out
+
+    def launch(self, theta, stream=None):
+        """Launch the evaluation of the rows of theta (at most chunk rows) on the stream (default: the current one) and
+        return at once; collect() gives the log-likelihoods."""
+        theta = self._theta(theta)
+        if theta.shape[0] > self.chunk:
+            raise ValueError(f"{theta.shape[0]} vectors for a likelihood of chunk {self.chunk}")
+        self._pending = theta.shape[0]
+        _lib.check(_lib.lib().bb_loglike_launch(self._ctx, theta.ctypes.data, theta.shape[0],
+                                                _stream_ptr(self.torch, stream)))
+
+    def collect(self):
+        """The log-likelihoods (B,) of the last launch (waits for it)."""
+        B = self._pending
+        out = np.empty(B)
+        dens = np.empty((B, self.rings0.nr), np.float32) if self.update_prof else None
+        _lib.check(_lib.lib().bb_loglike_collect(self._ctx, out.ctypes.data,
+                                                 None if dens is None else dens.ctypes.data))
+        if dens is not None:
+            self.last_dens = dens.astype(np.float64)
+        return out
+
+    def device_tables(self, theta):
+        """(sr (B, 16, nsub), sr_ext or None): the sub-ring tables of the model (and of the exterior annulus when the
+        profile is computed per vector) that the context builds on the device for the rows of theta."""
+        theta = self._theta(theta)
+        B = theta.shape[0]
+        nsub_t = self.plan.nsub * (2 if self.plan.thick else 1)
+        sr = np.empty((B, 16, nsub_t))
+        ext = None
+        if self.update_prof:
+            pe = self.profile.plan_ext
+            ext = np.empty((B, 16, pe.nsub * (2 if pe.thick else 1)))
+        _lib.check(_lib.lib().bb_ctx_tables(self._ctx, theta.ctypes.data, B, sr.ctypes.data,
+                                            None if ext is None else ext.ctypes.data))
+        return sr, ext
 
     def tables(self, theta):
-        """(sr (B, 16, nsub), values) of the rows of theta: the sub-ring tables of the model (density row 1 when
-        the profile is computed per vector) and the ring values of the free keys."""
+        """(sr (B, 16, nsub), values) of the rows of theta on the host (SubringPlan.tables): the sub-ring tables of the
+        model (density row 1 when the profile is computed per vector) and the ring values of the free keys."""
         vals = self.free.values(theta)
         if self.useNorm and not self.update_prof:
             vals["dens"] = self.dens0
@@ -219,182 +388,30 @@ class GpuLikelihood:
             sr[:, 5, :] = 1.
         return sr, vals
 
-    def _device_part(self, sr, d_sr, vals, slot, sr_ext=None, d_sr_ext=None, after_maps=None):
-        """Launch one chunk on the current stream with the buffers of slot: (ss, lg, dens) device tensors.
-        after_maps: an event recorded once the chunk's ring maps are launched (or, without a profile, at once)."""
-        B = sr.shape[0]
-        bufs = self._buffers(B, slot)
-        dens = None
-        if self.update_prof:
-            prof = self.profile
-            if sr_ext is None:
-                sr_ext = prof.ext_tables(vals if vals is not None else {}, B)
-            maps = prof.maps(sr, d_sr, sr_ext, None, d_sr_ext, slot)
-            if after_maps is not None:
-                after_maps.record()
-            dens = prof.set_dens(prof.solve(maps, None, slot), d_sr, None, slot)
-        elif after_maps is not None:
-            after_maps.record()
-        if self.dm.setup.galmodtype == "MC":
-            tab = sr
-            if dens is not None:
-                # the clouds of the MC model follow the densities: bound them with those of the profile
-                v2 = dict(vals or {})
-                v2["dens"] = dens.cpu().numpy().astype(np.float64)
-                tab = self.plan.tables(v2)[0]
-            mod = self.dm.array_batch_mc(sr, self.blo, self.bhi, convolve=True, out=bufs["mod"], work=bufs["dep"],
-                                         d_sr=d_sr, max_clouds=int(self.dm.mc_clouds(tab).max()))
-        else:
-            dep = self.dm.array_batch(sr, self.blo, self.bhi, convolve=False, work=bufs["dep"], d_sr=d_sr)
-            if self.dm.cfield is not None:
-                # the beam convolution and the residual in one pass: the convolved models are never stored
-                by, bx = self.bhi[1] - self.blo[1], self.bhi[0] - self.blo[0]
-                h = self.dm.handle(nzq_for(sr.reshape(16, -1)))
-                _lib.check(_lib.lib().bb_conv_residual_gauss(h, dep.data_ptr(), self.dvalid.data_ptr(), B, by, bx,
-                                                             self.sigma ** 2, self.eps ** 2, bufs["cwork"].data_ptr(),
-                                                             bufs["cwl"], bufs["ss"].data_ptr(),
-                                                             bufs["lg"].data_ptr(), _stream_ptr(self.torch)))
-                return bufs["ss"], bufs["lg"], dens
-            mod = dep
-        s2 = self.sigma ** 2
-        _lib.check(_lib.lib().bb_residual_gauss(mod.data_ptr(), self.dvalid.data_ptr(), B, self.nvox_box, s2,
-                                                self.eps ** 2, bufs["work"].data_ptr(), bufs["wl"],
-                                                bufs["ss"].data_ptr(), bufs["lg"].data_ptr(),
-                                                _stream_ptr(self.torch)))
-        return bufs["ss"], bufs["lg"], dens
+    def info(self):
+        """dict: nvox and the squared data of the region (as the context counts them), its deposit box, the vectors
+        whose profile did not converge and the MC launches repeated with larger buffers."""
+        nv, ss = C.c_double(), C.c_double()
+        box = (C.c_int32 * 4)()
+        nf, mr = C.c_int64(), C.c_int64()
+        _lib.check(_lib.lib().bb_ctx_info(self._ctx, C.byref(nv), C.byref(ss), box, C.byref(nf), C.byref(mr)))
+        return dict(nvox=nv.value, ss_total=ss.value, deposit_box=list(box), nnls_fail=nf.value, mc_retries=mr.value)
 
-    def device_batch(self, parts):
-        """Launch the chunks parts (dicts: sr, d_sr, vals, optionally sr_ext, d_sr_ext: host / device sub-ring
-        tables) on two streams in turn, so that the latency-bound kernels of one chunk (the profile's least
-        squares) run beside the deposit of the other; the current stream waits for all of them. Returns the
-        (ss, lg, dens) device tensors of every chunk (valid once the current stream reaches this point)."""
-        torch = self.torch
-        cur = torch.cuda.current_stream()
-        streams = self._stream_pair() if len(parts) > 1 else (cur,)
-        out = []
-        prev = None
-        for i, p in enumerate(parts):
-            s = streams[i % len(streams)]
-            if s is not cur:
-                s.wait_stream(cur)
-            if prev is not None:
-                # (the ring maps of this chunk start when those of the previous one are launched: its least
-                # squares, latency-bound, then run beside them)
-                s.wait_event(prev)
-            ev = torch.cuda.Event() if len(parts) > 1 else None
-            with torch.cuda.stream(s):
-                out.append(self._device_part(p["sr"], p["d_sr"], p.get("vals"), i % 2, p.get("sr_ext"),
-                                             p.get("d_sr_ext"), ev))
-            if s is not cur:
-                # (the tables are read on s: their memory must not return to the caller's stream before that)
-                for k in ("d_sr", "d_sr_ext"):
-                    if k in p:
-                        p[k].record_stream(s)
-            prev = ev
-        for s in streams:
-            if s is not cur:
-                cur.wait_stream(s)
-        return out
+    def set_halves(self, on):
+        _lib.check(_lib.lib().bb_ctx_set_halves(self._ctx, 1 if on else 0))
 
-    def split_parts(self, sr, vals, halves=True, pinned=False):
-        """The chunk (sr, vals) as two halves with their device tables (one part below 64 vectors or without
-        halves). pinned: the tables go through page-locked host buffers and are copied asynchronously on the
-        current stream (the host does not wait; use the parts on that stream)."""
-        torch = self.torch
-        B = sr.shape[0]
-        cuts = [0, B] if B < 64 or not halves else [0, B // 2, B]
+    def set_graphs(self, on):
+        """Batches as replays of a captured HIP graph per batch size (bb_ctx_set_graphs)."""
+        self.use_graphs = bool(on)
+        _lib.check(_lib.lib().bb_ctx_set_graphs(self._ctx, 1 if on else 0))
 
-        def up(a):
-            t = torch.from_numpy(np.ascontiguousarray(a))
-            return t.pin_memory().to(self.dev, non_blocking=True) if pinned else t.to(self.dev)
+    def graph_replays(self):
+        return int(_lib.lib().bb_ctx_graph_replays(self._ctx))
 
-        parts = []
-        for a, b in zip(cuts[:-1], cuts[1:]):
-            p = dict(sr=sr[a:b], d_sr=up(sr[a:b]), vals=vals)
-            if self.update_prof:
-                # (the values of the free keys are per vector, (B, nr))
-                v = {k: np.asarray(x)[a:b] for k, x in (vals or {}).items()}
-                p["vals"] = v
-                p["sr_ext"] = self.profile.ext_tables(v, b - a)
-                p["d_sr_ext"] = up(p["sr_ext"])
-            parts.append(p)
-        return parts
-
-    def _graph_batch(self, sr, vals):
-        """The launches of device_batch for the tables sr (and vals), as a HIP graph per batch size: captured on the
-        first batch of that size (after one eager run that allocates the buffers), replayed afterwards with the new
-        tables copied into the graph's input buffers. The ring orientation of the deposit tiles is that of the first
-        batch (it changes the speed, not the result). Returns device_batch's output tensors."""
-        torch = self.torch
-        B = sr.shape[0]
-        parts = self.split_parts(sr, vals)
-        g = self._graphs.get(B)
-        if g is None:
-            self.device_batch(parts)               # (eager: allocates the buffers, sets the kernel attributes)
-            torch.cuda.synchronize()
-            graph = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(graph):
-                out = self.device_batch(parts)
-            torch.cuda.synchronize()
-            # (the graph writes into the buffers of this batch size: hold them, since a batch of another size
-            # replaces them in _bufs)
-            g = (graph, parts, out, self.buffers_in_use())
-            if len(self._graphs) >= 4:                # (a few batch sizes: a sampler's batches vary in size)
-                self._graphs.pop(next(iter(self._graphs)))
-            self._graphs[B] = g
-        graph, static, out, _ = g
-        for p, q in zip(static, parts):
-            p["d_sr"].copy_(q["d_sr"])
-            if "d_sr_ext" in p:
-                p["d_sr_ext"].copy_(q["d_sr_ext"])
-        graph.replay()
-        return out
-
-    def residuals(self, sr, vals=None):
-        """(S_box, lg) per table: device launches for one chunk of sub-ring tables (B, 16, nsub), in parts on
-        two streams; with a per-vector profile, vals are the ring values of the free keys (for the annulus)."""
-        if self.use_graphs:
-            res = self._graph_batch(sr, vals)
-        else:
-            res = self.device_batch(self.split_parts(sr, vals))
-        self.torch.cuda.current_stream().synchronize()
-        return self.collect(res)
-
-    def launch(self, theta, halves=False):
-        """Launch the evaluation of the rows of theta (at most chunk rows, no negative positive-only parameter) on
-        the current stream and return at once: the device tensors for collect(), valid once that stream reaches
-        this point. halves: as residuals, two halves on two streams."""
-        sr, vals = self.tables(np.atleast_2d(np.asarray(theta, float)))
-        return self.device_batch(self.split_parts(sr, vals, halves))
-
-    def collect(self, res):
-        """(S_box, lg) of the device tensors of launch (or device_batch), as numpy arrays; the density profiles of
-        the chunk are kept in last_dens."""
-        ss = np.concatenate([r[0].cpu().numpy() for r in res])
-        lg = np.concatenate([r[1].cpu().numpy() for r in res])
-        if res[0][2] is not None:
-            self.last_dens = np.concatenate([r[2].cpu().numpy() for r in res])
-            if not np.all(np.isfinite(self.last_dens)):
-                raise RuntimeError("the density profile did not converge (non-negative least squares)")
-        return ss, lg
-
-    def loglike(self, theta):
-        """log-likelihood of each row of theta (B, ndim) -> (B,) float64 (-inf where a positive parameter
-        is negative)."""
-        theta = np.atleast_2d(np.asarray(theta, float))
-        out = np.full(theta.shape[0], -np.inf)
-        ok = ~self.free.negative(theta)
-        idx = np.flatnonzero(ok)
-        for c in range(0, len(idx), self.chunk):
-            ii = idx[c:c + self.chunk]
-            out[ii] = self.finish(*self.residuals(*self.tables(theta[ii])))
-        return out
-
-    def finish(self, ss, lg):
-        """The log-likelihoods of the residual sums (S_box, lg) of residuals (likelihood.py:567-619)."""
-        s2 = self.sigma ** 2
-        if self.eps > 0:
-            nll = 0.5 * (ss + self.ss_out / s2) + 0.5 * self.nvox * math.log(2 * math.pi * s2) + 0.5 * lg
-        else:
-            nll = 0.5 * (ss + self.ss_out) / s2 + 0.5 * self.nvox * math.log(2 * math.pi * s2)
-        return -self.likefactor * nll
+    def trace(self, events):
+        """Record 5 torch CUDA events at the stage boundaries of eager launches (None: stop)."""
+        if events is None:
+            _lib.check(_lib.lib().bb_ctx_trace(self._ctx, None))
+            return
+        arr = (C.c_void_p * 5)(*[e.cuda_event for e in events])
+        _lib.check(_lib.lib().bb_ctx_trace(self._ctx, arr))

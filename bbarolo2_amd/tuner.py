@@ -14,7 +14,6 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from .galmod import nzq_for
 
 # (TX, TY) with TX >= TY; the deposit orients each shape along the projected minor axis (DiskModel.orient_tile)
 TILE_SHAPES = ((4, 4), (8, 4), (8, 8), (16, 4), (16, 8), (32, 4), (16, 16), (32, 8), (64, 4), (32, 16))
@@ -22,9 +21,10 @@ TILE_SHAPES = ((4, 4), (8, 4), (8, 8), (16, 4), (16, 8), (32, 4), (16, 16), (32,
 _CACHE = {}
 
 
-def _key(like, B, nsub):
+def _key(like, B):
     st = like.dm.setup
-    return (like.bhi[0] - like.blo[0], like.bhi[1] - like.blo[1], st.nsubs, len(st.relint), int(nsub), int(B))
+    return (like.bhi[0] - like.blo[0], like.bhi[1] - like.blo[1], st.nsubs, len(st.relint), int(like.plan.nsub),
+            bool(like.plan.thick), int(B))
 
 
 def get_tile(h):
@@ -54,23 +54,22 @@ def tune_tiles(like, theta, reps=2, shapes=TILE_SHAPES, cache=_CACHE):
     Returns (shape, {shape: ms per batch})."""
     torch = like.torch
     theta = np.atleast_2d(np.asarray(theta, float))
-    sr, vals = like.tables(theta)
-    h = like.dm.handle(nzq_for(sr.reshape(16, -1)))
-    key = _key(like, sr.shape[0], sr.shape[2])
+    h = like.handle
+    key = _key(like, theta.shape[0])
     if key in cache:
         best, times = cache[key]
         _lib.check(_lib.lib().bb_model_set_tile(h, best[0], best[1]))
         return best, times
-    parts = like.split_parts(sr, vals, halves=False)
     times = {}
     L = _lib.lib()
     for s in tile_candidates(h, shapes):
         _lib.check(L.bb_model_set_tile(h, s[0], s[1]))
-        like.device_batch(parts)
+        like.loglike(theta)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(reps):
-            like.device_batch(parts)
+            like.launch(theta)
+            like.collect()
         e1.record()
         e1.synchronize()
         times[s] = e0.elapsed_time(e1) / reps

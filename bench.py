@@ -2,18 +2,18 @@
 of a Bayesian fit of the 64x64x40 configuration with and without the surrogate screen, and the evaluations per
 second of an uneven batch of 64 galaxies (tile shapes picked by the tuner; per-galaxy batch times p50 / p99).
 
-One step = one batch of B parameter vectors (free VROT, VDISP per ring, INC, PA: 82-D) evaluated by the
-model-cube likelihood, inputs resident on the device: the deconvolved density profile of every vector (ring maps,
-non-negative least squares), then ANALYTIC GALMOD -> beam convolution -> masked Gaussian residual, as
-GpuLikelihood.device_batch runs it (two halves of the batch on two streams). The synthetic cube is built on the
-GPU (ANALYTIC GALMOD of the truth rings, beam, noise 3% of the peak). Prints one JSON line (see --help).
+One step = one batch of B parameter vectors (free VROT, VDISP per ring, INC, PA: 82-D) through the likelihood
+context, from theta on the host to the log-likelihoods on the host (GpuLikelihood.loglike -> bb_loglike_batch): the
+sub-ring tables built on the device, the deconvolved density profile of every vector (ring maps, non-negative least
+squares), ANALYTIC GALMOD -> beam convolution fused with the masked Gaussian residual; two halves of the batch on two
+streams. Every step takes new vectors (a pool of prior draws). The synthetic cube is built on the GPU (ANALYTIC GALMOD
+of the truth rings, beam, noise 3% of the peak). Prints one JSON line (see --help).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-bayes] [--no-multi]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-bayes] [--no-multi] [--no-mc]
 """
 import argparse
 import ctypes as C
 import json
-import math
 import os
 import sys
 import time
@@ -28,6 +28,10 @@ PEAK_HBM = 8.0e12          # MI355X HBM3E, bytes/s
 # (FETCH_SIZE, WRITE_SIZE) scaled by the same counters of a plain 1 GiB copy (bench.py --copy-calibration)
 TRAFFIC_SHAPE = (256, 160, 160, 96)
 TRAFFIC_TILE_BYTES = (1837977.6 * 2.0 + 4709621.0 * 1.0) * 1024   # KiB x scale -> bytes
+# LDS instructions (wave level) of one such launch (rocprofv3 --pmc SQ_INSTS_LDS) and the rate of one CU for
+# 64-bit LDS atomics measured by a probe of ds_add_u64 (6.4-6.8 cycles per wave instruction): the deposit's limiter
+LDS_INSTS_TILE = 3.0e9
+LDS_ATOMIC_PEAK = 256 * 2.4e9 / 6.6          # wave instructions per second over the 256 CUs at 2.4 GHz
 
 
 def workload(n_theta, B, seed=1):
@@ -51,63 +55,48 @@ def _cpu_worker_init(wl):
     _W.update(wl)
 
 
-def _cpu_setup(w):
-    """Per-worker constants of the oracle leg: the setup, the sub-ring plan and the data maps of the profile
-    (likemask voxels, noise sigma sqrt(n), region), as the likelihood prepares them."""
-    from bbarolo2_amd.galmod import GalmodSetup, SubringPlan, zstep_for
-    from bbarolo2_amd.likelihood import float32_rings
-    st = GalmodSetup(w["h"], w["par"])
-    st.zstep = zstep_for(w["h"], st, True)
-    w["st"] = st
-    w["r32"] = float32_rings(w["truth"])
-    w["plan"] = SubringPlan(st, w["r32"], "CENTRED")
-    data, lm = w["data"], w["likemask"]
-    v = lm & np.isfinite(data) & (data != 0)
-    w["ymap"] = np.where(v, data, 0.).astype(np.float64).sum(axis=0)
-    nv = v.sum(axis=0)
-    w["smap"] = w["sigma"] * np.sqrt(nv.astype(np.float64))
-    w["okmap"] = w["region"] & (nv > 0)
-
-
 def _cpu_eval(i):
-    """One likelihood evaluation with the FP64 oracle (deconvolved profile with FFT convolutions, model, beam,
-    residual); returns its wall time."""
-    import oracle
-    from oracle import profile as op
-    from bbarolo2_amd.galmod import nzq_for, PHI_TAB
+    """One likelihood evaluation with the FP64 oracle pipeline (oracle.pipeline: deconvolved profile with FFT
+    convolutions, model in the vector's box, FFT beam convolution, residual); returns its wall time."""
+    from oracle import pipeline
     w = _W
-    if "st" not in w:
-        _cpu_setup(w)
-    st, plan = w["st"], w["plan"]
+    if "orc" not in w:
+        w["orc"] = pipeline.Likelihood(w["h"], w["par"], w["truth"], w["spec"], w["data"], w["likemask"], w["region"],
+                                       w["box"], w["sigma"], w["likefactor"], fft=True)
+    th = w["theta"][i % len(w["theta"])]
     t0 = time.perf_counter()
-    vals = w["free"].values(w["theta"][i % len(w["theta"])][None])
-    rr = w["r32"].copy()
-    for k, v in vals.items():
-        setattr(rr, k, v[0])
-    blo, bhi = w["box"]
-    sl = (slice(blo[1], bhi[1]), slice(blo[0], bhi[0]))
-    dens = op.deconvolved(st, w["kern"], 1.0, rr, (blo, bhi), w["ymap"][sl], w["smap"][sl], w["okmap"][sl],
-                          beam_arcsec=abs(w["h"].Bmaj()) * 3600., corr=1. / w["likefactor"], fft=True)
-    vals["dens"] = dens.astype(np.float32).astype(np.float64)[None]
-    sr, _ = plan.tables(vals)
-    m = oracle.galmod_analytic(st, sr[0], blo, bhi, nzq_for(sr[0]), PHI_TAB)
-    m = oracle.conv_fft(m, w["kern"])
-    d = w["data"][:, blo[1]:bhi[1], blo[0]:bhi[0]]
-    mk = w["likemask"][:, blo[1]:bhi[1], blo[0]:bhi[0]]
-    rg = w["region"][blo[1]:bhi[1], blo[0]:bhi[0]]
-    oracle.sumsq_residuals(np.ascontiguousarray(m.transpose(2, 0, 1)), d, mk, 2, rg)
+    w["orc"].loglike(th)
     return time.perf_counter() - t0
 
 
+def _host_cpus():
+    """(cores this process may run on, os.cpu_count(), CPU model name)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return avail, os.cpu_count(), model
+
+
 def cpu_baseline(wl, per_worker=3):
-    """The oracle timed on the host cores: P worker processes (one evaluation per core, as the reference's sampler
-    pools; started by spawn, since this process holds the GPU), each doing 1 untimed + per_worker timed
-    evaluations; evals/s = P / median time."""
+    """The oracle timed on the host cores: one worker process per core this process may run on (one evaluation per
+    core, as the reference's sampler pools; started by spawn, since this process holds the GPU), each doing 1
+    untimed + per_worker timed evaluations; evals/s = P / median time."""
     import multiprocessing as mp
-    P = max(1, min(16, os.cpu_count() or 1))
+    P, ncpu, model = _host_cpus()
     ctx = mp.get_context("spawn")
-    small = {k: wl[k] for k in ("h", "truth", "par", "data", "likemask", "region", "free", "theta", "box", "kern",
-                                "sigma", "likefactor")}
+    small = {k: wl[k] for k in ("h", "truth", "par", "data", "likemask", "region", "theta", "box", "sigma",
+                                "likefactor")}
+    small["spec"] = wl["free"].spec
     with ctx.Pool(P, initializer=_cpu_worker_init, initargs=(small,)) as pool:
         pool.map(_cpu_eval, range(P), chunksize=1)
         t0 = time.perf_counter()
@@ -116,10 +105,10 @@ def cpu_baseline(wl, per_worker=3):
     med = float(np.median(times))
     return dict(value=P / med, unit="evals/s", cores=P, kind="port",
                 sample=f"{P * per_worker} FP64 oracle likelihood evaluations (deconvolved density profile, ANALYTIC "
-                       f"model, beam, residual) "
-                       f"of the 160x160x96 workload on {P} worker processes, 1 untimed each; median "
+                       f"model, beam, residual) of the 160x160x96 workload on {P} worker processes (all the cores this "
+                       f"process may run on; os.cpu_count() {ncpu}; {model}), 1 untimed each; median "
                        f"{med * 1e3:.0f} ms/eval/core, wall {wall:.1f} s; single core {1 / med:.2f} evals/s",
-                single_core=1 / med)
+                single_core=1 / med, cpu_model=model, cpu_count=ncpu)
 
 
 # The reference's published time of one likelihood call on this geometry, ANALYTIC, INC and PA free, one BLAS
@@ -136,8 +125,9 @@ def cpu_calibration(n_eval=6, seed=1):
     fit = synthetic_fit("ngc2403", chunk=16)
     u = np.random.default_rng(seed).random((n_eval, fit["free"].ndim))
     fit["theta"] = fit["lo"] + u * (fit["hi"] - fit["lo"])
-    small = {k: fit[k] for k in ("h", "truth", "par", "data", "likemask", "region", "free", "theta", "box", "kern",
-                                 "sigma", "likefactor")}
+    small = {k: fit[k] for k in ("h", "truth", "par", "data", "likemask", "region", "theta", "box", "sigma",
+                                 "likefactor")}
+    small["spec"] = fit["free"].spec
     with mp.get_context("spawn").Pool(1, initializer=_cpu_worker_init, initargs=(small,)) as pool:
         pool.map(_cpu_eval, [0], chunksize=1)
         times = pool.map(_cpu_eval, range(n_eval), chunksize=n_eval)
@@ -160,6 +150,35 @@ def copy_calibration(nbytes=1 << 30, reps=5):
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}), flush=True)
 
 
+def stage_times(like, theta, reps):
+    """ms per batch of each stage (ring maps, profile, deposit, convolution + residual) with the batch on one stream,
+    and of the deposit kernel k_an_tile alone (events the library records around its launch)."""
+    import torch
+    from bbarolo2_amd import _lib
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+    tev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    for e in ev + tev:
+        e.record()
+    torch.cuda.synchronize()
+    like.set_halves(False)
+    like.loglike(theta)
+    like.trace(ev)
+    _lib.check(_lib.lib().bb_trace_deposit(C.c_void_p(tev[0].cuda_event), C.c_void_p(tev[1].cuda_event)))
+    st = np.zeros(4)
+    tile = 0.
+    try:
+        for _ in range(reps):
+            like.loglike(theta)
+            torch.cuda.synchronize()
+            st += [ev[k].elapsed_time(ev[k + 1]) for k in range(4)]
+            tile += tev[0].elapsed_time(tev[1])
+    finally:
+        _lib.check(_lib.lib().bb_trace_deposit(None, None))
+        like.trace(None)
+        like.set_halves(True)
+    return st / reps, tile / reps
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +188,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-bayes", action="store_true", help="skip the Bayesian fits (plain and surrogate-screened)")
     ap.add_argument("--no-multi", action="store_true", help="skip the uneven multi-galaxy batch")
+    ap.add_argument("--no-mc", action="store_true", help="skip the MC likelihood leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device buffers 5 times (the reference kernel of the HBM counter passes)")
     a = ap.parse_args()
@@ -181,143 +201,75 @@ def main():
         return
     B = a.batch
     import torch
-    from bbarolo2_amd import _lib
-    from bbarolo2_amd.engine import DiskModel, _stream_ptr
     torch.cuda.set_device(0)
-    wl = workload(max(200, B), B)
+    npool = B * 8
+    wl = workload(npool, B)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(wl)
         print(f"[bench] cpu baseline {cpu['value']:.2f} evals/s on {cpu['cores']} cores", file=sys.stderr, flush=True)
         cpu["calibration"] = cpu_calibration()
         print(f"[bench] cpu calibration {cpu['calibration']}", file=sys.stderr, flush=True)
-    dm, like = wl["dm"], wl["like"]
-    if like.profile is None or not like.update_prof:
+    like = wl["like"]
+    if not like.update_prof:
         raise RuntimeError("the workload fits the geometry without DENS: the profile must be per vector")
-    theta = np.resize(wl["theta"], (B, wl["free"].ndim))
-    sr, vals = like.tables(theta)
-    prof = like.profile
-    d_sr = torch.from_numpy(np.ascontiguousarray(sr)).cuda()
-    sr_ext = prof.ext_tables(vals, B)
-    d_sr_ext = torch.from_numpy(np.ascontiguousarray(sr_ext)).cuda()
-    bufs = like._buffers(B)
+    pool = wl["theta"]
+    steps = [pool[(k * B) % npool:(k * B) % npool + B] for k in range(max(a.steps, a.warmup, 1))]
     blo, bhi = like.blo, like.bhi
-    by, bx, nz = bhi[1] - blo[1], bhi[0] - blo[0], dm.setup.nsubs
+    by, bx, nz = bhi[1] - blo[1], bhi[0] - blo[0], wl["dm"].setup.nsubs
     nvox = by * bx * nz
-    L = _lib.lib()
-    nzq = 1 if not np.any(sr[:, 6] > 0) else 1024
-    h = dm.handle(nzq)
-    dm.orient_tile(sr, nzq)            # (as DiskModel.array_batch does for every batch)
-    plan = dm.azplan(sr[0, 0], nzq)
-    wb = int(L.bb_analytic_work_size(plan, B))
-    ws = torch.empty(wb, dtype=torch.uint8, device="cuda")
-    box = _lib.bb_box(blo[0], blo[1], bhi[0], bhi[1])
-    s2 = like.sigma ** 2
-    stream = torch.cuda.current_stream()
-    sp = _stream_ptr(torch)
-    names = ["ring_maps", "profile_solve", "galmod_analytic", "conv_residual"]
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
-    stage = np.zeros(len(names))
-    # the deposit kernel alone (k_an_tile), timed by events that the library records around its launch
-    tev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    for e in tev:
-        e.record(stream)               # (creates the HIP event)
-    torch.cuda.synchronize()
-    tile_ms = 0.0
-
-    def step(timed):
-        def mark(k):
-            if timed:
-                ev[k].record(stream)
-        mark(0)
-        maps = prof.maps(sr, d_sr, sr_ext, d_sr_ext=d_sr_ext)
-        mark(1)
-        prof.set_dens(prof.solve(maps), d_sr)
-        mark(2)
-        _lib.check(L.bb_galmod_analytic(h, plan, box, d_sr.data_ptr(), sr.shape[2], B, ws.data_ptr(), wb,
-                                        bufs["dep"].data_ptr(), 0, sp))
-        mark(3)
-        _lib.check(L.bb_conv_residual_gauss(h, bufs["dep"].data_ptr(), like.dvalid.data_ptr(), B, by, bx, s2, 0.0,
-                                            bufs["cwork"].data_ptr(), bufs["cwl"], bufs["ss"].data_ptr(),
-                                            bufs["lg"].data_ptr(), sp))
-        mark(4)
-
-    # the timed steps: the batch as GpuLikelihood evaluates it, two halves on two streams (device_batch)
-    parts = like.split_parts(sr, vals)
-    for _ in range(a.warmup):
-        like.device_batch(parts)
+    for k in range(a.warmup):
+        like.loglike(steps[k % len(steps)])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
-        like.device_batch(parts)
+    for k in range(a.steps):
+        ll = like.loglike(steps[k])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    if not np.all(np.isfinite(ll)):
+        raise RuntimeError("a log-likelihood of the timed batch is not finite")
     ms = dt * 1e3 / a.steps
     value = B * a.steps / dt
-    # diagnostics (not the value): the stages of the whole batch in sequence on one stream, and the deposit
-    # kernel alone (events recorded by the library around its launch)
-    nd = max(2, min(a.steps, 5))
-    step(False)
-    torch.cuda.synchronize()
-    _lib.check(L.bb_trace_deposit(C.c_void_p(tev[0].cuda_event), C.c_void_p(tev[1].cuda_event)))
-    for _ in range(nd):
-        step(True)
-        ev[-1].synchronize()
-        stage += [ev[k].elapsed_time(ev[k + 1]) for k in range(len(names))]
-        tile_ms += tev[0].elapsed_time(tev[1])
-    torch.cuda.synchronize()
-    _lib.check(L.bb_trace_deposit(None, None))
-    tile_ms /= nd
-    stage /= nd
-    # the Monte-Carlo model (GALMODTYPE MC, counter-based stream) of the same vectors and densities, timed alone
-    dens = prof.set_dens(prof.solve(prof.maps(sr, d_sr, sr_ext, d_sr_ext=d_sr_ext)), d_sr).cpu().numpy()
-    v2 = dict(vals)
-    v2["dens"] = dens.astype(np.float64)
-    sr_mc = like.plan.tables(v2)[0]
-    dm_mc = DiskModel(wl["h"], wl["par"], galmodtype="MC")
-    cap = int(dm_mc.mc_clouds(sr_mc).max())
-    dm_mc.array_batch_mc(sr_mc, blo, bhi, convolve=False, work=bufs["dep"], d_sr=d_sr, max_clouds=cap)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    nmc = 3
-    for _ in range(nmc):
-        dm_mc.array_batch_mc(sr_mc, blo, bhi, convolve=False, work=bufs["dep"], d_sr=d_sr, max_clouds=cap)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    mc_ms = e0.elapsed_time(e1) / nmc
-    # the same vectors through the host path (GpuLikelihood.loglike), for the record
-    ll = like.loglike(theta[:4])
-    nmap = prof.n * prof.by * prof.bx
-    # algorithmic HBM bytes per evaluation: the cube written once (deposit), read and written (convolution), read
-    # with the data (residual); the maps: counts written and read, FP64 maps written (finish) and read (Gram)
-    bytes_per_eval = {"ring_maps": 8 * nmap * 3, "profile_solve": 8 * nmap, "galmod_analytic": 4 * nvox,
-                      "conv_residual": 8 * nvox}
-    # the dominant kernel, k_an_tile: it writes the model cube once (4 B per voxel); everything else it touches
-    # (sub-ring records, boxes, LDS tables) is small and re-read from cache
+    print(f"[bench] {value:.1f} evals/s ({ms:.2f} ms per batch of {B}, theta -> log L)", file=sys.stderr, flush=True)
+    # diagnostics (not the value): the stages of one batch in sequence on one stream, the deposit kernel alone
+    stage, tile_ms = stage_times(like, steps[0], max(2, min(a.steps, 5)))
+    names = ["ring_maps", "profile_solve", "galmod_analytic", "conv_residual"]
+    # the MC likelihood of the same cube and vectors (GALMODTYPE MC, counter-based stream)
+    mc = None
+    if not a.no_mc:
+        mc = mc_likelihood(wl, steps, B)
+        print(f"[bench] MC likelihood {mc['evals_per_s']:.1f} evals/s", file=sys.stderr, flush=True)
+    nmap = (like.rings0.nr + 1) * by * bx
     ach = 4 * nvox * B / (tile_ms * 1e-3)
-    traffic = None
+    traffic = TRAFFIC_TILE_BYTES if (B, by, bx, nz) == TRAFFIC_SHAPE else None
+    lds = None
     if (B, by, bx, nz) == TRAFFIC_SHAPE:
-        traffic = TRAFFIC_TILE_BYTES
+        rate = LDS_INSTS_TILE / (tile_ms * 1e-3)
+        lds = {"insts_per_launch": LDS_INSTS_TILE, "achieved": rate, "peak": LDS_ATOMIC_PEAK,
+               "unit": "LDS wave instructions/s", "frac": rate / LDS_ATOMIC_PEAK}
     out = {
-        "metric": "likelihood evals/s (160\u00b2\u00d796 cube); Bayes fit wall-time, surrogate vs plain", "value": value, "unit": "evals/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 (f64 geometry and sums)", "data": "synthetic",
+        "metric": "likelihood evals/s (160\u00b2\u00d796 cube); Bayes fit wall-time, surrogate vs plain", "value": value,
+        "unit": "evals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 geometry and sums)",
+        "data": "synthetic",
         "config": {"workload": "GLOBAL likelihood, 160x160x96, 8\" pixels, 5.2 km/s, beam 30\", 40 rings, "
                                "ANALYTIC, free VROT VDISP INC PA (82-D), DENS from the deconvolved profile of "
-                               "every vector", "batch": B,
+                               "every vector; timed from theta on the host to log L on the host, new vectors every "
+                               "step", "batch": B,
                    "model_box": [int(blo[0]), int(blo[1]), int(bhi[0]), int(bhi[1])]},
         "stages_ms_per_step": dict(zip(names, [round(float(x), 4) for x in stage])),
-        "stages_note": "whole batch, stages in sequence on one stream (the timed steps run two halves on two "
-                       "streams)",
-        "galmod_mc_ms_per_step": round(float(mc_ms), 4),
+        "stages_note": "one batch, stages in sequence on one stream (the timed steps run two halves on two streams)",
         "roofline": {"bound": "hbm", "kernel": "k_an_tile", "kernel_ms": round(tile_ms, 4), "achieved": ach / 1e9,
                      "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": ach / PEAK_HBM, "traffic": traffic,
-                     "traffic_unit": "bytes per launch", "algorithmic_bytes": 4 * nvox * B},
+                     "traffic_unit": "bytes per launch", "algorithmic_bytes": 4 * nvox * B,
+                     "limiter": "LDS atomics and memory latency, not HBM (lds gives the LDS fraction)", "lds": lds},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample",
-                                                                      "calibration")},
-        "loglike_sample": [float(x) for x in ll],
+                                                                      "calibration", "cpu_model", "cpu_count")},
+        "loglike_sample": [float(x) for x in ll[:4]],
+        "map_pixels_per_eval": nmap,
     }
+    if mc is not None:
+        out["mc_likelihood"] = mc
     if cpu is not None:
         out["gpu_over_cpu"] = value / cpu["value"]
     if not a.no_bayes:
@@ -325,6 +277,29 @@ def main():
     if not a.no_multi:
         out["multi_galaxy"] = multi_galaxy_bench()
     print(json.dumps(out), flush=True)
+
+
+def mc_likelihood(wl, steps, B, n=3):
+    """The same cube, vectors and free parameters with the Monte-Carlo model (GALMODTYPE MC, counter-based stream):
+    evaluations per second of the likelihood from theta to log L (ring maps and profile as ANALYTIC, MC deposit)."""
+    import torch
+    from bbarolo2_amd.engine import DiskModel
+    from bbarolo2_amd.galmod import GalmodPar
+    from bbarolo2_amd.likelihood import GpuLikelihood
+    par = GalmodPar(**{k: getattr(wl["par"], k) for k in wl["par"].__dataclass_fields__})
+    par.GALMODTYPE = "MC"
+    dm = DiskModel(wl["h"], par)
+    like = GpuLikelihood(dm, wl["data"], wl["truth"], wl["free"], likemask=wl["likemask"], region=wl["region"],
+                         likebox=wl["box"], sigma=wl["sigma"], likefactor=wl["likefactor"], chunk=B)
+    like.loglike(steps[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(n):
+        ll = like.loglike(steps[k % len(steps)])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"evals_per_s": B * n / dt, "ms_per_step": dt * 1e3 / n, "steps": n, "finite": bool(np.all(np.isfinite(ll))),
+            "mc_retries": like.info()["mc_retries"]}
 
 
 def multi_galaxy_bench(n=64, B=32, steps=3, seed=11):
@@ -351,13 +326,16 @@ def multi_galaxy_bench(n=64, B=32, steps=3, seed=11):
         step_ms.append(ms)
         done.append(d)
         host.append(hm)
-    # the same step as one captured HIP graph (captured on its first step, replayed with new tables)
+    # the same step with every galaxy's chain replayed as a captured HIP graph (eager, captured, then replayed)
     from bbarolo2_amd.multi import GalaxyBatch
     gg = GalaxyBatch(gb.likes, n_streams=len(gb.streams), graphs=True)
+    gg.graph_step(thetas)
     ref = gg.graph_step(thetas)
     if not all(np.array_equal(a, b) for a, b in zip(ref, ll)):
         raise RuntimeError("multi-galaxy batch: the graph replay differs from the eager step")
     graph_ms = float(np.median([gg.timed_graph_step(thetas)[1] for _ in range(steps)]))
+    for l in gb.likes:
+        l.set_graphs(False)
     alone = gb.alone_ms(thetas)
     ms = float(np.median(step_ms))
     d = np.median(np.array(done), axis=0)
@@ -369,7 +347,8 @@ def multi_galaxy_bench(n=64, B=32, steps=3, seed=11):
           f"sum of alone {alone.sum():.1f} ms", file=sys.stderr, flush=True)
     return {"workload": f"{n} galaxies (seed {seed}), cube sides {min(f['h'].DimAx(0) for f in fits)}-"
                         f"{max(f['h'].DimAx(0) for f in fits)} px, {sum(vox) / 1e6:.1f} M voxels in all; GLOBAL "
-                        f"likelihood, {B} vectors per galaxy per step, four streams; also as one captured HIP graph",
+                        f"likelihood, {B} vectors per galaxy per step, four streams; also with every galaxy's chain as a "
+                        f"captured HIP graph",
             "evals_per_s": n * B / ms * 1e3, "ms_per_step": round(ms, 3), "steps": steps,
             "host_launch_ms": round(float(np.median(host)), 3),
             "graph_ms_per_step": round(graph_ms, 3), "graph_evals_per_s": n * B / graph_ms * 1e3,

@@ -241,6 +241,117 @@ int bb_dilate_mask(const uint8_t* mask, int nz, int ny, int nx, double radius, i
                    uint8_t* out, void* stream);
 int bb_dilate_rmax(void);
 
+/* ---- the fused likelihood of a fit --------------------------------------------------------------------------------
+ * Bayesian3DFIT._log_likelihood (bbarolo/bayesian/likelihood.py:567-619) of batches of parameter vectors, run by the
+ * library from theta to log L: the seam that nautilus's vectorized sampler calls (bayesian/sampling.py:235-241). Per
+ * vector: the free values into the float32 ring table (_update_rings, bayesian/fitter.py:1109-1136), zero likelihood
+ * for a negative positive-only value (likelihood.py:537-559), the sub-ring tables (GalmodSetup.subrings,
+ * galmod.py:761-851) built on the device, the deconvolved density profile when DENS is not fitted and the geometry is
+ * free (bayesian/density.py:13-88), the model in its own box (likelihood.py:429-435), the beam convolution fused with
+ * the masked residual, the squared data of the region outside the box (likelihood.py:437-440), and
+ *     log L = -likefactor [(S + S_out) / (2 sigma^2) + nvox ln(2 pi sigma^2) / 2]        (+ the fracerr terms).
+ * Ring keys (the rows of the ring table, in the reference's units: km/s, arcsec, degrees; SIGMA and FRACERR are
+ * single values of theta): */
+#define BB_NKEY 18
+enum {
+    BB_K_VROT = 0, BB_K_VRAD, BB_K_VVERT, BB_K_VDISP, BB_K_DENS, BB_K_Z0, BB_K_DVDZ, BB_K_ZCYL, BB_K_INC, BB_K_PHI,
+    BB_K_XPOS, BB_K_YPOS, BB_K_VSYS, BB_K_FTHICK, BB_K_Z0THICK, BB_K_VDISPTHICK, BB_K_SIGMA, BB_K_FRACERR
+};
+
+/* The sub-ring plan of a set of fixed radii (the part of GalmodSetup.subrings that does not depend on the ring values,
+ * galmod.py:805-832, with the ring convention of model_rings, galmod.py:1066-1119): HOST arrays. Model ring m takes the
+ * value of key k from fit ring ridx[m] when bit k of from_fit is set, else base[k][m]. Sub-ring s interpolates between
+ * model rings idx[s]-1 and idx[s] at dr[s] arcmin from the first (cst[s]: the values of model ring 0); rads[s] is its
+ * float32-accumulated radius. densw (nsub, may be NULL) multiplies the density (the disk edge); side (2, nsub_total)
+ * are the weights of the receding and approaching halves; thick: a second, thick layer (nsub_total = 2 nsub). */
+typedef struct bb_ring_plan {
+    int32_t nmr, nsub, thick;
+    uint32_t from_fit;
+    const int32_t* ridx;
+    const double* rad;
+    const double* base;
+    const int32_t* idx;
+    const double* dr;
+    const uint8_t* cst;
+    const double* rads;
+    const double* densw;
+    const double* side;
+} bb_ring_plan;
+
+typedef struct bb_like_desc {
+    int32_t nr;          /* rings of the fit */
+    int32_t ndim;        /* length of a parameter vector */
+    int32_t chunk;       /* most vectors of one launch (device buffers are sized for it) */
+    int32_t mc;          /* 1: the Monte-Carlo model (GALMODTYPE MC), 0: ANALYTIC */
+    int32_t nvpar;       /* MC: sub-clouds per cloud */
+    uint32_t seed;       /* MC: the seed of the counter-based stream */
+    int32_t box_free;    /* 1: each vector has its own model box (XPOS, YPOS or Z0 free) */
+    int32_t halves;      /* 1: the two halves of a launch on two streams */
+    bb_box likebox;      /* the box of the likelihood region (and of the profile's maps) */
+    bb_box fixbox;       /* box_free = 0: the model box of every vector (within likebox) */
+    double sigma;        /* noise per voxel (theta's SIGMA when free) */
+    double likefactor;   /* 1 / correlation volume of the noise */
+    double eps;          /* fixed fractional model error (theta's FRACERR when free; 0: none) */
+    double ext;          /* arcsec: max(last radius + RADSEP / 2, outer edge of the disk) (galmod.py:1231-1233) */
+    double pixas0, pixas1;   /* |CDELT1|, |CDELT2| in arcsec */
+    int32_t nx, ny;      /* the cube */
+    int64_t mc_cap;      /* MC: first bound of the clouds of one vector in its box (grown when exceeded) */
+} bb_like_desc;
+
+/* The deconvolved profile of every vector (NULL: DENS fitted, or the profile of the initial rings in the ring table):
+ * the profile object's description and HOST arrays as bb_profile_create, the ring weights of the model sub-rings and
+ * of the exterior annulus (as bb_ring_maps_deposit) and the fixed point of the maps. */
+typedef struct bb_like_profile {
+    bb_profile_desc d;
+    const double* wpix;
+    const double* ypix;
+    const double* lcoef;
+    const int32_t* lcol;
+    const int32_t* near;
+    const int32_t* widx;
+    const double* wval;
+    const int32_t* widx_e;
+    const double* wval_e;
+    double qunit, unit;
+} bb_like_profile;
+
+/* The context of a fit on the current device (Bayesian3DFIT's set-up, likelihood.py:83-440): m the model (its grid,
+ * beam and vertical-sample capacity), d, data (nz, ny, nx) float32, likemask (nz, ny, nx) uint8 (NULL: all), region
+ * (ny, nx) uint8 (NULL: all); rings (BB_NKEY, nr) the float32 ring table; free_off[k] the offset of key k in theta
+ * (-1: not free), free_ring[k] 1 for one value per ring; plan the model's sub-ring plan, annulus that of the exterior
+ * annulus (needed with prof). All HOST arrays, copied. A voxel enters the likelihood if it is in the likemask and the
+ * region and its data are finite and non-zero. */
+typedef struct bb_ctx bb_ctx;
+int bb_ctx_create(const bb_model* m, const bb_like_desc* d, const float* data, const uint8_t* likemask,
+                  const uint8_t* region, const double* rings, const int32_t* free_off, const int32_t* free_ring,
+                  const bb_ring_plan* plan, const bb_ring_plan* annulus, const bb_like_profile* prof, bb_ctx** out);
+int bb_ctx_destroy(bb_ctx* c);
+/* The log-likelihoods of B (<= chunk) parameter vectors theta (HOST, B x ndim), launched on stream (returns at once)
+ * and collected: loglike (HOST, B; -inf where the likelihood is zero: a negative positive-only value, FTHICK > 1,
+ * SIGMA <= 0, a profile that did not converge), dens (HOST, B x nr float32 densities of the profile, may be NULL).
+ * One launch at a time per context. */
+int bb_loglike_launch(bb_ctx* c, const double* theta, int B, void* stream);
+int bb_loglike_collect(bb_ctx* c, double* loglike, float* dens);
+/* launch + collect over chunks of any B */
+int bb_loglike_batch(bb_ctx* c, const double* theta, int B, double* loglike, float* dens, void* stream);
+/* The sub-ring tables of B (<= chunk) vectors as the context builds them on the device: sr (B, 16, nsub_total) and
+ * sr_ext (B, 16, nsub of the annulus; may be NULL), HOST arrays; synchronous. */
+int bb_ctx_tables(bb_ctx* c, const double* theta, int B, double* sr, double* sr_ext);
+/* nvox and the squared data of the region, the deposit box [blo0, blo1, bhi0, bhi1), the vectors whose profile did not
+ * converge and the MC launches repeated with larger buffers (any may be NULL). */
+int bb_ctx_info(const bb_ctx* c, double* nvox, double* ss_total, int32_t* deposit_box, int64_t* nnls_fail,
+                int64_t* mc_retries);
+/* 0: launches run on the caller's stream only (no halves on two streams). */
+int bb_ctx_set_halves(bb_ctx* c, int halves);
+/* 1: the device work of a launch runs as a captured HIP graph per batch size (the first launch of a size eagerly, the
+ * second captured, later ones replayed with the new vectors staged in the pinned buffer the graph copies from); 0:
+ * eager launches. bb_ctx_graph_replays counts the replays. */
+int bb_ctx_set_graphs(bb_ctx* c, int on);
+int64_t bb_ctx_graph_replays(const bb_ctx* c);
+/* Measurement, no counterpart in the reference: 5 hipEvent_t that eager launches record on the stream of their first
+ * part before the ring maps, after them, after the profile, after the deposit and after the residual (NULL: none). */
+int bb_ctx_trace(bb_ctx* c, void* const* ev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,7 +9,8 @@ path). The FP64 log-likelihood of one parameter vector, restated from Bayesian3D
   * DENS not fitted: the deconvolved density profile of the vector (bayesian/density.py:13-88 through
     oracle.profile), the densities float32 in the ring table;
   * the model box of the vector within the likelihood box (likelihood.py:429-435, DiskModel.box galmod.py:1224-1240);
-  * the ANALYTIC model of the sub-rings (GalmodSetup.subrings per vector, orc_galmod_analytic), the 'same' beam
+  * the ANALYTIC model of the sub-rings (GalmodSetup.subrings per vector, orc_galmod_analytic; or the counter-based MC
+    model, orc_galmod_mc, when the GalmodPar asks for it), the 'same' beam
     convolution with the 1e-12 cut and the flux scale (orc_conv_same, or the FFT form);
   * the masked residual in the box (orc_sumsq_residuals / orc_residuals_fracerr) plus the squared data of the
     region outside the box from a summed-area table (likelihood.py:437-440), and
@@ -30,7 +31,7 @@ import numpy as np
 
 import oracle
 from oracle import profile as op
-from bbarolo2_amd.galmod import GalmodSetup, model_rings, model_box, nzq_for, zstep_for, PHI_TAB
+from bbarolo2_amd.galmod import GalmodSetup, model_rings, model_box, nzq_for, zstep_for, seed32, PHI_TAB
 from bbarolo2_amd.smooth import beam_kernel
 
 # ring parameters that make the likelihood zero when negative (likelihood.py:526-535)
@@ -54,6 +55,7 @@ class Likelihood:
                  maskmode=2, smooth=0.3, likefactor2d=None, ringmode="CENTRED", fft=False):
         self.h = h
         self.st = GalmodSetup(h, par)
+        self.seed = seed32(par.ISEED)
         self.ringmode = ringmode
         kb = beam_kernel(h) if h.Bmaj() > 0 else None
         self.kern = None if kb is None else kb[0]
@@ -145,8 +147,11 @@ class Likelihood:
 
     def model(self, rings, blo, bhi):
         """The convolved ANALYTIC model (by, bx, nz) of the rings in the box."""
-        sr, _ = self.st.subrings(model_rings(rings, self.ringmode, self.st.radsep))
-        m = oracle.galmod_analytic(self.st, sr, blo, bhi, nzq_for(sr), PHI_TAB)
+        sr, nv = self.st.subrings(model_rings(rings, self.ringmode, self.st.radsep))
+        if self.st.galmodtype == "MC":
+            m = oracle.galmod_mc(self.st, sr, nv, blo, bhi, self.seed)[1]
+        else:
+            m = oracle.galmod_analytic(self.st, sr, blo, bhi, nzq_for(sr), PHI_TAB)
         if self.kern is not None:
             conv = oracle.conv_fft if self.fft else oracle.conv_same
             m = conv(m, self.kern, self.scale)

@@ -93,3 +93,23 @@ def test_argument_errors_of_the_mask_dilation():
     assert L.bb_dilate_mask(p, 4, 8, 8, 2.0, 1, C.c_void_p(96), p, None) != 0 and b"alias" in L.bb_last_error()
     assert L.bb_dilate_mask(None, 4, 8, 8, 2.0, 1, C.c_void_p(96), C.c_void_p(128), None) != 0
     assert L.bb_dilate_mask(None, 0, 8, 8, 2.0, 1, None, None, None) == 0          # (an empty cube: nothing to do)
+
+
+def test_argument_errors_of_the_likelihood_context():
+    L = _lib.lib()
+    out = C.c_void_p()
+    d = _lib.bb_like_desc()
+    # NULL handles and arguments are rejected before any device call
+    assert L.bb_ctx_create(None, C.byref(d), None, None, None, None, None, None, None, None, None, None) != 0
+    assert L.bb_ctx_create(None, C.byref(d), None, None, None, None, None, None, None, None, None,
+                           C.byref(out)) != 0 and b"NULL" in L.bb_last_error() and not out.value
+    assert L.bb_loglike_launch(None, None, 1, None) != 0
+    assert L.bb_loglike_collect(None, None, None) != 0
+    assert L.bb_loglike_batch(None, None, 1, None, None, None) != 0
+    assert L.bb_ctx_tables(None, None, 1, None, None) != 0
+    assert L.bb_ctx_info(None, None, None, None, None, None) != 0
+    assert L.bb_ctx_set_halves(None, 0) != 0 and L.bb_ctx_set_graphs(None, 1) != 0
+    assert L.bb_ctx_graph_replays(None) < 0
+    assert L.bb_ctx_trace(None, None) != 0
+    assert L.bb_ctx_destroy(None) == 0
+    assert L.bb_model_set_bin(None, 4) != 0

@@ -1,6 +1,7 @@
-"""GPU (MI355X): likelihood batches replayed as a captured HIP graph give the eager results, bit for bit, on every
-replay, with other eager work and synchronisations between the replays (the 64x64x40 configuration: ring maps,
-density profile, ANALYTIC deposit, beam, residual; two parts on two streams)."""
+"""GPU (MI355X): likelihood batches replayed as a captured HIP graph (bb_ctx_set_graphs: the first batch of a size
+eager, the second captured, later ones replayed) give the eager results, bit for bit, on every replay, with other
+eager work and synchronisations between the replays (the 64x64x40 configuration: device tables, ring maps, density
+profile, ANALYTIC deposit, beam, residual; two halves on two streams)."""
 import numpy as np
 import pytest
 
@@ -27,7 +28,7 @@ def test_graph_replays_match_eager():
         # eager work in between
         torch.ones(1 << 20, device="cuda").sum().item()
         torch.cuda.synchronize()
-    assert len(graph["like"]._graphs) == 1
+    assert graph["like"].graph_replays() == 3
 
 
 def test_graph_replays_with_alternating_batch_sizes():
@@ -46,4 +47,26 @@ def test_graph_replays_with_alternating_batch_sizes():
         lg = graph["like"].loglike(theta)
         print(f"batch {rep} ({n} rows): max |eager - graph| = {np.max(np.abs(le - lg)):.3e}", flush=True)
         assert np.array_equal(le, lg), rep
-    assert len(graph["like"]._graphs) == 2
+    assert graph["like"].graph_replays() == 3
+
+
+def test_graph_replays_mc():
+    """The MC model's chain (its buffers cleared by kernels, not memsets) replayed as a graph, bit for bit."""
+    import torch
+    from bbarolo2_amd.bayes import fit_of
+    from bbarolo2_amd.synth import config
+    torch.cuda.set_device(0)
+    h, truth, par, noise, seed, spec, desc = config(5)
+    par.GALMODTYPE = "MC"
+    cfg = (h, truth, par, noise, seed, [("vrot", "ring"), ("vdisp", "ring")], desc)
+    eager = fit_of(cfg, chunk=32)
+    graph = fit_of(cfg, chunk=32, graphs=True)
+    lo, hi = eager["lo"], eager["hi"]
+    rng = np.random.default_rng(6)
+    for rep in range(4):
+        theta = lo + rng.random((32, len(lo))) * (hi - lo)
+        le = eager["like"].loglike(theta)
+        lg = graph["like"].loglike(theta)
+        print(f"MC replay {rep}: max |eager - graph| = {np.max(np.abs(le - lg)):.3e}", flush=True)
+        assert np.array_equal(le, lg), rep
+    assert graph["like"].graph_replays() == 3
