@@ -88,6 +88,9 @@ _SIGS = {
     "bb_dilate_rmax": (C.c_int, []),
     "bb_conv_residual_gauss": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _P, C.c_int64,
                                          _P, _P, _P]),
+    "bb_conv_residual_noalloc_max": (C.c_int64, []),
+    "bb_conv_residual_gauss_noalloc": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _P,
+                                                 _P, _P]),
     "bb_mc_work_size": (C.c_int64, [_P, bb_box, C.c_int, C.c_int, C.c_int64]),
     "bb_galmod_mc": (C.c_int, [_P, bb_box, _P, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int64, _P, C.c_int64, _P,
                                _P, _P, C.c_int, _P]),

@@ -12,6 +12,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
+#include "../../include/bbgpu.h"
+#include "bb_internal.h"
+
 #define BB_CV_RY 4
 #define BB_CV_RX 4
 #define BB_CV_THREADS 256
@@ -277,4 +282,65 @@ extern "C" int bb_launch_conv2d_same(const float* in, float* out, int B, int by,
         default: return (int)hipErrorInvalidValue;
     }
     return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// The fused convolution-residual without a caller's work buffer: the per-block partial sums live in a file-scope
+// device array of BB_CR_SCRATCH doubles (8 MiB; each device that runs it holds its own copy until the process
+// ends). Its address on a device comes from hipGetSymbolAddress on the first call with that device current; host
+// code passes that address, never the array's name, to the kernels. Every block writes its own partial pair, so
+// nothing needs clearing; calls that overlap on one device (two streams, two threads) share the array and race.
+#define BB_CR_SCRATCH (1 << 20)
+#define BB_CR_MAX_DEVICES 64
+
+static __device__ __align__(256) double scratch[BB_CR_SCRATCH];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[BB_CR_MAX_DEVICES] = {nullptr};
+
+static int scratch_ptr(double** out) {
+    int dev = -1;
+    int rc = bb_hip_check(hipGetDevice(&dev), "bb_conv_residual_gauss_noalloc: hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= BB_CR_MAX_DEVICES)
+        return bb_fail(BB_ERR_ARG, "bb_conv_residual_gauss_noalloc: device %d beyond %d", dev, BB_CR_MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        rc = bb_hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                          "bb_conv_residual_gauss_noalloc: hipGetSymbolAddress");
+        if (rc) return rc;
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+extern "C" int64_t bb_conv_residual_noalloc_max(void) { return BB_CR_SCRATCH; }
+
+extern "C" int bb_conv_residual_gauss_noalloc(const bb_model* m, const float* dep, const float* dvalid, int B, int by,
+                                              int bx, double sigma2, double eps2, double* out_ss, double* out_lg,
+                                              void* stream) {
+    if (!m) return bb_fail(BB_ERR_ARG, "bb_conv_residual_gauss_noalloc: model is NULL");
+    if (!m->d_kern) return bb_fail(BB_ERR_ARG, "bb_conv_residual_gauss_noalloc: the model has no beam kernel");
+    if (B < 0 || by < 1 || bx < 1 || B > 65535) return bb_fail(BB_ERR_ARG, "bb_conv_residual_gauss_noalloc: bad sizes");
+    if (B == 0) return 0;
+    if (!dep || !dvalid || !out_ss) return bb_fail(BB_ERR_ARG, "bb_conv_residual_gauss_noalloc: NULL array");
+    if (eps2 > 0 && (!out_lg || !(sigma2 > 0)))
+        return bb_fail(BB_ERR_ARG, "bb_conv_residual_gauss_noalloc: fracerr needs sigma2 > 0 and out_lg");
+    const int64_t nblk = bb_conv_resid_blocks(by, bx, m->g.nz);
+    const int64_t need = (int64_t)B * nblk * 2;
+    if (need > BB_CR_SCRATCH)
+        return bb_fail(BB_ERR_ARG,
+                       "bb_conv_residual_gauss_noalloc: %d models of %d x %d x %d need %lld partial sums, the scratch "
+                       "holds %d (use bb_conv_residual_gauss with a work buffer, or smaller batches)",
+                       B, by, bx, m->g.nz, (long long)need, BB_CR_SCRATCH);
+    double* work = nullptr;
+    int rc = scratch_ptr(&work);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    rc = bb_launch_conv_resid(dep, dvalid, B, by, bx, m->g.nz, m->d_kern, m->ky, m->kx, (float)m->scale, sigma2, eps2,
+                              work, nullptr, nullptr, 0, st);
+    if (!rc) rc = bb_launch_residual_final(work, B, (int)nblk, out_ss, out_lg, st);
+    return rc ? bb_fail(BB_ERR_HIP, "bb_conv_residual_gauss_noalloc launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }

@@ -113,6 +113,14 @@ int64_t bb_conv_residual_work_size(const bb_model* m, int B, int by, int bx);
 int bb_conv_residual_gauss(const bb_model* m, const float* dep, const float* dvalid, int B, int by, int bx,
                            double sigma2, double eps2, double* work, int64_t work_len, double* out_ss, double* out_lg,
                            void* stream);
+/* bb_conv_residual_gauss without a work buffer and without any allocation: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops). Its partial sums live in a
+ * static device array of bb_conv_residual_noalloc_max() doubles (8 MiB), one per device, held until the process ends;
+ * a call needing more (B x blocks x 2, see bb_conv_residual_work_size) is rejected. Calls that overlap on one device
+ * share that array and race. Same results as bb_conv_residual_gauss, bit for bit. */
+int64_t bb_conv_residual_noalloc_max(void);
+int bb_conv_residual_gauss_noalloc(const bb_model* m, const float* dep, const float* dvalid, int B, int by, int bx,
+                                   double sigma2, double eps2, double* out_ss, double* out_lg, void* stream);
 
 /* Monte-Carlo GALMOD (_galmod_mc, bbarolo/galmod.py:92-171) of B sub-ring tables sr (B, 16, nsub) with a
  * counter-based stream: the draws of cloud ic of sub-ring s are Philox4x32-10 of counters (ic, block) under the
