@@ -103,8 +103,7 @@ __device__ double interp_row(const PlanDev& P, const FitDev& F, const double* th
 }
 
 // One thread per (sub-ring, vector): the 16 rows of the table (B, 16, nsub_t) of GalmodSetup.subrings
-// (galmod.py:761-851, with _finish_subrings: the edge weights of the density, the thin and thick layers, the side
-// weights).
+// (galmod.py:761-851: the edge weights of the density, the thin and thick layers, the side weights, galmod.py:833-851).
 __global__ void __launch_bounds__(64)
 k_like_tables(PlanDev P, FitDev F, const double* __restrict__ theta, double* __restrict__ srall) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;

@@ -69,168 +69,197 @@ def seed32(iseed):
     return int(iseed) & 0xFFFFFFFF
 
 
+# ---------------------------------------------------------------------------------------------------------------------
+# Header-derived constants (row A1)
+
+# sky-axis units -> arcmin: the first tag found in CUNIT decides
+_SKY_ARCMIN = (("deg", 60.), ("arcs", 1. / 60.), ("arcm", 1.))
+_FREQ_HZ = {"hz": 1, "khz": 1E03, "mhz": 1E06, "ghz": 1E09}
+_VELSYS = {"radio": 1, "optical": 2, "relativistic": 3}
+_FWHM_TO_SIGMA = 1. / (2 * math.sqrt(2 * math.log(2)))
+# Rayleigh-Jeans conversion of HI column density to surface brightness per axis type: (Boltzmann / Jy, the line
+# constant of the axis); the beam-area factor is per square arcminute (galmod.py:727-741)
+_RJ = (1.3806488E-23 / 1E-26)
+_NHI_CONST = {2: 2.588E04, 3: 3.8475E-06, 4: 1.823E-05}
+_ARCMIN2_SR = (math.pi / (180.0 * 60.0)) ** 2
+
+
+def _sky_arcmin(cunit):
+    cu = cunit.lower()
+    for tag, f in _SKY_ARCMIN:
+        if tag in cu:
+            return f
+    raise GalmodError(f"GALMOD error (unknown CUNIT for RA-DEC): cannot convert to ARCMIN.\n{cu}")
+
+
+def _line_weights(relint, nlines):
+    if len(relint) != nlines:
+        raise GalmodError(f"GALMOD ERROR: {nlines} lines, but RELINT has {len(relint)} values.")
+    return np.array(relint, float)
+
+
+class SpectralAxis:
+    """The third axis of a cube in SI units (m, Hz or m/s) with its reference frequency, the lines of the model
+    (reference frequency and relative intensity of each) and the velocity convention (galmod.py:640-712).
+
+    axtyp: 2 wavelength, 3 frequency, 4 velocity; velsys: 1 radio, 2 optical, 3 relativistic."""
+
+    def __init__(self, h, gm):
+        unit = h.Cunit(2).lower()
+        kind = h.SpectralType()
+        z1 = 1 + float(gm.REDSHIFT)
+        self.velsys = _VELSYS.get(h.VelDef(), 1)
+        self.relint = np.array([1.])
+        lines = None
+        crval, cdelt = h.Crval(2), h.Cdelt(2)
+        if kind == "wave":
+            if unit not in WAVE_UNITS:
+                raise GalmodError(f"GALMOD ERROR: unknown CUNIT3={unit}. Cannot convert to meters.")
+            self.axtyp = 2
+            to_si = 1. / WAVE_UNITS[unit]
+            crval, cdelt = crval * to_si, cdelt * to_si
+            rest = gm.RESTWAVE
+            self.freq0 = C_MS / (rest[0] * z1 * to_si) if rest[0] != -1 else C_MS / crval
+            if len(rest) > 1:
+                lines = C_MS / (np.asarray(rest, float) * z1 * to_si)
+                self.relint = _line_weights(gm.RELINT, len(rest))
+        elif kind == "freq":
+            if unit not in _FREQ_HZ:
+                raise GalmodError(f"GALMOD ERROR: unknown CUNIT3={unit}. Cannot convert to Hz.")
+            self.axtyp = 3
+            to_si = _FREQ_HZ[unit]
+            crval, cdelt = crval * to_si, cdelt * to_si
+            self.freq0 = h.Freq0() / z1 or HI_FREQ
+            if len(gm.RESTFREQ) > 1:
+                lines = np.asarray(gm.RESTFREQ, float) / z1
+                self.relint = _line_weights(gm.RELINT, len(gm.RESTFREQ))
+        elif "velo" in kind:
+            if unit not in VEL_UNITS:
+                raise GalmodError(f"GALMOD ERROR: unknown CUNIT3={unit}. Cannot convert to m/s.")
+            self.axtyp = 4
+            to_si = 1.0E03 / VEL_UNITS[unit]
+            self.freq0 = h.Freq0() / z1 or HI_FREQ
+            crval, cdelt = crval * to_si, cdelt * to_si
+        else:
+            raise GalmodError("GALMOD ERROR: unknown Spectral axis type. Check CTYPE3 and CUNIT3 keywords.")
+        self.crval, self.cdelt, self.crpix = crval, cdelt, h.Crpix(2)
+        self.lines = np.ascontiguousarray([self.freq0] if lines is None else lines, dtype=np.float64)
+        self.relint = np.ascontiguousarray(self.relint, dtype=np.float64)
+
+    def world(self, pix):
+        """The world coordinate (SI) at 0-based pixel positions."""
+        return (np.asarray(pix, float) + 1 - self.crpix) * self.cdelt + self.crval
+
+    def velocity(self, world, freq0):
+        """Velocity (m/s) of world coordinates for a line of reference frequency freq0 (the inverse of _velgrid)."""
+        if self.axtyp == 4:
+            return np.ascontiguousarray(world, dtype=np.float64)
+        ratio = (C_MS / world if self.axtyp == 2 else world) / freq0
+        if self.velsys == 1:
+            v = C_MS * (1 - ratio)
+        elif self.velsys == 2:
+            v = C_MS * (1 / ratio - 1)
+        else:
+            v = C_MS * (1 - ratio * ratio) / (1 + ratio * ratio)
+        return np.ascontiguousarray(v, dtype=np.float64)
+
+    def edges(self, nz, freq0=None):
+        """Velocity (m/s) of the nz + 1 channel boundaries (pixels k - 1/2) for the line freq0 (default the first)."""
+        return self.velocity(self.world(np.arange(nz + 1) - 0.5), self.freq0 if freq0 is None else freq0)
+
+    def column_density_factor(self, nz):
+        """Per channel: HI column density to Jy/beam per square arcminute, rounded to float32 (galmod.py:727-741)."""
+        w = self.world(np.arange(nz))
+        if self.axtyp == 2:
+            lam = w
+        elif self.axtyp == 3:
+            lam = C_MS / w
+        else:
+            if self.velsys == 2:
+                f = self.freq0 * (C_MS / (C_MS + w))
+            elif self.velsys == 3:
+                f = self.freq0 * np.sqrt((C_MS - w) / (C_MS + w))
+            else:
+                f = self.freq0 * (1 - w / C_MS)
+            lam = C_MS / f
+        per_sr = 2 * _RJ / (_NHI_CONST[self.axtyp] * lam * lam) / abs(self.cdelt)
+        return (per_sr * _ARCMIN2_SR).astype(np.float32).astype(np.float64)
+
+
 class GalmodSetup:
-    """Header-derived quantities of the models of a cube (row A1). Units: arcmin for the sky, m/s
-    for velocities, radians for angles."""
+    """Header-derived quantities of the models of a cube (row A1, GalmodSetup.__init__, galmod.py:613-726).
+    Units: arcmin for the sky, m/s for velocities, radians for angles."""
 
     def __init__(self, header, par=None, galmodtype=None, densflux=None):
         h = header
         gm = GalmodPar() if par is None else par
-        self.header = h
-        self.par = gm
+        self.header, self.par = h, gm
         self.nsubs = h.DimAx(2)
         self.dimx, self.dimy = h.DimAx(0), h.DimAx(1)
-        cdelt = [0., 0.]
-        for i in range(2):
-            cu = h.Cunit(i).lower()
-            if "deg" in cu: conv = 60.
-            elif "arcs" in cu: conv = 1 / 60.
-            elif "arcm" in cu: conv = 1.
-            else:
-                raise GalmodError(f"GALMOD error (unknown CUNIT for RA-DEC): cannot convert to ARCMIN.\n{cu}")
-            cdelt[i] = h.Cdelt(i) * conv
-        self.cdelt = cdelt
-        self.pixsize = [abs(cdelt[0]), abs(cdelt[1])]
+        # sky pixels
+        self.cdelt = [h.Cdelt(i) * _sky_arcmin(h.Cunit(i)) for i in (0, 1)]
+        self.pixsize = [abs(c) for c in self.cdelt]
         self.pixarea = self.pixsize[0] * self.pixsize[1]
         self.crota2 = h.Crota() * math.pi / 180.
+        self.radsep = 0.75 * min(self.pixsize)
         self.zstep = ZSTEP_FINE
-        reds = float(gm.REDSHIFT)
-        cunit3 = h.Cunit(2).lower()
-        self.cdelt3, self.crval3, self.crpix3 = h.Cdelt(2), h.Crval(2), h.Crpix(2)
-        self.velsys = {"radio": 1, "optical": 2, "relativistic": 3}.get(h.VelDef(), 1)
-        sptype = h.SpectralType()
-        lfreq0 = None
-        self.relint = np.array([1.])
-        self.freq0 = HI_FREQ
-        if sptype == "wave":
-            self.axtyp = 2
-            if cunit3 not in WAVE_UNITS:
-                raise GalmodError(f"GALMOD ERROR: unknown CUNIT3={cunit3}. Cannot convert to meters.")
-            mconv = 1. / WAVE_UNITS[cunit3]
-            self.crval3 *= mconv
-            self.cdelt3 *= mconv
-            restw = gm.RESTWAVE[0]
-            self.freq0 = C_MS / (restw * (1 + reds) * mconv) if restw != -1 else C_MS / self.crval3
-            if len(gm.RESTWAVE) > 1:
-                lfreq0 = C_MS / (np.asarray(gm.RESTWAVE, float) * (1 + reds) * mconv)
-                self.relint = self._relint(gm.RELINT, len(gm.RESTWAVE))
-        elif sptype == "freq":
-            self.axtyp = 3
-            hz = {"hz": 1, "khz": 1E03, "mhz": 1E06, "ghz": 1E09}.get(cunit3)
-            if hz is None:
-                raise GalmodError(f"GALMOD ERROR: unknown CUNIT3={cunit3}. Cannot convert to Hz.")
-            self.crval3 *= hz
-            self.cdelt3 *= hz
-            self.freq0 = h.Freq0() / (1 + reds) or HI_FREQ
-            if len(gm.RESTFREQ) > 1:
-                lfreq0 = np.asarray(gm.RESTFREQ, float) / (1 + reds)
-                self.relint = self._relint(gm.RELINT, len(gm.RESTFREQ))
-        elif "velo" in sptype:
-            self.axtyp = 4
-            if cunit3 not in VEL_UNITS:
-                raise GalmodError(f"GALMOD ERROR: unknown CUNIT3={cunit3}. Cannot convert to m/s.")
-            msconv = 1.0E03 / VEL_UNITS[cunit3]
-            self.freq0 = h.Freq0() / (1 + reds) or HI_FREQ
-            self.crval3 *= msconv
-            self.cdelt3 *= msconv
-        else:
-            raise GalmodError("GALMOD ERROR: unknown Spectral axis type. Check CTYPE3 and CUNIT3 keywords.")
-        nch = gm.SPECRES if gm.SPECRES != -1 else gm.LINEAR
-        if nch == -1:
-            nch = 1. / (2 * math.sqrt(2 * math.log(2)))
+        # spectral axis and lines
+        ax = SpectralAxis(h, gm)
+        self.axis = ax
+        self.axtyp, self.velsys, self.freq0 = ax.axtyp, ax.velsys, ax.freq0
+        self.crval3, self.cdelt3, self.crpix3 = ax.crval, ax.cdelt, ax.crpix
+        self.lfreq0, self.relint = ax.lines, ax.relint
+        # channel width and instrumental broadening (SPECRES, else LINEAR, else a channel of FWHM 1)
         self.chwidth = abs(delta_vel(h)) * 1000
-        self.sig_instr = nch * self.chwidth
+        res = next((v for v in (gm.SPECRES, gm.LINEAR) if v != -1), _FWHM_TO_SIGMA)
+        self.sig_instr = res * self.chwidth
+        # cloud and layer keywords (out-of-range CMODE / CDENS fall back to 1)
         self.ltype = int(gm.LTYPE)
         self.cmode = int(gm.CMODE) if 0 <= gm.CMODE <= 2 else 1
         self.cdens = float(gm.CDENS) if gm.CDENS >= 0 else 1.
-        self.iseed = gm.ISEED
-        self.empty = gm.EMPTY
-        self.radmaxa = float(gm.RADMAXA)
-        self.radmaxr = float(gm.RADMAXR)
+        self.iseed, self.empty, self.nvpar = gm.ISEED, gm.EMPTY, gm.NV
+        self.radmaxa, self.radmaxr = float(gm.RADMAXA), float(gm.RADMAXR)
         self.densIsFlux = gm.DENSFLUX if densflux is None else densflux
-        self.nvpar = gm.NV
         self.galmodtype = (galmodtype or gm.GALMODTYPE).upper()
-        self.radsep = 0.75 * min(self.pixsize)
+        # channel factors: flux per channel, or the column-density conversion
         if self.densIsFlux:
             self.cd2i = np.full(self.nsubs, np.float32(3600. / self.chwidth * 1000.), dtype=np.float64)
         else:
-            self.cd2i = self._nhi_to_rad()
-        self.lfreq0 = np.ascontiguousarray([self.freq0] if lfreq0 is None else lfreq0, dtype=np.float64)
-        self.relint = np.ascontiguousarray(self.relint, dtype=np.float64)
-        self.vedges = self._velocity_edges()
-        self.vedges_lines = np.ascontiguousarray([self._velocity_edges(f) for f in self.lfreq0])
+            self.cd2i = ax.column_density_factor(self.nsubs)
+        self.vedges = ax.edges(self.nsubs)
+        self.vedges_lines = np.ascontiguousarray([ax.edges(self.nsubs, f) for f in self.lfreq0])
         self.chmin = float(np.min(np.abs(np.diff(self.vedges)))) if self.nsubs > 0 else self.chwidth
-
-    @staticmethod
-    def _relint(relint, nlines):
-        if len(relint) != nlines:
-            raise GalmodError(f"GALMOD ERROR: {nlines} lines, but RELINT has {len(relint)} values.")
-        return np.array(relint, float)
-
-    def _nhi_to_rad(self):
-        """The channel factor of column-density models (galmod.py:727-741)."""
-        jy, K = 1E-26, 1.3806488E-23
-        isubs = np.arange(self.nsubs)
-        w = self.crval3 + self.cdelt3 * (isubs + 1 - self.crpix3)
-        if self.axtyp == 2:
-            lab, fac = w, 2.588E04
-        elif self.axtyp == 3:
-            lab, fac = C_MS / w, 3.8475E-06
-        else:
-            f = self.freq0 * (1 - w / C_MS)
-            if self.velsys == 2: f = self.freq0 * (C_MS / (C_MS + w))
-            if self.velsys == 3: f = self.freq0 * np.sqrt((C_MS - w) / (C_MS + w))
-            lab, fac = C_MS / f, 1.823E-05
-        ddum = 2 * (K / jy) / (fac * lab * lab) / abs(self.cdelt3)
-        return (ddum * (math.pi / (180.0 * 60.0)) ** 2).astype(np.float32).astype(np.float64)
-
-    def _velocity_edges(self, freq0=None):
-        """Velocity (m/s) of the channel boundaries, the inverse of the channel grid at k - 1/2
-        (galmod.py:743-758)."""
-        g = np.arange(self.nsubs + 1) - 0.5
-        w = (g + 1 - self.crpix3) * self.cdelt3 + self.crval3
-        if self.axtyp == 4:
-            return np.ascontiguousarray(w, dtype=np.float64)
-        f = C_MS / w if self.axtyp == 2 else w
-        r = f / (self.freq0 if freq0 is None else freq0)
-        if self.velsys == 1:
-            v = C_MS * (1 - r)
-        elif self.velsys == 2:
-            v = C_MS * (1 / r - 1)
-        else:
-            v = C_MS * (1 - r * r) / (1 + r * r)
-        return np.ascontiguousarray(v, dtype=np.float64)
 
     # ------------------------------------------------------------------
     def side_edges(self, rings):
-        """(edgeapp, edgerec) in arcsec, None for a half without an edge (galmod.py:853-863)."""
-        out = []
-        for attr, default in (("edgeapp", self.radmaxa), ("edgerec", self.radmaxr)):
-            e = getattr(rings, attr, None)
-            e = default if e is None else float(e)
-            out.append(e if e >= 0 else None)
-        return tuple(out)
+        """(approaching, receding) outer edges (arcsec) of the two halves of the disk: the rings' EDGEAPP / EDGEREC,
+        else RADMAXA / RADMAXR; None where there is none (negative) (galmod.py:853-863)."""
+        def edge(given, default):
+            e = default if given is None else float(given)
+            return e if e >= 0 else None
+        return (edge(getattr(rings, "edgeapp", None), self.radmaxa),
+                edge(getattr(rings, "edgerec", None), self.radmaxr))
 
     def subrings(self, rings):
         """Sub-rings 0.75 pixel wide interpolated from the rings (row A3, galmod.py:761-851).
         Returns (sr (16, nsub) float64, nv (nsub,) int64)."""
-        nr = rings.nr
         radsep = self.radsep
         rad = np.asarray(rings.radii, float) / 60.
-        for i in range(1, nr):
-            if rad[i - 1] + radsep >= rad[i]:
-                if rad[i - 1] > rad[i]:
-                    raise GalmodError("GALMOD ERROR: Radii not in increasing order.")
-                raise GalmodError("GALMOD ERROR: Radius separation too small.")
-        if np.any(np.asarray(rings.vdisp) < 0):
-            raise GalmodError("GALMOD ERROR: Negative velocity dispersion not allowed.")
-        if np.any(np.asarray(rings.dens) < 0):
-            raise GalmodError("GALMOD ERROR: Negative column-density or flux are not allowed.")
-        if np.any(np.asarray(rings.z0) < 0):
-            raise GalmodError("GALMOD ERROR: Negative scale height not allowed.")
+        gaps = rad[1:] - (rad[:-1] + radsep)
+        if np.any(gaps <= 0):
+            i = int(np.flatnonzero(gaps <= 0)[0]) + 1
+            raise GalmodError("GALMOD ERROR: Radii not in increasing order." if rad[i - 1] > rad[i]
+                              else "GALMOD ERROR: Radius separation too small.")
+        for key, what in (("vdisp", "Negative velocity dispersion not allowed."),
+                          ("dens", "Negative column-density or flux are not allowed."),
+                          ("z0", "Negative scale height not allowed.")):
+            if np.any(np.asarray(getattr(rings, key)) < 0):
+                raise GalmodError("GALMOD ERROR: " + what)
         thick = rings.thick()
-        if thick and np.any((np.asarray(rings.fthick) < 0) | (np.asarray(rings.fthick) > 1)):
-            raise GalmodError("GALMOD ERROR: The fraction of gas in the thick layer (FTHICK) must be 0 to 1.")
+        if thick:
+            f = np.asarray(rings.fthick)
+            if np.any((f < 0) | (f > 1)):
+                raise GalmodError("GALMOD ERROR: The fraction of gas in the thick layer (FTHICK) must be 0 to 1.")
         V = ring_values(rings, thick)
         empty = self.empty if rings.empty is None else rings.empty
         idx, dr, const, rads = subring_radii(rad, radsep, empty)
@@ -239,29 +268,26 @@ class GalmodSetup:
         sr[0] = rads
         if nsub:
             sr[1:] = interp_values(V, rad, idx, dr, const)
-        return self._finish_subrings(sr, rings, thick, nsub)
+        return self.finish_table(sr, rings, thick)
 
-    def _finish_subrings(self, sr, rings, thick, nsub):
+    def finish_table(self, sr, rings, thick):
+        """The 16-row table of interpolated values sr (14 rows, or 17 with a thick layer) with the disk edge applied to
+        the density, the thin and thick layers as two sets of sub-rings, and the weights of the two halves
+        (galmod.py:833-851); and the sub-clouds per sub-ring (NV, or the channels; 1 without dispersion)."""
         radsep = self.radsep
+        n = sr.shape[1]
         edge = getattr(rings, "edge", None)
-        if edge is not None and nsub:
+        if edge is not None and n:
             sr[5] *= edge_weights(sr[0] * 60., radsep * 60., float(edge), float(rings.edgewidth or 0.))
         if thick:
-            f = sr[14]
-            thin, thk = sr[:14].copy(), sr[:14].copy()
-            thin[5] *= 1. - f
-            thin[7] = 0.
-            thk[5] *= f
-            thk[6], thk[4] = sr[15], sr[16]
-            sr = np.concatenate([thin, thk], axis=1)
-        side = np.ones((2, sr.shape[1]))
-        for k, e in enumerate(self.side_edges(rings)[::-1]):        # (receding, approaching)
+            sr = split_layers(sr)
+        weights = np.ones((2, sr.shape[1]))
+        for row, e in zip((0, 1), self.side_edges(rings)[::-1]):          # row 0 receding, 1 approaching
             if e is not None and sr.shape[1]:
-                side[k] = edge_weights(sr[0] * 60., radsep * 60., e)
-        sr = np.ascontiguousarray(np.concatenate([sr[:14], side]))
-        nvtmp = self.nvpar if self.nvpar >= 1 else self.nsubs
-        nv = np.where(sr[4] == 0, 1, nvtmp).astype(np.int64)
-        return sr, nv
+                weights[row] = edge_weights(sr[0] * 60., radsep * 60., e)
+        out = np.ascontiguousarray(np.vstack([sr[:14], weights]))
+        per = self.nvpar if self.nvpar >= 1 else self.nsubs
+        return out, np.where(out[4] == 0, 1, per).astype(np.int64)
 
     def subcloud_flux(self, sr, nv):
         """Mean flux (before cd2i) of one MC sub-cloud over the sub-rings (galmod.py:879-897)."""
@@ -281,21 +307,48 @@ class GalmodSetup:
         return float(unit * np.sum(ri * ri) / np.sum(ri))
 
 
+def split_layers(sr):
+    """A 17-row table (rows 14-16: FTHICK, thick-layer height and dispersion) as the sub-rings of the thin layer
+    (density x (1 - FTHICK), no lag) followed by those of the thick one (density x FTHICK, its own height and
+    dispersion)."""
+    f = sr[14]
+    thin, thk = sr[:14].copy(), sr[:14].copy()
+    thin[5] *= 1. - f
+    thin[7] = 0.
+    thk[5] *= f
+    thk[6] = sr[15]
+    thk[4] = sr[16]
+    return np.concatenate([thin, thk], axis=1)
+
+
+def _thick_height(z0thick, z0):
+    """The thick layer's scale height: Z0THICK, or Z0 where Z0THICK is negative (tied to it)."""
+    return np.where(z0thick >= 0, z0thick, z0)
+
+
+# Rows 1-13 of a sub-ring table from the ring values in the reference's units: (key, conversion); rows 14-16 of a
+# thick layer follow (FTHICK, its height, its dispersion). galmod.py:793-804.
+TABLE_ROWS = (("vrot", lambda v: v * 1000.), ("vrad", lambda v: v * 1000.), ("vvert", lambda v: v * 1000.),
+              ("vdisp", lambda v: v * 1000.), ("dens", lambda v: v), ("z0", lambda v: v / 60.),
+              ("dvdz", lambda v: v * 1000 * 60), ("zcyl", lambda v: v / 60.), ("inc", np.radians),
+              ("phi", np.radians), ("xpos", lambda v: v), ("ypos", lambda v: v), ("vsys", lambda v: v * 1000.))
+
+
+def table_rows(get, thick):
+    """The rows 1.. of a sub-ring table (13, or 16 with a thick layer) from get(key) -> ring values (any shape)."""
+    rows = [conv(np.asarray(get(k), float)) for k, conv in TABLE_ROWS]
+    if thick:
+        vdt = np.asarray(get("vdispthick"), float)
+        vdt = np.where(vdt >= 0, vdt, np.asarray(get("vdisp"), float))
+        rows += [np.asarray(get("fthick"), float),
+                 _thick_height(np.asarray(get("z0thick"), float), np.asarray(get("z0"), float)) / 60., vdt * 1000.]
+    return rows
+
+
 def ring_values(rings, thick):
     """The ring values in internal units, rows 1..13 of the sub-ring table (+3 rows for a thick layer:
-    fthick, z0thick, vdispthick), galmod.py:793-804."""
-    V = np.vstack([np.asarray(rings.vrot) * 1000., np.asarray(rings.vrad) * 1000.,
-                   np.asarray(rings.vvert) * 1000., np.asarray(rings.vdisp) * 1000.,
-                   np.asarray(rings.dens, float), np.asarray(rings.z0) / 60.,
-                   np.asarray(rings.dvdz) * 1000 * 60, np.asarray(rings.zcyl) / 60.,
-                   np.radians(np.asarray(rings.inc, float)), np.radians(np.asarray(rings.phi, float)),
-                   np.asarray(rings.xpos, float), np.asarray(rings.ypos, float),
-                   np.asarray(rings.vsys) * 1000.]).astype(float)
-    if thick:
-        vdt = np.asarray(rings.vdispthick, float)
-        vdt = np.where(vdt >= 0, vdt, np.asarray(rings.vdisp, float))
-        V = np.vstack([V, np.asarray(rings.fthick, float), rings.thick_z0() / 60., vdt * 1000.])
-    return V
+    fthick, z0thick, vdispthick)."""
+    return np.vstack(table_rows(lambda k: getattr(rings, k), thick)).astype(float)
 
 
 def subring_radii(rad, radsep, empty):
@@ -502,15 +555,7 @@ class SubringPlan:
                     a = a[None]
                 return np.broadcast_to(a[:, ri], (B, nm))
             return np.broadcast_to(np.asarray(getattr(p, k), float)[None], (B, nm))
-        rows = [get("vrot") * 1000., get("vrad") * 1000., get("vvert") * 1000., get("vdisp") * 1000.,
-                get("dens"), get("z0") / 60., get("dvdz") * 1000 * 60, get("zcyl") / 60.,
-                np.radians(get("inc")), np.radians(get("phi")), get("xpos"), get("ypos"), get("vsys") * 1000.]
-        if self.thick:
-            z0t = get("z0thick")
-            z0t = np.where(z0t >= 0, z0t, get("z0"))
-            vdt = get("vdispthick")
-            vdt = np.where(vdt >= 0, vdt, get("vdisp"))
-            rows += [get("fthick"), z0t / 60., vdt * 1000.]
+        rows = [np.broadcast_to(r, (B, nm)) for r in table_rows(get, self.thick)]
         V = np.stack(rows, axis=1)                                  # (B, nrow, nr_model)
         sr = np.empty((B, V.shape[1] + 1, self.nsub))
         sr[:, 0] = self.rads
@@ -519,7 +564,7 @@ class SubringPlan:
         out = []
         nv = None
         for b in range(B) if (self.thick or self._edges()) else ():
-            s, nv = self.setup._finish_subrings(sr[b], p, self.thick, self.nsub)
+            s, nv = self.setup.finish_table(sr[b], p, self.thick)
             out.append(s)
         if out:
             return np.ascontiguousarray(np.stack(out)), nv
@@ -536,7 +581,7 @@ class SubringPlan:
 
     def weights(self):
         """(densw, side): the edge weights of the density of the nsub sub-rings of one layer (None without an edge
-        of the disk) and the weights (2, nsub_total) of the receding and approaching halves, as _finish_subrings
+        of the disk) and the weights (2, nsub_total) of the receding and approaching halves, as finish_table
         applies them (galmod.py:833-851)."""
         p = self.proto
         radsep = self.setup.radsep
