@@ -78,6 +78,10 @@ _SIGS = {
     "bb_azplan_destroy": (C.c_int, [_P]),
     "bb_analytic_work_size": (C.c_int64, [_P, C.c_int]),
     "bb_galmod_analytic": (C.c_int, [_P, _P, bb_box, _P, C.c_int, C.c_int, _P, C.c_int64, _P, C.c_int, _P]),
+    "bb_galmod_analytic_boxes": (C.c_int, [_P, _P, bb_box, _P, _P, C.c_int, C.c_int, _P, C.c_int64, _P, C.c_int,
+                                           _P]),
+    "bb_galmod_mc_boxes": (C.c_int, [_P, bb_box, _P, _P, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int64, _P,
+                                     C.c_int64, _P, _P, _P, C.c_int, _P]),
     "bb_trace_deposit": (C.c_int, [_P, _P]),
     "bb_conv2d_same": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, _P]),
     "bb_residual_work_size": (C.c_int64, [C.c_int, C.c_int64]),
@@ -97,6 +101,7 @@ _SIGS = {
     "bb_norm_work_size": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "bb_norm3dfit": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P,
                                _P, C.POINTER(bb_norm_par), _P, _P, _P, C.c_int, _P, _P, C.c_int64, _P, _P, _P, _P]),
+    "bb_local_norm": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P]),
     "bb_ring_maps_work_size": (C.c_int64, [_P, C.c_int]),
     "bb_ring_maps_deposit": (C.c_int, [_P, _P, bb_box, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, C.c_int,
                                        C.c_double, _P, C.c_int64, _P, _P]),

@@ -1,0 +1,144 @@
+"""3DFIT METHOD CLASSIC of a cube on the GPU: the first-stage fit of every ring and the final model
+(Galfit.galfit and _fit_one_ring, bbarolo/galfit.py:1097-1260; finalModel with LOCAL normalisation,
+bbarolo/galfit_out.py:128-159).
+
+A stage fits all the rings together (RingFit.fit): every ring's downhill simplex advances one step per round and the
+trial rings of all of them are modelled in one launch, each in its own box. As the reference does:
+  * the data and their spread are scaled by 10 / max(data) during the fit (NORMALCUBE, galfit.py:1172-1181);
+  * a ring with too few pixels in the mask is skipped, a fit that does not converge or whose VROT does not fit in the
+    band of the cube is not ok (galfit.py:1118-1142, _max_vrot galfit.py:1085-1093);
+  * after the last stage the rings that are not ok get DENS 0 (galfit.py:1250-1254);
+  * the final model is the model of the fitted rings with the ring convention, normalised LOCAL (each spectrum scaled
+    to the flux of the data in the mask, bb_local_norm) or not at all (NONE).
+Not built: the second stage (TWOSTAGE regularisation of the geometry, galfit.py:1296-), the REVERSE passes with the
+models of the surrounding rings (galfit.py:1231-1294) and the AZIM normalisation of the final model (it needs the
+radial profile of the data, Ellprof). A FREE with geometry and TWOSTAGE raises NotImplementedError.
+
+The mask: the reference builds it with its source finder (MASK SEARCH); search_mask is a simpler stand-in (the largest
+connected region above a growth threshold holding a voxel above the detection threshold).
+"""
+
+import math
+
+import numpy as np
+
+from . import _lib
+from .engine import DiskModel, _torch, _stream_ptr
+from .galfit import RingFit, Fit3DPar, free_flags, INC, PA, Z0, XPOS, YPOS, VSYS
+from .galmod import GalmodPar, model_rings
+
+
+def robust_spread(data):
+    """1.4826 x the median absolute deviation of the non-blank voxels (the cube's robust spread, stats.py:81-82)."""
+    v = np.asarray(data, np.float64)
+    v = v[np.isfinite(v) & (v != 0)]
+    if v.size == 0:
+        return 0.
+    return float(1.4826 * np.median(np.abs(v - np.median(v))))
+
+
+def search_mask(data, snr=5., grow=3.):
+    """A mask in place of the reference's source finder: the largest connected region (26-neighbours) of the voxels
+    above grow sigma that holds a voxel above snr sigma (sigma: the robust spread)."""
+    from scipy import ndimage
+    sig = robust_spread(data)
+    d = np.nan_to_num(np.asarray(data, np.float64))
+    lab, n = ndimage.label(d > grow * sig, structure=np.ones((3, 3, 3), bool))
+    if n == 0:
+        raise ValueError("MASKING error: No sources detected in the datacube. Cannot build mask!!!")
+    peaks = np.unique(lab[d > snr * sig])
+    peaks = peaks[peaks > 0]
+    if peaks.size == 0:
+        raise ValueError("MASKING error: No sources detected in the datacube. Cannot build mask!!!")
+    sizes = ndimage.sum(np.ones_like(lab), lab, index=peaks)
+    return lab == peaks[int(np.argmax(sizes))]
+
+
+class ClassicFit:
+    """3DFIT CLASSIC of the cube (h, data (nz, ny, nx) float32) from the initial rings inr.
+
+    par: Fit3DPar (FREE, FTYPE, WFUNC, NORM, ...); gpar: GalmodPar of the models (GALMODTYPE, LTYPE, ...); mask
+    (nz, ny, nx) bool (default search_mask); normalcube: scale the data during the fit; twostage: the reference's
+    TWOSTAGE (a second stage is not built); chan_noise (nz): the noise of each channel (default ones)."""
+
+    def __init__(self, h, data, inr, par=None, gpar=None, mask=None, normalcube=True, twostage=True, chan_noise=None):
+        self.torch = _torch()
+        self.h = h
+        self.par = par or Fit3DPar()
+        self.gpar = gpar or GalmodPar()
+        self.data = np.ascontiguousarray(data, np.float32)
+        R = np.asarray(inr.radii, float)
+        if np.any(np.diff(R) <= 0):
+            raise ValueError("3DFIT ERROR: Radii not in increasing order.")
+        if np.any(R < 0):
+            raise ValueError("3DFIT ERROR: Negative radius!!!")
+        self.inr = inr.copy()
+        self.mask = search_mask(self.data) if mask is None else np.asarray(mask, bool)
+        self.normalcube = bool(normalcube)
+        self.twostage = bool(twostage)
+        self.chan_noise = chan_noise
+        self.dm = DiskModel(h, self.gpar, densflux=True)
+        self.rings = None
+        self.fitok = None
+        self.results = None
+
+    def _band_vrot(self, r, ir):
+        """The largest VROT whose projection stays in the band of the cube for ring ir of r (_max_vrot)."""
+        st = self.dm.setup
+        ax = st.axis
+        nz = st.nsubs
+        v = ax.velocity(ax.world(np.array([0., nz - 1.])), ax.freq0) / 1000.
+        vmin, vmax = abs(v[0] - r.vsys[ir]), abs(v[1] - r.vsys[ir])
+        sini = math.sin(r.inc[ir] * math.pi / 180.)
+        return max(vmax / sini, vmin / sini) if sini != 0 else np.inf
+
+    def run(self):
+        """Fit the rings (one stage) and set rings, fitok and results; returns self."""
+        m = free_flags(self.par.FREE)
+        geometry = any(m[k] for k in (INC, PA, Z0, XPOS, YPOS, VSYS))
+        if geometry and self.twostage:
+            raise NotImplementedError("the second stage of 3DFIT (TWOSTAGE with the geometry free) is not built")
+        scaling = 10. / float(self.data.max()) if self.normalcube else 1.
+        data = (self.data * np.float32(scaling)).astype(np.float32) if self.normalcube else self.data
+        noise = robust_spread(self.data) * scaling
+        fit = RingFit(self.dm, data, self.mask, self.inr, par=self.par, data_noise=noise, chan_noise=self.chan_noise,
+                      vsys_range=self._vsys_range())
+        out, res = fit.fit(stage=1)
+        ok = []
+        for ir, r in enumerate(res):
+            good = r is not None and r[0] and out.vrot[ir] <= self._band_vrot(out, ir)
+            ok.append(bool(good))
+        for ir in range(out.nr):
+            if not ok[ir]:
+                out.dens[ir] = 0.
+        self.rings, self.fitok, self.results, self.nevals = out, ok, res, fit.nevals
+        return self
+
+    def _vsys_range(self):
+        st = self.dm.setup
+        ax = st.axis
+        v = ax.velocity(ax.world(np.array([0., st.nsubs - 1.])), ax.freq0) / 1000.
+        return (float(v.min()), float(v.max()))
+
+    def model(self, norm=None, rings=None):
+        """The final model (nz, ny, nx) float32 of the fitted rings (or of rings): the model of the disk with the ring
+        convention, convolved with the beam, normalised as norm (default the first NORM of par: LOCAL or NONE)."""
+        torch = self.torch
+        r = self.rings if rings is None else rings
+        norm = (norm or (self.par.NORM.split() or ["LOCAL"])[0]).upper()
+        if norm not in ("LOCAL", "NONE"):
+            raise NotImplementedError(f"the {norm} normalisation of the final model is not built")
+        dm, st = self.dm, self.dm.setup
+        nz, ny, nx = self.data.shape
+        sr, _ = st.subrings(model_rings(r, dm.ringmode, st.radsep))
+        if st.galmodtype == "MC":
+            m = dm.array_batch_mc(sr[None], (0, 0), (nx, ny), convolve=dm.cfield is not None)
+        else:
+            m = dm.array_batch(sr[None], (0, 0), (nx, ny), convolve=dm.cfield is not None)
+        if norm == "LOCAL":
+            dev = m.device
+            d = torch.from_numpy(np.ascontiguousarray(self.data.transpose(1, 2, 0))).to(dev)
+            k = torch.from_numpy(np.ascontiguousarray(self.mask.transpose(1, 2, 0)).astype(np.uint8)).to(dev)
+            _lib.check(_lib.lib().bb_local_norm(m.data_ptr(), d.data_ptr(), k.data_ptr(), ny, nx, nz, m.data_ptr(),
+                                                _stream_ptr(torch)))
+        return np.ascontiguousarray(m[0].cpu().numpy().transpose(2, 0, 1))

@@ -37,7 +37,6 @@ extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* s
 extern "C" size_t bb_rm_tile_lds(int nw, int TX, int TY);
 extern "C" int bb_launch_zero_planes(unsigned long long* counts, int B, int nmap, int map0, int nw, int64_t plane,
                                      hipStream_t st);
-extern "C" int bb_norm_blocks(int by, int bx);
 extern "C" int bb_launch_norm(int B, int by, int bx, int nz, int ny, int nx, int64_t blo0, int64_t blo1, double wpow,
                               int side, int ftype, double data_noise, double bweight, int normtype, int foot,
                               double hmax, const float* model, const float* data, const uint8_t* mask,
@@ -305,6 +304,13 @@ static int check_box(const bb_model* m, bb_box box) {
 extern "C" int bb_galmod_analytic(const bb_model* m, const bb_azplan* p, bb_box box, const double* sr, int nsub,
                                   int B, void* work, int64_t work_bytes, float* out, int accumulate, void* stream) {
     return bb_galmod_analytic_vb(m, p, box, nullptr, sr, nsub, B, work, work_bytes, out, accumulate, stream);
+}
+
+extern "C" int bb_galmod_analytic_boxes(const bb_model* m, const bb_azplan* p, bb_box box, const int32_t* vbox,
+                                        const double* sr, int nsub, int B, void* work, int64_t work_bytes, float* out,
+                                        int accumulate, void* stream) {
+    if (!vbox) return fail(BB_ERR_ARG, "bb_galmod_analytic_boxes: vbox is NULL");
+    return bb_galmod_analytic_vb(m, p, box, (const int4*)vbox, sr, nsub, B, work, work_bytes, out, accumulate, stream);
 }
 
 int bb_galmod_analytic_vb(const bb_model* m, const bb_azplan* p, bb_box box, const int4* vbox, const double* sr,
@@ -588,6 +594,14 @@ extern "C" int bb_galmod_mc(const bb_model* m, bb_box box, const double* sr, int
                            overflow, accumulate, stream);
 }
 
+extern "C" int bb_galmod_mc_boxes(const bb_model* m, bb_box box, const int32_t* vbox, const double* sr, int nsub,
+                                  int B, int nvpar, uint32_t seed, int64_t max_clouds, void* work, int64_t work_bytes,
+                                  float* out, uint64_t* counts, int* overflow, int accumulate, void* stream) {
+    if (!vbox) return fail(BB_ERR_ARG, "bb_galmod_mc_boxes: vbox is NULL");
+    return bb_galmod_mc_vb(m, box, (const int4*)vbox, sr, nsub, B, nvpar, seed, max_clouds, work, work_bytes, out,
+                           counts, overflow, accumulate, stream);
+}
+
 int bb_galmod_mc_vb(const bb_model* m, bb_box box, const int4* vbox, const double* sr, int nsub, int B, int nvpar,
                     uint32_t seed, int64_t max_clouds, void* work, int64_t work_bytes, float* out, uint64_t* counts,
                     int* overflow, int accumulate, void* stream) {
@@ -644,8 +658,7 @@ int bb_galmod_mc_vb(const bb_model* m, bb_box box, const int4* vbox, const doubl
 // The 3DFIT ring residual.
 extern "C" int64_t bb_norm_work_size(int B, int by, int bx, int nz, int with_support) {
     if (B < 0 || by < 1 || bx < 1 || nz < 1) return -1;
-    const int64_t nblk = bb_norm_blocks(by, bx);
-    int64_t n = (int64_t)B * nblk * 3 + B;
+    int64_t n = (int64_t)B * by * bx * 3 + B;
     if (with_support) n += (int64_t)B * by * (bx + 1) * nz;
     return n;
 }

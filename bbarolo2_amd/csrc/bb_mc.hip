@@ -328,8 +328,15 @@ k_mc_clouds(bb_grid g, McGeo G, const McSub* __restrict__ subs, int nsub, uint32
             McRec* __restrict__ recs, int64_t cap, int* __restrict__ overflow, const int4* __restrict__ vbox) {
     const int s = blockIdx.x, b = blockIdx.y;
     const McSub u = subs[(size_t)b * nsub + s];
-    // the box of the clouds kept: the layout box, or the vector's own model box inside it
-    const int4 vb = vbox ? vbox[b] : make_int4((int)G.blo0, (int)G.blo1, (int)G.bhi0, (int)G.bhi1);
+    // the box of the clouds kept: the layout box, or the vector's own model box within it
+    int4 vb = make_int4((int)G.blo0, (int)G.blo1, (int)G.bhi0, (int)G.bhi1);
+    if (vbox) {
+        const int4 v = vbox[b];
+        vb.x = max(vb.x, v.x);
+        vb.y = max(vb.y, v.y);
+        vb.z = min(vb.z, v.z);
+        vb.w = min(vb.w, v.w);
+    }
     for (int64_t ic = threadIdx.x; ic < u.nc; ic += blockDim.x) {
         int tile;
         McRec rec;

@@ -5,12 +5,17 @@
 //
 // B trial models (B, by, bx, nz) float32 yxz share their box; each trial has its own ring geometry. One
 // wavefront per pixel of the box: the lanes take the channels, the LOCAL factor is a wavefront reduction, and
-// the sums (FP64) and counts go to per-block partials reduced in a fixed order (deterministic). AZIM takes a
-// first pass for the ring's factor. The support's footprint sums use per-row prefix sums of the unconvolved
+// each pixel's sums (FP64) and counts are stored; one workgroup per trial then adds the pixels in an order fixed
+// by their coordinates in the cube (anchored_sum), so a trial gives the same value bit for bit in any box that
+// holds its ring (the CLASSIC stage models the trials of all its rings in the union of their boxes). AZIM takes
+// a first pass for the ring's factor. The support's footprint sums use per-row prefix sums of the unconvolved
 // model along x (FP64), as the reference does.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+
+#include "../../include/bbgpu.h"
+#include "bb_internal.h"
 
 #define BB_NM_THREADS 256
 #define BB_NM_WAVES (BB_NM_THREADS / 64)
@@ -54,53 +59,72 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// AZIM: per-block partial sums of the model and of the masked data over the ring's pixels
+// the sum over the pixels of one trial of K per-pixel values pix (npix, K), by the workgroup (BB_NM_THREADS) of
+// the trial, in an order fixed by the pixels' coordinates in the cube: lane l of a wavefront adds the columns
+// with (x + blo0) % 64 == l of a row, wavefront w the rows with (y + blo1) % 4 == w, the row sums going to the
+// lane (y + blo1) / 4 % 64 in increasing y; then wavefront sums and the 4 wavefronts in order. A larger box
+// around the same non-zero pixels only adds exact zeros (every value is >= 0), so the sum is unchanged.
+template <int K>
+__device__ void anchored_sum(const NormGeo& G, const double* __restrict__ pix, double* out) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int xr = (int)(G.blo0 & 63), yr = (int)(G.blo1 & 3);
+    double acc[K];
+    for (int k = 0; k < K; k++) acc[k] = 0.0;
+    for (int y = (wv - yr + 4) & 3; y < G.by; y += BB_NM_WAVES) {
+        double rs[K];
+        for (int k = 0; k < K; k++) rs[k] = 0.0;
+        const double* row = pix + (size_t)y * G.bx * K;
+        for (int x = (lane - xr + 64) & 63; x < G.bx; x += 64)
+            for (int k = 0; k < K; k++) rs[k] += row[(size_t)x * K + k];
+        for (int k = 0; k < K; k++) rs[k] = wave_sum(rs[k]);
+        if (lane == (int)(((y + G.blo1) >> 2) & 63))
+            for (int k = 0; k < K; k++) acc[k] += rs[k];
+    }
+    __shared__ double red[BB_NM_WAVES][K];
+    for (int k = 0; k < K; k++) acc[k] = wave_sum(acc[k]);
+    if (lane == 0)
+        for (int k = 0; k < K; k++) red[wv][k] = acc[k];
+    __syncthreads();
+    for (int k = 0; k < K; k++) {
+        double t = 0.0;
+        for (int i = 0; i < BB_NM_WAVES; i++) t += red[i][k];
+        out[k] = t;
+    }
+}
+
+// AZIM: per pixel of the ring, the sums of the model and of the masked data over the channels (0 elsewhere)
 __global__ void __launch_bounds__(BB_NM_THREADS)
 k_norm_azim(NormGeo G, const float* __restrict__ model, const float* __restrict__ data,
-            const uint8_t* __restrict__ mask, const double* __restrict__ ring, double* __restrict__ part) {
+            const uint8_t* __restrict__ mask, const double* __restrict__ ring, double* __restrict__ pix) {
     const int b = blockIdx.y, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int npix = G.by * G.bx;
     const double* rp = ring + (size_t)b * 6;
     const float* mb = model + (size_t)b * npix * G.nz;
-    double ms = 0.0, os = 0.0;
     for (int p = blockIdx.x * BB_NM_WAVES + wv; p < npix; p += gridDim.x * BB_NM_WAVES) {
         const int y = p / G.bx, x = p - y * G.bx;
-        double w;
-        if (!nm_inside(G, rp, x, y, w)) continue;
-        const size_t d0 = ((size_t)(y + G.blo1) * G.nx + (x + G.blo0)) * G.nz;
-        for (int z = lane; z < G.nz; z += 64) {
-            ms += (double)mb[(size_t)p * G.nz + z];
-            if (mask[d0 + z]) os += (double)data[d0 + z];
+        double w, ms = 0.0, os = 0.0;
+        if (nm_inside(G, rp, x, y, w)) {
+            const size_t d0 = ((size_t)(y + G.blo1) * G.nx + (x + G.blo0)) * G.nz;
+            for (int z = lane; z < G.nz; z += 64) {
+                ms += (double)mb[(size_t)p * G.nz + z];
+                if (mask[d0 + z]) os += (double)data[d0 + z];
+            }
+            ms = wave_sum(ms);
+            os = wave_sum(os);
         }
-    }
-    __shared__ double red[2][BB_NM_WAVES];
-    ms = wave_sum(ms);
-    os = wave_sum(os);
-    if (lane == 0) {
-        red[0][wv] = ms;
-        red[1][wv] = os;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = 0, c = 0;
-        for (int i = 0; i < BB_NM_WAVES; i++) {
-            a += red[0][i];
-            c += red[1][i];
+        if (lane == 0) {
+            pix[((size_t)b * npix + p) * 2 + 0] = ms;
+            pix[((size_t)b * npix + p) * 2 + 1] = os;
         }
-        part[((size_t)b * gridDim.x + blockIdx.x) * 2 + 0] = a;
-        part[((size_t)b * gridDim.x + blockIdx.x) * 2 + 1] = c;
     }
 }
 
-__global__ void k_norm_azim_final(const double* __restrict__ part, int nblk, double* __restrict__ factor) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= (int)gridDim.x * (int)blockDim.x) return;
-    double a = 0, c = 0;
-    for (int k = 0; k < nblk; k++) {
-        a += part[((size_t)b * nblk + k) * 2 + 0];
-        c += part[((size_t)b * nblk + k) * 2 + 1];
-    }
-    factor[b] = a != 0 ? c / a : 0.0;
+__global__ void __launch_bounds__(BB_NM_THREADS)
+k_norm_azim_final(NormGeo G, const double* __restrict__ pix, double* __restrict__ factor) {
+    const int b = blockIdx.x;
+    double s[2];
+    anchored_sum<2>(G, pix + (size_t)b * G.by * G.bx * 2, s);
+    if (threadIdx.x == 0) factor[b] = s[0] != 0 ? s[1] / s[0] : 0.0;
 }
 
 // prefix sums of the unconvolved model along x, per (trial, row, channel): P (B, by, bx+1, nz) FP64
@@ -120,23 +144,26 @@ __global__ void k_norm_prefix(NormGeo G, int B, const float* __restrict__ pre, d
     }
 }
 
-// the residual: per-block partials (sum, blanks, voxels)
+// the residual: per pixel of the box (sum, blanks, voxels), zeros outside the ring
 __global__ void __launch_bounds__(BB_NM_THREADS)
 k_norm_res(NormGeo G, const float* __restrict__ model, const float* __restrict__ data, const uint8_t* __restrict__ mask,
            const double* __restrict__ ring, const double* __restrict__ chan_noise, const double* __restrict__ azf,
            const double* __restrict__ P, const int* __restrict__ ilo, const int* __restrict__ ihi,
-           const double* __restrict__ thr, double* __restrict__ part) {
+           const double* __restrict__ thr, double* __restrict__ pix) {
     const int b = blockIdx.y, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int npix = G.by * G.bx, nz = G.nz;
     const double* rp = ring + (size_t)b * 6;
     const float* mb = model + (size_t)b * npix * nz;
     const double* Pb = P ? P + (size_t)b * G.by * (G.bx + 1) * nz : nullptr;
     const int hy = G.ky / 2;
-    double sum = 0.0, nb = 0.0, nt = 0.0;
     for (int p = blockIdx.x * BB_NM_WAVES + wv; p < npix; p += gridDim.x * BB_NM_WAVES) {
         const int y = p / G.bx, x = p - y * G.bx;
-        double wi;
-        if (!nm_inside(G, rp, x, y, wi)) continue;
+        double* o = pix + ((size_t)b * npix + p) * 3;
+        double wi, sum = 0.0, nb = 0.0, nt = 0.0;
+        if (!nm_inside(G, rp, x, y, wi)) {
+            if (lane == 0) o[0] = o[1] = o[2] = 0.0;
+            continue;
+        }
         const size_t d0 = ((size_t)(y + G.blo1) * G.nx + (x + G.blo0)) * nz;
         double factor = 1.0;
         if (G.normtype == 0) {
@@ -185,45 +212,28 @@ k_norm_res(NormGeo G, const float* __restrict__ model, const float* __restrict__
             else t = 0.0;
             sum += t;
         }
-    }
-    __shared__ double red[3][BB_NM_WAVES];
-    sum = wave_sum(sum);
-    nb = wave_sum(nb);
-    nt = wave_sum(nt);
-    if (lane == 0) {
-        red[0][wv] = sum;
-        red[1][wv] = nb;
-        red[2][wv] = nt;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = 0, c = 0, d = 0;
-        for (int i = 0; i < BB_NM_WAVES; i++) {
-            a += red[0][i];
-            c += red[1][i];
-            d += red[2][i];
+        sum = wave_sum(sum);
+        nb = wave_sum(nb);
+        nt = wave_sum(nt);
+        if (lane == 0) {
+            o[0] = sum;
+            o[1] = nb;
+            o[2] = nt;
         }
-        double* o = part + ((size_t)b * gridDim.x + blockIdx.x) * 3;
-        o[0] = a;
-        o[1] = c;
-        o[2] = d;
     }
 }
 
-__global__ void k_norm_final(const double* __restrict__ part, int nblk, int B, double bweight, double* __restrict__ val,
-                             int64_t* __restrict__ nbo, int64_t* __restrict__ nto) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B) return;
-    double s = 0, nb = 0, nt = 0;
-    for (int k = 0; k < nblk; k++) {
-        const double* p = part + ((size_t)b * nblk + k) * 3;
-        s += p[0];
-        nb += p[1];
-        nt += p[2];
-    }
+__global__ void __launch_bounds__(BB_NM_THREADS)
+k_norm_final(NormGeo G, const double* __restrict__ pix, double* __restrict__ val, int64_t* __restrict__ nbo,
+             int64_t* __restrict__ nto) {
+    const int b = blockIdx.x;
+    double s[3];
+    anchored_sum<3>(G, pix + (size_t)b * G.by * G.bx * 3, s);
+    if (threadIdx.x != 0) return;
+    const double nb = s[1], nt = s[2];
     if (nbo) nbo[b] = (int64_t)nb;
     if (nto) nto[b] = (int64_t)nt;
-    val[b] = nt - nb == 0 ? __builtin_inf() : pow(1.0 + nb / nt, bweight) * s / (nt - nb);
+    val[b] = nt - nb == 0 ? __builtin_inf() : pow(1.0 + nb / nt, G.bweight) * s[0] / (nt - nb);
 }
 
 // ---------------------------------------------------------------------------------------------------------
@@ -257,19 +267,58 @@ extern "C" int bb_launch_norm(int B, int by, int bx, int nz, int ny, int nx, int
     G.foot = foot;
     G.ky = pre ? ky : 0;
     const int nblk = bb_norm_blocks(by, bx);
-    double* part = work;                                   // B * nblk * 3
-    double* azf = part + (size_t)B * nblk * 3;             // B
-    double* P = azf + B;                                   // B * by * (bx+1) * nz (with a support)
+    double* pix = work;                                          // B * by * bx * 3
+    double* azf = pix + (size_t)B * by * bx * 3;                 // B
+    double* P = azf + B;                                         // B * by * (bx+1) * nz (with a support)
     if (normtype == 1) {
-        hipLaunchKernelGGL(k_norm_azim, dim3(nblk, B), dim3(BB_NM_THREADS), 0, st, G, model, data, mask, ring, part);
-        hipLaunchKernelGGL(k_norm_azim_final, dim3(B), dim3(1), 0, st, part, nblk, azf);
+        hipLaunchKernelGGL(k_norm_azim, dim3(nblk, B), dim3(BB_NM_THREADS), 0, st, G, model, data, mask, ring, pix);
+        hipLaunchKernelGGL(k_norm_azim_final, dim3(B), dim3(BB_NM_THREADS), 0, st, G, (const double*)pix, azf);
     }
     if (pre) {
         const int64_t nl = (int64_t)B * by * nz;
         hipLaunchKernelGGL(k_norm_prefix, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, st, G, B, pre, P);
     }
     hipLaunchKernelGGL(k_norm_res, dim3(nblk, B), dim3(BB_NM_THREADS), 0, st, G, model, data, mask, ring, chan_noise,
-                       azf, pre ? (const double*)P : (const double*)nullptr, ilo, ihi, thr, part);
-    hipLaunchKernelGGL(k_norm_final, dim3((B + 63) / 64), dim3(64), 0, st, part, nblk, B, bweight, val, nb, nt);
+                       azf, pre ? (const double*)P : (const double*)nullptr, ilo, ihi, thr, pix);
+    hipLaunchKernelGGL(k_norm_final, dim3(B), dim3(BB_NM_THREADS), 0, st, G, (const double*)pix, val, nb, nt);
     return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// LOCAL normalisation of a final model (galfit_out.py _local, bbarolo/galfit_out.py:128-135): every spectrum of the
+// model scaled by (sum over the channels of the data in the mask) / (sum of the model), 0 where the model sums to 0.
+// One wavefront per pixel, lanes along the channels (yxz layout), FP64 sums.
+__global__ void __launch_bounds__(256)
+k_local_norm(const float* __restrict__ model, const float* __restrict__ data, const uint8_t* __restrict__ mask,
+             int64_t npix, int nz, float* __restrict__ out) {
+    const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= npix) return;
+    const float* m = model + p * nz;
+    const float* d = data + p * nz;
+    const uint8_t* k = mask + p * nz;
+    double ms = 0., os = 0.;
+    for (int z = lane; z < nz; z += 64) {
+        ms += (double)m[z];
+        if (k[z]) os += (double)d[z];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ms += __shfl_xor(ms, off, 64);
+        os += __shfl_xor(os, off, 64);
+    }
+    const double f = ms != 0. ? os / ms : 0.;
+    for (int z = lane; z < nz; z += 64) out[p * nz + z] = (float)((double)m[z] * f);
+}
+
+extern "C" int bb_local_norm(const float* model, const float* data, const uint8_t* mask, int by, int bx, int nz,
+                             float* out, void* stream) {
+    if (by < 0 || bx < 0 || nz < 1) return bb_fail(BB_ERR_ARG, "bb_local_norm: bad sizes");
+    const int64_t npix = (int64_t)by * bx;
+    if (npix == 0) return 0;
+    if (!model || !data || !mask || !out) return bb_fail(BB_ERR_ARG, "bb_local_norm: NULL array");
+    const int64_t blocks = (npix + 3) / 4;
+    if (blocks > INT32_MAX) return bb_fail(BB_ERR_ARG, "bb_local_norm: too many pixels");
+    hipLaunchKernelGGL(k_local_norm, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, model, data, mask, npix,
+                       nz, out);
+    return bb_hip_check(hipGetLastError(), "bb_local_norm");
 }

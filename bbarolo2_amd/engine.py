@@ -231,11 +231,12 @@ class DiskModel:
         return (nc.sum(axis=1) + 2 * sr.shape[2]).astype(np.int64)
 
     def array_batch_mc(self, sr, blo, bhi, convolve=True, out=None, work=None, stream=None, d_sr=None, seed=None,
-                       counts=None, max_clouds=None):
+                       counts=None, max_clouds=None, vbox=None):
         """Monte-Carlo models (counter-based stream, bbgpu.h) of B sub-ring tables sr (B, 16, nsub) in the box
         [blo, bhi): a (B, by, bx, nz) float32 device tensor (yxz), convolved with the beam if convolve. counts:
         optional (B, by, bx, nz) int64 device tensor of the integer sums. max_clouds: clouds per vector to hold
-        (default: the bound of mc_clouds on sr)."""
+        (default: the bound of mc_clouds on sr). vbox (B, 4) int: the model box [blo0, blo1, bhi0, bhi1) of each
+        vector within [blo, bhi) (bb_galmod_mc_boxes; None: [blo, bhi) for all)."""
         torch = _torch()
         sr = np.ascontiguousarray(sr, np.float64)
         B, nrow, nsub = sr.shape
@@ -261,8 +262,14 @@ class DiskModel:
             d_sr = torch.from_numpy(sr).to(dev)
         nv = st.nvpar if st.nvpar >= 1 else st.nsubs
         s32 = seed32(st.iseed if seed is None else seed)
-        _lib.check(L.bb_galmod_mc(h, box, d_sr.data_ptr(), nsub, B, nv, s32, cap, ws.data_ptr(), wb, dep.data_ptr(),
-                                  None if counts is None else counts.data_ptr(), flag.data_ptr(), 0, sp))
+        if vbox is None:
+            _lib.check(L.bb_galmod_mc(h, box, d_sr.data_ptr(), nsub, B, nv, s32, cap, ws.data_ptr(), wb, dep.data_ptr(),
+                                      None if counts is None else counts.data_ptr(), flag.data_ptr(), 0, sp))
+        else:
+            d_vb = torch.from_numpy(np.ascontiguousarray(vbox, np.int32).reshape(B, 4)).to(dev)
+            _lib.check(L.bb_galmod_mc_boxes(h, box, d_vb.data_ptr(), d_sr.data_ptr(), nsub, B, nv, s32, cap,
+                                            ws.data_ptr(), wb, dep.data_ptr(),
+                                            None if counts is None else counts.data_ptr(), flag.data_ptr(), 0, sp))
         if int(flag.item()) != 0:
             raise _lib.BBError(f"bb_galmod_mc: more than {cap} clouds of a vector inside the box")
         if not convolve or self.cfield is None:
@@ -295,10 +302,11 @@ class DiskModel:
             _lib.check(L.bb_model_set_tile(h, want[0], want[1]))
         return want
 
-    def array_batch(self, sr, blo, bhi, convolve=True, out=None, work=None, stream=None, d_sr=None):
+    def array_batch(self, sr, blo, bhi, convolve=True, out=None, work=None, stream=None, d_sr=None, vbox=None):
         """ANALYTIC models of B sub-ring tables sr (B, 16, nsub) sharing their radii, in the box [blo, bhi):
         a (B, by, bx, nz) float32 device tensor (yxz), convolved with the beam if convolve. d_sr: the tables
-        already on the device (sr then gives their radii and heights: the density row may differ)."""
+        already on the device (sr then gives their radii and heights: the density row may differ). vbox (B, 4) int:
+        the model box [blo0, blo1, bhi0, bhi1) of each vector within [blo, bhi) (bb_galmod_analytic_boxes)."""
         torch = _torch()
         sr = np.ascontiguousarray(sr, np.float64)
         B, nrow, nsub = sr.shape
@@ -321,8 +329,13 @@ class DiskModel:
             wb = int(L.bb_analytic_work_size(p, B))
             ws = torch.empty(wb, dtype=torch.uint8, device=dev)
             box = _lib.bb_box(blo[0], blo[1], bhi[0], bhi[1])
-            _lib.check(L.bb_galmod_analytic(h, p, box, d_sr.data_ptr(), nsub, B, ws.data_ptr(), wb, dep.data_ptr(),
-                                            0, st))
+            if vbox is None:
+                _lib.check(L.bb_galmod_analytic(h, p, box, d_sr.data_ptr(), nsub, B, ws.data_ptr(), wb,
+                                                dep.data_ptr(), 0, st))
+            else:
+                d_vb = torch.from_numpy(np.ascontiguousarray(vbox, np.int32).reshape(B, 4)).to(dev)
+                _lib.check(L.bb_galmod_analytic_boxes(h, p, box, d_vb.data_ptr(), d_sr.data_ptr(), nsub, B,
+                                                      ws.data_ptr(), wb, dep.data_ptr(), 0, st))
         else:
             dep.zero_()
         if not convolve or self.cfield is None:

@@ -149,12 +149,12 @@ class RingSimplex:
         return d
 
     # ------------------------------------------------------------------ trials
-    def clamp(self, dring, zpar):
-        """func3D's bounds (galfit.py:832-889): the values of a trial point, clamped or redrawn, written back
-        into zpar; returns the dict of ring values."""
+    def clamp(self, dring, zpar, rng=None):
+        """func3D's bounds (galfit.py:832-889): the values of a trial point, clamped or redrawn (with rng, default the
+        fit's generator), written back into zpar; returns the dict of ring values."""
         p, inr, mins, maxs, m = self.par, self.inr, self.mins, self.maxs, self.mpar
         w = dring.id
-        fran = self.rng.random
+        fran = (self.rng if rng is None else rng).random
         vals = {k: float(getattr(dring, k)[0]) for k in KEYS.values()}
         n = 0
         if m[VROT]:
@@ -195,24 +195,31 @@ class RingSimplex:
                 n += 1
         return vals
 
-    def func3D_batch(self, dring, points):
-        """func3D of several points of one ring (the clamps in order, then one batch of device evaluations)."""
+    def trials(self, dring, points, rng=None):
+        """The ring of each point of a simplex (func3D's clamps in order, galfit.py:832-889)."""
         rings = []
         for zpar in points:
-            vals = self.clamp(dring, zpar)
+            vals = self.clamp(dring, zpar, rng)
             r = dring.copy()
             r.id = dring.id
             for k, v in vals.items():
                 getattr(r, k)[:] = v
             rings.append(r)
-        return self.values(rings)
+        return rings
+
+    def func3D_batch(self, dring, points):
+        """func3D of several points of one ring (the clamps in order, then one batch of evaluations)."""
+        return self.values(self.trials(dring, points))
 
     def func3D(self, dring, zpar):
         return float(self.func3D_batch(dring, [zpar])[0])
 
     # ------------------------------------------------------------------ minimisation
-    def minimize(self, dring):
-        """Downhill simplex of Galfit.minimize / _mtry (galfit.py:891-1003): (ok, minimum, pmin)."""
+    def simplex(self, dring, rng=None):
+        """The downhill simplex of Galfit.minimize / _mtry (galfit.py:891-1003) as a generator: it yields the list of
+        trial rings it needs evaluated and receives their residuals (an array), so that the simplexes of all the rings
+        advance together, one batch of evaluations per round; it returns (ok, minimum, pmin). rng: the generator of
+        the clamps' redraws (the ring's own, seeded [12345, stage, ring] by the caller)."""
         TINY = 1.0e-10
         tol = self.par.TOL
         ndim = self.nfree
@@ -249,10 +256,23 @@ class RingSimplex:
         p = np.tile(np.array(point, float), (mpts, 1))
         for i in range(1, mpts):
             p[i, i - 1] += dels[i - 1]
-        xs = [p[i].copy() for i in range(mpts)]
-        y = self.func3D_batch(dring, xs)
+        y = np.array((yield self.trials(dring, [p[i].copy() for i in range(mpts)], rng)), float)
         nfunc = 0
         psum = p.sum(axis=0)
+
+        def mtry(ihi, fac):
+            # the reflection, expansion or contraction of the worst point through the face opposite it (_mtry)
+            nonlocal psum
+            fac1 = (1.0 - fac) / ndim
+            fac2 = fac1 - fac
+            ptry = psum * fac1 - p[ihi] * fac2
+            ytry = float((yield self.trials(dring, [ptry.copy()], rng))[0])
+            if ytry < y[ihi]:
+                y[ihi] = ytry
+                psum += ptry - p[ihi]
+                p[ihi] = ptry
+            return ytry
+
         while True:
             ilo = 0
             ihi, inhi = (0, 1) if y[0] > y[1] else (1, 0)
@@ -272,40 +292,37 @@ class RingSimplex:
             if nfunc >= NMAX:
                 return False, float(y[ilo]), p[ilo].copy()
             nfunc += 2
-            ytry = self._mtry(dring, p, y, psum, ihi, -1.0)
+            ytry = yield from mtry(ihi, -1.0)
             if ytry <= y[ilo]:
-                ytry = self._mtry(dring, p, y, psum, ihi, 2.0)
+                ytry = yield from mtry(ihi, 2.0)
             elif ytry >= y[inhi]:
                 ysave = y[ihi]
-                ytry = self._mtry(dring, p, y, psum, ihi, 0.5)
+                ytry = yield from mtry(ihi, 0.5)
                 if ytry >= ysave:
                     idx = [i for i in range(mpts) if i != ilo]
                     for i in idx:
                         p[i] = 0.5 * (p[i] + p[ilo])
                     # (clamped copies are evaluated; the simplex keeps the unclamped points, galfit.py:978-981)
-                    y[idx] = self.func3D_batch(dring, [p[i].copy() for i in idx])
+                    y[idx] = np.asarray((yield self.trials(dring, [p[i].copy() for i in idx], rng)), float)
                     nfunc += ndim
                     psum = p.sum(axis=0)
             else:
                 nfunc -= 1
 
-    def _mtry(self, dring, p, y, psum, ihi, fac):
-        ndim = self.nfree
-        fac1 = (1.0 - fac) / ndim
-        fac2 = fac1 - fac
-        ptry = psum * fac1 - p[ihi] * fac2
-        ytry = self.func3D(dring, ptry)
-        if ytry < y[ihi]:
-            y[ihi] = ytry
-            psum += ptry - p[ihi]
-            p[ihi] = ptry
-        return ytry
+    def minimize(self, dring):
+        """The simplex of one ring evaluated on its own: (ok, minimum, pmin)."""
+        gen = self.simplex(dring)
+        try:
+            req = next(gen)
+            while True:
+                req = gen.send(self.values(req))
+        except StopIteration as e:
+            return e.value
 
     # ------------------------------------------------------------------ the fit
-    def fit_ring(self, ir, stage=1):
-        """The first-stage fit of ring ir (Galfit._fit_ring / _fit_one_ring, galfit.py:1097-1142): None if the ring
-        has too few pixels in the mask, else (ok, minimum, values of the free parameters)."""
-        self.rng = np.random.default_rng([12345, stage, ir])
+    def ring_start(self, ir, stage=1):
+        """The annulus of ring ir and its generator (seeded [12345, stage, ring], galfit.py:1097-1100), or None when
+        the ring has too few pixels in the mask (galfit.py:1118-1125)."""
         dring = self.make_dring(ir)
         blo, bhi = self.box(dring)
         ps = self.pixscale
@@ -313,23 +330,62 @@ class RingSimplex:
                                       dring.phi[-1], dring.inc[-1], dring.radii[0] / ps, dring.radii[-1] / ps)
         if nIn < 20 and (nIn == 0 or nIn < 0.7 * nTot):
             return None
-        return self.minimize(dring)
+        return dring, np.random.default_rng([12345, stage, ir])
 
-    def fit(self, stage=1):
-        """Fit every ring; returns (Rings with the fitted values, list of the ring results)."""
+    def fit_ring(self, ir, stage=1):
+        """The first-stage fit of ring ir alone (Galfit._fit_ring / _fit_one_ring, galfit.py:1097-1142): None if the
+        ring has too few pixels in the mask, else (ok, minimum, values of the free parameters)."""
+        st = self.ring_start(ir, stage)
+        if st is None:
+            return None
+        self.rng = st[1]
+        return self.minimize(st[0])
+
+    def values_rounds(self, groups):
+        """The residuals of the trial rings of several rings, groups = [(ring index, [trial rings])]: a list of
+        arrays. Here one ring after the other; RingFit evaluates all of them in one launch."""
+        return [self.values(rings) for _, rings in groups]
+
+    def fit(self, stage=1, rings=None):
+        """Fit the rings (all, or the indices given) of one stage: every ring's simplex advances one step per round
+        and the trial rings of all of them are evaluated together. Returns (Rings with the fitted values, the result
+        of each ring: None when skipped, else (ok, minimum, values of the free parameters))."""
         out = self.inr.copy()
-        res = []
-        for ir in range(self.inr.nr):
-            r = self.fit_ring(ir, stage)
-            res.append(r)
+        idx = list(range(self.inr.nr)) if rings is None else list(rings)
+        res = {ir: None for ir in idx}
+        gens, req = {}, {}
+        for ir in idx:
+            st = self.ring_start(ir, stage)
+            if st is None:
+                continue
+            gens[ir] = self.simplex(*st)
+            req[ir] = next(gens[ir])
+        self.begin_stage({ir: self.make_dring(ir) for ir in gens})
+        try:
+            while req:
+                order = sorted(req)
+                vals = self.values_rounds([(ir, req[ir]) for ir in order])
+                for ir, v in zip(order, vals):
+                    try:
+                        req[ir] = gens[ir].send(v)
+                    except StopIteration as e:
+                        res[ir] = e.value
+                        del req[ir]
+        finally:
+            self.begin_stage(None)
+        for ir in idx:
+            r = res[ir]
             if r is None or not r[0]:
                 continue
             n = 0
-            for idx in range(MAXPAR):
-                if self.mpar[idx]:
-                    getattr(out, KEYS[idx])[ir] = r[2][n]
+            for k in range(MAXPAR):
+                if self.mpar[k]:
+                    getattr(out, KEYS[k])[ir] = r[2][n]
                     n += 1
-        return out, res
+        return out, [res[ir] for ir in idx]
+
+    def begin_stage(self, drings):
+        """Called with the annuli of the rings of a stage before its rounds (None after them)."""
 
 
 class RingFit(RingSimplex):
@@ -343,6 +399,7 @@ class RingFit(RingSimplex):
         torch = _torch()
         nz, ny, nx = data.shape
         h = dm.header
+        self._layout = None
         ps = (abs(h.Cdelt(0)) * arcsconv(h.Cunit(0)) + abs(h.Cdelt(1)) * arcsconv(h.Cunit(1))) / 2.
         mask = np.asarray(mask, bool)
         super().__init__(inr, par, nx, ny, mask.sum(axis=0), ps, lambda r: dm.box(r, final=False), vsys_range)
@@ -380,41 +437,95 @@ class RingFit(RingSimplex):
             ihi = np.zeros(1, np.int32)
         return ilo, ihi, MC_SUPPORT_NMIN * unit * cd
 
+    def begin_stage(self, drings):
+        """The layout of a stage's rounds: the sub-rings of every ring's annulus, one block per ring, in one table whose
+        radii (and azimuth plan) stay the same through the stage (None: the layout of each call's rings)."""
+        self._layout = None
+        if drings:
+            st = self.dm.setup
+            self._layout = {ir: st.subrings(d)[0] for ir, d in sorted(drings.items())}
+
     def values(self, rings):
         """The 3DFIT residual of each ring of the list (Galfit.getFuncValue without the models of the other rings,
-        galfit.py:736-745), evaluated in batches of rings that share their box."""
-        out = np.empty(len(rings))
-        groups = {}
+        galfit.py:736-745), all evaluated in one launch."""
+        groups, order = {}, []
         for i, r in enumerate(rings):
-            blo, bhi = self.box(r)
-            groups.setdefault((tuple(blo), tuple(bhi)), []).append(i)
-        for (blo, bhi), idx in groups.items():
-            out[idx] = self._values_box([rings[i] for i in idx], blo, bhi)
-        self.nevals += len(rings)
+            groups.setdefault(r.id, []).append(i)
+        out = np.empty(len(rings))
+        keys = sorted(groups)
+        vals = self.values_rounds([(k, [rings[i] for i in groups[k]]) for k in keys])
+        for k, v in zip(keys, vals):
+            out[groups[k]] = v
         return out
 
-    def _values_box(self, rings, blo, bhi):
+    def values_rounds(self, groups):
+        """The residuals of the trial rings of several rings, groups = [(ring index, [trial rings])], in one launch:
+        every trial is a table of the stage's layout whose density is zero outside its own ring's block, modelled in its
+        own box (bb_galmod_analytic_boxes / bb_galmod_mc_boxes) within the union of the boxes and convolved there, then
+        each ring's trials go through the 3DFIT residual (bb_norm3dfit) with that ring's support."""
         torch = self.torch
         dm, st = self.dm, self.dm.setup
-        L = _lib.lib()
-        B = len(rings)
-        tabs = [st.subrings(r) for r in rings]
-        sr = np.stack([t[0] for t in tabs])
-        by, bx, nz = bhi[1] - blo[1], bhi[0] - blo[0], st.nsubs
-        stream = _stream_ptr(torch)
+        layout = self._layout
+        if layout is None or any(ir not in layout for ir, _ in groups):
+            layout = {ir: st.subrings(rs[0])[0] for ir, rs in groups}
+        order = sorted(layout)
+        offs, n = {}, 0
+        for ir in order:
+            offs[ir] = n
+            n += layout[ir].shape[1]
+        trials = [(ir, r) for ir, rs in groups for r in rs]
+        B = len(trials)
+        sr = np.empty((B, 16, n))
+        units, vbox = [], np.empty((B, 4), np.int64)
+        mc = st.galmodtype == "MC"
+        for b, (ir, r) in enumerate(trials):
+            own, nv = st.subrings(r)
+            if own.shape[1] != layout[ir].shape[1]:
+                raise RuntimeError("the sub-rings of a trial differ from those of its ring's annulus")
+            # ANALYTIC: the blocks in the layout's order (one azimuth plan for the batch); MC: the trial's own block
+            # first, so that its sub-rings keep the indices of the ring's own model (the keys of their random streams)
+            blocks = ([ir] + [j for j in order if j != ir]) if mc else order
+            a = 0
+            for jr in blocks:
+                w = layout[jr].shape[1]
+                if jr == ir:
+                    sr[b, :, a:a + w] = own
+                else:
+                    sr[b, :, a:a + w] = layout[jr]
+                    sr[b, 5, a:a + w] = 0.
+                a += w
+            units.append(st.subcloud_flux(own, nv) if own.shape[1] else 0.)
+            blo, bhi = self.box(r)
+            vbox[b] = (blo[0], blo[1], bhi[0], bhi[1])
+        ublo = (int(vbox[:, 0].min()), int(vbox[:, 1].min()))
+        ubhi = (int(vbox[:, 2].max()), int(vbox[:, 3].max()))
+        by, bx, nz = ubhi[1] - ublo[1], ubhi[0] - ublo[0], st.nsubs
         dep = torch.empty((B, by, bx, nz), dtype=torch.float32, device=self.dev)
         if st.galmodtype == "MC":
-            mod = dm.array_batch_mc(sr, blo, bhi, convolve=dm.cfield is not None, work=dep)
-            sup = None
+            mod = dm.array_batch_mc(sr, ublo, ubhi, convolve=dm.cfield is not None, work=dep, vbox=vbox)
         else:
-            mod = dm.array_batch(sr, blo, bhi, convolve=dm.cfield is not None, work=dep)
-            units = [st.subcloud_flux(t[0], t[1]) if t[0].shape[1] else 0. for t in tabs]
-            sup = self._support(units[0]) if len(set(units)) == 1 else None
-            if sup is None and any(u > 0 for u in units):
-                # (support units differ: evaluate one by one)
-                if B > 1:
-                    return np.array([self._values_box([r], blo, bhi)[0] for r in rings])
-                sup = self._support(units[0])
+            mod = dm.array_batch(sr, ublo, ubhi, convolve=dm.cfield is not None, work=dep, vbox=vbox)
+        self.nevals += B
+        out, a = [], 0
+        for ir, rs in groups:
+            k = len(rs)
+            out.append(self._norm_rows(mod, dep, a, rs, units[a:a + k], ublo, ubhi))
+            a += k
+        return out
+
+    def _norm_rows(self, mod, dep, a, rings, units, blo, bhi):
+        """bb_norm3dfit of the models a .. a + len(rings) of mod (the box [blo, bhi)): the trials of one ring, sharing
+        the support of its unit (trials of different units are split)."""
+        torch = self.torch
+        st = self.dm.setup
+        L = _lib.lib()
+        B = len(rings)
+        if st.galmodtype != "MC" and len(set(units)) > 1:
+            return np.concatenate([self._norm_rows(mod, dep, a + i, [r], [u], blo, bhi)
+                                   for i, (r, u) in enumerate(zip(rings, units))])
+        sup = self._support(units[0]) if st.galmodtype != "MC" else None
+        by, bx, nz = bhi[1] - blo[1], bhi[0] - blo[0], st.nsubs
+        vox = by * bx * nz
         ps = self.pixscale
         ring = np.array([[r.xpos[-1] - blo[0], r.ypos[-1] - blo[1], r.phi[-1], r.inc[-1], r.radii[0] / ps,
                           r.radii[-1] / ps] for r in rings], np.float64)
@@ -424,17 +535,15 @@ class RingFit(RingSimplex):
         wl = int(L.bb_norm_work_size(B, by, bx, nz, int(sup is not None)))
         work = torch.empty(wl, dtype=torch.float64, device=self.dev)
         val = torch.empty(B, dtype=torch.float64, device=self.dev)
+        args = (None, None, None, 0, None)
         if sup is not None:
             ilo, ihi, thr = sup
             d_ilo = torch.from_numpy(ilo).to(self.dev)
             d_ihi = torch.from_numpy(ihi).to(self.dev)
             d_thr = torch.from_numpy(np.ascontiguousarray(thr, np.float64)).to(self.dev)
-        _lib.check(L.bb_norm3dfit(mod.data_ptr(), B, by, bx, nz, self.d_data.data_ptr(), self.d_mask.data_ptr(), self.ny,
-                                  self.nx, int(blo[0]), int(blo[1]), d_ring.data_ptr(), self.d_cn.data_ptr(),
-                                  C.byref(par), dep.data_ptr() if sup is not None else None,
-                                  d_ilo.data_ptr() if sup is not None else None,
-                                  d_ihi.data_ptr() if sup is not None else None,
-                                  len(ilo) if sup is not None else 0,
-                                  d_thr.data_ptr() if sup is not None else None, work.data_ptr(), wl, val.data_ptr(),
-                                  None, None, stream))
+            args = (dep.data_ptr() + 4 * a * vox, d_ilo.data_ptr(), d_ihi.data_ptr(), len(ilo), d_thr.data_ptr())
+        _lib.check(L.bb_norm3dfit(mod.data_ptr() + 4 * a * vox, B, by, bx, nz, self.d_data.data_ptr(),
+                                  self.d_mask.data_ptr(), self.ny, self.nx, int(blo[0]), int(blo[1]), d_ring.data_ptr(),
+                                  self.d_cn.data_ptr(), C.byref(par), *args, work.data_ptr(), wl, val.data_ptr(), None,
+                                  None, _stream_ptr(torch)))
         return val.cpu().numpy()

@@ -84,6 +84,14 @@ int64_t bb_analytic_work_size(const bb_azplan* p, int B);
 int bb_galmod_analytic(const bb_model* m, const bb_azplan* p, bb_box box, const double* sr, int nsub, int B,
                        void* work, int64_t work_bytes, float* out, int accumulate, void* stream);
 
+/* bb_galmod_analytic with a model box per vector: vbox (B x 4 int32, DEVICE) holds [blo0, blo1, bhi0, bhi1) of vector
+ * b in absolute pixels; its model is the reference's model of that box (DiskModel.array(rings, blo, bhi),
+ * galmod.py:1242-1271: samples outside it are dropped, its vertical samples reach as far as that box needs) laid out
+ * in out (B, by, bx, nz) over box, zero outside it. So models of different boxes share one launch (the 3DFIT ring fits
+ * of all the rings). */
+int bb_galmod_analytic_boxes(const bb_model* m, const bb_azplan* p, bb_box box, const int32_t* vbox, const double* sr,
+                             int nsub, int B, void* work, int64_t work_bytes, float* out, int accumulate, void* stream);
+
 /* Measurement, no counterpart in the reference: two hipEvent_t (or both NULL to stop) that the calling host
  * thread's ANALYTIC deposits record on their stream just before and just after the deposit kernel (the tile
  * pass of bb_galmod_analytic), so that a benchmark times that kernel alone. */
@@ -135,6 +143,12 @@ int bb_galmod_mc(const bb_model* m, bb_box box, const double* sr, int nsub, int 
                  int64_t max_clouds, void* work, int64_t work_bytes, float* out, uint64_t* counts, int* overflow,
                  int accumulate, void* stream);
 
+/* bb_galmod_mc with a model box per vector (vbox as bb_galmod_analytic_boxes): the clouds of vector b outside its box
+ * are dropped. */
+int bb_galmod_mc_boxes(const bb_model* m, bb_box box, const int32_t* vbox, const double* sr, int nsub, int B, int nvpar,
+                       uint32_t seed, int64_t max_clouds, void* work, int64_t work_bytes, float* out, uint64_t* counts,
+                       int* overflow, int accumulate, void* stream);
+
 /* The 3DFIT residual of ring models (_norm + Galfit._norm_value / _support_args, bbarolo/galfit.py:39-188,
  * 685-733): B trial models (B, by, bx, nz) float32 yxz (convolved) in the box [blo, blo + (bx, by)) of the data
  * (ny, nx, nz) float32 yxz and mask (ny, nx, nz) uint8 (the transposed cube and mask of Galfit, data_T/mask_T);
@@ -159,6 +173,12 @@ int bb_norm3dfit(const float* model, int B, int by, int bx, int nz, const float*
                  int nx, int blo0, int blo1, const double* ring, const double* chan_noise, const bb_norm_par* par,
                  const float* pre, const int32_t* ilo, const int32_t* ihi, int ky, const double* thr, double* work,
                  int64_t work_len, double* value, int64_t* nblank, int64_t* ntot, void* stream);
+
+/* The LOCAL normalisation of a final 3DFIT model (bbarolo/galfit_out.py:128-135): out = model x (sum over the channels
+ * of the data where mask is set) / (sum over the channels of the model), per pixel (0 where the model sums to 0).
+ * model, data, out (by, bx, nz) float32 yxz, mask (by, bx, nz) uint8, all on the device (out may alias model). */
+int bb_local_norm(const float* model, const float* data, const uint8_t* mask, int by, int bx, int nz, float* out,
+                  void* stream);
 
 /* Unit-density ring maps (DiskModel.ring_maps / _ring_maps_analytic, bbarolo/galmod.py:1317-1345, 468-536) in
  * two steps. bb_ring_maps_deposit writes the planes map0 .. map0+nw-1 of counts (B, nmap, by, bx) uint64: the

@@ -151,3 +151,31 @@ def test_fit_skips_rings_outside_mask():
     q.mask2d = np.ones((64, 64))
     out, res = q.fit()
     assert all(r[0] for r in res) and np.allclose(out.vrot, 120., atol=0.5)
+
+
+def test_stage_rounds_equal_rings_alone():
+    """The stage's lock-step rounds (every ring's simplex one step per round, all their trials evaluated together)
+    give each ring the result of fitting it alone with its own generator [12345, stage, ring]: same minimum, point
+    and number of evaluations; and each round holds the trials of every ring still running."""
+    inr = _inr([10., 30., 50., 70.], vdisp=3.)
+    par = G.Fit3DPar(FREE="VROT VDISP INC", TOL=1e-4, DELTAVROT=8., DELTAINC=2.)
+    tgt = {"vrot": (109., 4.), "vdisp": (5., 1.), "inc": (62., 1.)}
+
+    class Rounds(Quad):
+        def values_rounds(self, groups):
+            self.rounds.append([len(g) for _, g in groups])
+            return super().values_rounds(groups)
+
+    q = Rounds(inr, par, tgt)
+    q.rounds = []
+    out, res = q.fit(stage=2)
+    assert q.rounds[0] == [4, 4, 4, 4] and len(q.rounds) > 10
+    for ir in range(4):
+        a = Quad(inr, par, tgt)
+        r = a.fit_ring(ir, stage=2)
+        assert r[0] == res[ir][0] and r[1] == res[ir][1] and np.array_equal(r[2], res[ir][2])
+        if r[0]:
+            assert out.vrot[ir] == r[2][0] and out.inc[ir] == r[2][2]
+        else:
+            assert out.vrot[ir] == inr.vrot[ir]
+    assert sum(r[0] for r in res) >= 1

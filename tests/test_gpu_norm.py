@@ -96,3 +96,26 @@ def test_norm_support_parity():
         print(f"[norm support] trial {b}: blanks {nb[b]}/{rnb} voxels {nt[b]}/{rnt}", flush=True)
         assert nb[b] == rnb and nt[b] == rnt and 0 < rnb < rnt
         assert abs(val[b] - ref) <= 1e-9 * abs(ref)
+
+
+@pytest.mark.parametrize("k", [0, 2, 3, 4])
+def test_norm_same_in_larger_box(k):
+    """The same trials modelled in a larger box (zeros around them) give the same values bit for bit: the pixel sums
+    are added in an order fixed by their coordinates in the cube, not by the box."""
+    data, mask, models, _, cn = _case(k)
+    rings = np.array([[18.3, 17.6, 35., 50., 4.2, 11.7], [20.1, 16.2, 120., 40., 0., 9.3],
+                      [17.7, 18.9, 250., 20., 6.5, 13.0]])
+    c = CASES[k]
+    par = dict(side=c["side"], ftype=c["ftype"], normtype=c["normtype"], foot=0, wpow=c["wpow"], data_noise=0.37,
+               bweight=1.5, hmax=0.)
+    blo, dx, dy = (7, 9), 3, 3
+    B, by, bx, nz = models.shape
+    big = np.zeros((B, by + 8, bx + 10, nz), np.float32)
+    big[:, dy:dy + by, dx:dx + bx] = models
+    rb = rings.copy()
+    rb[:, 0] += dx
+    rb[:, 1] += dy
+    v1, nb1, nt1 = _gpu_norm(models, data, mask, blo, rings, cn, par)
+    v2, nb2, nt2 = _gpu_norm(big, data, mask, (blo[0] - dx, blo[1] - dy), rb, cn, par)
+    print(f"[norm box {k}] {v1} {v2}", flush=True)
+    assert np.array_equal(v1, v2) and np.array_equal(nb1, nb2) and np.array_equal(nt1, nt2)
