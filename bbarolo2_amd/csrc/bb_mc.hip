@@ -65,6 +65,8 @@ __device__ __forceinline__ float mc_sqrt(float x) {
     return fmaf(0.5f * y, fmaf(-s, s, x), s);
 }
 
+// log(x), x > 0 normal: x = m 2^e, m in [sqrt(1/2), sqrt(2)), log(1 + y) = y P(y) with P the float32-rounded
+// least-squares fit that mc_log_fit.py (beside this file) computes
 __device__ __forceinline__ float mc_log(float x) {
     const uint32_t i = __float_as_uint(x);
     int e = (int)((i >> 23) & 0xFF) - 127;

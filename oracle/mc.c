@@ -80,7 +80,8 @@ static inline float sqrt_f(float x) {
     return fmaf(0.5f * y, fmaf(-s, s, x), s);
 }
 
-/* log(x), x > 0 normal: x = m 2^e, m in [sqrt(1/2), sqrt(2)), log(1 + y) = y P(y) (least-squares fit) */
+/* log(x), x > 0 normal: x = m 2^e, m in [sqrt(1/2), sqrt(2)), log(1 + y) = y P(y), P the float32-rounded
+   least-squares fit of bbarolo2_amd/csrc/mc_log_fit.py (the same constants as the kernel's mc_log) */
 static inline float log_f(float x) {
     uint32_t i = i_of(x);
     int e = (int)((i >> 23) & 0xFF) - 127;

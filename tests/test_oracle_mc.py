@@ -107,3 +107,30 @@ def test_mc_flux_per_subcloud(small_header):
     sr, _ = st.subrings(model_rings(r, "CENTRED", st.radsep))
     expect = np.sum(sr[5] * 2 * np.pi * sr[0] * st.radsep) * st.cd2i[0]
     assert abs(mc.sum() / expect - 1) < 1e-4
+
+
+def test_mc_log_coefficients_from_their_fit():
+    """The coefficients of mc_log in the kernel and in the oracle are the float32-rounded least-squares fit that
+    bbarolo2_amd/csrc/mc_log_fit.py states, and the evaluation stays within 2e-6 of log."""
+    import importlib.util
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("mc_log_fit", os.path.join(root, "bbarolo2_amd", "csrc",
+                                                                             "mc_log_fit.py"))
+    fit = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(fit)
+    c = fit.coefficients()
+    want = [np.float32(c[k]) for k in range(9, 0, -1)]
+    for path, fn in ((("bbarolo2_amd", "csrc", "bb_mc.hip"), "mc_log"), (("oracle", "mc.c"), "log_f")):
+        src = open(os.path.join(root, *path)).read()
+        body = src[src.index(f"float {fn}(float x)"):]
+        body = body[:body.index("return")]
+        got = [np.float32(float(v)) for v in re.findall(r"p = (?:fmaf\(p, y, )?(-?[0-9.]+e[-+][0-9]+)f", body)]
+        assert got == want, (path, got)
+    assert c[0] == np.float32(1.0)
+    x = np.exp(np.linspace(-20., 20., 200001)).astype(np.float32)
+    err = np.abs(fit.evaluate(x, c).astype(np.float64) - np.log(x.astype(np.float64)))
+    assert err.max() < 2e-6
+    o = np.array([oracle.lib().orc_mc_log(float(v)) for v in x[::997]], np.float64)
+    assert np.abs(o - np.log(x[::997].astype(np.float64))).max() < 2e-6
