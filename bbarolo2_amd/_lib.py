@@ -131,6 +131,7 @@ _SIGS = {
     "bb_mlp_forward": (C.c_int, [_P, _P, C.c_int, _P, _P]),
     "bb_mlp_grad": (C.c_int, [_P, _P, _P, C.c_int, _P, _P, _P]),
     "bb_mlp_train": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_float, _P, _P]),
+    "bb_mlp_walk": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

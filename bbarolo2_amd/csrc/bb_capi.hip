@@ -712,7 +712,15 @@ extern "C" int bb_launch_mlp_tick(int* count, float* cc, const float* loss_slot,
 extern "C" int bb_launch_mlp_cast(const float* W, int rows, int cols, unsigned short* Wb, unsigned short* WbT,
                                   hipStream_t st);
 
+extern "C" int bb_launch_walk_init(int* cnt, int steps, unsigned char* moved, int n, hipStream_t st);
+extern "C" int bb_launch_walk_prop(const float* x, const float* zl, int n, int din, int s, float* sc, const int* cnt,
+                                   const float* xc, const float* xs, float* prop, float* pn, hipStream_t st);
+extern "C" int bb_launch_walk_accept(const float* prop, const float* y, int n, int din, const float* thr, float* x,
+                                     unsigned char* moved, int* cnt_s, hipStream_t st);
+extern "C" int bb_launch_walk_final(float* sc, const int* cnt, int steps, int n, hipStream_t st);
+
 #define BB_MLP_DP 32          // padded input width (one MFMA K step)
+#define BB_WALK_MAX_STEPS 4096
 
 struct bb_mlp {
     int device;
@@ -737,6 +745,11 @@ struct bb_mlp {
     hipGraphExec_t gx;        // one captured training step (forward, backward, Adam, BF16 copies) for gx_rows rows
     float gx_lr;
     int gx_rows;
+    float *wprop, *wpn;       // device: proposals and standardised inputs of a walk step (nmax x din)
+    int* wcnt;                // device: acceptances per walk step (BB_WALK_MAX_STEPS)
+    hipGraphExec_t wx;        // the last captured walk, for the arguments in wkey
+    const void* wkey[8];
+    int wrows, wsteps;
 };
 
 #define BB_MLP_HIST 4096      // most steps of one bb_mlp_train call
@@ -757,8 +770,9 @@ extern "C" int bb_mlp_destroy(bb_mlp* n) {
     if (!n) return 0;
     if (n->gx) (void)hipGraphExecDestroy(n->gx);
     if (n->cs) (void)hipStreamDestroy(n->cs);
+    if (n->wx) (void)hipGraphExecDestroy(n->wx);
     void* ptrs[] = {n->P, n->Gr, n->Mo, n->Ve, n->Wb, n->X, n->XT, n->Z, n->A, n->AT, n->D, n->DT, n->y, n->t,
-                    n->cnt, n->cc, n->loss_slot, n->hist, n->part};
+                    n->cnt, n->cc, n->loss_slot, n->hist, n->part, n->wprop, n->wpn, n->wcnt};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete n;
@@ -983,6 +997,63 @@ extern "C" int bb_mlp_train(bb_mlp* n, const float* x, const float* t, int rows,
     for (int s = 0; s < steps && !rc; s++) rc = (int)hipGraphLaunch(n->gx, st);
     if (!rc && loss) rc = (int)hipMemcpyAsync(loss, n->hist, (size_t)steps * sizeof(float), hipMemcpyDeviceToDevice, st);
     return rc ? fail(BB_ERR_HIP, "bb_mlp_train: %s", hipGetErrorString((hipError_t)rc)) : 0;
+}
+
+// The walk's launches: counters cleared, then per step the proposals, the forward pass and the acceptance.
+static int mlp_walk_body(bb_mlp* n, float* x, const float* zl, int rows, int steps, const float* thr, const float* xc,
+                         const float* xs, float* sc, unsigned char* moved, hipStream_t st) {
+    int rc = bb_launch_walk_init(n->wcnt, steps, moved, rows, st);
+    for (int s = 0; s < steps && !rc; s++) {
+        rc = bb_launch_walk_prop(x, zl, rows, n->din, s, sc, n->wcnt, xc, xs, n->wprop, n->wpn, st);
+        if (!rc) rc = mlp_forward(n, n->wpn, rows, st);
+        if (!rc) rc = bb_launch_walk_accept(n->wprop, n->y, rows, n->din, thr, x, moved, n->wcnt + s, st);
+    }
+    if (!rc) rc = bb_launch_walk_final(sc, n->wcnt, steps, rows, st);
+    return rc;
+}
+
+extern "C" int bb_mlp_walk(bb_mlp* n, float* x, const float* zl, int rows, int steps, const float* thr,
+                           const float* xc, const float* xs, float* sc, unsigned char* moved, void* stream) {
+    int rc = mlp_check_rows(n, x, rows, "bb_mlp_walk");
+    if (rc) return rc;
+    if (steps < 1 || steps > BB_WALK_MAX_STEPS)
+        return fail(BB_ERR_ARG, "bb_mlp_walk: steps = %d (1..%d)", steps, BB_WALK_MAX_STEPS);
+    if (!zl || !thr || !xc || !xs || !sc || !moved) return fail(BB_ERR_ARG, "bb_mlp_walk: NULL argument");
+    hipStream_t st = (hipStream_t)stream;
+    if (!n->wprop) {
+        rc = hip_check(hipMalloc((void**)&n->wprop, (size_t)n->nmax * n->din * sizeof(float)), "bb_mlp_walk");
+        if (!rc) rc = hip_check(hipMalloc((void**)&n->wpn, (size_t)n->nmax * n->din * sizeof(float)), "bb_mlp_walk");
+        if (!rc) rc = hip_check(hipMalloc((void**)&n->wcnt, BB_WALK_MAX_STEPS * sizeof(int)), "bb_mlp_walk");
+        if (rc) return rc;
+    }
+    const void* key[8] = {x, zl, thr, xc, xs, sc, moved, nullptr};
+    bool same = n->wx && n->wrows == rows && n->wsteps == steps;
+    for (int i = 0; i < 8 && same; i++) same = key[i] == n->wkey[i];
+    if (!same) {
+        // capture the whole walk on the private stream (counters are cleared by a kernel: no memset node)
+        if (n->wx) {
+            (void)hipGraphExecDestroy(n->wx);
+            n->wx = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(n->cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            const int r2 = mlp_walk_body(n, x, zl, rows, steps, thr, xc, xs, sc, moved, n->cs);
+            e = hipStreamEndCapture(n->cs, &g);
+            if (e == hipSuccess && r2) e = (hipError_t)r2;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&n->wx, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            n->wx = nullptr;
+            return fail(BB_ERR_HIP, "bb_mlp_walk: graph capture: %s", hipGetErrorString(e));
+        }
+        memcpy(n->wkey, key, sizeof key);
+        n->wrows = rows;
+        n->wsteps = steps;
+    }
+    rc = (int)hipGraphLaunch(n->wx, st);
+    return rc ? fail(BB_ERR_HIP, "bb_mlp_walk: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
 
 // ---------------------------------------------------------------------------------------------------------

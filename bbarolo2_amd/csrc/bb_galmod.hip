@@ -457,16 +457,20 @@ __device__ __forceinline__ float lds_phi(double t, const float* ph, int ntab) {
     return fmaf(fr, ph[k + 1] - a, a);
 }
 
+// Phi at table coordinate x = 512 (t + 6) from the LDS copy of the table whose first entry is 0 and last 1: x is
+// clamped to the table, so the ends need no test (x <= 0: entry 0; x >= ntab - 1: entry ntab - 2 + 1 x its step).
+__device__ __forceinline__ float lds_phi_x(float x, const float* ph, int ntab) {
+    const float xc = fminf(fmaxf(x, 0.0f), (float)(ntab - 1));
+    const int k = min((int)xc, ntab - 2);
+    const float fr = xc - (float)k;
+    const float a = ph[k];
+    return fmaf(fr, ph[k + 1] - a, a);
+}
+
 // The same with the argument in float32: t is a difference of velocities over sigma (the subtraction is FP64),
 // and x = 512 (t + 6) keeps 2.4e-4 of a table step, far below the table's own interpolation error.
 __device__ __forceinline__ float lds_phi_f(float t, const float* ph, int ntab) {
-    const float x = fmaf(t, 512.0f, 3072.0f);
-    const float xc = fminf(fmaxf(x, 0.0f), (float)(ntab - 2));     // (branch-free: the ends are selected below)
-    const int k = (int)xc;
-    const float fr = xc - (float)k;
-    const float a = ph[k];
-    const float v = fmaf(fr, ph[k + 1] - a, a);
-    return x > 0.0f ? (x < (float)(ntab - 1) ? v : 1.0f) : 0.0f;
+    return lds_phi_x(fmaf(t, 512.0f, 3072.0f), ph, ntab);
 }
 
 // aw x the channel-integrated line of centre v (galmod.py:194-214) into pixel slot of the tile: the rare
@@ -615,7 +619,14 @@ __device__ __forceinline__ void an_lines(const bb_grid& g, const Acc& A, const d
             return LIN ? ((double)e + 0.5 - g.crpix3) * g.cdelt3 + g.crval3 : ve[e];
         };
         const float cf0 = scf[0];
-        float pp = k1 > k0 ? lds_phi_f((float)(edge((int)k0) - vplane) * inv, sph, ntab) : 0.f;
+        // LIN: the table coordinate of edge k0 + e is x0 + e dx, one FP32 fma from the window's first edge (x0 from
+        // the FP64 edge as above; the fma's error, ~1e-6 in t over a window of 64 channels, is that of the FP32
+        // cast of the FP64 difference); the edge index is clamped to the window, so the channels past it take
+        // two equal values and add 0
+        const float x0 = k1 > k0 ? fmaf((float)(edge((int)k0) - vplane) * inv, 512.0f, 3072.0f) : 0.f;
+        const float dx = (float)(g.cdelt3 / u.sig * 512.0);
+        const int ne = (int)(k1 - k0);
+        float pp = k1 > k0 ? lds_phi_x(x0, sph, ntab) : 0.f;
         // blocks of BB_CB channels: the block's profile values first (independent table reads, one wait), then
         // every run adds the block (one address per run, the channels as immediate offsets)
         for (int kb = (int)k0; kb < (int)k1; kb += BB_CB) {
@@ -623,15 +634,23 @@ __device__ __forceinline__ void an_lines(const bb_grid& g, const Acc& A, const d
             float pe[BB_CB], prof[BB_CB];
 #pragma unroll
             for (int j = 0; j < BB_CB; j++) {            // (independent lookups: the loads issue together)
-                const int k = kb + j < nz ? kb + j : nz - 1;      // (past k1: computed, not added)
-                pe[j] = lds_phi_f((float)(edge(k + 1) - vplane) * inv, sph, ntab);
+                if (LIN) {
+                    const int e = min(kb - (int)k0 + j + 1, ne);
+                    pe[j] = lds_phi_x(fmaf((float)e, dx, x0), sph, ntab);
+                } else {
+                    const int k = kb + j < nz ? kb + j : nz - 1;      // (past k1: computed, not added)
+                    pe[j] = lds_phi_f((float)(edge(k + 1) - vplane) * inv, sph, ntab);
+                }
             }
 #pragma unroll
             for (int j = 0; j < BB_CB; j++) {
-                const int k = kb + j < nz ? kb + j : nz - 1;
-                const float c = LIN ? cf0 : scf[k];
-                const float v = fabsf(pe[j] - (j ? pe[j - 1] : pp)) * c;
-                prof[j] = j < nk ? v : 0.f;
+                if (LIN) {
+                    prof[j] = fabsf(pe[j] - (j ? pe[j - 1] : pp)) * cf0;
+                } else {
+                    const int k = kb + j < nz ? kb + j : nz - 1;
+                    const float v = fabsf(pe[j] - (j ? pe[j - 1] : pp)) * scf[k];
+                    prof[j] = j < nk ? v : 0.f;
+                }
             }
             pp = pe[BB_CB - 1];
 #pragma unroll
@@ -764,8 +783,9 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
 // is called for each, e = (chunk << 5) | azimuth lane, one per thread. The list holds neighbouring azimuths of
 // one sub-ring next to each other; lane l of wavefront w takes entry l * W + w of each window of blockDim
 // entries (W wavefronts), so the lanes of one LDS atomic are W azimuths apart (different pixels: same-address
-// atomics of a wavefront serialise) on neighbouring sub-rings (similar work). Every thread of the block calls
-// it (it synchronises). SCAP: the entries of slist.
+// atomics of a wavefront serialise) on neighbouring sub-rings (similar work). The samples of successive batches
+// of chunks accumulate in slist until it is nearly full or the tile's chunks are done (larger windows: fuller
+// wavefronts). Every thread of the block calls it (it synchronises). SCAP: the entries of slist.
 template <int SCAP, class Dep>
 __device__ __forceinline__ void tile_samples(int nchunks, const short4* __restrict__ bb, const short4* __restrict__ sb,
                                              const int* __restrict__ tlist, int tlo, int thi, int tx0, int tx1,
@@ -774,6 +794,20 @@ __device__ __forceinline__ void tile_samples(int nchunks, const short4* __restri
     // the candidate chunks: entries [tlo, thi) of the tile's chunk list, or all the chunks (tlist NULL)
     const int lo = tlist ? tlo : 0, hi = tlist ? thi : nchunks;
     const int lane = threadIdx.x % BB_CS, grp = threadIdx.x / BB_CS;
+    // 3. the ns samples gathered, one per thread (every thread calls it)
+    auto flush = [&](int ns) {
+        const int W = blockDim.x >> 6;
+        for (int base2 = 0; base2 < ns; base2 += blockDim.x) {
+            // (a last, partial window fills as few wavefronts as it can, its lanes still Wa entries apart)
+            const int m = ns - base2;
+            const int Wa = m >= (int)blockDim.x ? W : (m + 63) >> 6;
+            const int idx = (int)(threadIdx.x >> 6) < Wa ? base2 + (threadIdx.x & 63) * Wa + (threadIdx.x >> 6) : ns;
+            if (idx < ns) dep(slist[idx]);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) nsamp = 0;
+        __syncthreads();
+    };
     for (int base = lo; base < hi && active; base += blockDim.x) {
         // 1. the chunks whose bounding box meets the tile
         if (base + (int)threadIdx.x < hi) {
@@ -820,21 +854,15 @@ __device__ __forceinline__ void tile_samples(int nchunks, const short4* __restri
             __syncthreads();
             const int ns = nsamp;
             __syncthreads();
-            if (ns > SCAP - (int)blockDim.x || j0 + ngrp >= n) {
-                // 3. the samples gathered, one per thread
-                const int W = blockDim.x >> 6;
-                for (int base2 = 0; base2 < ns; base2 += blockDim.x) {
-                    const int idx = base2 + (threadIdx.x & 63) * W + (threadIdx.x >> 6);
-                    if (idx < ns) dep(slist[idx]);
-                }
-                __syncthreads();
-                if (threadIdx.x == 0) nsamp = 0;
-                __syncthreads();
-            }
+            if (ns > SCAP - (int)blockDim.x) flush(ns);
         }
         if (threadIdx.x == 0) nlist = 0;
         __syncthreads();
     }
+    // the samples still gathered (every thread reads nsamp after the barrier that ended the loop)
+    const int ns = nsamp;
+    __syncthreads();
+    if (ns > 0) flush(ns);
 }
 
 template <bool LIN>
@@ -872,7 +900,7 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
         scd[i] = cd2i[i];
         scf[i] = (float)cd2i[i];
     }
-    for (int i = threadIdx.x; i < ntab; i += blockDim.x) sph[i] = (float)ptab[i];
+    for (int i = threadIdx.x; i < ntab; i += blockDim.x) sph[i] = i == 0 ? 0.0f : (i == ntab - 1 ? 1.0f : (float)ptab[i]);
     // the fixed-point scale of this vector: amax bounds any single contribution
     double am = 0.;
     for (int s = threadIdx.x; s < nsub; s += blockDim.x) {

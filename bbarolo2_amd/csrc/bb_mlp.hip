@@ -301,3 +301,81 @@ extern "C" int bb_launch_mlp_cast(const float* W, int rows, int cols, unsigned s
                        WbT);
     return (int)hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------------------------------------
+// Random walks on the region {network(x) >= thr} (bb_mlp_walk): per step the proposals x + scale zl[s] and their
+// standardised inputs, the network's forward pass, then the acceptance. The scale of step s follows from that of
+// step s - 1 and its acceptance (every thread computes it; thread 0 records it for the next launches), so the whole
+// walk runs on the device.
+__device__ __forceinline__ float walk_scale(float prev, int acc, int n) {
+    return fminf(fmaxf(prev * expf((float)acc / (float)n - 0.3f), 1e-4f), 3.0f);
+}
+
+__global__ void k_walk_init(int* cnt, int steps, unsigned char* moved, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < steps) cnt[i] = 0;
+    if (i < n) moved[i] = 0;
+}
+
+__global__ void k_walk_prop(const float* __restrict__ x, const float* __restrict__ zl, int n, int din, int s,
+                            float* __restrict__ sc, const int* __restrict__ cnt, const float* __restrict__ xc,
+                            const float* __restrict__ xs, float* __restrict__ prop, float* __restrict__ pn) {
+    const float scale = s > 0 ? walk_scale(sc[s - 1], cnt[s - 1], n) : sc[0];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > 0 && i == 0) sc[s] = scale;
+    if (i >= (int64_t)n * din) return;
+    const int j = (int)(i % din);
+    const float p = x[i] + zl[(size_t)s * n * din + i] * scale;
+    prop[i] = p;
+    pn[i] = (p - xc[j]) / xs[j];
+}
+
+__global__ void k_walk_accept(const float* __restrict__ prop, const float* __restrict__ y, int n, int din,
+                              const float* __restrict__ thr, float* __restrict__ x, unsigned char* __restrict__ moved,
+                              int* __restrict__ cnt_s) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool acc = false;
+    if (i < n) {
+        bool inside = true;
+        for (int j = 0; j < din; j++) {
+            const float p = prop[(size_t)i * din + j];
+            inside = inside && p >= 0.0f && p <= 1.0f;
+        }
+        acc = inside && y[i] >= thr[0];
+        if (acc) {
+            for (int j = 0; j < din; j++) x[(size_t)i * din + j] = prop[(size_t)i * din + j];
+            moved[i] = 1;
+        }
+    }
+    const unsigned long long m = __ballot(acc);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(cnt_s, (int)__popcll(m));
+}
+
+__global__ void k_walk_final(float* sc, const int* cnt, int steps, int n) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc[steps] = walk_scale(sc[steps - 1], cnt[steps - 1], n);
+}
+
+extern "C" int bb_launch_walk_init(int* cnt, int steps, unsigned char* moved, int n, hipStream_t st) {
+    const int m = steps > n ? steps : n;
+    hipLaunchKernelGGL(k_walk_init, dim3((m + 255) / 256), dim3(256), 0, st, cnt, steps, moved, n);
+    return (int)hipGetLastError();
+}
+
+extern "C" int bb_launch_walk_prop(const float* x, const float* zl, int n, int din, int s, float* sc, const int* cnt,
+                                   const float* xc, const float* xs, float* prop, float* pn, hipStream_t st) {
+    const int64_t m = (int64_t)n * din;
+    hipLaunchKernelGGL(k_walk_prop, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, x, zl, n, din, s, sc, cnt, xc,
+                       xs, prop, pn);
+    return (int)hipGetLastError();
+}
+
+extern "C" int bb_launch_walk_accept(const float* prop, const float* y, int n, int din, const float* thr, float* x,
+                                     unsigned char* moved, int* cnt_s, hipStream_t st) {
+    hipLaunchKernelGGL(k_walk_accept, dim3((n + 255) / 256), dim3(256), 0, st, prop, y, n, din, thr, x, moved, cnt_s);
+    return (int)hipGetLastError();
+}
+
+extern "C" int bb_launch_walk_final(float* sc, const int* cnt, int steps, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_walk_final, dim3(1), dim3(64), 0, st, sc, cnt, steps, n);
+    return (int)hipGetLastError();
+}

@@ -26,8 +26,11 @@ on every exactly evaluated proposal, its targets clipped one live-set range belo
 threshold. Its margin is screen_sigma x its RMS error on the newest evaluations it was not trained on; while that
 error is larger than half the live set's range of log-likelihood, or the training set is small, nothing is
 screened (the plain sampler). With the ellipsoid, proposals it predicts below threshold - margin are not
-evaluated. With random walks, the chains walk on the surrogate's region {prediction >= threshold - margin} and
-only their end points are evaluated. Every accepted point is judged by the exact likelihood.
+evaluated. With random walks, the chains walk on the surrogate's region {prediction >= threshold - margin} (on the
+device when the surrogate offers walk(): one captured graph per walk) and only their end points are evaluated; those
+walks take surrogate_walks steps (default 4 x walks): a walk that has not mixed over the region leaves the end points
+that pass too far inside it (the margin shell takes mass from near the boundary), which biases log Z upwards. Every
+accepted point is judged by the exact likelihood.
 """
 
 import math
@@ -95,7 +98,8 @@ class Sampler:
 
     def __init__(self, prior, likelihood, n_dim, n_live=2000, n_batch=None, enlarge_per_dim=1.1, n_update=None,
                  seed=None, surrogate=None, screen_sigma=3.0, train_steps=300, retrain_every=None, train_max=16384,
-                 bound="ellipsoid", walks=25, retrain_growth=1.0, walk_min_pass=0.2, low_pass="exact"):
+                 bound="ellipsoid", walks=25, retrain_growth=1.0, walk_min_pass=0.2, low_pass="exact",
+                 surrogate_walks=None):
         self.prior, self.likelihood = prior, likelihood
         self.n_dim, self.n_live = int(n_dim), int(n_live)
         self.n_batch = int(n_batch) if n_batch else max(100, self.n_live // 4)
@@ -111,6 +115,7 @@ class Sampler:
         if bound not in ("ellipsoid", "rwalk"):
             raise ValueError(f"bound {bound!r}: 'ellipsoid' or 'rwalk'")
         self.bound, self.walks, self.scale = bound, int(walks), 1.0
+        self.surrogate_walks = int(surrogate_walks) if surrogate_walks else 4 * self.walks
         self._last_keep = None
         self._walk_pass = 1.0        # fraction of the last surrogate walks' end points above the threshold
         self.walk_min_pass = float(walk_min_pass)
@@ -120,6 +125,7 @@ class Sampler:
         self._retrain_now = False    # (low_pass "retrain": a low pass rate asks for a training at the next batch)
         self._low_streak = 0         # surrogate walks in a row with a low pass rate
         self.n_train = 0             # trainings of the surrogate
+        self.train_log = []          # (evaluations, threshold, live range, error on fresh points, training rows)
         self.n_walk_judged = 0       # end points of surrogate walks judged by the exact likelihood
         self.n_walk_passed = 0
         self.n_like = 0            # exact likelihood evaluations
@@ -167,6 +173,7 @@ class Sampler:
             U, Y = U[-self.train_max:], Y[-self.train_max:]
         self.surrogate.fit(U, Y, steps=self.train_steps)
         self.n_train += 1
+        self.train_log.append((self.n_like, float(lmin), float(lrange), self.screen_err, len(Y)))
         self._trained_upto = sum(len(a) for a in self._eval_l)
         self.t_surrogate += time.perf_counter() - t0
 
@@ -297,6 +304,14 @@ class Sampler:
         screened = self._screen_ready(lrange) and self._walk_pass > self.walk_min_pass
         margin = self.screen_sigma * self.screen_err if screened else 0.0
         t_sur = 0.0
+        if screened and hasattr(self.surrogate, "walk"):
+            # the surrogate walks the chains on its own device (no host round trip per step)
+            t0 = time.perf_counter()
+            x, moved, self.scale = self.surrogate.walk(x, chol, self.scale, lmin - margin, self.surrogate_walks,
+                                                       int(self.rng.integers(1 << 62)))
+            self.t_surrogate += time.perf_counter() - t0
+            self.n_screened += self.n_batch * self.surrogate_walks
+            return self._judge(x[moved], lmin)
         for _ in range(self.walks):
             prop = x + self.scale * (self.rng.normal(size=x.shape) @ chol.T)
             idx = np.flatnonzero(np.all((prop >= 0) & (prop <= 1), axis=1))
@@ -320,8 +335,11 @@ class Sampler:
         self.t_surrogate += t_sur
         if not screened:
             return x[moved], lx[moved]
-        # the exact likelihood judges every end point
-        xm = x[moved]
+        return self._judge(x[moved], lmin)
+
+    def _judge(self, xm, lmin):
+        """The exact likelihood judges the end points xm of surrogate walks: those above lmin are returned (with
+        their log-likelihoods); the pass rate decides whether the next walks use the surrogate (low_pass)."""
         if len(xm) == 0:
             self._walk_pass = 0.0
             return xm, np.zeros(0)

@@ -63,6 +63,7 @@ class MLPSurrogate:
         """Fresh Glorot parameters (and Adam state)."""
         self.set_params(glorot_params(self.din, self.hidden, self.nhidden, self.seed))
         self.mu, self.sd = 0.0, 1.0
+        self.xc, self.xs = np.zeros(self.din), np.ones(self.din)
 
     def set_params(self, p):
         p = np.ascontiguousarray(p, np.float32)
@@ -116,16 +117,58 @@ class MLPSurrogate:
         return ls[:steps].cpu().numpy()
 
     def fit(self, x, y, steps=400, lr=3e-3, restart=False):
-        """Train on unit-cube points x and values y (standardised: mean 0, deviation 1); returns the last loss."""
+        """Train on unit-cube points x and values y; both are standardised (the inputs per axis to the mean and
+        spread of x, the targets to mean 0 and deviation 1), so that the network sees the region the points fill
+        (a posterior can fill 1e-3 of the cube along an axis) at unit scale. Returns the last loss."""
+        x = np.asarray(x, np.float64)
         y = np.asarray(y, np.float64).ravel()
         if restart:
             self.reset()
         self.mu, self.sd = float(y.mean()), float(y.std()) or 1.0
-        ls = self.train_raw(x, (y - self.mu) / self.sd, steps, lr)
+        self.xc = x.mean(axis=0)
+        self.xs = np.maximum(x.std(axis=0), 1e-12)
+        ls = self.train_raw(self._norm(x), (y - self.mu) / self.sd, steps, lr)
         return float(ls[-1]) if len(ls) else float("nan")
 
+    def _norm(self, x):
+        return ((np.asarray(x, np.float64) - self.xc) / self.xs).astype(np.float32)
+
+    def walk(self, x0, chol, scale, thr, steps, seed):
+        """Random walks on the region {prediction >= thr} on the device (bb_mlp_walk, one captured graph): from the
+        unit-cube points x0 (n, din), `steps` Gaussian steps x + scale z chol^T (z standard normal, torch's generator
+        seeded with seed, drawn before the walk); a step is kept if it stays in the unit cube and its prediction is
+        >= thr; after each step the scale is multiplied by exp(acceptance - 0.3) (clamped to [1e-4, 3]). Returns
+        (end points (n, din) float64, moved (n,) bool, final scale); one synchronisation at the end."""
+        t = self.torch
+        x0 = self._rows(x0)
+        n, d = x0.shape
+        steps = int(steps)
+        if n == 0:
+            return np.zeros((0, d)), np.zeros(0, bool), float(scale)
+        key = (n, steps)
+        if getattr(self, "_wbuf_key", None) != key:
+            f32 = dict(dtype=t.float32, device=self.dev)
+            self._wbuf = dict(x=t.empty((n, d), **f32), zl=t.empty((steps * n, d), **f32), sc=t.empty(steps + 1, **f32),
+                              thr=t.empty(1, **f32), xc=t.empty(d, **f32), xs=t.empty(d, **f32),
+                              moved=t.empty(n, dtype=t.uint8, device=self.dev))
+            self._wbuf_key = key
+        b = self._wbuf
+        g = t.Generator(device=self.dev)
+        g.manual_seed(int(seed))
+        L = self._dev(np.asarray(chol, np.float64).T)
+        t.matmul(t.randn((steps * n, d), generator=g, device=self.dev), L, out=b["zl"])
+        b["x"].copy_(t.from_numpy(x0))
+        b["xc"].copy_(t.from_numpy(np.asarray(self.xc, np.float32)))
+        b["xs"].copy_(t.from_numpy(np.asarray(self.xs, np.float32)))
+        b["sc"][:1].fill_(float(scale))
+        b["thr"].fill_(float((thr - self.mu) / self.sd))
+        _lib.check(_lib.lib().bb_mlp_walk(self.handle, b["x"].data_ptr(), b["zl"].data_ptr(), n, steps,
+                                          b["thr"].data_ptr(), b["xc"].data_ptr(), b["xs"].data_ptr(),
+                                          b["sc"].data_ptr(), b["moved"].data_ptr(), _stream_ptr(t)))
+        return b["x"].double().cpu().numpy(), b["moved"].cpu().numpy().astype(bool), float(b["sc"][steps].item())
+
     def predict(self, x):
-        x = np.atleast_2d(np.asarray(x, np.float32))
+        x = self._norm(np.atleast_2d(x))
         out = np.empty(x.shape[0])
         for a in range(0, x.shape[0], self.nmax):
             out[a:a + self.nmax] = self.raw_forward(x[a:a + self.nmax])

@@ -361,35 +361,49 @@ def multi_galaxy_bench(n=64, B=32, steps=3, seed=11):
             "tune_s": round(t_tune, 2), "tiles": shapes}
 
 
-def bayes_fits(n_live=500, walks=25, seed=3):
-    """The second half of the metric: wall time of the Bayesian fit of the 64x64x40 cube (6 rings, free VROT VDISP
-    INC: 13-D), the nested sampler with constrained random walks, plain and with its walks screened by the BF16
-    surrogate trained on the device; and how far the screened posterior is from the plain one."""
+SCREEN = dict(low_pass="retrain", walk_min_pass=0.1, surrogate_walks=100)
+
+
+def _bayes_pair(cfg, n_live, walks, seed, train_max, graphs):
+    """The plain and the screened nested sampling run of configuration cfg (same seed): wall times and samplers."""
     from bbarolo2_amd import bayes
-    fit = bayes.synthetic_fit(5, chunk=n_live)
+    fit = bayes.synthetic_fit(cfg, chunk=n_live, graphs=graphs)
     res = {}
     for scr in (False, True):
         t0 = time.perf_counter()
-        kw = dict(low_pass="retrain", walk_min_pass=0.1, train_max=8192) if scr else {}
+        kw = dict(SCREEN, train_max=train_max) if scr else {}
         s = bayes.nested_fit(fit, n_live=n_live, n_batch=n_live, seed=seed, surrogate=scr, bound="rwalk",
                              walks=walks, **kw)
         res[scr] = (time.perf_counter() - t0, s)
-        print(f"[bench] bayes fit surrogate={scr}: {res[scr][0]:.1f} s, {s.n_like} likelihood evaluations",
-              file=sys.stderr, flush=True)
+        print(f"[bench] config {cfg} bayes fit surrogate={scr}: {res[scr][0]:.1f} s, {s.n_like} likelihood "
+              "evaluations", file=sys.stderr, flush=True)
     (tp, sp), (ts, ss) = res[False], res[True]
     shift, width = bayes.compare(sp, ss)
-    return {"workload": "64x64x40 cube, 20 km/s channels, beam 4x3 px at PA 30, 6 rings, ANALYTIC, free VROT VDISP "
-                        "INC (13-D), DENS from the deconvolved profile of every vector; nested sampling, "
-                        f"n_live = n_batch = {n_live}, random walks of {walks} steps, f_live 0.01; screened: walks on "
-                        "the surrogate's region, end points judged exactly, retrained at once when under 10% pass "
-                        "(most recent 8192 points)",
-            "plain_s": round(tp, 3), "surrogate_s": round(ts, 3), "speedup": round(tp / ts, 3),
-            "plain_n_like": sp.n_like, "surrogate_n_like": ss.n_like, "surrogate_screened": ss.n_screened,
-            "surrogate_train_predict_s": round(ss.t_surrogate, 3), "surrogate_trainings": ss.n_train,
+    return {"plain_s": round(tp, 3), "surrogate_s": round(ts, 3), "speedup": round(tp / ts, 3),
+            "plain_n_like": sp.n_like, "surrogate_n_like": ss.n_like, "surrogate_walk_steps": ss.n_screened,
+            "surrogate_train_walk_s": round(ss.t_surrogate, 3), "surrogate_trainings": ss.n_train,
             "walk_end_points_judged": ss.n_walk_judged, "walk_end_points_passed": ss.n_walk_passed,
-            "plain_like_s": round(sp.t_like, 3),
-            "surrogate_like_s": round(ss.t_like, 3), "log_z": [sp.log_z, ss.log_z],
-            "median_shift_sigma_max": shift, "ci68_width_change_max": width}
+            "plain_like_s": round(sp.t_like, 3), "surrogate_like_s": round(ss.t_like, 3),
+            "log_z": [sp.log_z, ss.log_z], "median_shift_sigma_max": shift, "ci68_width_change_max": width}
+
+
+def bayes_fits(walks=25, seed=3):
+    """The second half of the metric: wall time of the batched Bayesian fit of the 128x128x64 cube (8 rings, free
+    VROT VDISP: 16-D; 2000 live points per batch, batches as captured HIP graphs), the nested sampler with
+    constrained random walks, plain and with its walks screened by the BF16 surrogate trained on the device; and how
+    far the screened posterior is from the plain one. The same for the 64x64x40 surrogate configuration (13-D,
+    500 live points), where an exact evaluation costs ~3 us and the network's overhead is of the same order."""
+    out = {"workload": "128x128x64 cube, 10 km/s channels, beam 40\", 8 rings, ANALYTIC, free VROT VDISP (16-D), DENS "
+                       "from the deconvolved profile of every vector; nested sampling, n_live = n_batch = 2000, "
+                       f"random walks of {walks} steps, f_live 0.01, batches as captured HIP graphs; screened: walks "
+                       "on the surrogate's region on the device, end points judged exactly, retrained at once when "
+                       "under 10% pass (most recent 16384 points)"}
+    out.update(_bayes_pair(3, 2000, walks, seed, 16384, True))
+    small = {"workload": "64x64x40 cube, 20 km/s channels, beam 4x3 px at PA 30, 6 rings, free VROT VDISP INC "
+                         "(13-D), n_live = n_batch = 500, otherwise as above (most recent 8192 points)"}
+    small.update(_bayes_pair(5, 500, walks, seed, 8192, False))
+    out["config_64x64x40"] = small
+    return out
 
 
 if __name__ == "__main__":

@@ -258,6 +258,16 @@ int bb_mlp_grad(bb_mlp* net, const float* x, const float* t, int rows, float* gr
  * step) may be NULL. Deterministic (fixed-order reductions, no atomics). */
 int bb_mlp_train(bb_mlp* net, const float* x, const float* t, int rows, int steps, float lr, float* loss,
                  void* stream);
+/* Random walks on the network's region {net(x) >= thr} for a sampler's proposals (the screened random walks of the
+ * nested sampler; the reference samples with nautilus's neural bounds instead, bbarolo/bayesian/nautilus/bounds/
+ * neural.py, so this call has no counterpart there). rows walkers x (rows, din) float32 take `steps` steps: the
+ * proposal x + sc[s] zl[s] (zl (steps, rows, din): the Gaussian steps, already multiplied by the Cholesky factor)
+ * is kept if it lies in the unit cube and net((proposal - xc) / xs) >= thr[0]; x is updated in place and moved
+ * (rows, uint8) marks the walkers that moved. sc (steps + 1 floats): sc[0] the first step's scale; the call writes
+ * sc[s] = clamp(sc[s-1] exp(acceptance of step s-1 - 0.3), 1e-4, 3) for s = 1..steps. All DEVICE arrays; the walk
+ * is one captured HIP graph, replayed while the arguments are the same. steps <= 4096. */
+int bb_mlp_walk(bb_mlp* net, float* x, const float* zl, int rows, int steps, const float* thr, const float* xc,
+                const float* xs, float* sc, uint8_t* moved, void* stream);
 
 /* ---- per-fit set-up: the likelihood mask -----------------------------------------------------------------------
  * Replaces dilate_mask (bbarolo/bayesian/noise.py:18-30; called by _setup_likemask, likelihood.py:160-175, with the

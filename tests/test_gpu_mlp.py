@@ -92,3 +92,59 @@ def test_training_fits_a_smooth_function():
     pred = net.predict(xv)
     rms = np.sqrt(np.mean((pred - tv) ** 2))
     assert rms < 0.1 * tv.std(), (rms, tv.std())
+
+
+@pytest.mark.parametrize("n,steps", [(37, 5), (600, 40)])
+def test_walk_matches_its_steps(n, steps):
+    """bb_mlp_walk (the whole walk as one captured graph) against its steps replayed from the host with the same
+    Gaussian steps and the scales the walk recorded: the same proposals, the network's forward pass on each, the same
+    acceptances, end points and moved flags, bit for bit; a second call with other values replays the graph."""
+    import torch
+    from bbarolo2_amd import _lib
+    net = _net(seed=4)
+    x, t = _data(2048, seed=5)
+    net.fit(x, t, steps=200)
+    rng = np.random.default_rng(n)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    for call in range(2):
+        x0 = (0.3 + 0.4 * rng.random((n, DIN))).astype(np.float32)
+        zl = (rng.normal(size=(steps, n, DIN)) * 0.05).astype(np.float32)
+        thr = np.float32(np.median(net.raw_forward(x0)) - 0.2)
+        xc, xs = net.xc.astype(np.float32), net.xs.astype(np.float32)
+        dx = torch.from_numpy(x0.copy()).to(dev)
+        dz = torch.from_numpy(zl).to(dev)
+        dthr = torch.tensor([thr], dtype=torch.float32, device=dev)
+        dxc, dxs = torch.from_numpy(xc).to(dev), torch.from_numpy(xs).to(dev)
+        dsc = torch.zeros(steps + 1, dtype=torch.float32, device=dev)
+        dsc[0] = 0.7 + 0.2 * call
+        dmv = torch.empty(n, dtype=torch.uint8, device=dev)
+        if call == 1:                       # the same buffers as a replay would use: the walk writes in place
+            dx, dz, dthr, dxc, dxs, dsc, dmv = keep
+            dx.copy_(torch.from_numpy(x0))
+            dz.copy_(torch.from_numpy(zl))
+            dthr.fill_(float(thr))
+            dsc.zero_()
+            dsc[0] = 0.9
+        keep = (dx, dz, dthr, dxc, dxs, dsc, dmv)
+        _lib.check(_lib.lib().bb_mlp_walk(net.handle, dx.data_ptr(), dz.data_ptr(), n, steps, dthr.data_ptr(),
+                                          dxc.data_ptr(), dxs.data_ptr(), dsc.data_ptr(), dmv.data_ptr(), None))
+        torch.cuda.synchronize()
+        sc = dsc.cpu().numpy()
+        xh = x0.copy()
+        mv = np.zeros(n, bool)
+        nacc = 0
+        for s in range(steps):
+            prop = (xh + zl[s] * sc[s]).astype(np.float32)
+            pn = ((prop - xc) / xs).astype(np.float32)
+            y = net.raw_forward(pn).astype(np.float32)
+            acc = np.all((prop >= 0) & (prop <= 1), axis=1) & (y >= thr)
+            xh[acc] = prop[acc]
+            mv |= acc
+            nacc += int(acc.sum())
+            want = np.float32(min(max(np.float32(sc[s]) * np.exp(np.float32(acc.sum() / n) - np.float32(0.3)),
+                                      1e-4), 3.0))
+            assert abs(sc[s + 1] - want) <= 4e-7 * want, (s, sc[s + 1], want)
+        print(f"[walk n={n} steps={steps} call {call}] moved {mv.sum()}, {nacc} of {n * steps} steps accepted, "
+              f"final scale {sc[-1]:.4f}", flush=True)
+        assert 0 < nacc < n * steps
+        assert np.array_equal(dx.cpu().numpy(), xh) and np.array_equal(dmv.cpu().numpy().astype(bool), mv)
