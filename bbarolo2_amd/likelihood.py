@@ -329,6 +329,12 @@ class GpuLikelihood:
             raise ValueError(f"theta has {theta.shape[1]} columns, the fit {self.free.ndim}")
         return theta
 
+    @property
+    def ctx(self):
+        """The bb_ctx of the fit (a C pointer): bb_loglike_batch(ctx, theta, B, loglike, dens, stream) evaluates a
+        batch with no Python in the way."""
+        return self._ctx
+
     def loglike(self, theta):
         """log-likelihood of each row of theta (B, ndim) -> (B,) float64 (-inf where the likelihood is zero)."""
         theta = self._theta(theta)
