@@ -32,8 +32,8 @@ KEYS = {VROT: "vrot", VDISP: "vdisp", DENS: "dens", Z0: "z0", INC: "inc", PA: "p
 
 @dataclass
 class Fit3DPar:
-    """The 3DFIT parameters of the ring fits (the defaults of bbarolo/params.py:170-200)."""
-    FREE: str = "VROT VDISP"
+    """The 3DFIT parameters of the ring fits (the defaults of bbarolo/params.py:177-216)."""
+    FREE: str = "VROT VDISP INC PA"
     DELTAVROT: float = 1000.0
     DELTAINC: float = 5.0
     DELTAPHI: float = 15.0
@@ -43,7 +43,7 @@ class Fit3DPar:
     WFUNC: int = 2
     TOL: float = 1e-3
     BWEIGHT: float = 1.0
-    NORM: str = "LOCAL"
+    NORM: str = "AZIM"
     SIDE: str = "B"
 
 

@@ -6,8 +6,9 @@
     FTYPE 2, NORM LOCAL, WFUNC 2): VROT and VDISP recovered, and the LOCAL final model equal to the oracle's
     normalisation (galfit_out.py _local) of the same model;
   * the reference's own quick fit of its NGC 2403 cube (tests/test_bbarolo.py:306-317: rings 105:400:90", free VROT
-    VDISP, WFUNC 2, no second stage) on the committed crop of that cube (tests/golden/ngc2403_crop.npz), with the
-    stand-in mask of the source finder: the reference's ranges (VROT rising, 60 < VROT[0] < 100, 110 < VROT[-1] < 135).
+    VDISP, WFUNC 2, no second stage, the other keywords at their defaults), read from a parameter file, on the
+    committed crop of that cube (tests/golden/ngc2403_crop.npz) with the stand-in mask of the source finder: the
+    reference's ranges (VROT rising, 60 < VROT[0] < 100, 110 < VROT[-1] < 135).
 """
 import json
 import os
@@ -35,7 +36,7 @@ def _ringfit(small_header, gtype):
     data = m.transpose(2, 0, 1).astype(np.float32)
     mask = data > 0.02 * data.max()
     inr = small_rings(8, z0=5., vrot=130., vdisp=20.)
-    return RingFit(dm, data, mask, inr, par=Fit3DPar(FREE="VROT VDISP"), data_noise=0.01 * float(data.max()))
+    return RingFit(dm, data, mask, inr, par=Fit3DPar(FREE="VROT VDISP", NORM="LOCAL"), data_noise=0.01 * float(data.max()))
 
 
 @pytest.mark.parametrize("gtype", ["ANALYTIC", "MC"])
@@ -100,18 +101,33 @@ def _ngc2403():
     return Header(cards), data, int(z["offset"])
 
 
-def test_ngc2403_quick_ranges():
+def test_ngc2403_quick_ranges(tmp_path):
+    """From a parameter file with the reference's defaults (NORM AZIM, MC models, FTYPE 2): the quick fit's rings,
+    centre in the coordinates of the crop."""
     import torch
-    from bbarolo2_amd.classic import ClassicFit
-    from bbarolo2_amd.galfit import Fit3DPar
-    from bbarolo2_amd.rings import Rings
+    from bbarolo2_amd.params import ParFile
     torch.cuda.set_device(0)
     h, data, off = _ngc2403()
-    inr = Rings().setRings(radii=np.arange(105., 400., 90.), xpos=77. - off, ypos=77. - off, vsys=132.8, vrot=120.,
-                           vdisp=8., vrad=0., z0=10., inc=60., phi=123.7, dens=1.)
-    fp = Fit3DPar(FREE="VROT VDISP", WFUNC=2)
-    cf = ClassicFit(h, data, inr, par=fp, gpar=GalmodPar(GALMODTYPE="MC"), twostage=False).run()
+    par = tmp_path / "ngc2403_quick.par"
+    par.write_text(f"""FITSFILE  ngc2403_crop.fits
+3DFIT     true
+RADII     range(105,400,90)
+XPOS      {77 - off}
+YPOS      {77 - off}
+VSYS      132.8
+VROT      120
+VDISP     8
+VRAD      0
+Z0        10
+INC       60
+PA        123.7
+FREE      VROT VDISP
+WFUNC     2
+TWOSTAGE  false
+""")
+    cf = ParFile.read(str(par)).classic_fit(h, data).run()
     v = np.asarray(cf.rings.vrot)
     print("NGC 2403 vrot", np.round(v, 2), "vdisp", np.round(cf.rings.vdisp, 2), "ok", cf.fitok, flush=True)
+    assert cf.par.NORM == "AZIM" and cf.gpar.GALMODTYPE == "MC"
     assert all(cf.fitok)
     assert np.all(np.diff(v) > 0) and 60 < v[0] < 100 and 110 < v[-1] < 135

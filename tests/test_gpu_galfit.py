@@ -27,7 +27,7 @@ def _setup(small_header, galmodtype="ANALYTIC"):
     data = m.transpose(2, 0, 1).astype(np.float32)
     mask = data > 0.02 * data.max()
     inr = small_rings(8, z0=5., vrot=130., vdisp=20.)
-    fit = RingFit(dm, data, mask, inr, par=Fit3DPar(FREE="VROT VDISP"), data_noise=0.01 * float(data.max()))
+    fit = RingFit(dm, data, mask, inr, par=Fit3DPar(FREE="VROT VDISP", NORM="LOCAL"), data_noise=0.01 * float(data.max()))
     return dm, fit, data, mask
 
 
