@@ -90,6 +90,20 @@ _SIGS = {
     "bb_conv_residual_work_size": (C.c_int64, [_P, C.c_int, C.c_int, C.c_int]),
     "bb_dilate_mask": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _P, _P, _P]),
     "bb_dilate_rmax": (C.c_int, []),
+    "bb_noise_flags": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P]),
+    "bb_masked_median_work_size": (C.c_int64, []),
+    "bb_masked_median": (C.c_int, [_P, _P, C.c_int64, _P, _P, _P, _P, _P]),
+    "bb_masked_std_work_size": (C.c_int64, []),
+    "bb_masked_std": (C.c_int, [_P, _P, C.c_int64, _P, _P, _P, _P]),
+    "bb_flag_subsample_work_size": (C.c_int64, [C.c_int64, C.c_int64]),
+    "bb_flag_subsample": (C.c_int, [_P, C.c_int64, C.c_int64, _P, C.c_int64, _P, C.c_int64, _P, _P]),
+    "bb_box_dilate": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P]),
+    "bb_lag_xy_max": (C.c_int, []),
+    "bb_lag_xy_work_size": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int]),
+    "bb_lag_xy": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _P, C.c_int64, _P, _P, _P]),
+    "bb_lag_z_work_size": (C.c_int64, [C.c_int, C.c_int, C.c_int]),
+    "bb_lag_z": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _P, C.c_int64, _P, _P,
+                           _P]),
     "bb_conv_residual_gauss": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _P, C.c_int64,
                                          _P, _P, _P]),
     "bb_conv_residual_noalloc_max": (C.c_int64, []),

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbbgpu.so")
 SOURCES = ["bb_capi.hip", "bb_like.hip", "bb_galmod.hip", "bb_conv.hip", "bb_resid.hip", "bb_profile.hip", "bb_mc.hip",
-           "bb_norm.hip", "bb_mlp.hip", "bb_setup.hip"]
+           "bb_norm.hip", "bb_mlp.hip", "bb_setup.hip", "bb_noise.hip"]
 OBJDIR = os.path.join(HERE, "_build")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -279,6 +279,58 @@ int bb_dilate_mask(const uint8_t* mask, int nz, int ny, int nx, double radius, i
                    uint8_t* out, void* stream);
 int bb_dilate_rmax(void);
 
+/* ---- per-fit set-up: the noise ------------------------------------------------------------------------------------
+ * The statistics of Bayesian3DFIT._setup_noise (the noise rms, bbarolo/bayesian/likelihood.py:92-111) and of
+ * noise_correlation (bbarolo/bayesian/noise.py:33-186, called by _set_likefactor, likelihood.py:262-268). Cubes are
+ * DEVICE (nz, ny, nx) arrays, data float32; every call is asynchronous on stream.
+ *
+ * The voxels used: finite data (and non-zero when zero_blank: the blanks of likelihood.py:496-499) outside emis
+ * (may be NULL: nothing excluded; noise.py:75-90 with the dilated mask, likelihood.py:105-108 with the mask). used
+ * (uint8 0 / 1) per voxel; chan_mask and chan_used (uint64, nz) the voxels of mask (may be NULL) and the used voxels
+ * per channel; pix_emis (uint8, ny x nx) whether any channel of the pixel is in emis and pix_used (uint32) its used
+ * voxels (both may be NULL). */
+int bb_noise_flags(const float* data, const uint8_t* mask, const uint8_t* emis, int nz, int ny, int nx, int zero_blank,
+                   uint8_t* used, uint64_t* chan_mask, uint64_t* chan_used, uint8_t* pix_emis, uint32_t* pix_used,
+                   void* stream);
+/* numpy.median of the float32 values x[i] with sel[i] != 0 (i < n), or of |x[i] - center[0]| (float32) when center
+ * (DEVICE, 1 float) is not NULL (likelihood.py:109; noise.py:93-96): out (DEVICE float) the middle value or the float32
+ * mean of the two middle values, NaN when nothing is selected; count (DEVICE int64, may be NULL) the number selected.
+ * Exact (radix selection). work: bb_masked_median_work_size() DEVICE bytes. */
+int64_t bb_masked_median_work_size(void);
+int bb_masked_median(const float* x, const uint8_t* sel, int64_t n, const float* center, void* work, float* out,
+                     int64_t* count, void* stream);
+/* The standard deviation of the selected values (ddof 0), the fallback of likelihood.py:110-111 (numpy.nanstd): FP64
+ * sums rounded to float32; count (DEVICE int64) the number selected (as bb_masked_median returns it). */
+int64_t bb_masked_std_work_size(void);
+int bb_masked_std(const float* x, const uint8_t* sel, int64_t n, const int64_t* count, void* work, float* out,
+                  void* stream);
+/* The subsample of noise.py:94-95: out[i] = 1 for the flagged voxels (flags[i] != 0, nflag of them) whose rank among
+ * the flagged voxels (C order) is one of ranks (DEVICE int64, nr of them, from numpy's
+ * default_rng(0).choice(nflag, 2000000, replace=False)). work: bb_flag_subsample_work_size(n, nflag) DEVICE bytes. */
+int64_t bb_flag_subsample_work_size(int64_t n, int64_t nflag);
+int bb_flag_subsample(const uint8_t* flags, int64_t n, int64_t nflag, const int64_t* ranks, int64_t nr, void* work,
+                      int64_t work_bytes, uint8_t* out, void* stream);
+/* The binary dilation of m by a box of half-sizes (rz, ry, rx) (_dilate, noise.py:8-15; zero outside the cube): out
+ * (uint8 0 / 1); tmp (same size) is needed with more than one non-zero half-size. */
+int bb_box_dilate(const uint8_t* m, int nz, int ny, int nx, int rz, int ry, int rx, uint8_t* tmp, uint8_t* out,
+                  void* stream);
+/* The spatial autocorrelation sums of noise.py:103-119 without FFTs: for the lags (dy, dx) of the half-plane dy > 0
+ * or (dy = 0, 0 <= dx <= L), in the order (0, 0..L), (1, -L..L), ..., (L, -L..L) (2L^2 + 2L + 1 of them),
+ *     c[l] = sum over the channels with chan_use[z] and the pixels p of w(z, p) w(z, p + lag) (data - med)(data' - med)
+ *     wsum[l] = the number of such pairs,
+ * FP64 (DEVICE arrays; the other half-plane has the same sums); w the used voxels. work: bb_lag_xy_work_size doubles.
+ * 1 <= L <= bb_lag_xy_max(). */
+int bb_lag_xy_max(void);
+int64_t bb_lag_xy_work_size(int nz, int ny, int nx, int L);
+int bb_lag_xy(const float* data, const uint8_t* w, const uint8_t* chan_use, int nz, int ny, int nx, int L, double med,
+              double* work, int64_t work_len, double* c, double* wsum, void* stream);
+/* The spectral sums of noise.py:136-173: for t = 0..K (K <= 7), c[t] = sum over the pixels with pix_use (NULL: all)
+ * and the pairs of channels (z, z + t) of the same run (runid[z] >= 0, DEVICE int32, nz) of w w' (data - med)
+ * (data' - med), wsum[t] the number of pairs. work: bb_lag_z_work_size doubles. */
+int64_t bb_lag_z_work_size(int ny, int nx, int K);
+int bb_lag_z(const float* data, const uint8_t* w, const uint8_t* pix_use, const int32_t* runid, int nz, int ny, int nx,
+             int K, double med, double* work, int64_t work_len, double* c, double* wsum, void* stream);
+
 /* ---- the fused likelihood of a fit --------------------------------------------------------------------------------
  * Bayesian3DFIT._log_likelihood (bbarolo/bayesian/likelihood.py:567-619) of batches of parameter vectors, run by the
  * library from theta to log L: the seam that nautilus's vectorized sampler calls (bayesian/sampling.py:235-241). Per
