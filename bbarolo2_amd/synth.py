@@ -9,7 +9,8 @@ these simplifications, stated once here:
   * the emission mask is the truth model above 2 noise rms (the reference's default is MASK SEARCH);
   * the likelihood mask dilates it by one beam FWHM in each channel and +-3 channels (noise.dilate_mask on the
     GPU, bayesian/noise.py:18-30, the default LIKEOPTS dilation (1, 3));
-  * sigma is 1.4826 x the MAD of the data outside the emission mask (likelihood.py:107-114);
+  * sigma is 1.4826 x the MAD of the data outside the emission mask (noise.noise_rms on the GPU,
+    likelihood.py:105-111);
   * likefactor is 1 / beam area in pixels (LIKEOPTS noisecorr='beam');
   * the region is the box of the model of the initial rings (DiskModel.box, final=True).
 """
@@ -21,7 +22,7 @@ import numpy as np
 from .header import emptyheader, beam_area_pix
 from .rings import Rings
 from .galmod import GalmodPar
-from .noise import dilate_mask
+from .noise import dilate_mask, noise_rms as device_noise_rms
 
 
 def config(n):
@@ -161,8 +162,7 @@ def likelihood_inputs(h, truth_zyx, data, noise_rms, box):
     mask = truth_zyx > 2. * noise_rms
     bpix = abs(h.Bmaj() / h.Cdelt(0)) if h.Bmaj() else 1.
     likemask = dilate_mask(mask, bpix, 3)          # (on the GPU)
-    out = data[~mask & (data != 0)]
-    sigma = float(1.4826 * np.median(np.abs(out - np.median(out)))) if out.size else float(noise_rms)
+    sigma = device_noise_rms(data, mask)           # (on the GPU)
     nz, ny, nx = data.shape
     region = np.zeros((ny, nx), bool)
     (blo, bhi) = box
