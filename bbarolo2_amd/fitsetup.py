@@ -16,6 +16,7 @@ import math
 
 import numpy as np
 
+from .engine import _torch
 from .galmod import model_box
 from .likelihood import GpuLikelihood, float32_rings
 
@@ -131,8 +132,8 @@ def likelihood_region(header, rings, free, lo, hi, ringmode="CENTRED", side=None
 def likelihood_mask(header, mask, likemask="dilated", dilation=(1., 3)):
     """The voxels of the likelihood (_setup_likemask, likelihood.py:155-175) as a uint8 device tensor: the mask
     dilated by dilation[0] beam FWHM in each channel and +-dilation[1] channels ('dilated'), or every voxel ('none')."""
-    import torch
     from .noise import dilate_mask_device, _flags
+    torch = _torch()
     mode = str(likemask).lower()
     if mode not in ("dilated", "none"):
         raise ValueError(f"ERROR! likemask must be 'dilated' or 'none', not {likemask!r}.")
@@ -156,11 +157,16 @@ class LikelihoodSetup:
     def __init__(self, dm, data, mask, rings, free, lo, hi, options=None, side=None):
         from .noise import setup_noise
         from .params import LIKE_DEFAULTS
+        from .noise import _cube, _flags
+        torch = _torch()
         opts = dict(LIKE_DEFAULTS)
         opts.update(options or {})
         h = dm.header
-        self.noise = setup_noise(h, data, mask, opts["noisecorr"])
-        self.likemask = likelihood_mask(h, mask, opts["likemask"], opts["dilation"])
+        d = _cube(torch, data)                           # (the cube and the mask on the device once)
+        m = _flags(torch, mask, d.shape)
+        self.noise = setup_noise(h, d, m, opts["noisecorr"])
+        self.likemask = likelihood_mask(h, m, opts["likemask"], opts["dilation"])
+        del d, m
         self.likebox, self.region = likelihood_region(h, rings, free, lo, hi, dm.ringmode, side)
         self.sigma = float(opts["sigma"]) if opts.get("sigma") else self.noise["noiserms"]
         self.fracerr = opts.get("fracerr")

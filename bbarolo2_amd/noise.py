@@ -368,6 +368,10 @@ def setup_noise(header, data, mask=None, noisecorr="measured"):
     dict, or None when not measured or not measurable), likefactor, likefactor2d and note. The zeros of the cube are
     blanks; the correlation uses the beam FWHM in pixels, |BMAJ / CDELT1|."""
     from .header import beam_area_pix
+    torch = _torch()
+    data = _cube(torch, data)                        # (on the device once, for both passes)
+    if mask is not None:
+        mask = _flags(torch, mask, data.shape)
     rms = noise_rms(data, mask)
     barea = beam_area_pix(header)
     nc = None
