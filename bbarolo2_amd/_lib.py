@@ -98,6 +98,7 @@ _SIGS = {
     "bb_flag_subsample_work_size": (C.c_int64, [C.c_int64, C.c_int64]),
     "bb_flag_subsample": (C.c_int, [_P, C.c_int64, C.c_int64, _P, C.c_int64, _P, C.c_int64, _P, _P]),
     "bb_box_dilate": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P]),
+    "bb_like_maps": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, bb_box, _P, _P, _P, _P, _P]),
     "bb_lag_xy_max": (C.c_int, []),
     "bb_lag_xy_work_size": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "bb_lag_xy": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _P, C.c_int64, _P, _P, _P]),

@@ -580,30 +580,55 @@ extern "C" int bb_ctx_create(const bb_model* m, const bb_like_desc* d, const flo
     c->bx = c->db.bhi0 - c->db.blo0;
     c->by = c->db.bhi1 - c->db.blo1;
     c->vox = (int64_t)c->by * c->bx * nz;
-    std::vector<double> dm2((size_t)ny * nx, 0.);
-    std::vector<float> dv((size_t)c->vox, __builtin_nanf(""));
+    // one pass over the cube on the device (bb_like_maps): the data of the likelihood in the box and, per pixel, the
+    // squared data and number of the likelihood voxels; then the table on the host, summed down the columns first
+    // and along the rows next, as numpy's cumsum(cumsum(a, 0), 1) (likelihood.py:222-229)
+    const size_t npix = (size_t)ny * nx, nvx = npix * nz;
+    float* t_data = nullptr;
+    uint8_t *t_lm = nullptr, *t_reg = nullptr;
+    double* t_dm2 = nullptr;
+    int32_t* t_nv = nullptr;
+    float* dvp = nullptr;
+    rc = dalloc(c, &dvp, (size_t)c->vox, "bb_ctx_create: data");
+    c->dvalid = dvp;
+    if (!rc) rc = bb_hip_check(hipMalloc((void**)&t_data, nvx * sizeof(float)), "bb_ctx_create: staging");
+    if (!rc && likemask) rc = bb_hip_check(hipMalloc((void**)&t_lm, nvx), "bb_ctx_create: staging");
+    if (!rc && region) rc = bb_hip_check(hipMalloc((void**)&t_reg, npix), "bb_ctx_create: staging");
+    if (!rc) rc = bb_hip_check(hipMalloc((void**)&t_dm2, npix * sizeof(double)), "bb_ctx_create: staging");
+    if (!rc) rc = bb_hip_check(hipMalloc((void**)&t_nv, npix * sizeof(int32_t)), "bb_ctx_create: staging");
+    if (!rc) rc = bb_hip_check(hipMemcpy(t_data, data, nvx * sizeof(float), hipMemcpyHostToDevice), "bb_ctx_create");
+    if (!rc && t_lm) rc = bb_hip_check(hipMemcpy(t_lm, likemask, nvx, hipMemcpyHostToDevice), "bb_ctx_create");
+    if (!rc && t_reg) rc = bb_hip_check(hipMemcpy(t_reg, region, npix, hipMemcpyHostToDevice), "bb_ctx_create");
+    if (!rc) {
+        const bb_box db = {c->db.blo0, c->db.blo1, c->db.bhi0, c->db.bhi1};
+        rc = bb_like_maps(t_data, t_lm, t_reg, nz, ny, nx, db, dvp, t_dm2, t_nv, nullptr, nullptr);
+    }
+    std::vector<double> dm2(npix, 0.);
+    std::vector<int32_t> nvm(npix, 0);
+    if (!rc) rc = bb_hip_check(hipMemcpy(dm2.data(), t_dm2, npix * sizeof(double), hipMemcpyDeviceToHost),
+                               "bb_ctx_create");
+    if (!rc) rc = bb_hip_check(hipMemcpy(nvm.data(), t_nv, npix * sizeof(int32_t), hipMemcpyDeviceToHost),
+                               "bb_ctx_create");
+    for (void* q : {(void*)t_data, (void*)t_lm, (void*)t_reg, (void*)t_dm2, (void*)t_nv})
+        if (q) (void)hipFree(q);
+    if (rc) {
+        bb_ctx_destroy(c);
+        return rc;
+    }
     double nvox = 0.;
-    for (int z = 0; z < nz; z++)
-        for (int y = 0; y < ny; y++)
-            for (int x = 0; x < nx; x++) {
-                const size_t i = ((size_t)z * ny + y) * nx + x;
-                const float v = data[i];
-                const bool use = (!likemask || likemask[i]) && v == v && v != 0.f && !isinf(v);
-                if (!use) continue;
-                const bool inreg = !region || region[(size_t)y * nx + x];
-                if (inreg) {
-                    dm2[(size_t)y * nx + x] += (double)v * (double)v;
-                    nvox += 1.;
-                }
-                if (inreg && x >= c->db.blo0 && x < c->db.bhi0 && y >= c->db.blo1 && y < c->db.bhi1)
-                    dv[(((size_t)(y - c->db.blo1)) * c->bx + (x - c->db.blo0)) * nz + z] = v;
-            }
+    for (size_t q = 0; q < npix; q++) {
+        const bool inreg = !region || region[q];
+        if (!inreg) dm2[q] = 0.;
+        else nvox += (double)nvm[q];
+    }
     c->sat.assign((size_t)(ny + 1) * (nx + 1), 0.);
+    std::vector<double> col(nx, 0.);
     for (int y = 0; y < ny; y++) {
         double row = 0.;
         for (int x = 0; x < nx; x++) {
-            row += dm2[(size_t)y * nx + x];
-            c->sat[(size_t)(y + 1) * (nx + 1) + x + 1] = c->sat[(size_t)y * (nx + 1) + x + 1] + row;
+            col[x] += dm2[(size_t)y * nx + x];
+            row += col[x];
+            c->sat[(size_t)(y + 1) * (nx + 1) + x + 1] = row;
         }
     }
     c->sat_total = c->sat[(size_t)ny * (nx + 1) + nx];
@@ -614,9 +639,6 @@ extern "C" int bb_ctx_create(const bb_model* m, const bb_like_desc* d, const flo
                c->sat[(size_t)y0 * W + x0];
     };
     c->ss_out_fix = d->box_free ? 0. : std::max(c->sat_total - boxsum(F.blo0, F.blo1, F.bhi0, F.bhi1), 0.);
-    const float* dvp = nullptr;
-    rc = dupload(c, &dvp, dv.data(), dv.size(), "bb_ctx_create: data");
-    c->dvalid = (float*)dvp;
     // azimuth plans of the sub-ring radii (rows 0 of the tables: one layer, or thin then thick)
     std::vector<double> R0(c->nsub_t);
     for (int s = 0; s < c->nsub_t; s++) R0[s] = plan->rads[s % plan->nsub];

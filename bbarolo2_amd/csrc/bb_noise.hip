@@ -15,6 +15,10 @@
 //   draws 2e6 of them with numpy when there are more): flags of the voxels whose rank is chosen.
 // * bb_box_dilate: the binary dilation by a box of half-sizes (rz, ry, rx) (_dilate, noise.py:8-15: maximum filters
 //   along each axis, zero outside the cube).
+// * bb_like_maps: the maps of the likelihood voxels (likemask, not blank) of _setup_likemask (likelihood.py:176-185):
+//   per pixel the sum of the squared data, their number and the sum of the data, in FP64, summed as the reference
+//   sums them (sequentially within chunks of 16 channels, the chunks added in order: the same bits), and the data of
+//   the likelihood in a box as the context keeps it ((by, bx, nz), the voxels not used NaN).
 // * bb_lag_xy, bb_lag_z: the autocorrelation sums of the centred noise that the reference forms with zero-padded
 //   FFTs, as direct FP64 sums over the lags it keeps: sum over pairs of used voxels of c c' and the number of such
 //   pairs (c = data - median). The padding of the FFTs exceeds the largest lag, so these are the same sums.
@@ -505,6 +509,42 @@ __global__ void k_lag_final(const double* __restrict__ part, int64_t nlag, int64
     wsum[l] = b;
 }
 
+// one thread per pixel: its spectrum in chunks of 16 channels
+#define BB_MAP_CHUNK 16
+__global__ void __launch_bounds__(BB_NS_THREADS)
+k_like_maps(const float* __restrict__ data, const uint8_t* __restrict__ likemask, const uint8_t* __restrict__ region,
+            int nz, int ny, int nx, int bx0, int by0, int bx1, int by1, float* __restrict__ dvalid,
+            double* __restrict__ dm2, int32_t* __restrict__ nvox, double* __restrict__ mom0) {
+    const int64_t plane = (int64_t)ny * nx;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= plane) return;
+    const int x = (int)(p % nx), y = (int)(p / nx);
+    const bool inbox = dvalid && x >= bx0 && x < bx1 && y >= by0 && y < by1;
+    const bool inreg = !region || region[p];
+    float* dv = inbox ? dvalid + ((int64_t)(y - by0) * (bx1 - bx0) + (x - bx0)) * nz : nullptr;
+    double a2 = 0.0, a1 = 0.0;
+    int32_t n = 0;
+    for (int z0 = 0; z0 < nz; z0 += BB_MAP_CHUNK) {
+        double c2 = 0.0, c1 = 0.0;
+        const int z1 = min(nz, z0 + BB_MAP_CHUNK);
+        for (int z = z0; z < z1; z++) {
+            const int64_t i = (int64_t)z * plane + p;
+            const float d = data[i];
+            const bool v = (!likemask || likemask[i]) && used_value(d, 1);
+            const double dd = v ? (double)d : 0.0;
+            c2 += dd * dd;
+            c1 += dd;
+            n += v;
+            if (dv) dv[z] = (v && inreg) ? d : __int_as_float(0x7fc00000);
+        }
+        a2 += c2;
+        a1 += c1;
+    }
+    if (dm2) dm2[p] = a2;
+    if (nvox) nvox[p] = n;
+    if (mom0) mom0[p] = a1;
+}
+
 int grid_of(int64_t n) {
     int64_t g = (n + BB_NS_THREADS - 1) / BB_NS_THREADS;
     return (int)(g < 1 ? 1 : (g > 65535 ? 65535 : g));
@@ -688,4 +728,18 @@ extern "C" int bb_lag_z(const float* data, const uint8_t* w, const uint8_t* pix_
                        plane, K, med, work);
     hipLaunchKernelGGL(k_lag_final, dim3(1), dim3(64), 0, st, (const double*)work, (int64_t)(K + 1), nb, c, wsum);
     return bb_hip_check(hipGetLastError(), "bb_lag_z");
+}
+
+extern "C" int bb_like_maps(const float* data, const uint8_t* likemask, const uint8_t* region, int nz, int ny, int nx,
+                            bb_box box, float* dvalid, double* dm2, int32_t* nvox, double* mom0, void* stream) {
+    if (nz < 1 || ny < 1 || nx < 1) return bb_fail(BB_ERR_ARG, "bb_like_maps: bad sizes %d %d %d", nz, ny, nx);
+    if (!data) return bb_fail(BB_ERR_ARG, "bb_like_maps: data is NULL");
+    if (dvalid && (box.blo0 < 0 || box.blo1 < 0 || box.bhi0 > nx || box.bhi1 > ny || box.bhi0 <= box.blo0 ||
+                   box.bhi1 <= box.blo1))
+        return bb_fail(BB_ERR_ARG, "bb_like_maps: box outside the cube or empty");
+    const int64_t plane = (int64_t)ny * nx;
+    const int64_t nb = (plane + BB_NS_THREADS - 1) / BB_NS_THREADS;
+    hipLaunchKernelGGL(k_like_maps, dim3((unsigned)nb), dim3(BB_NS_THREADS), 0, (hipStream_t)stream, data, likemask,
+                       region, nz, ny, nx, box.blo0, box.blo1, box.bhi0, box.bhi1, dvalid, dm2, nvox, mom0);
+    return bb_hip_check(hipGetLastError(), "bb_like_maps");
 }

@@ -172,18 +172,15 @@ class GpuLikelihood:
             region[Lb[1]:Hb[1], Lb[0]:Hb[0]] = True
         region = np.asarray(region, bool)
         if maskmode == 2:
-            lm = np.ones(data.shape, bool) if likemask is None else np.asarray(likemask, bool)
+            lm = None if likemask is None else np.ascontiguousarray(np.asarray(likemask, bool)).view(np.uint8)
         elif maskmode == 0:
-            lm = np.ones(data.shape, bool)
+            lm = None
         else:
             raise NotImplementedError("maskmode 1 (the data times the mask) is not supported by the likelihood context")
         data = np.ascontiguousarray(data, np.float32)
-        blank = ~np.isfinite(data) | (data == 0)
-        v = lm & ~blank
-        nvmap = v.sum(axis=0)
+        nvmap, dm2, ymap = self._maps(data, lm)
         self.nvox = int(nvmap[region].sum())
-        d2 = np.where(v, data, 0.).astype(np.float64) ** 2
-        dm2 = np.where(region, d2.sum(axis=0), 0.)
+        dm2 = np.where(region, dm2, 0.)
         sat = np.zeros((ny + 1, nx + 1))
         sat[1:, 1:] = np.cumsum(np.cumsum(dm2, axis=0), axis=1)
         self._sat = sat
@@ -206,7 +203,6 @@ class GpuLikelihood:
         self.dens0 = None
         self.last_dens = None
         if self.useNorm:
-            ymap = np.where(v, data, 0.).astype(np.float64).sum(axis=0)
             smap = self.sigma * np.sqrt(nvmap.astype(np.float64))
             sl = (slice(Lb[1], Hb[1]), slice(Lb[0], Hb[0]))
             ok = region[sl] & (nvmap[sl] > 0)
@@ -238,6 +234,22 @@ class GpuLikelihood:
             self.set_graphs(True)
 
     # ------------------------------------------------------------------ set-up
+    def _maps(self, data, lm):
+        """(nvox_map int, dm2_map, mom0) of the likelihood voxels (likemask lm, uint8 or None: all; not blank) per
+        pixel, from one pass over the cube on the device (bb_like_maps: the reference's sums, likelihood.py:176-185)."""
+        torch = self.torch
+        nz, ny, nx = data.shape
+        dev = torch.device("cuda", torch.cuda.current_device())
+        d = torch.from_numpy(data).to(dev)
+        m = None if lm is None else torch.from_numpy(lm).to(dev)
+        dm2 = torch.empty((ny, nx), dtype=torch.float64, device=dev)
+        nv = torch.empty((ny, nx), dtype=torch.int32, device=dev)
+        m0 = torch.empty((ny, nx), dtype=torch.float64, device=dev)
+        _lib.check(_lib.lib().bb_like_maps(d.data_ptr(), None if m is None else m.data_ptr(), None, nz, ny, nx,
+                                           _lib.bb_box(0, 0, nx, ny), None, dm2.data_ptr(), nv.data_ptr(),
+                                           m0.data_ptr(), _stream_ptr(torch)))
+        return nv.cpu().numpy(), dm2.cpu().numpy(), m0.cpu().numpy()
+
     def _boxsum(self, blo, bhi):
         S = self._sat
         return float(S[bhi[1], bhi[0]] - S[blo[1], bhi[0]] - S[bhi[1], blo[0]] + S[blo[1], blo[0]])
@@ -302,11 +314,11 @@ class GpuLikelihood:
             d.mc_cap = max(2 * int(dm.mc_clouds(sr0).max()), 4096)
         off, ring = free.offsets()
         rt = np.ascontiguousarray(ring_table(r))
-        lmu = np.ascontiguousarray(lm, np.uint8)
         rgu = np.ascontiguousarray(region, np.uint8)
-        keep += [rt, off, ring, lmu, rgu, data]
+        keep += [rt, off, ring, lm, rgu, data]
         out = C.c_void_p()
-        _lib.check(_lib.lib().bb_ctx_create(h, C.byref(d), data.ctypes.data, lmu.ctypes.data, rgu.ctypes.data,
+        _lib.check(_lib.lib().bb_ctx_create(h, C.byref(d), data.ctypes.data, None if lm is None else lm.ctypes.data,
+                                            rgu.ctypes.data,
                                             rt.ctypes.data, off.ctypes.data, ring.ctypes.data, C.byref(main),
                                             C.byref(ann) if ann is not None else None,
                                             C.byref(prof) if prof is not None else None, C.byref(out)))

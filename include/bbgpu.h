@@ -314,6 +314,13 @@ int bb_flag_subsample(const uint8_t* flags, int64_t n, int64_t nflag, const int6
  * (uint8 0 / 1); tmp (same size) is needed with more than one non-zero half-size. */
 int bb_box_dilate(const uint8_t* m, int nz, int ny, int nx, int rz, int ry, int rx, uint8_t* tmp, uint8_t* out,
                   void* stream);
+/* The maps of the likelihood voxels (likemask (may be NULL: all), not blank: finite and non-zero) of _setup_likemask
+ * (likelihood.py:176-185): per pixel dm2 the sum of their squared data, nvox their number, mom0 the sum of their data
+ * (FP64, summed sequentially within chunks of 16 channels and the chunks in order, as the reference's numpy sums:
+ * the same bits); and dvalid, the data of the likelihood in box as the context holds it, (by, bx, nz) yxz with NaN
+ * where the voxel is not used or its pixel is outside region (may be NULL). DEVICE arrays; any output may be NULL. */
+int bb_like_maps(const float* data, const uint8_t* likemask, const uint8_t* region, int nz, int ny, int nx, bb_box box,
+                 float* dvalid, double* dm2, int32_t* nvox, double* mom0, void* stream);
 /* The spatial autocorrelation sums of noise.py:103-119 without FFTs: for the lags (dy, dx) of the half-plane dy > 0
  * or (dy = 0, 0 <= dx <= L), in the order (0, 0..L), (1, -L..L), ..., (L, -L..L) (2L^2 + 2L + 1 of them),
  *     c[l] = sum over the channels with chan_use[z] and the pixels p of w(z, p) w(z, p + lag) (data - med)(data' - med)

@@ -43,6 +43,26 @@ GLOBAL_KEYS = ('sigma', 'fracerr')
 PROFILE_KEYS = ('inc', 'phi', 'xpos', 'ypos', 'radmax', 'z0', 'fthick')
 
 
+def like_maps(data, likemask=None):
+    """(dm2, nvox_map, mom0) of _setup_likemask (likelihood.py:176-185): per pixel the sum of the squared data of the
+    likelihood voxels (likemask, not blank: NaN or 0), their number and the sum of their data, in FP64, summed over
+    chunks of 16 channels in order as the reference does."""
+    data = np.asarray(data, np.float32)
+    nz, ny, nx = data.shape
+    dm2, mom0 = np.zeros((ny, nx)), np.zeros((ny, nx))
+    nvox = np.zeros((ny, nx), np.int64)
+    for k in range(0, nz, 16):
+        dk = data[k:k + 16]
+        v = ~(~np.isfinite(dk) | (dk == 0))
+        if likemask is not None:
+            v &= np.asarray(likemask[k:k + 16], bool)
+        d = np.where(v, dk, 0.).astype(np.float64)
+        dm2 += np.sum(d * d, axis=0)
+        nvox += v.sum(axis=0)
+        mom0 += d.sum(axis=0)
+    return dm2, nvox, mom0
+
+
 class Likelihood:
     """The likelihood of a cube for the free parameters spec ([(key, 'ring' | 'single'), ...]) around rings0.
 
@@ -78,12 +98,9 @@ class Likelihood:
         self.lf2d = self.likefactor if likefactor2d is None else float(likefactor2d)
         lm = np.ones(self.data.shape, bool) if likemask is None else np.asarray(likemask, bool)
         self.likemask = lm
-        v = lm & ~(~np.isfinite(self.data) | (self.data == 0))
-        d = np.where(v, self.data, 0.).astype(np.float64)
-        self.nvox_map = v.sum(axis=0)
-        self.mom0 = d.sum(axis=0)
+        dm2, self.nvox_map, self.mom0 = like_maps(self.data, lm)
         self.map_rms = self.sigma * np.sqrt(self.nvox_map.astype(np.float64))
-        a = np.where(self.region, (d * d).sum(axis=0), 0.)
+        a = np.where(self.region, dm2, 0.)
         self.sat = np.zeros((ny + 1, nx + 1))
         self.sat[1:, 1:] = np.cumsum(np.cumsum(a, axis=0), axis=1)
         self.nvox = int(self.nvox_map[self.region].sum())

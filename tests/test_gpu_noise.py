@@ -244,3 +244,37 @@ def test_setup_noise_likefactor():
     assert s["likefactor"] == pytest.approx(1. / (sp * max(nc["spectral"], nc["spectral_core"], 1.)), rel=1e-9)
     assert s["likefactor2d"] == pytest.approx(1. / sp, rel=1e-9)
     assert setup_noise(h, d, mask, noisecorr="beam")["likefactor"] == 1. / ba
+
+
+@pytest.mark.parametrize("shape", [(37, 29, 41), (16, 64, 64), (1, 7, 300), (100, 12, 9)])
+def test_like_maps_exact(shape):
+    # the per-pixel maps of the likelihood voxels bit for bit against the reference's chunked sums, and the data of
+    # the likelihood in a box (NaN outside the voxels used and the region)
+    torch = _torch0()
+    from bbarolo2_amd import _lib
+    from oracle.pipeline import like_maps
+    rng = np.random.default_rng(shape[0])
+    d = (rng.standard_normal(shape) * 3).astype(np.float32)
+    d[rng.random(shape) < 0.05] = 0.
+    d[rng.random(shape) < 0.02] = np.nan
+    lm = rng.random(shape) < 0.7
+    reg = rng.random(shape[1:]) < 0.8
+    nz, ny, nx = shape
+    box = (1, 2, nx - 1, ny)                              # [blo0, blo1, bhi0, bhi1)
+    dt = torch.from_numpy(d).cuda()
+    lt = torch.from_numpy(lm.view(np.uint8)).cuda()
+    rt = torch.from_numpy(reg.view(np.uint8)).cuda()
+    by, bx = box[3] - box[1], box[2] - box[0]
+    dv = torch.empty((by, bx, nz), dtype=torch.float32, device="cuda")
+    dm2 = torch.empty((ny, nx), dtype=torch.float64, device="cuda")
+    nv = torch.empty((ny, nx), dtype=torch.int32, device="cuda")
+    m0 = torch.empty((ny, nx), dtype=torch.float64, device="cuda")
+    _lib.check(_lib.lib().bb_like_maps(dt.data_ptr(), lt.data_ptr(), rt.data_ptr(), nz, ny, nx, _lib.bb_box(*box),
+                                       dv.data_ptr(), dm2.data_ptr(), nv.data_ptr(), m0.data_ptr(), None))
+    torch.cuda.synchronize()
+    e2, en, e0 = like_maps(d, lm)
+    assert np.array_equal(dm2.cpu().numpy(), e2) and np.array_equal(nv.cpu().numpy(), en)
+    assert np.array_equal(m0.cpu().numpy(), e0)
+    v = lm & np.isfinite(d) & (d != 0) & reg[None]
+    want = np.where(v, d, np.nan)[:, box[1]:box[3], box[0]:box[2]].transpose(1, 2, 0)
+    assert np.array_equal(dv.cpu().numpy(), want, equal_nan=True)

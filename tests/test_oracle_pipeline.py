@@ -82,3 +82,22 @@ def test_pipeline_fracerr_limit():
     th = np.concatenate([[110.], truth.dens])
     a, b = g.loglike(th), e.loglike(th)
     assert abs(a - b) <= 1e-9 * abs(a)
+
+
+def test_like_maps_sums():
+    # small integers: every partial sum exact, so the chunked sums equal the plain ones bit for bit; random data:
+    # within FP64 rounding of an exactly rounded sum; counts exact; blanks (NaN, 0) and the mask left out
+    from oracle.pipeline import like_maps
+    rng = np.random.default_rng(8)
+    d = rng.integers(-9, 10, (37, 5, 6)).astype(np.float32)
+    d[3, 1, 1] = np.nan
+    lm = rng.random(d.shape) < 0.8
+    dm2, nv, m0 = like_maps(d, lm)
+    v = lm & np.isfinite(d) & (d != 0)
+    dd = np.where(v, d, 0.).astype(np.float64)
+    assert np.array_equal(dm2, (dd * dd).sum(0)) and np.array_equal(m0, dd.sum(0)) and np.array_equal(nv, v.sum(0))
+    r = rng.standard_normal((50, 4, 4)).astype(np.float32)
+    dm2, nv, m0 = like_maps(r)
+    import math
+    exact = np.array([[math.fsum(float(x) ** 2 for x in r[:, i, j]) for j in range(4)] for i in range(4)])
+    assert np.allclose(dm2, exact, rtol=1e-14, atol=0) and np.all(nv == 50)
