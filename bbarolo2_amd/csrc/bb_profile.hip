@@ -302,14 +302,15 @@ __device__ __forceinline__ void nn_backward(const NnLds& S, int n, int k, int la
     }
 }
 
-// Append index j as row k of the factor; false (nothing changed) if its pivot is not positive.
+// Append index j as row k of the factor; false (nothing changed) if its pivot is at the round-off of its own
+// computation (16 eps of the diagonal) or below: the column is, to working precision, in the span of the factor's.
 __device__ bool nn_append(const NnLds& S, int n, int& k, int lane, int j) {
     double b0 = 0.0, b1 = 0.0;
     if (lane < k) b0 = S.M[(size_t)S.ord[lane] * n + j];
     if (lane + 64 < k) b1 = S.M[(size_t)S.ord[lane + 64] * n + j];
     nn_forward(S, n, k, lane, b0, b1);
     const double d2 = S.M[(size_t)j * n + j] - nn_sum((lane < k ? b0 * b0 : 0.0) + (lane + 64 < k ? b1 * b1 : 0.0));
-    if (!(d2 > 0.0)) return false;
+    if (!(d2 > 16.0 * 2.220446049250313e-16 * S.M[(size_t)j * n + j])) return false;
     const double d = sqrt(d2);
     double* Lk = S.L + (size_t)k * n;
     if (lane < k) Lk[lane] = b0;
@@ -324,11 +325,19 @@ __device__ bool nn_append(const NnLds& S, int n, int& k, int lane, int j) {
     return true;
 }
 
-// The factor of the passive set from scratch, in index order; false if a pivot is not positive.
+// The factor of the passive set from scratch, in index order; a column dependent on the earlier ones leaves the
+// passive set (its value 0: the others span its contribution).
 __device__ bool nn_rebuild(const NnLds& S, int n, int& k, int lane) {
     k = 0;
-    for (int j = 0; j < n; j++)
-        if (S.P[j] && !nn_append(S, n, k, lane, j)) return false;
+    for (int j = 0; j < n; j++) {
+        if (!S.P[j] || nn_append(S, n, k, lane, j)) continue;
+        __syncthreads();
+        if (lane == (j & 63)) {
+            S.P[j] = 0;
+            S.x[j] = 0.0;
+        }
+        __syncthreads();
+    }
     return true;
 }
 
@@ -398,7 +407,12 @@ __device__ bool nn_solve(const NnLds& S, int n, int lane, const double* warm = n
         nn_argmax(gv, best);
         if (best < 0) break;
         const int j = best;
-        if (!nn_append(S, n, k, lane, j)) return false;
+        if (!nn_append(S, n, k, lane, j)) {
+            // dependent on the passive columns: it cannot lower the residual; barred until the solution moves
+            if (lane == (j & 63)) S.ban[j] = 1;
+            __syncthreads();
+            continue;
+        }
         if (lane == (j & 63)) S.P[j] = 1;
         __syncthreads();
         for (int inner = 0;; inner++) {

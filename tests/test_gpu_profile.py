@@ -99,6 +99,27 @@ def test_profile_solve(name):
     assert err <= tol
 
 
+@pytest.mark.parametrize("eps", [0., 1e-12, 1e-9])
+def test_profile_solve_near_dependent_columns(eps):
+    # two ring maps equal to working precision (eps = 0: identical): the solve gives a finite non-negative profile
+    # whose fitted map is the oracle's (QR), instead of failing on a pivot at round-off
+    rng = np.random.default_rng(12)
+    A = rng.random((60, 6))
+    A[:, 4] = A[:, 3] * (1. + eps * rng.standard_normal(60))
+    x = np.array([1., 0.5, 2., 1., 1., 0.3])
+    b = A @ x + 0.01 * rng.standard_normal(60)
+    radii = [1., 2., 3., 4., 5., 6.5]
+    for smooth in (0., 0.3):
+        ref = op.solve(A, b, radii, smooth, corr=2.)
+        got = _solve_gpu(A, b, radii, smooth, corr=2.)
+        fr, fg = A @ ref, A @ got
+        err = np.abs(fg - fr).max() / np.abs(fr).max()
+        print(f"near-dependent eps {eps:g} smooth {smooth}: fitted map max rel {err:.2e}, x {np.round(got, 4)}",
+              flush=True)
+        assert np.all(np.isfinite(got)) and np.all(got >= 0)
+        assert err <= 1e-6
+
+
 def _profile_cube():
     """A noisy model cube of rings narrower than the beam (tests/test_bbarolo.py:107-140)."""
     h = emptyheader([64, 64, 40], [-10. / 3600., 10. / 3600., 10.], beam=30. / 3600., bunit="JY/BEAM", obj="MOCK")
