@@ -12,6 +12,7 @@ without a GPU or without libbbgpu.so these methods raise.
 
 import ctypes as C
 import math
+from collections import OrderedDict
 
 import numpy as np
 
@@ -21,6 +22,7 @@ from .galmod import (GalmodSetup, GalmodPar, model_rings, model_box, ringmode_na
 from .smooth import beam_kernel
 
 AN_THREADS = 256
+PLAN_CACHE = 32          # azimuth plans kept per DiskModel
 
 
 def _torch():
@@ -119,24 +121,27 @@ class DiskModel:
         return h
 
     def azplan(self, R0, nzq=ZSAMPLES_MAX):
-        """The bb_azplan of the sub-ring radii R0 (arcmin) on the current device (cached by radii)."""
+        """The bb_azplan of the sub-ring radii R0 (arcmin) on the current device, cached by radii: the PLAN_CACHE most
+        recently used plans are kept, the least recently used one is destroyed when a new one would exceed them. The
+        plans serve eager launches only (the likelihood context and its captured graphs own plans of their own); a
+        destroyed plan's device memory is freed after the launches queued before it (hipFree waits for them)."""
         torch = _torch()
         R0 = np.ascontiguousarray(R0, np.float64)
         key = (torch.cuda.current_device(), int(nzq), R0.tobytes())
         if self._plans is None:
-            self._plans = {}
+            self._plans = OrderedDict()
         p = self._plans.get(key)
-        if p is None:
-            h = self.handle(nzq)
-            out = C.c_void_p()
-            _lib.check(_lib.lib().bb_azplan_create(h, R0.ctypes.data, len(R0), C.byref(out)))
-            if len(self._plans) > 32:
-                for old in self._plans.values():
-                    _lib.lib().bb_azplan_destroy(old)
-                self._plans = {}
-            p = out
-            self._plans[key] = p
-        return p
+        if p is not None:
+            self._plans.move_to_end(key)
+            return p
+        h = self.handle(nzq)
+        out = C.c_void_p()
+        _lib.check(_lib.lib().bb_azplan_create(h, R0.ctypes.data, len(R0), C.byref(out)))
+        while len(self._plans) >= PLAN_CACHE:
+            _, old = self._plans.popitem(last=False)
+            _lib.lib().bb_azplan_destroy(old)
+        self._plans[key] = out
+        return out
 
     def __del__(self):
         try:

@@ -167,3 +167,35 @@ def test_profile_prior_rows_match_oracle():
         assert dmed == np.median(np.diff(np.sort(np.asarray(R)[:-1])))
         gaps = np.diff(Rs)
         assert np.array_equal(near.astype(bool), np.minimum(gaps[:-1], gaps[1:]) < 0.25 * dmed)
+
+
+def test_azplan_cache_evicts_least_recently_used(monkeypatch):
+    # the plan cache of DiskModel keeps the PLAN_CACHE most recently used plans and destroys only the oldest
+    import types
+    from bbarolo2_amd import engine
+    made, destroyed = [], []
+
+    class FakeLib:
+        def bb_azplan_create(self, h, r0, n, out):
+            made.append(len(made) + 1)
+            out._obj.value = made[-1]
+            return 0
+
+        def bb_azplan_destroy(self, p):
+            destroyed.append(p.value)
+            return 0
+
+    fake_torch = types.SimpleNamespace(cuda=types.SimpleNamespace(current_device=lambda: 0))
+    monkeypatch.setattr(engine, "_torch", lambda: fake_torch)
+    monkeypatch.setattr(engine._lib, "lib", lambda: FakeLib())
+    dm = engine.DiskModel.__new__(engine.DiskModel)
+    dm._plans = None
+    dm.handle = lambda nzq=None: None
+    n = engine.PLAN_CACHE
+    plans = [dm.azplan(np.array([float(i)])) for i in range(n)]
+    assert dm.azplan(np.array([0.])) is plans[0] and not destroyed          # (a hit: plan 0 most recent)
+    dm.azplan(np.array([float(n)]))                                        # (a new plan: the oldest, 1, goes)
+    assert destroyed == [2] and len(dm._plans) == n
+    assert dm.azplan(np.array([0.])) is plans[0]
+    dm.azplan(np.array([float(n + 1)]))
+    assert destroyed == [2, 3]
