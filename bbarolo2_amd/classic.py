@@ -10,9 +10,10 @@ trial rings of all of them are modelled in one launch, each in its own box. As t
   * after the last stage the rings that are not ok get DENS 0 (galfit.py:1250-1254);
   * the final model is the model of the fitted rings with the ring convention, normalised LOCAL (each spectrum scaled
     to the flux of the data in the mask, bb_local_norm) or not at all (NONE).
-Not built: the second stage (TWOSTAGE regularisation of the geometry, galfit.py:1296-), the REVERSE passes with the
-models of the surrounding rings (galfit.py:1231-1294) and the AZIM normalisation of the final model (it needs the
-radial profile of the data, Ellprof). A FREE with geometry and TWOSTAGE raises NotImplementedError.
+  * with TWOSTAGE and a free geometry, the second stage: the fitted INC, PA, VSYS, XPOS, YPOS, Z0 regularised
+    (median, Bezier or polynomial: regularize.py) and fixed, the rings fitted again (galfit.py:1300-1393).
+Not built: the REVERSE passes with the models of the surrounding rings (galfit.py:1231-1294) and the AZIM
+normalisation of the final model (it needs the radial profile of the data, Ellprof).
 
 The mask: the reference builds it with its source finder (MASK SEARCH); search_mask is a simpler stand-in (the largest
 connected region above a growth threshold holding a voxel above the detection threshold).
@@ -59,7 +60,7 @@ class ClassicFit:
 
     par: Fit3DPar (FREE, FTYPE, WFUNC, NORM, ...); gpar: GalmodPar of the models (GALMODTYPE, LTYPE, ...); mask
     (nz, ny, nx) bool (default search_mask); normalcube: scale the data during the fit; twostage: the reference's
-    TWOSTAGE (a second stage is not built); chan_noise (nz): the noise of each channel (default ones)."""
+    TWOSTAGE (a second stage when the geometry is free); chan_noise (nz): the noise of each channel (default ones)."""
 
     def __init__(self, h, data, inr, par=None, gpar=None, mask=None, normalcube=True, twostage=True, chan_noise=None):
         self.torch = _torch()
@@ -92,27 +93,68 @@ class ClassicFit:
         sini = math.sin(r.inc[ir] * math.pi / 180.)
         return max(vmax / sini, vmin / sini) if sini != 0 else np.inf
 
+    def _stage(self, fit, stage):
+        """One stage of the ring fits: (rings, ok per ring, results)."""
+        out, res = fit.fit(stage=stage)
+        ok = [bool(r is not None and r[0] and out.vrot[ir] <= self._band_vrot(out, ir)) for ir, r in enumerate(res)]
+        return out, ok, res
+
     def run(self):
-        """Fit the rings (one stage) and set rings, fitok and results; returns self."""
+        """Fit the rings: the first stage and, with TWOSTAGE and a free geometry, the second (the geometry of the
+        first regularised, then fixed: SecondStage, galfit.py:1300-1393); set rings, fitok, results (those of the last
+        stage), stage1 (rings, fitok, results of the first) and regularised (the regularisation used, or None);
+        returns self."""
         m = free_flags(self.par.FREE)
-        geometry = any(m[k] for k in (INC, PA, Z0, XPOS, YPOS, VSYS))
-        if geometry and self.twostage:
-            raise NotImplementedError("the second stage of 3DFIT (TWOSTAGE with the geometry free) is not built")
+        geometry = [k for k in (INC, PA, VSYS, XPOS, YPOS, Z0) if m[k]]
         scaling = 10. / float(self.data.max()) if self.normalcube else 1.
         data = (self.data * np.float32(scaling)).astype(np.float32) if self.normalcube else self.data
         noise = robust_spread(self.data) * scaling
         fit = RingFit(self.dm, data, self.mask, self.inr, par=self.par, data_noise=noise, chan_noise=self.chan_noise,
                       vsys_range=self._vsys_range())
-        out, res = fit.fit(stage=1)
-        ok = []
-        for ir, r in enumerate(res):
-            good = r is not None and r[0] and out.vrot[ir] <= self._band_vrot(out, ir)
-            ok.append(bool(good))
+        out, ok, res = self._stage(fit, 1)
+        self.stage1 = (out.copy(), list(ok), res)
+        self.regularised = None
+        if self.twostage and geometry:
+            inr2, how = self._regularise(out, ok, m)
+            if inr2 is not None:
+                self.regularised = how
+                fit.inr = inr2
+                fit.mpar = [v and k not in (INC, PA, VSYS, XPOS, YPOS, Z0) for k, v in enumerate(m)]
+                fit.nfree = sum(fit.mpar)
+                out, ok, res = self._stage(fit, 2)
         for ir in range(out.nr):
             if not ok[ir]:
                 out.dens[ir] = 0.
         self.rings, self.fitok, self.results, self.nevals = out, ok, res, fit.nevals
         return self
+
+    def _regularise(self, out, ok, m):
+        """The initial rings of the second stage: the first stage's rings, the rings not fitted back at their initial
+        VROT, VDISP, DENS and VRAD, the free geometry regularised (regularize.regtypes / regularize): (rings, [(key,
+        type, auto)]), or (None, None) when a regularisation fails or no ring was fitted (the first stage is final)."""
+        from .regularize import regtypes, regularize
+        good = np.asarray(ok, bool)
+        if not good.any():
+            return None, None
+        types = regtypes(self.par.REGTYPE, out.inc, out.phi, good)
+        r2 = out.copy()
+        for k in ("vrot", "vdisp", "dens", "vrad"):
+            getattr(r2, k)[~good] = getattr(self.inr, k)[~good]
+        how = []
+        for (flag, key), (rtype, auto) in zip(((INC, "inc"), (PA, "phi"), (VSYS, "vsys"), (XPOS, "xpos"),
+                                                 (YPOS, "ypos"), (Z0, "z0")), types):
+            if not m[flag]:
+                continue
+            x, y = np.asarray(r2.radii, float), np.asarray(getattr(r2, key), float)
+            if good.all():
+                done, vals = regularize(x, y, rtype)
+            else:
+                done, vals = regularize(x[good], y[good], rtype, xout=x)
+            if not done:
+                return None, None
+            setattr(r2, key, np.asarray(vals, float))
+            how.append((key, rtype, auto))
+        return r2, how
 
     def _vsys_range(self):
         st = self.dm.setup

@@ -45,6 +45,7 @@ class Fit3DPar:
     BWEIGHT: float = 1.0
     NORM: str = "AZIM"
     SIDE: str = "B"
+    REGTYPE: str = "auto"
 
 
 def free_flags(free):
@@ -266,7 +267,9 @@ class RingSimplex:
             fac1 = (1.0 - fac) / ndim
             fac2 = fac1 - fac
             ptry = psum * fac1 - p[ihi] * fac2
-            ytry = float((yield self.trials(dring, [ptry.copy()], rng))[0])
+            # (the clamps write the values evaluated into ptry, which replaces the worst point: func3D's zpar,
+            # galfit.py:964-973)
+            ytry = float((yield self.trials(dring, [ptry], rng))[0])
             if ytry < y[ihi]:
                 y[ihi] = ytry
                 psum += ptry - p[ihi]

@@ -161,6 +161,7 @@ KEYWORDS = {
     "SIDE": ("ustr", "B", ()),
     "BWEIGHT": ("float", 1.0, ()),
     "TWOSTAGE": ("flag", True, ()),
+    "REGTYPE": ("lstr", "auto", ()),
     "FLAGERRORS": ("flag", False, ("ERRORS",)),
     "NORM": ("ustr", "AZIM", ()),
     "REVERSE": ("lstr", "false", ()),
@@ -428,7 +429,8 @@ class ParFile:
         return Fit3DPar(FREE=self.get("FREE"), DELTAVROT=self.get("DELTAVROT"), DELTAINC=self.get("DELTAINC"),
                         DELTAPHI=self.get("DELTAPA"), MINVDISP=self.get("MINVDISP"), MAXVDISP=self.get("MAXVDISP"),
                         FTYPE=self.get("FTYPE"), WFUNC=self.get("WFUNC"), TOL=self.get("TOL"),
-                        BWEIGHT=self.get("BWEIGHT"), NORM=self.get("NORM"), SIDE=self.get("SIDE"))
+                        BWEIGHT=self.get("BWEIGHT"), NORM=self.get("NORM"), SIDE=self.get("SIDE"),
+                        REGTYPE=self.get("REGTYPE"))
 
     def like_options(self):
         return like_options(self.get("LIKEOPTS"))

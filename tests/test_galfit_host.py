@@ -179,3 +179,14 @@ def test_stage_rounds_equal_rings_alone():
         else:
             assert out.vrot[ir] == inr.vrot[ir]
     assert sum(r[0] for r in res) >= 1
+
+
+def test_reflection_keeps_the_clamped_point():
+    """The reflected, expanded or contracted point enters the simplex as evaluated, clamped to the bounds (func3D
+    writes the values back into its trial vector, galfit.py:832-889, 964-973): with the minimum of INC beyond its
+    band, the fit's INC stays inside the band (at its edge), not where the unclamped simplex would wander."""
+    inr = _inr([10., 30., 50.])
+    par = G.Fit3DPar(FREE="VROT INC", TOL=1e-7, DELTAINC=5.)
+    q = Quad(inr, par, {"vrot": (110., 10.), "inc": (80., 2.)})
+    ok, fmin, p = q.minimize(q.make_dring(1))
+    assert ok and 55. <= p[1] <= 65. and abs(p[1] - 65.) < 0.05 and abs(p[0] - 110.) < 0.5

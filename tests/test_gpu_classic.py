@@ -131,3 +131,41 @@ TWOSTAGE  false
     assert cf.par.NORM == "AZIM" and cf.gpar.GALMODTYPE == "MC"
     assert all(cf.fitok)
     assert np.all(np.diff(v) > 0) and 60 < v[0] < 100 and 110 < v[-1] < 135
+
+
+def test_config1_two_stages_free_inclination():
+    # FREE VROT VDISP INC with TWOSTAGE (the reference's default second stage): the first stage's inclinations
+    # regularised (REGTYPE auto: their median when they scatter by less than 3 degrees) and fixed, the rings fitted
+    # again; the second stage's inclination near the truth and VROT recovered on the inner rings
+    import torch
+    from bbarolo2_amd.classic import ClassicFit
+    from bbarolo2_amd.engine import DiskModel
+    from bbarolo2_amd.galfit import Fit3DPar
+    from bbarolo2_amd.regularize import regtypes, regularize
+    from bbarolo2_amd.synth import config, add_noise
+    torch.cuda.set_device(0)
+    h, truth, par, noise, seed, _, _ = config(1)
+    dm = DiskModel(h, par=par, densflux=True)
+    sr, _ = dm.subrings(truth)
+    mod = dm.array_batch_mc(sr[None], (0, 0), (128, 128))[0].double().cpu().numpy().transpose(2, 0, 1)
+    data = add_noise(mod, noise * mod.max(), dm.cfield, seed)
+    inr = truth.copy()
+    inr.vrot = np.full(truth.nr, 120.)
+    inr.vdisp = np.full(truth.nr, 20.)
+    inr.inc = np.full(truth.nr, 56.)
+    fp = Fit3DPar(FREE="VROT VDISP INC", FTYPE=2, NORM="LOCAL", WFUNC=2)
+    cf = ClassicFit(h, data, inr, par=fp, gpar=par).run()
+    r1, ok1, _ = cf.stage1
+    r = cf.rings
+    print("stage 1 inc", np.round(r1.inc, 2), "ok", ok1, "\nregularised", cf.regularised, "\nstage 2 inc",
+          np.round(r.inc, 2), "\nvrot", np.round(r.vrot, 2), "\ntruth", np.round(truth.vrot, 2), "\nok", cf.fitok,
+          f"\n{cf.nevals} trial models", flush=True)
+    assert cf.regularised is not None and cf.regularised[0][0] == "inc"
+    rtype = regtypes("auto", r1.inc, r1.phi, ok1)[0][0]
+    good = np.asarray(ok1)
+    want = regularize(r1.radii[good], r1.inc[good], rtype, xout=r1.radii)[1]
+    assert np.allclose(r.inc, want)                      # (fixed in the second stage at the regularised values)
+    assert np.all(np.abs(r.inc - 60.) < 3.)
+    inner = slice(1, truth.nr - 1)
+    assert all(cf.fitok[inner])
+    assert np.all(np.abs(r.vrot[inner] - truth.vrot[inner]) < 10.)
