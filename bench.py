@@ -27,10 +27,10 @@ PEAK_HBM = 8.0e12          # MI355X HBM3E, bytes/s
 # HBM bytes per launch of k_an_tile at batch 256 on the 160x160x96 box, from two rocprofv3 counter passes
 # (FETCH_SIZE, WRITE_SIZE) scaled by the same counters of a plain 1 GiB copy (bench.py --copy-calibration)
 TRAFFIC_SHAPE = (256, 160, 160, 96)
-TRAFFIC_TILE_BYTES = (1837977.6 * 2.0 + 4709621.0 * 1.0) * 1024   # KiB x scale -> bytes
+TRAFFIC_TILE_BYTES = (2780729.8 * 2.0 + 6629300.9 * 1.0) * 1024   # KiB x scale -> bytes
 # LDS instructions (wave level) of one such launch (rocprofv3 --pmc SQ_INSTS_LDS) and the rate of one CU for
 # 64-bit LDS atomics measured by a probe of ds_add_u64 (6.4-6.8 cycles per wave instruction): the deposit's limiter
-LDS_INSTS_TILE = 3.0e9
+LDS_INSTS_TILE = 2.92e9
 LDS_ATOMIC_PEAK = 256 * 2.4e9 / 6.6          # wave instructions per second over the 256 CUs at 2.4 GHz
 
 
