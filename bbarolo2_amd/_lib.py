@@ -74,6 +74,7 @@ _SIGS = {
     "bb_model_set_tile": (C.c_int, [_P, C.c_int, C.c_int]),
     "bb_model_get_tile": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "bb_model_set_bin": (C.c_int, [_P, C.c_int]),
+    "bb_model_set_deposit": (C.c_int, [_P, C.c_int]),
     "bb_azplan_create": (C.c_int, [_P, _P, C.c_int, C.POINTER(_P)]),
     "bb_azplan_destroy": (C.c_int, [_P]),
     "bb_analytic_work_size": (C.c_int64, [_P, C.c_int]),

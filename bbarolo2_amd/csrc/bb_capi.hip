@@ -18,7 +18,7 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
                                          int TX, int TY, double* sub, double* wtab, short4* bbox, short4* sbox,
                                          const double* cd2i, const double* vedges, const double* ptab, int ntab,
                                          int lin_axis, float* out, int accumulate, int* bin, int bin_cap,
-                                         const int4* vbox, hipStream_t st);
+                                         const int4* vbox, int variant, hipStream_t st);
 extern "C" void bb_set_tile_events(void* ev_begin, void* ev_end);
 extern "C" size_t bb_analytic_sub_bytes();
 extern "C" size_t bb_analytic_wt_bytes();
@@ -202,6 +202,12 @@ extern "C" int bb_model_set_bin(bb_model* m, int per_chunk) {
     return 0;
 }
 
+extern "C" int bb_model_set_deposit(bb_model* m, int variant) {
+    if (!m || variant < 0 || variant > 1) return fail(BB_ERR_ARG, "bb_model_set_deposit: variant %d (0 or 1)", variant);
+    m->dep_variant = variant;
+    return 0;
+}
+
 extern "C" int bb_model_get_tile(const bb_model* m, int* TX, int* TY) {
     if (!m || !TX || !TY) return fail(BB_ERR_ARG, "bb_model_get_tile: NULL argument");
     *TX = m->TX;
@@ -341,7 +347,7 @@ int bb_galmod_analytic_vb(const bb_model* m, const bb_azplan* p, bb_box box, con
     rc = bb_launch_galmod_analytic(&m->g, box, sr, nsub, B, p->d_naz, p->d_chunks, p->nchunks, p->d_az, m->TX, m->TY,
                                    sub, wtab, bbox, sbox, m->d_cd2i, m->d_vedges, m->d_ptab, m->ntab, m->lin, out,
                                    accumulate, work_bin(m, p, B, sbox, ntiles), (int)(p->nchunks * (int64_t)m->bin_per), vbox,
-                                   (hipStream_t)stream);
+                                   m->dep_variant, (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_galmod_analytic launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
 

@@ -458,11 +458,13 @@ __device__ __forceinline__ float lds_phi(double t, const float* ph, int ntab) {
 }
 
 // Phi at table coordinate x = 512 (t + 6) from the LDS copy of the table whose first entry is 0 and last 1: x is
-// clamped to the table, so the ends need no test (x <= 0: entry 0; x >= ntab - 1: entry ntab - 2 + 1 x its step).
+// clamped to the table, so the ends need no test (x <= 0: entry 0; x >= ntab - 1: the last entry).
+// The copy has one more entry, ph[ntab] = 1, so that x = ntab - 1 reads it with a zero weight (the value there, 1, is
+// that of the last interval's end, as ph[ntab - 2] and ph[ntab - 1] are both 1 in float32).
 __device__ __forceinline__ float lds_phi_x(float x, const float* ph, int ntab) {
-    const float xc = fminf(fmaxf(x, 0.0f), (float)(ntab - 1));
-    const int k = min((int)xc, ntab - 2);
-    const float fr = xc - (float)k;
+    const float xc = __builtin_amdgcn_fmed3f(x, 0.0f, (float)(ntab - 1));     // (x clamped to [0, ntab - 1])
+    const int k = (int)xc;
+    const float fr = __builtin_amdgcn_fractf(xc);       // (xc - k, exactly)
     const float a = ph[k];
     return fmaf(fr, ph[k + 1] - a, a);
 }
@@ -513,6 +515,8 @@ __device__ __forceinline__ double uw_(const Sub& u, const double* __restrict__ w
 #define BB_AN_SAMP_CAP 8192  // samples meeting the tile, gathered before they are deposited (ANALYTIC deposit; the
                              // LDS it takes is worth more here than float copies of the vertical weights: 66.1 ms
                              // against 68.0 with 4096 samples and the weights in LDS)
+#define BB_AN_WSAMP_CAP 2048 // the same for the wavefront deposit
+#define BB_TILE_PAD 64       // words after the LDS tile: the wavefront deposit's lanes past a line window add 0 there
 #define BB_RM_SAMP_CAP 3072  // the same for the ring maps (a longer list is slower there: 24.9 ms against 23.5)
 
 // Odd stride of the pixel spectra in the LDS tile: consecutive pixels start in different banks.
@@ -785,8 +789,10 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
 // entries (W wavefronts), so the lanes of one LDS atomic are W azimuths apart (different pixels: same-address
 // atomics of a wavefront serialise) on neighbouring sub-rings (similar work). The samples of successive batches
 // of chunks accumulate in slist until it is nearly full or the tile's chunks are done (larger windows: fuller
-// wavefronts). Every thread of the block calls it (it synchronises). SCAP: the entries of slist.
-template <int SCAP, class Dep>
+// wavefronts). Every thread of the block calls it (it synchronises). SCAP: the entries of slist. WAVE: the
+// wavefronts take 64 consecutive entries at a time instead, and every lane of a wavefront calls dep together
+// (e = -1 past the end): the deposit that works across the lanes of a wavefront (an_wave).
+template <int SCAP, bool WAVE = false, class Dep>
 __device__ __forceinline__ void tile_samples(int nchunks, const short4* __restrict__ bb, const short4* __restrict__ sb,
                                              const int* __restrict__ tlist, int tlo, int thi, int tx0, int tx1,
                                              int ty0, int ty1, int* clist, int* slist, int& nlist, int& nsamp,
@@ -797,12 +803,19 @@ __device__ __forceinline__ void tile_samples(int nchunks, const short4* __restri
     // 3. the ns samples gathered, one per thread (every thread calls it)
     auto flush = [&](int ns) {
         const int W = blockDim.x >> 6;
-        for (int base2 = 0; base2 < ns; base2 += blockDim.x) {
-            // (a last, partial window fills as few wavefronts as it can, its lanes still Wa entries apart)
-            const int m = ns - base2;
-            const int Wa = m >= (int)blockDim.x ? W : (m + 63) >> 6;
-            const int idx = (int)(threadIdx.x >> 6) < Wa ? base2 + (threadIdx.x & 63) * Wa + (threadIdx.x >> 6) : ns;
-            if (idx < ns) dep(slist[idx]);
+        if (WAVE) {
+            for (int g0 = (int)(threadIdx.x >> 6) * 64; g0 < ns; g0 += blockDim.x) {
+                const int idx = g0 + (int)(threadIdx.x & 63);
+                dep(idx < ns ? slist[idx] : -1);
+            }
+        } else {
+            for (int base2 = 0; base2 < ns; base2 += blockDim.x) {
+                // (a last, partial window fills as few wavefronts as it can, its lanes still Wa entries apart)
+                const int m = ns - base2;
+                const int Wa = m >= (int)blockDim.x ? W : (m + 63) >> 6;
+                const int idx = (int)(threadIdx.x >> 6) < Wa ? base2 + (threadIdx.x & 63) * Wa + (threadIdx.x >> 6) : ns;
+                if (idx < ns) dep(slist[idx]);
+            }
         }
         __syncthreads();
         if (threadIdx.x == 0) nsamp = 0;
@@ -865,7 +878,234 @@ __device__ __forceinline__ void tile_samples(int nchunks, const short4* __restri
     if (ns > 0) flush(ns);
 }
 
-template <bool LIN>
+// ---------------------------------------------------------------------------------------------------------
+// The deposit across the lanes of a wavefront (an_wave), for 64 consecutive samples of the tile's list.
+//   Phase A, one lane per sample: the sample's line window (line 0) and its runs in the tile, walked BB_SEG
+//   closing positions at a time (a segment of its column) into registers indexed by the position p: the run that
+//   ends at vertical sample kbase + p sets bit p of m, its amplitude aw_p[p] and the LDS offset of its spectrum
+//   sb_p[p]. No register array is indexed by a run count, so no select chains and no spills.
+//   Phase B, the lanes along the channels: for each sample of the wavefront that has runs, lane l computes the
+//   Phi table value at edge E0 + l of its window; a channel's lower edge comes from the lane below by a DPP shift
+//   (lane 0's from the previous block of 64 edges), and every run of the sample adds the profile to its pixel's
+//   spectrum, 64 consecutive channels per LDS atomic: no bank conflicts, and no idle lanes but past the end of
+//   the window. The sample's parameters come to the wavefront by readlane.
+// The values added, and so the model, are those of the lane-per-sample deposit (an_sample / an_lines) bit for
+// bit: the same FP32 operations on the same operands, integer sums. Samples whose velocity depends on height take
+// the lane-per-sample path (an_sample_lag) in Phase A.
+#define BB_SEG 16
+#define BB_WRUNS 256   // runs of a wavefront's samples held in its LDS buffer at a time
+
+// v of the lane below (lane l receives lane l - 1's v; lane 0 receives lane0): DPP wave_shr:1
+__device__ __forceinline__ float from_lane_below(float v, float lane0) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(lane0), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float readlane_f(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// NL1: one spectral line (the window and amplitudes of Phase A are those of line 0); otherwise every line's window
+// is computed in Phase B from the sample's velocity.
+template <bool LIN, bool NL1>
+__device__ __forceinline__ void an_wave(const bb_grid& g, const bb_box& box, const TileGeo& T, const Acc& A,
+                                        const double* sve, const double* scd, const float* scf, const float* sph,
+                                        int ntab, const int4* __restrict__ chunks, const double2* __restrict__ az,
+                                        const double* __restrict__ sub, const double* __restrict__ wtab, uint2* rbuf,
+                                        int e) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int nz = g.nz;
+    // ---- Phase A: one lane per sample
+    Sub u;
+    bool ok = e >= 0;
+    double xs = 0., xc = 0., yb = 0., vplane = 0., afl = 0.;
+    const double* wt = nullptr;
+    Track tr;
+    tr.X0 = tr.dX = tr.Y0 = tr.dY = 0.;
+    u.nzz = 0;
+    u.K = 0;
+    u.sig = 1.;
+    if (ok) {
+        const int4 ch = chunks[e >> 5];
+        const int s = ch.x >> 1;
+        const double* q = sub + (size_t)s * BB_NSF;
+        load_sub(u, q, ch.x & 1);
+        const double2 cs = az[ch.w + (e & 31)];
+        wt = wtab + (size_t)s * BB_ZCAP;
+        if (u.zdep) {
+            an_sample_lag(g, box, T, A, sve, scd, sph, ntab, u, wt, cs.x, cs.y);
+            ok = false;
+        } else {
+            // (the operations of an_sample)
+            const double wside = cs.x >= 0.0 ? u.wrec : u.wapp;
+            const double x = u.R * cs.x;
+            yb = u.R * cs.y * u.cinc;
+            xs = x * u.spa;
+            xc = x * u.cpa;
+            vplane = u.vsys + (u.vrot * cs.x + u.vrad * cs.y) * u.sinc + u.vvert * u.cinc;
+            afl = u.flux * wside;
+            tr = utrack(u, g, xs, xc, yb);
+        }
+    }
+    // the window of line 0 (an_lines): first channel k0, ne channels, the table coordinate x0 of its first edge
+    int hk0 = 0, hne = 0;
+    float hx0 = 0.f, ha = 0.f;
+    const float hinv = (float)(1.0 / u.sig), hdx = (float)(g.cdelt3 / u.sig * 512.0);
+    if (NL1 && ok) {
+        const int64_t kc = bb_lround(bb_velgrid(vplane, g.axtyp, g.velsys, g.lfreq0[0], g.crval3, g.cdelt3, g.crpix3));
+        int64_t k0 = kc - u.K;
+        if (k0 < 0) k0 = 0;
+        int64_t k1 = kc + u.K + 1;
+        if (k1 > nz) k1 = nz;
+        if (k1 > k0) {
+            hk0 = (int)k0;
+            hne = (int)(k1 - k0);
+        }
+        ha = (float)(g.relint[0] * afl);
+        if (LIN && hne > 0)
+            hx0 = fmaf((float)((((double)hk0 + 0.5 - g.crpix3) * g.cdelt3 + g.crval3) - vplane) * hinv, 512.0f, 3072.0f);
+        if (hne == 0 || ha == 0.f) ok = false;        // (the lane-per-sample deposit adds zeros)
+    }
+    // the closing positions of the sample's runs: the whole column, or the part of a long one that can meet the tile
+    int ka = 0, kb = -1;
+    if (ok) {
+        if (u.nzz <= BB_SEG) kb = u.nzz - 1;
+        else if (!tile_zrange(u, tr, T, ka, kb)) ok = false;
+    }
+    int c0 = 0, c1 = 0;
+    double ws = 0.;
+    if (ok) {
+        upix_fast(u, g, tr, xs, xc, yb, uz(u, ka), c0, c1);
+        ws = uw_(u, wt, ka, g.ltype);
+    }
+    const float cf0 = scf[0];
+    for (int kbase = ka; ; kbase += BB_SEG) {
+        const bool more = ok && kbase <= kb;
+        if (__ballot(more) == 0) break;        // (uniform: the wavefront's samples are all walked)
+        unsigned m = 0;
+        float aw_p[BB_SEG];
+        int sb_p[BB_SEG];
+#pragma unroll
+        for (int j = 0; j < BB_SEG; j++) {
+            const int kz = kbase + j + 1;      // the run open at kz - 1 closes unless kz is in its pixel
+            if (more && kz - 1 <= kb) {
+                int h0 = 0, h1 = 0;
+                bool same = false;
+                if (kz <= kb) {
+                    upix_fast(u, g, tr, xs, xc, yb, uz(u, kz), h0, h1);
+                    same = h0 == c0 && h1 == c1;
+                }
+                if (same) {
+                    ws += uw_(u, wt, kz, g.ltype);
+                } else {
+                    const int sl = T.slot(c0, c1);
+                    if (sl >= 0) {
+                        m |= 1u << j;
+                        aw_p[j] = NL1 ? ha * (float)ws * A.inv_q : (float)ws;
+                        sb_p[j] = sl * A.nzp + (NL1 ? hk0 : 0);
+                    }
+                    if (kz <= kb) {
+                        c0 = h0;
+                        c1 = h1;
+                        ws = uw_(u, wt, kz, g.ltype);
+                    }
+                }
+            }
+        }
+        // ---- the runs of the segment, compacted in the wavefront's LDS buffer (rbuf, BB_WRUNS entries: {aw bits,
+        // spectrum offset}) in lane order; more than BB_WRUNS runs go through in rounds of whole lanes
+        const int nr = __popc(m);
+        const unsigned long long below = (1ull << lane) - 1ull;
+        int pre = 0;                             // the runs of the lanes below this one
+#pragma unroll
+        for (int bit = 0; bit < 5; bit++) pre += __popcll(__ballot((nr >> bit) & 1) & below) << bit;
+        unsigned long long pending = __ballot(nr > 0);
+        int r0 = 0;
+        while (pending) {
+            const bool inr = nr > 0 && pre >= r0 && pre - r0 + nr <= BB_WRUNS;
+            if (inr) {
+                int p = pre - r0;
+#pragma unroll
+                for (int j = 0; j < BB_SEG; j++)
+                    if (m & (1u << j)) {
+                        rbuf[p] = make_uint2(__float_as_uint(aw_p[j]), (unsigned)sb_p[j]);
+                        p++;
+                    }
+            }
+            unsigned long long lm = __ballot(inr);
+            pending &= ~lm;
+            const int rbn = (pre - r0) | (nr << 16);       // (the sample's first record and run count, packed)
+            // ---- Phase B: the lanes along the channels, one sample of the round at a time
+            while (lm) {
+                const int L = __builtin_ctzll(lm);
+                lm &= lm - 1;
+                const int pk = __builtin_amdgcn_readlane(rbn, L), rb = pk & 0xffff, rn = pk >> 16;
+                const float dx = readlane_f(hdx, L), inv = (LIN && NL1) ? 0.f : readlane_f(hinv, L);
+                const double vpl = (LIN && NL1) ? 0. : readlane_d(vplane, L);
+                const double aflL = NL1 ? 0. : readlane_d(afl, L);
+                const int KL = NL1 ? 0 : (int)__builtin_amdgcn_readlane((int)u.K, L);
+                for (int nl = 0; nl < (NL1 ? 1 : g.nlines); nl++) {
+                    const double* ve = sve + (size_t)nl * (nz + 1);
+                    int k0, ne;
+                    float x0 = 0.f, a = 0.f;
+                    if (NL1) {
+                        const int kn = __builtin_amdgcn_readlane(hk0 | (hne << 16), L);
+                        k0 = kn & 0xffff;
+                        ne = kn >> 16;
+                        if (LIN) x0 = readlane_f(hx0, L);
+                    } else {
+                        const int64_t kc = bb_lround(bb_velgrid(vpl, g.axtyp, g.velsys, g.lfreq0[nl], g.crval3, g.cdelt3,
+                                                                g.crpix3));
+                        int64_t q0 = kc - KL;
+                        if (q0 < 0) q0 = 0;
+                        int64_t q1 = kc + KL + 1;
+                        if (q1 > nz) q1 = nz;
+                        if (q1 <= q0) continue;
+                        k0 = (int)q0;
+                        ne = (int)(q1 - q0);
+                        a = (float)(g.relint[nl] * aflL);
+                        if (LIN)
+                            x0 = fmaf((float)((((double)k0 + 0.5 - g.crpix3) * g.cdelt3 + g.crval3) - vpl) * inv, 512.0f,
+                                      3072.0f);
+                    }
+                    float carry = 0.f;
+                    for (int E0 = 0; E0 <= ne; E0 += 64) {
+                        // lane l: edge min(E0 + l, ne) of the window (exact in float32); the channel below it is
+                        // c = E0 + l - 1. Past the window both edges are edge ne and the profile is 0, and lane 0 of
+                        // the first block takes its own value as the edge below (channel -1 adds 0): no lane mask,
+                        // the zeros go to distinct words of the padded tile.
+                        const float Ef = (float)min(E0 + lane, ne);
+                        const float q = LIN ? lds_phi_x(fmaf(Ef, dx, x0), sph, ntab)
+                                            : lds_phi_f((float)(ve[k0 + (int)Ef] - vpl) * inv, sph, ntab);
+                        if (E0 == 0) carry = readlane_f(q, 0);
+                        const float qlo = from_lane_below(q, carry);
+                        carry = readlane_f(q, 63);
+                        const int c = E0 + lane - 1;
+                        const float prof = fabsf(q - qlo) * (LIN ? cf0 : scf[min(max(k0 + c, 0), nz - 1)]);
+                        unsigned long long* t = A.tile + (NL1 ? 0 : k0) + c;
+                        auto dep = [&](uint2 rec) {
+                            const float aw = NL1 ? __uint_as_float(rec.x) : a * __uint_as_float(rec.x) * A.inv_q;
+                            atomicAdd(t + (int)rec.y, (unsigned long long)(unsigned int)fmaf(aw, prof, 0.5f));
+                        };
+                        int r = 0;
+                        for (; r + 1 < rn; r += 2) {      // (two records per LDS read)
+                            const uint2 r1 = rbuf[rb + r], r2 = rbuf[rb + r + 1];
+                            dep(r1);
+                            dep(r2);
+                        }
+                        if (r < rn) dep(rbuf[rb + r]);
+                    }
+                }
+            }
+            if (pending) r0 = __builtin_amdgcn_readlane(pre, __builtin_ctzll(pending));
+        }
+    }
+}
+
+// MODE 0: one lane per sample (an_sample); 1: the wavefront deposit of one line (an_wave<LIN, true>); 2: the
+// wavefront deposit of several lines.
+template <bool LIN, int MODE>
 __global__ void __launch_bounds__(BB_TILE_THREADS)
 k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks, int nchunks,
           const double2* __restrict__ az, const double* __restrict__ suball, const double* __restrict__ wtaball,
@@ -873,10 +1113,15 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
           const double* __restrict__ cd2i, const double* __restrict__ vedges, const double* __restrict__ ptab,
           int ntab, float* __restrict__ outall, int accumulate, const int* __restrict__ bin, int bin_cap,
           const int4* __restrict__ vbox) {
-    // dynamic LDS: [vedges (nlines, nz+1) f64][cd2i (nz) f64][cd2i (nz) f32][Phi (ntab) f32][tile u64]
+    // dynamic LDS: [vedges (nlines, nz+1) f64][cd2i (nz) f64][cd2i (nz) f32][Phi (ntab + 1) f32][pad, tile u64, pad]
     extern __shared__ double dyn[];
+    // the wavefront deposit holds a smaller sample list and, in the bytes this frees, each wavefront's run buffer
+    constexpr int SCAP = MODE == 0 ? BB_AN_SAMP_CAP : BB_AN_WSAMP_CAP;
+    constexpr int NRB = MODE == 0 ? 1 : (BB_TILE_THREADS / 64) * BB_WRUNS;
+    static_assert(SCAP * sizeof(int) + NRB * sizeof(uint2) <= (BB_AN_SAMP_CAP + 2) * sizeof(int), "LDS budget");
     __shared__ int clist[BB_LIST_CAP];
-    __shared__ int slist[BB_AN_SAMP_CAP];
+    __shared__ int slist[SCAP];
+    __shared__ uint2 rbuf[NRB];
     __shared__ int nlist, nsamp;
     __shared__ double red[BB_TILE_THREADS / 64];
     int b, tid;
@@ -890,7 +1135,8 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     double* scd = dyn + (size_t)nl_ * (nz + 1);
     float* scf = (float*)(scd + nz);
     float* sph = scf + ((nz + 1) & ~1);
-    unsigned long long* tile = (unsigned long long*)(sph + ((ntab + 1) & ~1));
+    // (one word of padding before the tile and BB_TILE_PAD after it)
+    unsigned long long* tile = (unsigned long long*)(sph + ((ntab + 2) & ~1)) + 1;
     const int ntile = TX * TY * nzp;
     const double* sub = suball + (size_t)b * nsub * BB_NSF;
     const double* wtab = wtaball + (size_t)b * nsub * BB_ZCAP;
@@ -900,7 +1146,7 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
         scd[i] = cd2i[i];
         scf[i] = (float)cd2i[i];
     }
-    for (int i = threadIdx.x; i < ntab; i += blockDim.x) sph[i] = i == 0 ? 0.0f : (i == ntab - 1 ? 1.0f : (float)ptab[i]);
+    for (int i = threadIdx.x; i <= ntab; i += blockDim.x) sph[i] = i == 0 ? 0.0f : (i >= ntab - 1 ? 1.0f : (float)ptab[i]);
     // the fixed-point scale of this vector: amax bounds any single contribution
     double am = 0.;
     for (int s = threadIdx.x; s < nsub; s += blockDim.x) {
@@ -950,15 +1196,19 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     const int* tl;
     int tlo, thi;
     tile_list(bin, gridDim.y, gridDim.x, bin_cap, b, tid, tl, tlo, thi);
-    tile_samples<BB_AN_SAMP_CAP>(nchunks, bb, sb, tl, tlo, thi, tx0, tx1, ty0, ty1, clist, slist, nlist, nsamp, live,
-                                 [&](int e) {
-        const int4 ch = chunks[e >> 5];
-        const int s = ch.x >> 1;
-        Sub u;
-        load_sub(u, sub + (size_t)s * BB_NSF, ch.x & 1);
-        const double2 cs = az[ch.w + (e & 31)];
-        an_sample<LIN>(g, box, T, A, sve, scd, scf, sph, ntab, u,
-                  wtab + (size_t)s * BB_ZCAP, cs.x, cs.y);
+    tile_samples<SCAP, MODE != 0>(nchunks, bb, sb, tl, tlo, thi, tx0, tx1, ty0, ty1, clist, slist, nlist,
+                                            nsamp, live, [&](int e) {
+        if constexpr (MODE == 0) {
+            const int4 ch = chunks[e >> 5];
+            const int s = ch.x >> 1;
+            Sub u;
+            load_sub(u, sub + (size_t)s * BB_NSF, ch.x & 1);
+            const double2 cs = az[ch.w + (e & 31)];
+            an_sample<LIN>(g, box, T, A, sve, scd, scf, sph, ntab, u, wtab + (size_t)s * BB_ZCAP, cs.x, cs.y);
+        } else {
+            an_wave<LIN, MODE == 1>(g, box, T, A, sve, scd, scf, sph, ntab, chunks, az, sub, wtab,
+                                    rbuf + (threadIdx.x >> 6) * BB_WRUNS, e);
+        }
     });
     __syncthreads();
     float* out = outall + (size_t)b * by * bx * nz;
@@ -973,10 +1223,11 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     }
 }
 
-// LDS bytes of the dynamic part of k_an_tile: channel edges and factors (f64), Phi table (f32), the tile (u64).
+// LDS bytes of the dynamic part of k_an_tile: channel edges and factors (f64), Phi table (f32), the tile (u64) with
+// its padding.
 extern "C" size_t bb_analytic_lds_bytes(int nz, int nlines, int TX, int TY, int ntab) {
-    return ((size_t)nlines * (nz + 1) + nz) * sizeof(double) + (size_t)(((nz + 1) & ~1) + ((ntab + 1) & ~1)) * sizeof(float) +
-           ((size_t)TX * TY * bb_tile_stride(nz) + BB_CB) * sizeof(unsigned long long);
+    return ((size_t)nlines * (nz + 1) + nz) * sizeof(double) + (size_t)(((nz + 1) & ~1) + ((ntab + 2) & ~1)) * sizeof(float) +
+           ((size_t)TX * TY * bb_tile_stride(nz) + 1 + BB_TILE_PAD) * sizeof(unsigned long long);
 }
 
 extern "C" size_t bb_analytic_lds_static() {
@@ -997,7 +1248,7 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
                                          int TX, int TY, double* sub, double* wtab, short4* bbox, short4* sbox,
                                          const double* cd2i, const double* vedges, const double* ptab, int ntab,
                                          int lin_axis, float* out, int accumulate, int* bin, int bin_cap,
-                                         const int4* vbox, hipStream_t st) {
+                                         const int4* vbox, int variant, hipStream_t st) {
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
     hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, naz, sub, wtab, 0,
                        vbox);
@@ -1009,29 +1260,31 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
         if (e) return e;
     }
     const size_t lds = bb_analytic_lds_bytes(g->nz, g->nlines, TX, TY, ntab);
-    // a linear velocity axis with one channel factor: the channel edges and factors computed, not read
-    const bool lin = lin_axis != 0;
+    // a linear velocity axis with one channel factor: the channel edges and factors computed, not read. variant 0:
+    // one lane per sample, 1: the wavefront deposit (one line, or several)
+    using Kern = void (*)(bb_grid, bb_box, int, int, const int4*, int, const double2*, const double*, const double*,
+                          int, const short4*, const short4*, const double*, const double*, const double*, int, float*,
+                          int, const int*, int, const int4*);
+    static const Kern kern[2][3] = {{k_an_tile<false, 0>, k_an_tile<false, 1>, k_an_tile<false, 2>},
+                                    {k_an_tile<true, 0>, k_an_tile<true, 1>, k_an_tile<true, 2>}};
     static bool attr = false;
     if (!attr) {
         const int dyn = (int)(160 * 1024 - bb_analytic_lds_static());
-        hipError_t e = hipFuncSetAttribute((const void*)k_an_tile<true>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void*)k_an_tile<false>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-        if (e != hipSuccess) return (int)e;
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 3; j++) {
+                const hipError_t e = hipFuncSetAttribute((const void*)kern[i][j], hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+                if (e != hipSuccess) return (int)e;
+            }
         attr = true;
     }
     if (t_tile_ev[0] && t_tile_ev[1]) {
         hipError_t e = hipEventRecord(t_tile_ev[0], st);
         if (e != hipSuccess) return (int)e;
     }
-    if (lin)
-        hipLaunchKernelGGL(k_an_tile<true>, dim3(ntiles, B), dim3(BB_TILE_THREADS), lds, st, *g, box, TX, TY, chunks,
-                           nchunks, az, sub, wtab, nsub, bbox, sbox, cd2i, vedges, ptab, ntab, out, accumulate, bin,
-                           bin_cap, vbox);
-    else
-        hipLaunchKernelGGL(k_an_tile<false>, dim3(ntiles, B), dim3(BB_TILE_THREADS), lds, st, *g, box, TX, TY, chunks,
-                           nchunks, az, sub, wtab, nsub, bbox, sbox, cd2i, vedges, ptab, ntab, out, accumulate, bin,
-                           bin_cap, vbox);
+    const int mode = variant == 0 ? 0 : (g->nlines == 1 ? 1 : 2);
+    hipLaunchKernelGGL(kern[lin_axis != 0][mode], dim3(ntiles, B), dim3(BB_TILE_THREADS), lds, st, *g, box, TX, TY,
+                       chunks, nchunks, az, sub, wtab, nsub, bbox, sbox, cd2i, vedges, ptab, ntab, out, accumulate, bin,
+                       bin_cap, vbox);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && t_tile_ev[0] && t_tile_ev[1]) e = hipEventRecord(t_tile_ev[1], st);
     return (int)e;

@@ -40,6 +40,7 @@ struct bb_model {
     double scale;
     int TX, TY;        // pixel tile of the ANALYTIC deposit (its spectra fit the LDS budget)
     int bin_per;       // capacity of the tiles' chunk lists, in entries per chunk (0: no lists, every tile scans)
+    int dep_variant;   // ANALYTIC deposit: 0 one lane per sample, 1 the wavefront deposit (bit-identical models)
     int lin;           // linear velocity axis whose edges are exactly ((k - 1/2) + 1 - crpix3) cdelt3 + crval3 and
                        // one channel factor cd2i for all channels: the deposit computes them instead of reading them
 };

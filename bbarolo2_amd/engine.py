@@ -66,6 +66,7 @@ class DiskModel:
         self._plans = None
         self._device = device
         self._bin = None
+        self._deposit = None
 
     def set_bin(self, per_chunk):
         """The capacity of the deposits' per-tile chunk lists (bb_model_set_bin): 0..8 entries per chunk, 0 for
@@ -73,6 +74,13 @@ class DiskModel:
         self._bin = int(per_chunk)
         for h in (self._handle or {}).values():
             _lib.check(_lib.lib().bb_model_set_bin(h, self._bin))
+
+    def set_deposit(self, variant):
+        """The ANALYTIC deposit kernel (bb_model_set_deposit): 0 one lane per sample (default), 1 the wavefront
+        deposit. The models are bit-identical."""
+        self._deposit = int(variant)
+        for h in (self._handle or {}).values():
+            _lib.check(_lib.lib().bb_model_set_deposit(h, self._deposit))
 
     # ------------------------------------------------------------------ device objects
     def grid(self, nzq):
@@ -117,6 +125,8 @@ class DiskModel:
             h = out
             if self._bin is not None:
                 _lib.check(L.bb_model_set_bin(h, int(self._bin)))
+            if self._deposit is not None:
+                _lib.check(L.bb_model_set_deposit(h, int(self._deposit)))
             self._handle[key] = h
         return h
 
@@ -252,7 +262,7 @@ class DiskModel:
         st = self.setup
         by, bx, nz = bhi[1] - blo[1], bhi[0] - blo[0], st.nsubs
         dev = torch.device("cuda", torch.cuda.current_device())
-        h = self.handle(nzq_for(sr.reshape(16, -1)))
+        h = self.handle(nzq_for(sr))
         sp = _stream_ptr(torch, stream)
         L = _lib.lib()
         box = _lib.bb_box(blo[0], blo[1], bhi[0], bhi[1])
@@ -291,7 +301,7 @@ class DiskModel:
         sr = np.asarray(sr, np.float64).reshape(-1, 16, np.shape(sr)[-1])
         st = self.setup
         L = _lib.lib()
-        h = self.handle(nzq_for(sr.reshape(16, -1)) if nzq is None else nzq)
+        h = self.handle(nzq_for(sr) if nzq is None else nzq)
         tx, ty = C.c_int(), C.c_int()
         _lib.check(L.bb_model_get_tile(h, C.byref(tx), C.byref(ty)))
         lo, hi = sorted((tx.value, ty.value))
@@ -321,7 +331,7 @@ class DiskModel:
             raise ValueError("sub-ring tables hold NaN or infinite values")
         by, bx, nz = bhi[1] - blo[1], bhi[0] - blo[0], self.setup.nsubs
         dev = torch.device("cuda", torch.cuda.current_device())
-        nzq = nzq_for(sr.reshape(16, -1))
+        nzq = nzq_for(sr)
         h = self.handle(nzq)
         st = _stream_ptr(torch, stream)
         dep = torch.empty((B, by, bx, nz), dtype=torch.float32, device=dev) if work is None else work

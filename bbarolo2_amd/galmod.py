@@ -499,8 +499,8 @@ def zstep_for(h, setup, smoothed):
 
 def nzq_for(sr):
     """Vertical-sample capacity of the ANALYTIC kernels: 1 for thin disks, else ZSAMPLES_MAX
-    (galmod.py:909)."""
-    return 1 if not np.any(sr[6] > 0) else ZSAMPLES_MAX
+    (galmod.py:909). sr: one sub-ring table (16, nsub) or a batch of them (B, 16, nsub)."""
+    return 1 if not np.any(np.asarray(sr)[..., 6, :] > 0) else ZSAMPLES_MAX
 
 
 class SubringPlan:

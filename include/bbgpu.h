@@ -67,6 +67,10 @@ int bb_model_get_tile(const bb_model* m, int* TX, int* TY);
  * tiles test every chunk box; 0 turns the lists off. The results do not depend on it (exact integer sums). No
  * counterpart in the reference. */
 int bb_model_set_bin(bb_model* m, int per_chunk);
+/* The ANALYTIC deposit's kernel: 0 (default) one lane per sample for the whole deposit, 1 the wavefront deposit (one
+ * lane per sample walks its column, then the wavefront's lanes along the channels add each sample's line profile to
+ * its runs). Both add the same values: the models are bit-identical. No counterpart in the reference. */
+int bb_model_set_deposit(bb_model* m, int variant);
 
 /* The azimuth plan of a set of sub-ring radii R0 (arcmin, HOST array, nsub): the ANALYTIC samples of every
  * sub-ring (naz = max(int(3 circumference / pixel + 0.5), 16), two radial samples, galmod.py:339-345) and the
