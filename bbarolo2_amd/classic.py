@@ -12,8 +12,13 @@ trial rings of all of them are modelled in one launch, each in its own box. As t
     to the flux of the data in the mask, bb_local_norm) or not at all (NONE).
   * with TWOSTAGE and a free geometry, the second stage: the fitted INC, PA, VSYS, XPOS, YPOS, Z0 regularised
     (median, Bezier or polynomial: regularize.py) and fixed, the rings fitted again (galfit.py:1300-1393).
-Not built: the REVERSE passes with the models of the surrounding rings (galfit.py:1231-1294) and the AZIM
-normalisation of the final model (it needs the radial profile of the data, Ellprof).
+  * REVERSE (true, or auto for a median inclination above 75 degrees): the rings regridded to the beam's width if they
+    are narrower (galfit.py:503-532), then after the first pass of each stage SWEEPS more passes from the outermost
+    ring inwards, each ring fitted alone with the model of all the others around it (Gauss-Seidel: the rings outside
+    at their values of this pass, those inside at those of the previous one; densities from the deconvolved profile
+    of the previous pass), in the strip of the sky between the ring's radii along the major axis (RINGREGION STRIP)
+    or its annulus (_sweep, _ring_context, _segment, _context, galfit.py:786-808, 1013-1074, 1263-1294).
+Not built: the AZIM normalisation of the final model (it needs the radial profile of the data, Ellprof).
 
 The mask: the reference builds it with its source finder (MASK SEARCH); search_mask is a simpler stand-in (the largest
 connected region above a growth threshold holding a voxel above the detection threshold).
@@ -25,8 +30,10 @@ import numpy as np
 
 from . import _lib
 from .engine import DiskModel, _torch, _stream_ptr
-from .galfit import RingFit, Fit3DPar, free_flags, INC, PA, Z0, XPOS, YPOS, VSYS
+from .galfit import (RingFit, Fit3DPar, free_flags, context_dens, INC, PA, Z0, XPOS, YPOS, VSYS, KEYS, MAXPAR,
+                     CS_NONE, CS_OK, CS_FAILED)
 from .galmod import GalmodPar, model_rings
+from .rings import Rings
 
 
 def robust_spread(data):
@@ -36,6 +43,22 @@ def robust_spread(data):
     if v.size == 0:
         return 0.
     return float(1.4826 * np.median(np.abs(v - np.median(v))))
+
+
+def beam_rings(inr, beam):
+    """The rings of REVERSE when they are narrower than the beam (beam FWHM, arcsec): as wide as the beam out to the
+    last radius, with the initial values interpolated at the new radii (the position angle unwrapped;
+    galfit.py:516-532)."""
+    R = np.asarray(inr.radii, float)
+    nr = int(math.ceil(R[-1] / beam))
+    rad = np.arange(nr) * beam + beam / 2.
+    vals = {k: np.interp(rad, R, np.asarray(getattr(inr, k), float))
+            for k in ("xpos", "ypos", "vsys", "vrot", "vdisp", "vrad", "vvert", "dvdz", "zcyl", "dens", "z0", "inc",
+                      "z0thick", "fthick", "vdispthick")}
+    vals["phi"] = np.interp(rad, R, np.degrees(np.unwrap(np.radians(np.asarray(inr.phi, float)))))
+    r = Rings().setRings(radii=rad, radsep=beam, **vals)
+    r.twolayer = inr.twolayer
+    return r
 
 
 def search_mask(data, snr=5., grow=3.):
@@ -74,6 +97,13 @@ class ClassicFit:
         if np.any(R < 0):
             raise ValueError("3DFIT ERROR: Negative radius!!!")
         self.inr = inr.copy()
+        self.beam = abs(float(h.Bmaj())) * 3600. if h.Bmaj() else 0.        # (arcsec)
+        rev = str(self.par.REVERSE).lower()
+        self.reverse = "true" in rev or ("auto" in rev and float(np.median(self.inr.inc)) > 75.)
+        if self.reverse and self.beam > 0 and self.inr.radsep < self.beam:
+            self.inr = beam_rings(self.inr, self.beam)
+        self.sweeps = max(int(self.par.SWEEPS), 1) if self.reverse else 0
+        self.strip = self.reverse and str(self.par.RINGREGION).upper() == "STRIP"
         self.mask = search_mask(self.data) if mask is None else np.asarray(mask, bool)
         self.normalcube = bool(normalcube)
         self.twostage = bool(twostage)
@@ -94,10 +124,63 @@ class ClassicFit:
         return max(vmax / sini, vmin / sini) if sini != 0 else np.inf
 
     def _stage(self, fit, stage):
-        """One stage of the ring fits: (rings, ok per ring, results)."""
+        """One stage of the ring fits: (rings, ok per ring, results). The first pass fits all the rings together; with
+        REVERSE, the passes after it (_sweep)."""
         out, res = fit.fit(stage=stage)
         ok = [bool(r is not None and r[0] and out.vrot[ir] <= self._band_vrot(out, ir)) for ir, r in enumerate(res)]
+        cstat = np.array([CS_NONE if r is None else (CS_OK if o else CS_FAILED) for r, o in zip(res, ok)])
+        for pas in range(1, self.sweeps + 1):
+            res = self._sweep(fit, out, cstat, res, stage, pas)
+            ok = [bool(c == CS_OK) for c in cstat]
         return out, ok, res
+
+    def _sweep(self, fit, out, cstat, res, stage, pas):
+        """One pass of REVERSE after the first (Galfit._sweep and _ring_context, galfit.py:1263-1294): the rings
+        fitted again one at a time from the outermost inwards, each from its values of the previous pass and with the
+        model of all the other rings on the device (those outside at their values of this pass, those inside at the
+        previous pass's; the densities: the deconvolved profile of the previous pass scaled to the ring's own). out
+        (the rings), cstat (their status) and the results are updated in place; returns the results."""
+        nr = fit.inr.nr
+        prev = out.copy()
+        for ir in range(nr):
+            if cstat[ir] != CS_OK:
+                # (a ring not fitted starts again from its initial values)
+                for k in KEYS.values():
+                    getattr(prev, k)[ir] = getattr(fit.inr, k)[ir]
+        prof = fit.relative_profile(prev, self.beam or None)
+        sub = self.dm.setup.radsep * 60.
+        res = list(res)
+        for ir in range(nr - 1, -1, -1):
+            dens = context_dens(prof, ir, prev.dens[ir])
+            segs = []
+            if ir > 0:
+                segs.append(fit.segment(0, ir - 1, dens, out, cstat, sub))
+            if ir < nr - 1:
+                segs.append(fit.segment(ir + 1, nr - 1, dens, out, cstat, sub))
+            cstat[ir] = CS_NONE
+            st = fit.ring_start(ir, stage, start=prev, pas=pas)
+            if st is None:
+                res[ir] = None
+                continue
+            ctx = fit.context(segs, out) if segs else None
+            if ctx is not None and self.strip:
+                ctx.hmax = fit.strip_hmax(ir, st[0].radii[-1], self.beam)
+            fit.ctx, fit.rng = ctx, st[1]
+            try:
+                r = fit.minimize(st[0])
+            finally:
+                fit.ctx = None
+            res[ir] = r
+            if not r[0]:
+                cstat[ir] = CS_FAILED
+                continue
+            n = 0
+            for k in range(MAXPAR):
+                if fit.mpar[k]:
+                    getattr(out, KEYS[k])[ir] = r[2][n]
+                    n += 1
+            cstat[ir] = CS_OK if out.vrot[ir] <= self._band_vrot(out, ir) else CS_FAILED
+        return res
 
     def run(self):
         """Fit the rings: the first stage and, with TWOSTAGE and a free geometry, the second (the geometry of the

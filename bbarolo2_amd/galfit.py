@@ -28,6 +28,19 @@ SIDECODE = {"R": 1, "A": 2}
 MC_SUPPORT_NMIN = math.exp(-0.5772156649015329)
 KEYS = {VROT: "vrot", VDISP: "vdisp", DENS: "dens", Z0: "z0", INC: "inc", PA: "phi", XPOS: "xpos", YPOS: "ypos",
         VSYS: "vsys", VRAD: "vrad"}
+# REVERSE: the status of each ring for the models of the rings around a ring fit (galfit.py:1010-1011): not fitted (or
+# too few pixels in the mask), fitted, failed (not converged or rejected)
+CS_NONE, CS_OK, CS_FAILED = 0, 1, 2
+# the values a failed ring of a segment takes from the fitted ones (galfit.py:1030-1032)
+SEG_KEYS = ("xpos", "ypos", "vsys", "vrot", "vdisp", "vrad", "vvert", "dvdz", "zcyl", "z0", "inc", "phi", "z0thick",
+            "fthick", "vdispthick")
+
+
+def context_dens(prof, ir, dens):
+    """The densities of the rings in the model around ring ir whose own density is dens: the profile prof scaled to
+    dens at ring ir (Galfit._ctx_dens, galfit.py:1069-1074; only the ratios matter with NORM LOCAL or AZIM)."""
+    prof = np.asarray(prof, float)
+    return dens * prof / max(prof[ir], 1.E-3 * float(prof.max()))
 
 
 @dataclass
@@ -46,6 +59,9 @@ class Fit3DPar:
     NORM: str = "AZIM"
     SIDE: str = "B"
     REGTYPE: str = "auto"
+    REVERSE: str = "false"
+    SWEEPS: int = 1
+    RINGREGION: str = "STRIP"
 
 
 def free_flags(free):
@@ -148,6 +164,56 @@ class RingSimplex:
         d.radsep = drads[1] - drads[0]
         d.id = ir
         return d
+
+    def ring_edges(self):
+        """The inner and outer edges (arcsec) of the annuli of the ring fits (Galfit._ring_edges, galfit.py:1005-1008)."""
+        e = np.array([self.make_dring(ir).radii for ir in range(self.inr.nr)])
+        return e[:, 0], e[:, 1]
+
+    def segment(self, lo, hi, dens, rings, cstat, sub):
+        """The rings lo..hi of rings as one model for the passes of REVERSE (Galfit._segment, galfit.py:1013-1046):
+        nodes at the radii, linear in between, from the inner edge of the annulus of ring lo to the outer edge of that
+        of ring hi (the end values kept to the edges), so that the segments and the annulus of a ring fit tile the
+        disk. dens: the densities of all the rings; cstat: the status of every ring (CS_NONE rings have no emission,
+        CS_FAILED ones take the values interpolated from the CS_OK ones); sub: the width of the model's sub-rings
+        (arcsec). The inner part of a segment that does not start at the first ring is empty."""
+        cstat = np.asarray(cstat)
+        idx = np.arange(lo, hi + 1)
+        seg = rings.subset(idx)
+        seg.dens = np.where(cstat[idx] == CS_NONE, 0., np.asarray(dens, float)[idx])
+        good = np.flatnonzero(cstat == CS_OK)
+        bad = cstat[idx] == CS_FAILED
+        if bad.any() and good.size:
+            R = np.asarray(rings.radii, float)
+            for k in SEG_KEYS:
+                getattr(seg, k)[bad] = np.interp(seg.radii[bad], R[good], np.asarray(getattr(rings, k), float)[good])
+        elo, ehi = self.ring_edges()
+        nodes, rad = list(range(seg.nr)), list(seg.radii)
+        if elo[lo] < rad[0] - sub:
+            nodes.insert(0, 0)
+            rad.insert(0, elo[lo])
+        if ehi[hi] > rad[-1] + sub:
+            nodes.append(seg.nr - 1)
+            rad.append(ehi[hi])
+        seg = seg.subset(np.array(nodes))
+        seg.radii = np.array(rad, float)
+        seg.empty = None if lo == 0 else True
+        return seg
+
+    def strip_hmax(self, ir, rout, beam_arcsec):
+        """The half height (pixels) of the strip of ring ir whose annulus ends at rout (arcsec): the projected ring at
+        its initial inclination plus three scale heights and half the beam (Galfit._region, galfit.py:760-778)."""
+        inr = self.inr
+        hm = (rout * abs(math.cos(math.radians(inr.inc[ir]))) + 3 * float(inr.layer_height()[ir]) + beam_arcsec / 2.)
+        return hm / self.pixscale
+
+    def strip_box(self, dring, blo, bhi, hmax):
+        """The box of a ring grown to hold its strip (Galfit._strip_box, galfit.py:780-787)."""
+        ext = int(math.ceil(max(dring.radii[-1] / self.pixscale, hmax))) + 1
+        xpos, ypos = int(math.ceil(dring.xpos[-1])), int(math.ceil(dring.ypos[-1]))
+        lo = [max(min(blo[0], xpos - ext), 0), max(min(blo[1], ypos - ext), 0)]
+        hi = [min(max(bhi[0], xpos + ext + 1), self.nx), min(max(bhi[1], ypos + ext + 1), self.ny)]
+        return lo, hi
 
     # ------------------------------------------------------------------ trials
     def clamp(self, dring, zpar, rng=None):
@@ -323,17 +389,18 @@ class RingSimplex:
             return e.value
 
     # ------------------------------------------------------------------ the fit
-    def ring_start(self, ir, stage=1):
-        """The annulus of ring ir and its generator (seeded [12345, stage, ring], galfit.py:1097-1100), or None when
-        the ring has too few pixels in the mask (galfit.py:1118-1125)."""
-        dring = self.make_dring(ir)
+    def ring_start(self, ir, stage=1, start=None, pas=0):
+        """The annulus of ring ir (its values from the rings start, default the initial rings) and its generator
+        (seeded [12345, stage, ring], and the pass of REVERSE after the first, galfit.py:1097-1100), or None when the
+        ring has too few pixels in the mask (galfit.py:1118-1125)."""
+        dring = self.make_dring(ir, start)
         blo, bhi = self.box(dring)
         ps = self.pixscale
         nTot, nIn = count_ring_pixels(self.mask2d, blo, bhi, dring.xpos[-1] - blo[0], dring.ypos[-1] - blo[1],
                                       dring.phi[-1], dring.inc[-1], dring.radii[0] / ps, dring.radii[-1] / ps)
         if nIn < 20 and (nIn == 0 or nIn < 0.7 * nTot):
             return None
-        return dring, np.random.default_rng([12345, stage, ir])
+        return dring, np.random.default_rng([12345, stage, ir] + ([pas] if pas else []))
 
     def fit_ring(self, ir, stage=1):
         """The first-stage fit of ring ir alone (Galfit._fit_ring / _fit_one_ring, galfit.py:1097-1142): None if the
@@ -391,6 +458,27 @@ class RingSimplex:
         """Called with the annuli of the rings of a stage before its rounds (None after them)."""
 
 
+class RingContext:
+    """The model of the rings around a ring fit in a pass of REVERSE, on the device (Galfit._context and _Context,
+    galfit.py:436-452, 786-808): conv (by, bx, nz) float32, the model of the segments around the ring convolved with
+    the beam, in the box [blo, bhi); counts (by, bx, nz) float32, the expected numbers of Monte-Carlo sub-clouds of
+    its non-convolved ANALYTIC model (the support of the ring's residual; None for MC models); grow (x, y): the box
+    of a trial ring grows by the kernel's half sizes (ANALYTIC with a beam), within the context's box; hmax: the half
+    height (pixels) of the ring's strip (0: the annulus, RINGREGION ANNULUS)."""
+
+    def __init__(self, blo, bhi, conv, counts, grow=(0, 0), hmax=0.):
+        self.blo, self.bhi, self.conv, self.counts, self.grow, self.hmax = list(blo), list(bhi), conv, counts, grow, hmax
+
+    def ringbox(self, box):
+        (rlo, rhi), (gx, gy) = box, self.grow
+        blo = [max(rlo[0] - gx, self.blo[0]), max(rlo[1] - gy, self.blo[1])]
+        bhi = [min(rhi[0] + gx, self.bhi[0]), min(rhi[1] + gy, self.bhi[1])]
+        return blo, [max(bhi[0], blo[0] + 1), max(bhi[1], blo[1] + 1)]
+
+    def slices(self, blo, bhi):
+        return (slice(blo[1] - self.blo[1], bhi[1] - self.blo[1]), slice(blo[0] - self.blo[0], bhi[0] - self.blo[0]))
+
+
 class RingFit(RingSimplex):
     """The first-stage 3DFIT of the rings of a cube, the residuals evaluated on the current device.
 
@@ -415,6 +503,61 @@ class RingFit(RingSimplex):
         cn = np.ones(nz) if chan_noise is None else np.asarray(chan_noise, float)
         self.d_cn = torch.from_numpy(np.ascontiguousarray(cn, np.float64)).to(self.dev)
         self.data_noise = float(data_noise)
+        self.ctx = None            # (the model of the rings around the ring being fitted: a RingContext, REVERSE)
+
+    # ------------------------------------------------------------------ REVERSE
+    def context(self, segs, rings):
+        """The model of the segments segs (Rings, see segment) in the box of the rings, built on the device and
+        convolved once (the model is linear in the rings): a RingContext, or None without segments (Galfit._context,
+        galfit.py:786-808)."""
+        torch = self.torch
+        dm, st = self.dm, self.dm.setup
+        blo, bhi = self.box(rings)
+        analytic = st.galmodtype != "MC"
+        mod = counts = None
+        for seg in segs:
+            sr, nv = st.subrings(seg)
+            if not sr.shape[1]:
+                continue
+            if analytic:
+                m = dm.array_batch(sr[None], blo, bhi, convolve=False)[0]
+                unit = st.subcloud_flux(sr, nv)
+                if unit > 0:
+                    counts = m / unit if counts is None else counts + m / unit
+            else:
+                m = dm.array_batch_mc(sr[None], blo, bhi, convolve=False)[0]
+            mod = m.clone() if mod is None else mod + m
+        if mod is None:
+            return None
+        conv = dm.convolve_batch(mod[None])[0] if dm.cfield is not None else mod
+        grow = (dm.cfield.shape[1] // 2, dm.cfield.shape[0] // 2) if (analytic and dm.cfield is not None) else (0, 0)
+        return RingContext(blo, bhi, conv, counts, grow)
+
+    def relative_profile(self, rings, beam_arcsec=None):
+        """The deconvolved density profile of the data for the geometry of rings, the densities of the models around
+        the ring fits of REVERSE (Galfit._profile, galfit.py:1048-1067): the beam-convolved unit-density maps of the
+        rings and of the exterior annulus (ring maps on the device) against the total-intensity map of the data in
+        the mask, in the box of the rings, solved with the relative smoothness prior 0.1 (solve_profile_relative);
+        ones if there are too few pixels or no positive density."""
+        from .profile import exterior_annulus, solve_profile_relative
+        torch = self.torch
+        dm, st = self.dm, self.dm.setup
+        blo, bhi = self.box(rings)
+        sl = (slice(blo[1], bhi[1]), slice(blo[0], bhi[0]))
+        m = self.d_mask[sl]
+        mom0 = (self.d_data[sl].double() * m.double()).sum(dim=2).cpu().numpy()
+        nvox = m.sum(dim=2).cpu().numpy()
+        rms = float(self.data_noise) * np.sqrt(nvox)
+        K = dm.ring_maps(rings, blo, bhi)
+        ext, rext = exterior_annulus(st, rings, dm.ringmode, beam_arcsec)
+        K = np.concatenate([K, dm.ring_maps(ext, blo, bhi, ringmode="CENTRED")])
+        ok = (nvox > 0) & (rms > 0) & np.isfinite(mom0)
+        prof = None
+        if ok.sum() >= 2 * rings.nr:
+            A = K[:, ok].T / rms[ok, None]
+            b = mom0[ok] / rms[ok]
+            prof = solve_profile_relative(A, b, np.r_[np.asarray(rings.radii, float), rext], 0.1)[:rings.nr]
+        return np.ones(rings.nr) if prof is None or not np.any(prof > 0) else np.asarray(prof, float)
 
     # ------------------------------------------------------------------ device evaluation
     def _support(self, unit):
@@ -499,6 +642,11 @@ class RingFit(RingSimplex):
                 a += w
             units.append(st.subcloud_flux(own, nv) if own.shape[1] else 0.)
             blo, bhi = self.box(r)
+            if self.ctx is not None:
+                # (REVERSE: the ring's box grown to hold its strip and the kernel, within the context's box)
+                if self.ctx.hmax > 0:
+                    blo, bhi = self.strip_box(r, blo, bhi, self.ctx.hmax)
+                blo, bhi = self.ctx.ringbox((blo, bhi))
             vbox[b] = (blo[0], blo[1], bhi[0], bhi[1])
         ublo = (int(vbox[:, 0].min()), int(vbox[:, 1].min()))
         ubhi = (int(vbox[:, 2].max()), int(vbox[:, 3].max()))
@@ -509,6 +657,19 @@ class RingFit(RingSimplex):
         else:
             mod = dm.array_batch(sr, ublo, ubhi, convolve=dm.cfield is not None, work=dep, vbox=vbox)
         self.nevals += B
+        ctx = self.ctx
+        if ctx is not None:
+            # the model of the rings around the ring added to each trial in its own box, and their sub-clouds to the
+            # support of an ANALYTIC trial (getFuncValue with a context, galfit.py:746-758)
+            if mod.data_ptr() == dep.data_ptr():
+                mod = mod.clone()
+            for b in range(B):
+                lo, hi = vbox[b, :2], vbox[b, 2:]
+                ys, xs = slice(lo[1] - ublo[1], hi[1] - ublo[1]), slice(lo[0] - ublo[0], hi[0] - ublo[0])
+                cy, cx = ctx.slices(lo, hi)
+                mod[b, ys, xs] += ctx.conv[cy, cx]
+                if ctx.counts is not None and units[b] > 0:
+                    dep[b, ys, xs] += float(units[b]) * ctx.counts[cy, cx]
         out, a = [], 0
         for ir, rs in groups:
             k = len(rs)
@@ -533,8 +694,9 @@ class RingFit(RingSimplex):
         ring = np.array([[r.xpos[-1] - blo[0], r.ypos[-1] - blo[1], r.phi[-1], r.inc[-1], r.radii[0] / ps,
                           r.radii[-1] / ps] for r in rings], np.float64)
         d_ring = torch.from_numpy(ring).to(self.dev)
-        par = _lib.bb_norm_par(SIDECODE.get(self.par.SIDE, 0), int(self.par.FTYPE), self.norm, 0, float(self.par.WFUNC),
-                               self.data_noise, float(self.par.BWEIGHT), 0.)
+        hmax = self.ctx.hmax if self.ctx is not None else 0.
+        par = _lib.bb_norm_par(SIDECODE.get(self.par.SIDE, 0), int(self.par.FTYPE), self.norm, int(hmax > 0),
+                               float(self.par.WFUNC), self.data_noise, float(self.par.BWEIGHT), float(hmax))
         wl = int(L.bb_norm_work_size(B, by, bx, nz, int(sup is not None)))
         work = torch.empty(wl, dtype=torch.float64, device=self.dev)
         val = torch.empty(B, dtype=torch.float64, device=self.dev)

@@ -89,6 +89,27 @@ def prior_rows(radii):
     return lcoef, lcol, near, norm, dmed
 
 
+def solve_profile_relative(A, b, radii, smooth=0.1):
+    """Non-negative least squares of A dens = b (the rings, then the exterior annulus) with the curvature prior of
+    weight smooth relative to the data's (smooth x trace(A^T A) / the squared coefficients of the rows, prior_rows),
+    through the QR of the stacked system (profile.py:153-185): the profile of the models around the ring fits of
+    REVERSE (a host solve once per pass)."""
+    from scipy.optimize import nnls
+    A = np.asarray(A, float)
+    b = np.asarray(b, float)
+    n = A.shape[1]
+    alpha = float(smooth or 0.)
+    if alpha > 0 and n >= 3:
+        lcoef, lcol, _, norm, _ = prior_rows(radii)
+        L = np.zeros((n - 2, n))
+        for i in range(n - 2):
+            L[i, lcol[i]] = lcoef[i]
+        L *= math.sqrt(alpha * float(np.sum(A * A)) / max(norm, 1e-300))
+        A, b = np.vstack([A, L]), np.r_[b, np.zeros(n - 2)]
+    q, r = np.linalg.qr(A)
+    return nnls(r, q.T @ b)[0]
+
+
 class ProfileSetup:
     """The per-fit constants of the deconvolved density profile (host arrays, handed to the likelihood context).
 

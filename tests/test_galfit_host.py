@@ -190,3 +190,77 @@ def test_reflection_keeps_the_clamped_point():
     q = Quad(inr, par, {"vrot": (110., 10.), "inc": (80., 2.)})
     ok, fmin, p = q.minimize(q.make_dring(1))
     assert ok and 55. <= p[1] <= 65. and abs(p[1] - 65.) < 0.05 and abs(p[0] - 110.) < 0.5
+
+
+# ---------------------------------------------------------------------- REVERSE (host pieces)
+def test_context_dens():
+    """The profile scaled to the ring's own density; a ring whose profile is below 1e-3 of the peak is scaled as if
+    it were 1e-3 of it (no division by ~0)."""
+    prof = np.array([4., 2., 1., 0.])
+    assert np.allclose(G.context_dens(prof, 1, 3.), [6., 3., 1.5, 0.])
+    assert np.allclose(G.context_dens(prof, 3, 2.), prof * 2. / 4e-3)
+
+
+def test_segment_nodes_edges_and_status():
+    """The rings lo..hi as nodes, extended to the edges of the annuli of rings lo and hi (more than a sub-ring away);
+    CS_NONE rings have no emission; a CS_FAILED ring takes the values interpolated from the CS_OK ones; the inner
+    part is empty unless the segment starts at the first ring."""
+    inr = _inr([10., 30., 50., 70., 90.], vrot=[50., 80., 999., 120., 130.])
+    q = Quad(inr, G.Fit3DPar(), {})
+    elo, ehi = q.ring_edges()
+    assert np.allclose(elo, [0., 20., 40., 60., 80.]) and np.allclose(ehi, [20., 40., 60., 80., 100.])
+    cst = np.array([G.CS_OK, G.CS_OK, G.CS_FAILED, G.CS_OK, G.CS_NONE])
+    dens = np.array([1., 2., 3., 4., 5.])
+    seg = q.segment(1, 4, dens, inr, cst, 6.)
+    assert np.allclose(seg.radii, [20., 30., 50., 70., 90., 100.])        # inner edge 20, outer edge 100
+    assert np.allclose(seg.dens, [2., 2., 3., 4., 0., 0.])                # ring 4 CS_NONE: no emission
+    assert seg.vrot[2] == 100. and seg.vrot[3] == 120.                    # ring 2 failed: halfway between 80 and 120
+    assert seg.empty is True
+    seg0 = q.segment(0, 1, dens, inr, cst, 6.)
+    assert seg0.empty is None and np.allclose(seg0.radii, [0., 10., 30., 40.])    # (from the centre: inner edge 0)
+    near = q.segment(1, 1, dens, inr, cst, 15.)                           # (edges within a sub-ring: no extra node)
+    assert np.allclose(near.radii, [30.])
+
+
+def test_strip_region():
+    """Half height of the strip: the outer radius projected at the initial inclination + 3 scale heights + half the
+    beam, in pixels; the box grown to hold it around the centre, within the cube."""
+    inr = _inr([10., 30., 50.], z0=4., inc=80.)
+    q = Quad(inr, G.Fit3DPar(), {})
+    h = q.strip_hmax(1, 40., 30.)
+    assert abs(h - (40. * math.cos(math.radians(80.)) + 12. + 15.) / 10.) < 1e-12
+    d = q.make_dring(1)
+    lo, hi = q.strip_box(d, (25, 27), (36, 34), h)
+    ext = int(math.ceil(max(40. / 10., h))) + 1
+    assert lo == [30 - ext, 30 - ext] and hi == [30 + ext + 1, 30 + ext + 1]
+    lo, hi = q.strip_box(d, (0, 0), (64, 64), 100.)
+    assert lo == [0, 0] and hi == [64, 64]
+
+
+def test_beam_rings():
+    """REVERSE with rings narrower than the beam: rings one beam wide out to the last radius, the initial values
+    interpolated at the new radii, the position angle unwrapped across 360."""
+    from bbarolo2_amd.classic import beam_rings
+    inr = _inr([5., 15., 25., 35., 45., 55.], vrot=[10., 20., 30., 40., 50., 60.], phi=[350., 355., 359., 3., 8., 12.])
+    r = beam_rings(inr, 20.)
+    assert np.allclose(r.radii, [10., 30., 50.]) and r.radsep == 20.
+    assert np.allclose(r.vrot, [15., 35., 55.])
+    assert np.allclose(r.phi, [352.5, 361., 370.])
+
+
+def test_solve_profile_relative_vs_oracle():
+    """The relative-prior NNLS of the REVERSE profile (stack then QR) against the oracle's first solution of the
+    profile (QR then stack): the same least-squares problem."""
+    from scipy.optimize import nnls
+    import oracle.profile as op
+    from bbarolo2_amd.profile import solve_profile_relative
+    rng = np.random.default_rng(5)
+    radii = np.r_[np.arange(8) * 30. + 15., 260.]
+    A = np.abs(rng.normal(size=(200, 9)))
+    b = A @ np.linspace(2., 0.2, 9) + 0.05 * rng.normal(size=200)
+    got = solve_profile_relative(A, b, radii, 0.1)
+    Q, T = np.linalg.qr(A)
+    L, srel, _ = op.relative_scale(radii, float(np.sum(A * A)), 0.1)
+    ref = nnls(np.vstack([T, srel * L]), np.concatenate([Q.T @ b, np.zeros(7)]))[0]
+    assert np.allclose(got, ref, rtol=1e-9, atol=1e-12)
+    assert np.allclose(solve_profile_relative(A, b, radii, 0.), nnls(A, b)[0], rtol=1e-9, atol=1e-12)
