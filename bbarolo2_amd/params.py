@@ -165,6 +165,8 @@ KEYWORDS = {
     "FLAGERRORS": ("flag", False, ("ERRORS",)),
     "NORM": ("ustr", "AZIM", ()),
     "REVERSE": ("lstr", "false", ()),
+    "SWEEPS": ("int", 1, ()),
+    "RINGREGION": ("ustr", "STRIP", ()),
     "NORMALCUBE": ("flag", True, ()),
     "STARTRAD": ("int", 0, ()),
     "METHOD": ("ustr", "", ()),
@@ -430,7 +432,8 @@ class ParFile:
                         DELTAPHI=self.get("DELTAPA"), MINVDISP=self.get("MINVDISP"), MAXVDISP=self.get("MAXVDISP"),
                         FTYPE=self.get("FTYPE"), WFUNC=self.get("WFUNC"), TOL=self.get("TOL"),
                         BWEIGHT=self.get("BWEIGHT"), NORM=self.get("NORM"), SIDE=self.get("SIDE"),
-                        REGTYPE=self.get("REGTYPE"))
+                        REGTYPE=self.get("REGTYPE"), REVERSE=self.get("REVERSE"), SWEEPS=self.get("SWEEPS"),
+                        RINGREGION=self.get("RINGREGION"))
 
     def like_options(self):
         return like_options(self.get("LIKEOPTS"))

@@ -72,6 +72,9 @@ def test_values_and_defaults(tmp_path):
     assert (gp.GALMODTYPE, gp.LTYPE, gp.CMODE, gp.CDENS, gp.ISEED, gp.NV, gp.SM, gp.DENSFLUX) == \
         ("MC", 1, 1, 10.0, -1, -1, True, True)
     assert (fp.FTYPE, fp.WFUNC, fp.NORM, fp.BWEIGHT, fp.TOL, fp.FREE) == (2, 2, "AZIM", 1.0, 1e-3, "VROT VDISP PA INC")
+    assert (fp.REVERSE, fp.SWEEPS, fp.RINGREGION) == ("false", 1, "STRIP")       # (params.py:200-202)
+    r = P.ParFile({"REVERSE": "AUTO", "SWEEPS": "2", "RINGREGION": "annulus"}).fit3d_par()
+    assert (r.REVERSE, r.SWEEPS, r.RINGREGION) == ("auto", 2, "ANNULUS")
     # C++-style reads: the leading number of the first token, '1' or 'true' for a flag
     q = P.ParFile({"NRADII": "12abc", "RADSEP": "2.5e1x", "TWOSTAGE": "0", "SM": "TRUE", "CDENS": "12.7",
                    "RELINT": "1, 0.5 x 3"})
