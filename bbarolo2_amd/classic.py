@@ -18,7 +18,9 @@ trial rings of all of them are modelled in one launch, each in its own box. As t
     at their values of this pass, those inside at those of the previous one; densities from the deconvolved profile
     of the previous pass), in the strip of the sky between the ring's radii along the major axis (RINGREGION STRIP)
     or its annulus (_sweep, _ring_context, _segment, _context, galfit.py:786-808, 1013-1074, 1263-1294).
-Not built: the AZIM normalisation of the final model (it needs the radial profile of the data, Ellprof).
+  * the AZIM final model: the density of every fitted ring set to the face-on surface density of the data's
+    intensity map in its annulus (face_on_density, the radial profile of Ellprof), scaled by a power of ten to 0.1-10
+    and the model divided by it again (radialProfile and finalModel, galfit_out.py:86-159).
 
 The mask: the reference builds it with its source finder (MASK SEARCH); search_mask is a simpler stand-in (the largest
 connected region above a growth threshold holding a voxel above the detection threshold).
@@ -33,6 +35,7 @@ from .engine import DiskModel, _torch, _stream_ptr
 from .galfit import (RingFit, Fit3DPar, free_flags, context_dens, INC, PA, Z0, XPOS, YPOS, VSYS, KEYS, MAXPAR,
                      CS_NONE, CS_OK, CS_FAILED)
 from .galmod import GalmodPar, model_rings
+from .header import arcsconv
 from .rings import Rings
 
 
@@ -59,6 +62,92 @@ def beam_rings(inr, beam):
     r = Rings().setRings(radii=rad, radsep=beam, **vals)
     r.twolayer = inr.twolayer
     return r
+
+
+def face_on_density(im, rings, h, subpix=2):
+    """The face-on surface density of the map im (ny, nx) in the annulus of every ring (Ellprof.RadialProfile and
+    getSurfDensFaceOn, ellprof.py:78-226): the annuli halfway to the next radius (the last as wide as the one before),
+    around the mean centre of the rings, in the plane of each ring's inclination and position angle; every pixel
+    sampled at subpix x subpix points (float32 offsets), a point inside several annuli counting 1/n in each; per
+    ring the sum of the valid values over the points inside it, over their area (arcsec^2), times cos(inc) (an
+    inclination with |cos| <= 1e-4 taken as 89 deg). Returns (nr,) float64."""
+    im = np.asarray(im)
+    nr = rings.nr
+    R = np.asarray(rings.radii, float)
+    W = np.empty(nr)
+    for r in range(nr):
+        if nr == 1:
+            W[r] = rings.radsep if rings.radsep > 0 else R[0] / 2.
+        elif r == nr - 1:
+            W[r] = R[r] - R[r - 1]
+        else:
+            W[r] = R[r + 1] - R[r]
+    ang = np.radians(-1.0 * (np.asarray(rings.phi, float) + 90.0))
+    cp, sp = np.cos(ang), np.sin(ang)
+    inc = np.asarray(rings.inc, float).copy()
+    ci = np.cos(np.radians(inc))
+    bad = np.abs(ci) <= 0.0001
+    ci[bad] = math.cos(math.radians(89.))
+    ann = np.stack([np.maximum(R - 0.5 * W, 0.), np.maximum(R + 0.5 * W, 0.)], 1).astype(np.float32)
+    rmax = float(np.maximum(R + 0.5 * W, 0.).max())
+    P = [float(np.mean(rings.xpos)), float(np.mean(rings.ypos))]
+    ny, nx = im.shape
+    if not (0 <= P[0] < nx and 0 <= P[1] < ny):
+        raise ValueError("ELLPROF ERROR: The center of galaxy must be inside the map. Exiting...")
+    dx = h.Cdelt(0) * arcsconv(h.Cunit(0))
+    dy = h.Cdelt(1) * arcsconv(h.Cunit(1))
+    adx, ady = abs(dx), abs(dy)
+    x0, x1 = max(0, int(P[0] - rmax / adx - 1)), min(nx - 1, int(P[0] + rmax / adx + 1))
+    y0, y1 = max(0, int(P[1] - rmax / ady - 1)), min(ny - 1, int(P[1] + rmax / ady + 1))
+    ys, xs = np.mgrid[y0:y1 + 1, x0:x1 + 1]
+    xs, ys = xs.ravel(), ys.ravel()
+    v = im[ys, xs].astype(np.float32)
+    Xr = (adx * (xs - P[0])).astype(np.float32)
+    Yr = (ady * (ys - P[1])).astype(np.float32)
+
+    def offsets(step, absd):
+        o, pos = [], np.float32(0.5 * (step - absd))
+        while pos < np.float32(0.5 * absd):
+            o.append(float(pos))
+            pos = np.float32(pos + np.float32(step))
+        return np.array(o)
+    ox, oy = offsets(adx / float(subpix), adx), offsets(ady / float(subpix), ady)
+    OX, OY = np.meshgrid(ox, oy, indexing="ij")
+    px = (Xr[:, None] + OX.ravel()[None]).astype(np.float32)
+    py = (Yr[:, None] + OY.ravel()[None]).astype(np.float32)
+
+    def inside(r):
+        Xx = px * cp[r] - py * sp[r]
+        Yy = (px * sp[r] + py * cp[r]) / ci[r]
+        R2 = (Xx * Xx + Yy * Yy).astype(np.float32)
+        return (ann[r, 0] * ann[r, 0] <= R2) & (R2 < ann[r, 1] * ann[r, 1])
+    over = np.zeros(px.shape, np.int64)
+    for r in range(nr):
+        over += inside(r)
+    val = np.broadcast_to(v[:, None], px.shape).astype(np.float64)
+    val = np.where(over > 1, val / np.maximum(over, 1), val)
+    valid = np.broadcast_to((~np.isnan(v))[:, None], px.shape)
+    sub = float(len(ox) * len(oy))
+    out = np.zeros(nr)
+    for r in range(nr):
+        good = inside(r) & valid
+        area = good.sum() / sub * abs(dx * dy)
+        out[r] = ci[r] * (val[good].sum() / sub / area) if area > 0 else 0.
+    return out
+
+
+def pow10_to_unit(vals):
+    """The power of ten (float32 steps from 1) that brings the smallest of the positive values vals between 0.1 and 10
+    (galfit_out.py:93-106); 1 with none."""
+    vmin = min(vals) if len(vals) else np.finfo(np.float32).max
+    s = np.float32(1.)
+    while vmin < 0.1:
+        vmin *= 10
+        s *= np.float32(10)
+    while vmin > 10:
+        vmin /= 10
+        s /= np.float32(10)
+    return float(s)
 
 
 def search_mask(data, snr=5., grow=3.):
@@ -251,10 +340,18 @@ class ClassicFit:
         torch = self.torch
         r = self.rings if rings is None else rings
         norm = (norm or (self.par.NORM.split() or ["LOCAL"])[0]).upper()
-        if norm not in ("LOCAL", "NONE"):
-            raise NotImplementedError(f"the {norm} normalisation of the final model is not built")
+        norm = "AZIM" if norm == "AZIMUTHAL" else norm
+        if norm not in ("LOCAL", "NONE", "AZIM"):
+            raise ValueError(f"NORM {norm}: LOCAL, AZIM or NONE")
         dm, st = self.dm, self.dm.setup
         nz, ny, nx = self.data.shape
+        sf = 1.
+        if norm == "AZIM":
+            r = r.copy()
+            surf = face_on_density(self.intensity_map(), r, self.h)
+            sf = pow10_to_unit([v for v in surf if np.isfinite(v) and v > 0])
+            on = np.asarray(r.dens, float) > 0
+            r.dens = np.where(on, sf * np.abs(surf), r.dens)
         sr, _ = st.subrings(model_rings(r, dm.ringmode, st.radsep))
         if st.galmodtype == "MC":
             m = dm.array_batch_mc(sr[None], (0, 0), (nx, ny), convolve=dm.cfield is not None)
@@ -266,4 +363,13 @@ class ClassicFit:
             k = torch.from_numpy(np.ascontiguousarray(self.mask.transpose(1, 2, 0)).astype(np.uint8)).to(dev)
             _lib.check(_lib.lib().bb_local_norm(m.data_ptr(), d.data_ptr(), k.data_ptr(), ny, nx, nz, m.data_ptr(),
                                                 _stream_ptr(torch)))
-        return np.ascontiguousarray(m[0].cpu().numpy().transpose(2, 0, 1))
+        out = np.ascontiguousarray(m[0].cpu().numpy().transpose(2, 0, 1))
+        return (out / np.float32(sf)).astype(np.float32) if norm == "AZIM" else out
+
+    def intensity_map(self):
+        """The total-intensity map (ny, nx) float32 of the data in the mask: the sum over the channels times the
+        channel width (km/s), NaN where the sum is 0 (the moment-0 map without flux conversion, moments.py:86-110)."""
+        from .header import delta_vel
+        s = (self.data.astype(np.float64) * self.mask).sum(axis=0)
+        m0 = s * abs(float(delta_vel(self.h)))
+        return np.where(s == 0, np.nan, m0).astype(np.float32)

@@ -264,3 +264,52 @@ def test_solve_profile_relative_vs_oracle():
     ref = nnls(np.vstack([T, srel * L]), np.concatenate([Q.T @ b, np.zeros(7)]))[0]
     assert np.allclose(got, ref, rtol=1e-9, atol=1e-12)
     assert np.allclose(solve_profile_relative(A, b, radii, 0.), nnls(A, b)[0], rtol=1e-9, atol=1e-12)
+
+
+# ---------------------------------------------------------------------- AZIM final model (host pieces)
+def _hdr(n=201, pix=1.):
+    from bbarolo2_amd.header import emptyheader
+    return emptyheader([n, n, 8], [-pix / 3600., pix / 3600., 10.])
+
+
+def test_face_on_density_uniform_map():
+    """A uniform map of c per pixel: the surface density of every annulus is c / pixel area, and its face-on value c
+    cos(inc) / pixel area, at any position angle."""
+    from bbarolo2_amd.classic import face_on_density
+    h = _hdr(201, 2.)
+    im = np.full((201, 201), 3., np.float32)
+    for inc, pa in ((0., 0.), (60., 30.), (45., 200.)):
+        r = _inr([20., 40., 60., 80.], xpos=100., ypos=100., inc=inc, phi=pa)
+        got = face_on_density(im, r, h)
+        assert np.allclose(got, 3. / 4. * math.cos(math.radians(inc)), rtol=1e-6)
+
+
+def test_face_on_density_overlapping_annuli():
+    """Uneven radii 10, 20, 40: annuli [5, 15], [10, 30], [30, 50] (half the spacing to the next radius on both
+    sides); points in two annuli count 1/2 in each, so on a uniform face-on map the densities are the covered area
+    fractions: (75 + 125/2) / 200, (800 - 125/2) / 800 and 1, to the pixel sampling."""
+    from bbarolo2_amd.classic import face_on_density
+    h = _hdr(201, 0.5)
+    im = np.ones((201, 201), np.float32)
+    r = _inr([10., 20., 40.], xpos=100., ypos=100., inc=0., phi=0.)
+    got = face_on_density(im, r, h) * 0.25
+    assert np.allclose(got, [137.5 / 200., 737.5 / 800., 1.], rtol=2e-3)
+
+
+def test_face_on_density_blanks_and_centre():
+    """Blank (NaN) pixels are left out of the sums and of the areas; the centre must be inside the map."""
+    from bbarolo2_amd.classic import face_on_density
+    h = _hdr(101, 1.)
+    im = np.full((101, 101), 2., np.float32)
+    im[:50] = np.nan
+    r = _inr([10., 20.], xpos=50., ypos=50., inc=0., phi=0.)
+    assert np.allclose(face_on_density(im, r, h), 2.)
+    with pytest.raises(ValueError):
+        face_on_density(im, _inr([10., 20.], xpos=150., ypos=50.), h)
+
+
+def test_pow10_to_unit():
+    from bbarolo2_amd.classic import pow10_to_unit
+    assert pow10_to_unit([0.003, 5.]) == float(np.float32(100.))      # (0.003 -> 0.3)
+    assert pow10_to_unit([25.]) == float(np.float32(1.) / np.float32(10.))
+    assert pow10_to_unit([0.5, 9.]) == 1.
