@@ -169,3 +169,38 @@ def test_config1_two_stages_free_inclination():
     inner = slice(1, truth.nr - 1)
     assert all(cf.fitok[inner])
     assert np.all(np.abs(r.vrot[inner] - truth.vrot[inner]) < 10.)
+
+
+def test_azim_final_model(small_header):
+    """The AZIM final model (finalModel, galfit_out.py:86-159): the model of the fitted rings whose densities are the
+    face-on surface densities of the data's intensity map in their annuli (scaled by a power of ten and divided by it
+    again: the NONE model of those densities to float32 rounding), whose own intensity map then has about the shape
+    of the data's radial profile in the inner rings (the model's units are the deposit's, not the map's)."""
+    import torch
+    from bbarolo2_amd.classic import ClassicFit, face_on_density
+    from bbarolo2_amd.header import delta_vel
+    from bbarolo2_amd.engine import DiskModel
+    from bbarolo2_amd.galfit import Fit3DPar
+    torch.cuda.set_device(0)
+    dm = DiskModel(small_header, par=GalmodPar(GALMODTYPE="ANALYTIC"), densflux=True)
+    truth = small_rings(8, z0=5.)
+    sr, _ = dm.subrings(truth)
+    data = dm.array_batch(sr[None], (0, 0), (64, 64))[0].double().cpu().numpy().transpose(2, 0, 1).astype(np.float32)
+    mask = data > 0.01 * data.max()
+    inr = small_rings(8, z0=5., vrot=130., vdisp=20.)
+    cf = ClassicFit(small_header, data, inr, par=Fit3DPar(FREE="VROT VDISP", NORM="AZIM"),
+                    gpar=GalmodPar(GALMODTYPE="ANALYTIC"), mask=mask, twostage=False).run()
+    az = cf.model()
+    surf = face_on_density(cf.intensity_map(), cf.rings, small_header)
+    r = cf.rings.copy()
+    on = np.asarray(r.dens) > 0
+    r.dens = np.where(on, np.abs(surf), 0.)
+    plain = cf.model(norm="NONE", rings=r)
+    print("face-on densities of the data", np.round(surf, 4), "ok", cf.fitok, flush=True)
+    assert np.all(surf[on] > 0)
+    assert np.abs(az - plain).max() <= 1e-5 * np.abs(plain).max()
+    mom = (az.astype(np.float64) * mask).sum(axis=0) * abs(float(delta_vel(small_header)))
+    surf_model = face_on_density(np.where(mom == 0, np.nan, mom).astype(np.float32), cf.rings, small_header)
+    ratio = surf_model[1:6] / surf[1:6]
+    print("model/data face-on densities, rings 2-6", np.round(ratio, 3), flush=True)
+    assert np.all(np.abs(ratio / np.median(ratio) - 1.) < 0.15)
