@@ -35,7 +35,7 @@ class bb_profile_desc(C.Structure):
                 ("dmed", C.c_double)]
 
 
-NKEY = 18      # ring keys of the likelihood context (bbgpu.h BB_K_*)
+NKEY = 21      # keys of the likelihood context (bbgpu.h BB_K_*)
 
 
 class bb_ring_plan(C.Structure):
@@ -50,13 +50,16 @@ class bb_like_desc(C.Structure):
                 ("seed", C.c_uint32), ("box_free", C.c_int32), ("halves", C.c_int32), ("likebox", bb_box),
                 ("fixbox", bb_box), ("sigma", C.c_double), ("likefactor", C.c_double), ("eps", C.c_double),
                 ("ext", C.c_double), ("pixas0", C.c_double), ("pixas1", C.c_double), ("nx", C.c_int32),
-                ("ny", C.c_int32), ("mc_cap", C.c_int64)]
+                ("ny", C.c_int32), ("mc_cap", C.c_int64), ("edgewidth", C.c_double), ("oedge", C.c_double),
+                ("edge0", C.c_double), ("edgea0", C.c_double), ("edger0", C.c_double), ("annw", C.c_double),
+                ("ann_cap", C.c_int32), ("empty", C.c_int32)]
 
 
 class bb_like_profile(C.Structure):
     _fields_ = [("d", bb_profile_desc), ("wpix", C.c_void_p), ("ypix", C.c_void_p), ("lcoef", C.c_void_p),
                 ("lcol", C.c_void_p), ("near", C.c_void_p), ("widx", C.c_void_p), ("wval", C.c_void_p),
-                ("widx_e", C.c_void_p), ("wval_e", C.c_void_p), ("qunit", C.c_double), ("unit", C.c_double)]
+                ("widx_e", C.c_void_p), ("wval_e", C.c_void_p), ("qunit", C.c_double), ("unit", C.c_double),
+                ("radii", C.c_void_p), ("dmin", C.c_double)]
 
 
 class bb_norm_par(C.Structure):

@@ -32,8 +32,8 @@ extern "C" int bb_residual_blocks(int64_t nvox);
 extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* sr, int nsub, int B, const int* naz,
                                    const int4* chunks, int nchunks, const double2* az, int TX, int TY, double* sub,
                                    double* wtab, short4* bbox, short4* sbox, const int* widx, const double* wval,
-                                   int nmap, int map0, int nw, double qunit, unsigned long long* counts, int* bin,
-                                   int bin_cap, hipStream_t st);
+                                   const double* ew, int nmap, int map0, int nw, double qunit,
+                                   unsigned long long* counts, int* bin, int bin_cap, hipStream_t st);
 extern "C" size_t bb_rm_tile_lds(int nw, int TX, int TY);
 extern "C" int bb_launch_zero_planes(unsigned long long* counts, int B, int nmap, int map0, int nw, int64_t plane,
                                      hipStream_t st);
@@ -63,10 +63,11 @@ extern "C" size_t bb_nnls_lds(int n);
 extern "C" int bb_nnls_max();
 extern "C" int bb_launch_profile_solve(int n, int nr, int nok, double smooth, double corr, double floor_, double norm,
                                        double dmed, const double* maps, const double* wpix, const double* ypix,
-                                       int npix, const double* lcoef, const int* lcol, const int* near, int B,
-                                       double* partial, double* gram, double* Mall, double* x, hipStream_t st);
-extern "C" int bb_launch_set_dens(const double* x, int B, int n, int nr, const int* widx, const double* wval, int nsub,
-                                  double* sr, float* dens, hipStream_t st);
+                                       int npix, const double* lcoef, const int* lcol, const int* near,
+                                       const double* normv, int B, double* partial, double* gram, double* Mall,
+                                       double* x, hipStream_t st);
+extern "C" int bb_launch_set_dens(const double* x, int B, int n, int nr, const int* widx, const double* wval,
+                                  const double* ew, int nsub, double* sr, float* dens, hipStream_t st);
 extern "C" int bb_launch_dilate(const uint8_t* mask, int nz, int ny, int nx, double radius, int nchan, uint8_t* tmp,
                                 uint8_t* out, hipStream_t st);
 extern "C" int bb_launch_residual(const float* model, const float* dvalid, int B, int64_t nvox, double sigma2,
@@ -430,6 +431,13 @@ extern "C" int64_t bb_ring_maps_work_size(const bb_azplan* p, int B) { return bb
 extern "C" int bb_ring_maps_deposit(const bb_model* m, const bb_azplan* p, bb_box box, const double* sr, int nsub,
                                     int B, const int32_t* widx, const double* wval, int nmap, int map0, int nw,
                                     double qunit, void* work, int64_t work_bytes, uint64_t* counts, void* stream) {
+    return bb_ring_maps_deposit_ew(m, p, box, sr, nsub, B, widx, wval, nullptr, nmap, map0, nw, qunit, work, work_bytes,
+                                   counts, stream);
+}
+
+int bb_ring_maps_deposit_ew(const bb_model* m, const bb_azplan* p, bb_box box, const double* sr, int nsub, int B,
+                            const int32_t* widx, const double* wval, const double* ew, int nmap, int map0, int nw,
+                            double qunit, void* work, int64_t work_bytes, uint64_t* counts, void* stream) {
     if (!m || !p) return fail(BB_ERR_ARG, "bb_ring_maps_deposit: model or plan is NULL");
     int rc = check_box(m, box);
     if (rc) return rc;
@@ -471,7 +479,7 @@ extern "C" int bb_ring_maps_deposit(const bb_model* m, const bb_azplan* p, bb_bo
     short4* sbox = (short4*)w;
     const long long ntiles = (long long)((bx + TX - 1) / TX) * ((by + TY - 1) / TY);
     rc = bb_launch_ring_maps(&m->g, box, sr, nsub, B, p->d_naz, p->d_chunks, p->nchunks, p->d_az, TX, TY, sub, wtab,
-                             bbox, sbox, (const int*)widx, wval, nmap, map0, nw, qunit, (unsigned long long*)counts,
+                             bbox, sbox, (const int*)widx, wval, ew, nmap, map0, nw, qunit, (unsigned long long*)counts,
                              work_bin(m, p, B, sbox, ntiles), (int)(p->nchunks * (int64_t)m->bin_per), (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_ring_maps_deposit launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
@@ -546,6 +554,12 @@ extern "C" int64_t bb_profile_work_size(const bb_profile* p, int B) {
 
 extern "C" int bb_profile_solve(const bb_profile* p, const double* maps, int B, void* work, int64_t work_bytes,
                                 double* x, void* stream) {
+    return bb_profile_solve_pv(p, maps, B, work, work_bytes, x, nullptr, nullptr, nullptr, nullptr, stream);
+}
+
+int bb_profile_solve_pv(const bb_profile* p, const double* maps, int B, void* work, int64_t work_bytes, double* x,
+                        const double* lcoef, const int32_t* lcol, const int32_t* near, const double* normv,
+                        void* stream) {
     if (!p) return fail(BB_ERR_ARG, "bb_profile_solve: profile is NULL");
     if (B < 0 || B > 65535) return fail(BB_ERR_ARG, "bb_profile_solve: B=%d out of range", B);
     if (B == 0) return 0;
@@ -558,18 +572,27 @@ extern "C" int bb_profile_solve(const bb_profile* p, const double* maps, int B, 
     double* gram = partial + (size_t)B * bb_gram_blocks(p->d.npix) * bb_gram_entries(n);
     double* Mall = gram + (size_t)B * m * m;
     const bool prior = p->d.smooth > 0 && n >= 3;
+    // (the prior's rows of every vector when normv is given, else the profile's)
+    const bool pv = normv != nullptr;
+    if (pv && prior && (!lcoef || !lcol || !near)) return fail(BB_ERR_ARG, "bb_profile_solve: NULL prior rows");
     int rc = bb_launch_profile_solve(n, p->d.nr, p->d.nok, prior ? p->d.smooth : 0.0, p->d.corr, p->d.floor, p->d.norm,
-                                     p->d.dmed, maps, p->d_wpix, p->d_ypix, p->d.npix, p->d_lcoef, p->d_lcol, p->d_near,
-                                     B, partial, gram, Mall, x, (hipStream_t)stream);
+                                     p->d.dmed, maps, p->d_wpix, p->d_ypix, p->d.npix, pv ? lcoef : p->d_lcoef,
+                                     pv ? (const int*)lcol : p->d_lcol, pv ? (const int*)near : p->d_near,
+                                     pv ? normv : nullptr, B, partial, gram, Mall, x, (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_profile_solve launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
 
 extern "C" int bb_profile_set_dens(const double* x, int B, int n, int nr, const int32_t* widx, const double* wval,
                                    int nsub, double* sr, float* dens, void* stream) {
+    return bb_profile_set_dens_ew(x, B, n, nr, widx, wval, nullptr, nsub, sr, dens, stream);
+}
+
+int bb_profile_set_dens_ew(const double* x, int B, int n, int nr, const int32_t* widx, const double* wval,
+                           const double* ew, int nsub, double* sr, float* dens, void* stream) {
     if (B < 0 || B > 65535 || nr < 1 || n < nr || nsub < 0) return fail(BB_ERR_ARG, "bb_profile_set_dens: bad sizes");
     if (B == 0 || nsub == 0) return 0;
     if (!x || !widx || !wval || !sr) return fail(BB_ERR_ARG, "bb_profile_set_dens: NULL array");
-    int rc = bb_launch_set_dens(x, B, n, nr, (const int*)widx, wval, nsub, sr, dens, (hipStream_t)stream);
+    int rc = bb_launch_set_dens(x, B, n, nr, (const int*)widx, wval, ew, nsub, sr, dens, (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_profile_set_dens launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
 

@@ -33,7 +33,7 @@
 
 enum {
     F_R0 = 0, F_R1, F_XPOS, F_YPOS, F_SPA, F_CPA, F_SINC, F_CINC, F_VSYS, F_VROT, F_VRAD, F_VVERT, F_SIG, F_K,
-    F_FLUX, F_WREC, F_WAPP, F_Z0, F_DVDZ, F_ZCYL, F_ZDEP, F_NZZ, F_NIN, F_STEP, F_ZLIM, F_DENS, F_HALF
+    F_FLUX, F_WREC, F_WAPP, F_Z0, F_DVDZ, F_ZCYL, F_ZDEP, F_NZZ, F_NIN, F_STEP, F_ZLIM, F_DENS, F_HALF, F_NAZ
 };
 
 // ---------------------------------------------------------------------------------------------------------
@@ -73,7 +73,16 @@ k_an_prep(bb_grid g, bb_box box, const double* __restrict__ srall, int nsub, con
     double* wt = wtaball + ((size_t)b * nsub + s) * BB_ZCAP;
     auto S = [&](int row) { return sr[(size_t)row * nsub + s]; };
     const double R0 = S(0), dens = unit_dens ? 1.0 : S(5);    // (unit_dens: the ring maps)
-    const int64_t naz = naz_tab[s];
+    // azimuths of the sub-ring: the plan's, or from its radius as the plan makes them (a table whose radii differ per
+    // vector: the exterior annulus with a free edge)
+    int64_t naz;
+    if (naz_tab) {
+        naz = naz_tab[s];
+    } else {
+        naz = (int64_t)(3 * (2 * BB_PI * R0 / sqrt(g.pixarea)) + 0.5);
+        naz = naz < 16 ? 16 : naz > (1 << 24) ? (1 << 24) : naz;
+    }
+    q[F_NAZ] = (double)naz;
     const double ring_area_pix = 2 * BB_PI * R0 * g.radsep / g.pixarea;
     const double r_inner = R0 - 0.5 * g.radsep;
     q[F_R0] = sqrt(r_inner * r_inner + 2.0 * g.radsep * R0 * 0.25);
@@ -1306,8 +1315,8 @@ __global__ void __launch_bounds__(BB_RM_THREADS)
 k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks, int nchunks,
           const double2* __restrict__ az, const double* __restrict__ suball, const double* __restrict__ wtaball,
           int nsub, const short4* __restrict__ bball, const short4* __restrict__ sball, const int2* __restrict__ widx,
-          const double2* __restrict__ wval, int nmap, int map0, int nw, double inv_q,
-          unsigned long long* __restrict__ counts, const int* __restrict__ bin, int bin_cap, int wlds) {
+          const double2* __restrict__ wval, const double* __restrict__ ewall, int nmap, int map0, int nw,
+          double inv_q, unsigned long long* __restrict__ counts, const int* __restrict__ bin, int bin_cap, int wlds) {
     extern __shared__ double dyn[];
     __shared__ int clist[BB_LIST_CAP];
     __shared__ int slist[BB_RM_SAMP_CAP];
@@ -1330,6 +1339,7 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     __syncthreads();
     const double* sub = suball + (size_t)b * nsub * BB_NSF;
     const double* wtab = wtaball + (size_t)b * nsub * BB_ZCAP;
+    const double* ew = ewall ? ewall + (size_t)b * nsub : nullptr;     // (the vector's edge weights of the density)
     if (wlds)
         for (int i = threadIdx.x; i < nsub * BB_RM_WCAP; i += blockDim.x)
             swd[i] = wtab[(size_t)(i / BB_RM_WCAP) * BB_ZCAP + i % BB_RM_WCAP];
@@ -1363,7 +1373,11 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
         const double2 cs = az[ch.w + (e & 31)];
         const double wside = cs.x >= 0.0 ? u.wrec : u.wapp;
         if (wside == 0.0) return;
-        const double2 wv = wval[s];
+        double2 wv = wval[s];
+        if (ew) {
+            wv.x *= ew[s];
+            wv.y *= ew[s];
+        }
         const double x = u.R * cs.x, yb = u.R * cs.y * u.cinc, xs = x * u.spa, xc = x * u.cpa;
         const Track tr = utrack(u, g, xs, xc, yb);
         int ka, kb;
@@ -1404,6 +1418,61 @@ k_rm_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     }
 }
 
+// The map of the exterior annulus when its radii follow each vector's edge (a table per vector, no shared azimuth
+// plan): one workgroup per (sub-ring, radial sample, vector), a thread per azimuth (cos and sin of the azimuth itself
+// rather than the plan's rotation recurrence), the runs of equal-pixel vertical samples of the column added to the
+// vector's counts with global 64-bit atomics, as k_rm_tile adds them (the annulus is a few sub-rings beside hundreds).
+// Sub-rings [0, nlay) are the thin layer (ring weight wlay.x), the rest the thick one (wlay.y).
+__global__ void __launch_bounds__(256)
+k_rm_annulus(bb_grid g, bb_box box, const double* __restrict__ suball, const double* __restrict__ wtaball, int nsub,
+             int nlay, double2 wlay, double inv_q, int nmap, int map0, unsigned long long* __restrict__ counts) {
+    const int s = blockIdx.x >> 1, kr = blockIdx.x & 1, b = blockIdx.y;
+    const double* q = suball + ((size_t)b * nsub + s) * BB_NSF;
+    Sub u;
+    load_sub(u, q, kr);
+    const double wv = s < nlay ? wlay.x : wlay.y;
+    if ((u.wrec == 0.0 && u.wapp == 0.0) || wv == 0.0) return;
+    const int64_t naz = (int64_t)q[F_NAZ];
+    const double* wt = wtaball + ((size_t)b * nsub + s) * BB_ZCAP;
+    const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
+    unsigned long long* plane = counts + ((size_t)b * nmap + map0) * (size_t)by * bx;
+    const double daz = 2 * BB_PI / (double)naz;
+    const double az0 = 2 * BB_PI * (0.5 * ((kr + 1) % 2) + 0.25) / (double)naz;
+    for (int64_t ka = threadIdx.x; ka < naz; ka += blockDim.x) {
+        double saz, caz;
+        sincos(az0 + (double)ka * daz, &saz, &caz);
+        const double wside = caz >= 0.0 ? u.wrec : u.wapp;
+        if (wside == 0.0) continue;
+        const double x = u.R * caz, yb = u.R * saz * u.cinc, xs = x * u.spa, xc = x * u.cpa;
+        const Track tr = utrack(u, g, xs, xc, yb);
+        int c0, c1;
+        upix_fast(u, g, tr, xs, xc, yb, uz(u, 0), c0, c1);
+        double ws = uw_(u, wt, 0, g.ltype);
+        for (int kz = 1; kz <= u.nzz; kz++) {
+            int h0 = 0, h1 = 0;
+            bool same = false;
+            if (kz < u.nzz) {
+                upix_fast(u, g, tr, xs, xc, yb, uz(u, kz), h0, h1);
+                same = h0 == c0 && h1 == c1;
+            }
+            if (same) {
+                ws += uw_(u, wt, kz, g.ltype);
+                continue;
+            }
+            if (c0 >= box.blo0 && c0 < box.bhi0 && c1 >= box.blo1 && c1 < box.bhi1) {
+                const double base = u.flux * ws;
+                atomicAdd(plane + (size_t)(c1 - box.blo1) * bx + (c0 - box.blo0),
+                          (unsigned long long)(base * wv * wside * inv_q + 0.5));
+            }
+            if (kz < u.nzz) {
+                c0 = h0;
+                c1 = h1;
+                ws = uw_(u, wt, kz, g.ltype);
+            }
+        }
+    }
+}
+
 // Zero planes [map0, map0 + nw) of every vector's counts (B, nmap, plane): a kernel rather than a memset, so that the
 // launch can be captured in a HIP graph.
 __global__ void k_zero_planes(unsigned long long* __restrict__ counts, int nmap, int map0, int nw, int64_t plane) {
@@ -1429,8 +1498,8 @@ extern "C" size_t bb_rm_tile_static() { return (BB_LIST_CAP + BB_RM_SAMP_CAP) * 
 extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* sr, int nsub, int B, const int* naz,
                                    const int4* chunks, int nchunks, const double2* az, int TX, int TY, double* sub,
                                    double* wtab, short4* bbox, short4* sbox, const int* widx, const double* wval,
-                                   int nmap, int map0, int nw, double qunit, unsigned long long* counts, int* bin,
-                                   int bin_cap, hipStream_t st) {
+                                   const double* ew, int nmap, int map0, int nw, double qunit,
+                                   unsigned long long* counts, int* bin, int bin_cap, hipStream_t st) {
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
     hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, naz, sub, wtab, 1,
                        (const int4*)nullptr);
@@ -1452,7 +1521,22 @@ extern "C" int bb_launch_ring_maps(const bb_grid* g, bb_box box, const double* s
     const size_t lt = bb_rm_tile_lds(nw, TX, TY), lw = (size_t)nsub * BB_RM_WCAP * sizeof(double);
     const int wl = lt + lw + bb_rm_tile_static() <= 160 * 1024 ? 1 : 0;
     hipLaunchKernelGGL(k_rm_tile, dim3(ntiles, B), dim3(BB_RM_THREADS), lt + (wl ? lw : 0), st, *g, box, TX, TY, chunks,
-                       nchunks, az, sub, wtab, nsub, bbox, sbox, (const int2*)widx, (const double2*)wval, nmap, map0,
-                       nw, 1.0 / qunit, counts, bin, bin_cap, wl);
+                       nchunks, az, sub, wtab, nsub, bbox, sbox, (const int2*)widx, (const double2*)wval, ew, nmap,
+                       map0, nw, 1.0 / qunit, counts, bin, bin_cap, wl);
+    return (int)hipGetLastError();
+}
+
+// The exterior annulus of every vector from its own table sr (B, 16, nsub): the sub-ring records (azimuths from each
+// radius), the plane map0 of counts cleared, then k_rm_annulus. sub, wtab: B nsub records of work.
+extern "C" int bb_launch_rm_annulus(const bb_grid* g, bb_box box, const double* sr, int nsub, int nlay, int B,
+                                    double w_thin, double w_thick, double* sub, double* wtab, int nmap, int map0,
+                                    double qunit, unsigned long long* counts, hipStream_t st) {
+    const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
+    int e = bb_launch_zero_planes(counts, B, nmap, map0, 1, (int64_t)by * bx, st);
+    if (e || nsub == 0 || B == 0) return e;
+    hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, (const int*)nullptr,
+                       sub, wtab, 1, (const int4*)nullptr);
+    hipLaunchKernelGGL(k_rm_annulus, dim3(2 * nsub, B), dim3(256), 0, st, *g, box, sub, wtab, nsub, nlay,
+                       make_double2(w_thin, w_thick), 1.0 / qunit, nmap, map0, counts);
     return (int)hipGetLastError();
 }

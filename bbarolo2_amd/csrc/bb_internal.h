@@ -77,7 +77,26 @@ int bb_galmod_mc_vb(const bb_model* m, bb_box box, const int4* vbox, const doubl
                     uint32_t seed, int64_t max_clouds, void* work, int64_t work_bytes, float* out, uint64_t* counts,
                     int* overflow, int accumulate, void* stream);
 
+// The ring maps, profile and densities with per-vector parts (the likelihood context with a free edge of the disk):
+// ew (B, nsub) multiplies the ring weights of each vector's sub-rings (the edge weights of the density; NULL: 1); the
+// prior's rows of every vector (lcoef, lcol: B (n-2) 3; near: B (n-2); normv: B) replace the profile's when normv is
+// given.
+int bb_ring_maps_deposit_ew(const bb_model* m, const bb_azplan* p, bb_box box, const double* sr, int nsub, int B,
+                            const int32_t* widx, const double* wval, const double* ew, int nmap, int map0, int nw,
+                            double qunit, void* work, int64_t work_bytes, uint64_t* counts, void* stream);
+int bb_profile_solve_pv(const bb_profile* p, const double* maps, int B, void* work, int64_t work_bytes, double* x,
+                        const double* lcoef, const int32_t* lcol, const int32_t* near, const double* normv,
+                        void* stream);
+int bb_profile_set_dens_ew(const double* x, int B, int n, int nr, const int32_t* widx, const double* wval,
+                           const double* ew, int nsub, double* sr, float* dens, void* stream);
+
 // kernel launchers shared with bb_like.hip
+extern "C" int bb_launch_rm_annulus(const bb_grid* g, bb_box box, const double* sr, int nsub, int nlay, int B,
+                                    double w_thin, double w_thick, double* sub, double* wtab, int nmap, int map0,
+                                    double qunit, unsigned long long* counts, hipStream_t st);
+extern "C" int bb_launch_zero_u32(void* p, int64_t n, hipStream_t st);
+extern "C" size_t bb_analytic_sub_bytes();
+extern "C" size_t bb_analytic_wt_bytes();
 extern "C" int bb_launch_conv_resid(const float* in, const float* dvalid, int B, int by, int bx, int nz,
                                     const float* kern, int ky, int kx, float scale, double sigma2, double eps2,
                                     double* partial, const int4* vbox, const float2* sv, int frac, hipStream_t st);

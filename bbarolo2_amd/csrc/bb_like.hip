@@ -89,6 +89,24 @@ __device__ double row_value(const PlanDev& P, const FitDev& F, const double* th,
     }
 }
 
+// The edges of the disk that theta moves (_update_rings, bayesian/fitter.py:1112-1120): RADMAX the edge of the disk (a
+// sharp edge, or a Gaussian one of width `width`), RADMAXA / RADMAXR those of the approaching / receding halves (sharp;
+// negative: none, galmod.py:853-863). ew (B, nsub_t), when given, receives the density edge weight of every sub-ring
+// (the ring maps and the profile's densities take it, as the reference's ring weights hold it, galmod.py:1285-1314).
+struct EdgeDev {
+    int oe, oa, orr;          // offsets in theta (-1: not free)
+    double width;             // arcsec
+    double dr;                // arcsec: the sub-ring width (the setup's radsep)
+    double* ew;
+};
+
+// edge_weights (galmod.py:1122-1131): the fraction of the sub-ring [R - dr/2, R + dr/2] inside the edge, or the
+// Gaussian edge's 1/2 erfc((R - edge) / (sqrt(2) width))
+__device__ __forceinline__ double edge_weight(double R, double dr, double edge, double width) {
+    if (width > 0) return 0.5 * erfc((R - edge) / (sqrt(2.) * width));
+    return fmin(fmax((edge - (R - dr / 2.)) / dr, 0.), 1.);
+}
+
 // linear interpolation of row r at sub-ring s between the model rings of its segment (interp_values)
 __device__ double interp_row(const PlanDev& P, const FitDev& F, const double* th, int r, int s) {
     if (P.cst[s]) return row_value(P, F, th, r, 0);
@@ -105,7 +123,7 @@ __device__ double interp_row(const PlanDev& P, const FitDev& F, const double* th
 // One thread per (sub-ring, vector): the 16 rows of the table (B, 16, nsub_t) of GalmodSetup.subrings
 // (galmod.py:761-851: the edge weights of the density, the thin and thick layers, the side weights, galmod.py:833-851).
 __global__ void __launch_bounds__(64)
-k_like_tables(PlanDev P, FitDev F, const double* __restrict__ theta, double* __restrict__ srall) {
+k_like_tables(PlanDev P, FitDev F, EdgeDev E, const double* __restrict__ theta, double* __restrict__ srall) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
     if (s >= P.nsub) return;
     const double* th = theta + (size_t)b * F.ndim;
@@ -114,12 +132,33 @@ k_like_tables(PlanDev P, FitDev F, const double* __restrict__ theta, double* __r
     v[0] = P.rads[s];
     for (int r = 1; r <= 13; r++) v[r] = interp_row(P, F, th, r, s);
     double dens = v[5];
-    if (P.densw) dens = dens * P.densw[s];
+    const double Ras = v[0] * 60.;
+    if (E.oe >= 0) {
+        const double w = edge_weight(Ras, E.dr, th[E.oe], E.width);
+        dens = dens * w;
+        if (E.ew) {
+            E.ew[(size_t)b * P.nsub_t + s] = w;
+            if (P.thick) E.ew[(size_t)b * P.nsub_t + P.nsub + s] = w;
+        }
+    } else if (P.densw) {
+        dens = dens * P.densw[s];
+    }
+    // the side weights: the plan's, or those of the vector's own edges (both layers at the sub-ring's radius)
+    double wrec = P.side[s], wapp = P.side[P.nsub_t + s];
+    double wrec_t = P.thick ? P.side[P.nsub + s] : 0., wapp_t = P.thick ? P.side[P.nsub_t + P.nsub + s] : 0.;
+    if (E.orr >= 0) {
+        const double e = th[E.orr];
+        wrec = wrec_t = e >= 0 ? edge_weight(Ras, E.dr, e, 0.) : 1.0;
+    }
+    if (E.oa >= 0) {
+        const double e = th[E.oa];
+        wapp = wapp_t = e >= 0 ? edge_weight(Ras, E.dr, e, 0.) : 1.0;
+    }
     auto put = [&](int row, int col, double x) { sr[(size_t)row * P.nsub_t + col] = x; };
     if (!P.thick) {
         for (int r = 0; r <= 13; r++) put(r, s, r == 5 ? dens : v[r]);
-        put(14, s, P.side[s]);
-        put(15, s, P.side[P.nsub_t + s]);
+        put(14, s, wrec);
+        put(15, s, wapp);
         return;
     }
     const double f = interp_row(P, F, th, 14, s);
@@ -141,10 +180,150 @@ k_like_tables(PlanDev P, FitDev F, const double* __restrict__ theta, double* __r
         put(r, s, thin);
         put(r, t, thk);
     }
-    put(14, s, P.side[s]);
-    put(15, s, P.side[P.nsub_t + s]);
-    put(14, t, P.side[t]);
-    put(15, t, P.side[P.nsub_t + t]);
+    put(14, s, wrec);
+    put(15, s, wapp);
+    put(14, t, wrec_t);
+    put(15, t, wapp_t);
+}
+
+// The exterior annulus of the profile when an edge of the disk is free (profile.py:45-64, galmod.py:761-832): per
+// vector its radius rext = min(outer edge, RADMAX, max(RADMAXA, RADMAXR) when both halves have one) + width / 2, the
+// model rings of one CENTRED ring that wide (model_rings, galmod.py:1066-1119), the float32-accumulated sub-ring radii
+// (capacity cap per layer; past the last, zero side weights), every sub-ring with the values of the last ring (as the
+// annulus plan gives them: its model rings are equal, so the interpolation gives their values exactly), and the rows
+// of the curvature prior on the radii of the rings and of the annulus (prior_rows, profile.py:81-150), which move with
+// rext. One workgroup per vector; lane 0 does the sequential parts.
+struct AnnDev {
+    int oe, oa, orr;          // theta offsets of RADMAX, RADMAXA, RADMAXR (-1: not free)
+    double edge0, ea0, er0;   // the fixed edges (edge0 NaN: none; ea0, er0 negative: none)
+    double oedge, w;          // arcsec: the outer edge of the rings, the width of the annulus
+    double radsep;            // arcmin: the sub-ring width
+    int empty, cap;           // EMPTY, sub-rings per layer
+    int nr;                   // rings (the prior has n = nr + 1 columns)
+    const double* radii;      // (nr) arcsec
+    double dmin, dmed;        // a quarter of the smallest ring spacing, the median spacing
+    double* lcoef;            // (B, n - 2, 3)
+    int* lcol;                // (B, n - 2, 3)
+    int* near;                // (B, n - 2)
+    double* norm;             // (B)
+    int* overflow;            // set when a vector's annulus has more sub-rings than cap
+};
+
+#define BB_ANN_CAP 2048
+
+__global__ void __launch_bounds__(64)
+k_ann_tables(PlanDev P, FitDev F, AnnDev A, const double* __restrict__ theta, double* __restrict__ srall) {
+#pragma clang fp contract(off)
+    __shared__ double rads[BB_ANN_CAP];
+    __shared__ int nsh;
+    __shared__ double rows[17];
+    const int b = blockIdx.x, lane = threadIdx.x;
+    const double* th = theta + (size_t)b * F.ndim;
+    const int nst = A.cap * (P.thick ? 2 : 1);
+    double* sr = srall + (size_t)b * 16 * nst;
+    if (lane == 0) {
+        double edge = A.oedge;
+        const double e = A.oe >= 0 ? th[A.oe] : A.edge0;
+        if (e == e) edge = fmin(edge, e);
+        const double ea = A.oa >= 0 ? th[A.oa] : A.ea0, er = A.orr >= 0 ? th[A.orr] : A.er0;
+        if (ea >= 0 && er >= 0) edge = fmin(edge, fmax(ea, er));
+        const double rext = edge + A.w / 2.;
+        // model_rings of one CENTRED ring of width w
+        double rad[3];
+        int nm = 0;
+        const double rin = fmax(rext - A.w / 2., 0.), rout = rext + A.w / 2.;
+        if (rin / 60. + A.radsep < rext / 60.) rad[nm++] = rin / 60.;
+        rad[nm++] = rext / 60.;
+        if (rext / 60. + A.radsep < rout / 60.) rad[nm++] = rout / 60.;
+        // subring_radii: float32 accumulation
+        int n = 0;
+        const bool inner = !(rad[0] != 0 && A.empty);
+        float r = inner ? (float)(A.radsep / 2.0) : (float)(rad[0] + A.radsep / 2.0);
+        if (inner)
+            for (const float r0 = (float)rad[0]; r < r0; r = (float)((double)r + A.radsep)) {
+                if (n < A.cap) rads[n] = (double)r;
+                if (++n > A.cap) break;
+            }
+        for (int i = 1; i < nm && n <= A.cap; i++)
+            for (const float ri = (float)rad[i]; r < ri; r = (float)((double)r + A.radsep)) {
+                if (n < A.cap) rads[n] = (double)r;
+                if (++n > A.cap) break;
+            }
+        if (n > A.cap) {
+            *A.overflow = 1;
+            n = A.cap;
+        }
+        nsh = n;
+        for (int k = 1; k <= 13; k++) rows[k] = row_value(P, F, th, k, 0);
+        if (P.thick)
+            for (int k = 14; k <= 16; k++) rows[k] = row_value(P, F, th, k, 0);
+        // prior_rows on the radii of the rings and rext, sorted (stable: rext after the radii equal to it)
+        const int nr = A.nr, nc = nr + 1, nrow = nc > 2 ? nc - 2 : 0;
+        int p = 0;
+        while (p < nr && A.radii[p] <= rext) p++;
+        auto Rs = [&](int i) { return i < p ? A.radii[i] : i == p ? rext : A.radii[i - 1]; };
+        auto ord = [&](int i) { return i < p ? i : i == p ? nr : i - 1; };
+        const double ptp = fmax(A.radii[nr - 1], rext) - fmin(A.radii[0], rext);
+        const double tiny = 1e-6 * fmax(ptp, 1e-300);
+        const double dmin = nc > 2 ? fmax(A.dmin, tiny) : tiny;
+        double* lc = A.lcoef + (size_t)b * 3 * nrow;
+        int* lk = A.lcol + (size_t)b * 3 * nrow;
+        int* nearb = A.near + (size_t)b * nrow;
+        double norm = 0.;
+        for (int i = 0; i < nrow; i++) {
+            const double ra = Rs(i + 1) - Rs(i), rb = Rs(i + 2) - Rs(i + 1);
+            const double a = fmax(ra, tiny), bb = fmax(rb, tiny);
+            const double c = 2. / (a + bb);
+            lc[3 * i] = c * (1. / a);
+            lc[3 * i + 1] = c * (-(1. / a + 1. / bb));
+            lc[3 * i + 2] = c * (1. / bb);
+            lk[3 * i] = ord(i);
+            lk[3 * i + 1] = ord(i + 1);
+            lk[3 * i + 2] = ord(i + 2);
+            const double a2 = fmax(a, dmin), b2 = fmax(bb, dmin);
+            const double c2 = 2. / (a2 + b2);
+            const double m0 = c2 * (1. / a2), m1 = c2 * (-(1. / a2 + 1. / b2)), m2 = c2 * (1. / b2);
+            norm += m0 * m0 + m1 * m1 + m2 * m2;
+            nearb[i] = fmin(ra, rb) < 0.25 * A.dmed ? 1 : 0;
+        }
+        A.norm[b] = norm;
+    }
+    __syncthreads();
+    const int n = nsh;
+    auto put = [&](int row, int col, double x) { sr[(size_t)row * nst + col] = x; };
+    for (int k = lane; k < A.cap; k += 64) {
+        const bool on = k < n;
+        const double R0 = on ? rads[k] : 0.0;
+        const double dens = rows[5];
+        if (!P.thick) {
+            put(0, k, R0);
+            for (int r = 1; r <= 13; r++) put(r, k, rows[r]);
+            put(14, k, on ? 1.0 : 0.0);
+            put(15, k, on ? 1.0 : 0.0);
+            continue;
+        }
+        const int t = A.cap + k;
+        const double f = rows[14];
+        for (int r = 0; r <= 13; r++) {
+            double thin = r == 0 ? R0 : rows[r], thk = thin;
+            if (r == 5) {
+                thin = dens * (1. - f);
+                thk = dens * f;
+            } else if (r == 7) {
+                thin = 0.;
+            } else if (r == 6) {
+                thk = rows[15];
+            } else if (r == 4) {
+                thk = rows[16];
+            }
+            put(r, k, thin);
+            put(r, t, thk);
+        }
+        put(14, k, on ? 1.0 : 0.0);
+        put(15, k, on ? 1.0 : 0.0);
+        put(14, t, on ? 1.0 : 0.0);
+        put(15, t, on ? 1.0 : 0.0);
+    }
 }
 
 }  // namespace
@@ -160,6 +339,10 @@ struct bb_ctx {
     int64_t vox;               // by bx nz
     FitDev fit;
     PlanDev plan, ann;
+    EdgeDev edge;                // the model's edges moved by theta (offsets -1: fixed, in the plan)
+    bool ann_free = false;       // an edge is free and the profile is computed: the annulus of every vector
+    AnnDev annd;
+    double w_thin = 0., w_thick = 0.;   // the annulus's ring weights of its thin and thick sub-rings
     std::vector<void*> dev_allocs;
     // host copies for the validity and boxes
     std::vector<double> rings;   // (BB_NKEY, nr)
@@ -182,6 +365,8 @@ struct bb_ctx {
     size_t stage_bytes = 0;
     double* sr = nullptr;
     double* sr_e = nullptr;
+    double* e_sub = nullptr;     // the annulus's sub-ring records (an edge free)
+    double* e_wt = nullptr;
     float* dep = nullptr;
     uint64_t* counts = nullptr;
     double* maps = nullptr;
@@ -289,10 +474,20 @@ int copy_plan(bb_ctx* c, const bb_ring_plan* p, PlanDev* P, const char* what) {
     return rc;
 }
 
-int launch_tables(bb_ctx* c, const PlanDev& P, const double* theta, int B, double* out, hipStream_t st) {
+// the tables of the model (with its edges moved by theta) or of the fixed exterior annulus (no edges)
+int launch_tables(bb_ctx* c, const PlanDev& P, bool model, const double* theta, int B, double* out, hipStream_t st) {
     if (P.nsub == 0 || B == 0) return 0;
-    hipLaunchKernelGGL(k_like_tables, dim3((P.nsub + 63) / 64, B), dim3(64), 0, st, P, c->fit, theta, out);
+    EdgeDev E = {-1, -1, -1, 0., 0., nullptr};
+    if (model) E = c->edge;
+    hipLaunchKernelGGL(k_like_tables, dim3((P.nsub + 63) / 64, B), dim3(64), 0, st, P, c->fit, E, theta, out);
     return bb_hip_check(hipGetLastError(), "k_like_tables");
+}
+
+// the exterior annulus of every vector (an edge free): its tables and the rows of the prior
+int launch_ann(bb_ctx* c, const double* theta, int B, hipStream_t st) {
+    if (B == 0) return 0;
+    hipLaunchKernelGGL(k_ann_tables, dim3(B), dim3(64), 0, st, c->ann, c->fit, c->annd, theta, c->sr_e);
+    return bb_hip_check(hipGetLastError(), "k_ann_tables");
 }
 
 // the stage layout: theta (chunk ndim doubles), vbox_abs, vbox_rel (chunk int4 each), sv (chunk float2)
@@ -327,20 +522,36 @@ int run_part(bb_ctx* c, int p, int a, int n, const Stage& D, hipStream_t st, hip
         const int nm = c->nprof;
         uint64_t* counts = c->counts + (size_t)a * nm * P;
         double* maps = c->maps + (size_t)a * nm * P;
-        rc = bb_ring_maps_deposit(m, c->pm, c->d.likebox, sr, c->nsub_t, n, c->widx, c->wval, nm, 0, c->d.nr, c->qunit,
-                                  c->awork[p], c->awork_b[p], counts, st);
-        if (!rc)
-            rc = bb_ring_maps_deposit(m, c->pe, c->d.likebox, c->sr_e + (size_t)a * 16 * c->nsub_e, c->nsub_e, n,
-                                      c->widx_e, c->wval_e, nm, c->d.nr, 1, c->qunit, c->awork[p], c->awork_b[p], counts,
-                                      st);
+        const double* ew = c->edge.ew ? c->edge.ew + (size_t)a * c->nsub_t : nullptr;
+        rc = bb_ring_maps_deposit_ew(m, c->pm, c->d.likebox, sr, c->nsub_t, n, c->widx, c->wval, ew, nm, 0, c->d.nr,
+                                     c->qunit, c->awork[p], c->awork_b[p], counts, st);
+        double* sre = c->sr_e + (size_t)a * 16 * c->nsub_e;
+        if (!rc && c->ann_free) {
+            const size_t nse = (size_t)a * c->nsub_e;
+            rc = bb_launch_rm_annulus(&m->g, c->d.likebox, sre, c->nsub_e, c->annd.cap, n, c->w_thin, c->w_thick,
+                                      c->e_sub + nse * (bb_analytic_sub_bytes() / sizeof(double)),
+                                      c->e_wt + nse * (bb_analytic_wt_bytes() / sizeof(double)), nm, c->d.nr,
+                                      c->qunit, (unsigned long long*)counts, st);
+            if (rc) rc = bb_fail(BB_ERR_HIP, "bb_loglike_launch: annulus maps: %s", hipGetErrorString((hipError_t)rc));
+        } else if (!rc) {
+            rc = bb_ring_maps_deposit(m, c->pe, c->d.likebox, sre, c->nsub_e, n, c->widx_e, c->wval_e, nm, c->d.nr, 1,
+                                      c->qunit, c->awork[p], c->awork_b[p], counts, st);
+        }
         const int lby = c->d.likebox.bhi1 - c->d.likebox.blo1, lbx = c->d.likebox.bhi0 - c->d.likebox.blo0;
         if (!rc) rc = bb_ring_maps_finish(m, counts, n * nm, lby, lbx, c->unit, m->d_kern64 != nullptr, maps, st);
         if (!rc) rc = mark(1);
         if (!rc && after_maps) rc = bb_hip_check(hipEventRecord(after_maps, st), "bb_loglike_launch");
-        if (!rc) rc = bb_profile_solve(c->prof, maps, n, c->pwork[p], c->pwork_b[p], c->x + (size_t)a * nm, st);
+        if (!rc && c->ann_free) {
+            const size_t nrow = nm > 2 ? (size_t)(nm - 2) : 0;
+            rc = bb_profile_solve_pv(c->prof, maps, n, c->pwork[p], c->pwork_b[p], c->x + (size_t)a * nm,
+                                     c->annd.lcoef + (size_t)a * 3 * nrow, c->annd.lcol + (size_t)a * 3 * nrow,
+                                     c->annd.near + (size_t)a * nrow, c->annd.norm + a, st);
+        } else if (!rc) {
+            rc = bb_profile_solve(c->prof, maps, n, c->pwork[p], c->pwork_b[p], c->x + (size_t)a * nm, st);
+        }
         if (!rc)
-            rc = bb_profile_set_dens(c->x + (size_t)a * nm, n, nm, c->d.nr, c->widx, c->wval, c->nsub_t, sr,
-                                     c->dens + (size_t)a * c->d.nr, st);
+            rc = bb_profile_set_dens_ew(c->x + (size_t)a * nm, n, nm, c->d.nr, c->widx, c->wval, ew, c->nsub_t, sr,
+                                        c->dens + (size_t)a * c->d.nr, st);
     } else {
         rc = mark(1);
         if (!rc && after_maps) rc = bb_hip_check(hipEventRecord(after_maps, st), "bb_loglike_launch");
@@ -380,8 +591,13 @@ int enqueue_body(bb_ctx* c, hipStream_t st) {
     Stage D = stage_of(c, c->stage_d);
     int rc = bb_hip_check(hipMemcpyAsync(c->stage_d, c->stage_h, c->stage_bytes, hipMemcpyHostToDevice, st),
                           "bb_loglike_launch: upload");
-    if (!rc) rc = launch_tables(c, c->plan, D.th, nv, c->sr, st);
-    if (!rc && c->prof) rc = launch_tables(c, c->ann, D.th, nv, c->sr_e, st);
+    if (!rc) rc = launch_tables(c, c->plan, true, D.th, nv, c->sr, st);
+    if (!rc && c->ann_free) {
+        rc = bb_hip_check((hipError_t)bb_launch_zero_u32(c->mcflag + 2, 1, st), "bb_loglike_launch");
+        if (!rc) rc = launch_ann(c, D.th, nv, st);
+    } else if (!rc && c->prof) {
+        rc = launch_tables(c, c->ann, false, D.th, nv, c->sr_e, st);
+    }
     if (rc) return rc;
     const bool halves = c->d.halves && nv >= 64;
     c->nparts = halves ? 2 : 1;
@@ -409,8 +625,8 @@ int enqueue_body(bb_ctx* c, hipStream_t st) {
     if (!rc && c->prof)
         rc = bb_hip_check(hipMemcpyAsync(dens_h, c->dens, (size_t)nv * c->d.nr * sizeof(float), hipMemcpyDeviceToHost,
                                          st), "download");
-    if (!rc && c->d.mc)
-        rc = bb_hip_check(hipMemcpyAsync(flag_h, c->mcflag, 2 * sizeof(int), hipMemcpyDeviceToHost, st), "download");
+    if (!rc && (c->d.mc || c->ann_free))
+        rc = bb_hip_check(hipMemcpyAsync(flag_h, c->mcflag, 3 * sizeof(int), hipMemcpyDeviceToHost, st), "download");
     return rc;
 }
 
@@ -539,8 +755,21 @@ extern "C" int bb_ctx_create(const bb_model* m, const bb_like_desc* d, const flo
         const int n = free_ring[k] ? d->nr : 1;
         if (free_off[k] >= 0 && free_off[k] + n > d->ndim)
             return bb_fail(BB_ERR_ARG, "bb_ctx_create: free key %d beyond theta", k);
-        if ((k == BB_K_SIGMA || k == BB_K_FRACERR) && free_off[k] >= 0 && free_ring[k])
-            return bb_fail(BB_ERR_ARG, "bb_ctx_create: SIGMA and FRACERR are single values");
+        if (k >= BB_K_SIGMA && free_off[k] >= 0 && free_ring[k])
+            return bb_fail(BB_ERR_ARG, "bb_ctx_create: SIGMA, FRACERR, RADMAX, RADMAXA and RADMAXR are single values");
+    }
+    const bool edges_free = free_off[BB_K_RADMAX] >= 0 || free_off[BB_K_RADMAXA] >= 0 || free_off[BB_K_RADMAXR] >= 0;
+    if (free_off[BB_K_RADMAX] >= 0 && !(d->edgewidth >= 0))
+        return bb_fail(BB_ERR_ARG, "bb_ctx_create: the width of the edge must be >= 0");
+    if (prof && edges_free) {
+        if (!(d->annw > 0) || !(d->oedge >= 0) || d->ann_cap < 1 || d->ann_cap > BB_ANN_CAP)
+            return bb_fail(BB_ERR_ARG, "bb_ctx_create: a free edge needs the annulus width, the outer edge and an "
+                                       "annulus capacity of 1..%d sub-rings (%d)", BB_ANN_CAP, d->ann_cap);
+        if (!prof->radii || (d->nr > 1 && !(prof->dmin > 0)))
+            return bb_fail(BB_ERR_ARG, "bb_ctx_create: a free edge needs the radii of the rings and their spacing");
+        for (int i = 1; i < d->nr; i++)
+            if (!(prof->radii[i] > prof->radii[i - 1]))
+                return bb_fail(BB_ERR_ARG, "bb_ctx_create: the radii must increase");
     }
     if (prof && !annulus) return bb_fail(BB_ERR_ARG, "bb_ctx_create: the profile needs the annulus plan");
     bb_ctx* c = new bb_ctx();
@@ -561,10 +790,51 @@ extern "C" int bb_ctx_create(const bb_model* m, const bb_like_desc* d, const flo
     if (!rc) rc = copy_plan(c, plan, &c->plan, "model");
     c->thick = plan->thick != 0;
     c->nsub_t = c->plan.nsub_t;
+    c->edge = {free_off[BB_K_RADMAX], free_off[BB_K_RADMAXA], free_off[BB_K_RADMAXR], d->edgewidth, m->g.radsep * 60.,
+               nullptr};
     if (!rc && prof) {
         rc = copy_plan(c, annulus, &c->ann, "annulus");
         c->nsub_e = c->ann.nsub_t;
     }
+    c->ann_free = prof && edges_free;
+    if (!rc && c->ann_free) {
+        // the annulus of every vector: cap sub-rings per layer; its ring weights are those of the fixed annulus's
+        // first thin and thick sub-rings (the same for all of its sub-rings: equal model rings, no edge)
+        c->nsub_e = d->ann_cap * (c->ann.thick ? 2 : 1);
+        if (annulus->nsub < 1 || !prof->wval_e || !prof->widx_e || prof->widx_e[0] != 0)
+            rc = bb_fail(BB_ERR_ARG, "bb_ctx_create: the annulus plan has no sub-ring of its ring");
+        else {
+            c->w_thin = prof->wval_e[0];
+            c->w_thick = c->ann.thick ? prof->wval_e[2 * annulus->nsub] : 0.;
+        }
+        AnnDev& A = c->annd;
+        A.oe = free_off[BB_K_RADMAX];
+        A.oa = free_off[BB_K_RADMAXA];
+        A.orr = free_off[BB_K_RADMAXR];
+        A.edge0 = d->edge0;
+        A.ea0 = d->edgea0;
+        A.er0 = d->edger0;
+        A.oedge = d->oedge;
+        A.w = d->annw;
+        A.radsep = m->g.radsep;
+        A.empty = d->empty ? 1 : 0;
+        A.cap = d->ann_cap;
+        A.nr = d->nr;
+        A.dmin = prof->dmin;
+        A.dmed = prof->d.dmed;
+        const size_t C = (size_t)d->chunk, nrow = d->nr + 1 > 2 ? (size_t)(d->nr - 1) : 0;
+        if (!rc) rc = dupload(c, &A.radii, prof->radii, (size_t)d->nr, "bb_ctx_create: radii");
+        if (!rc) rc = dalloc(c, &A.lcoef, C * 3 * nrow, "bb_ctx_create: prior rows");
+        if (!rc) rc = dalloc(c, &A.lcol, C * 3 * nrow, "bb_ctx_create: prior rows");
+        if (!rc) rc = dalloc(c, &A.near, C * nrow, "bb_ctx_create: prior rows");
+        if (!rc) rc = dalloc(c, &A.norm, C, "bb_ctx_create: prior rows");
+        if (!rc) rc = dalloc(c, &c->e_sub, C * c->nsub_e * (bb_analytic_sub_bytes() / sizeof(double)),
+                             "bb_ctx_create: annulus records");
+        if (!rc) rc = dalloc(c, &c->e_wt, C * c->nsub_e * (bb_analytic_wt_bytes() / sizeof(double)),
+                             "bb_ctx_create: annulus records");
+    }
+    if (!rc && prof && free_off[BB_K_RADMAX] >= 0) rc = dalloc(c, &c->edge.ew, (size_t)d->chunk * c->nsub_t,
+                                                               "bb_ctx_create: edge weights");
     if (rc) {
         bb_ctx_destroy(c);
         return rc;
@@ -683,8 +953,9 @@ extern "C" int bb_ctx_create(const bb_model* m, const bb_like_desc* d, const flo
     if (!rc) rc = dalloc(c, &c->cwork, C * c->cper, "bb_ctx_create");
     if (!rc) rc = dalloc(c, &c->ss, C, "bb_ctx_create");
     if (!rc) rc = dalloc(c, &c->lg, C, "bb_ctx_create");
-    if (!rc) rc = dalloc(c, &c->mcflag, 2, "bb_ctx_create");
-    if (!rc) rc = bb_hip_check(hipMemset(c->mcflag, 0, 2 * sizeof(int)), "bb_ctx_create");
+    if (!rc) rc = dalloc(c, &c->mcflag, 4, "bb_ctx_create");
+    if (!rc) rc = bb_hip_check(hipMemset(c->mcflag, 0, 4 * sizeof(int)), "bb_ctx_create");
+    c->annd.overflow = c->mcflag + 2;
     for (int p = 0; p < 2 && !rc; p++) {
         const int nmax = p == 0 ? d->chunk : d->chunk / 2 + 1;
         int64_t a = bb_analytic_work_size(c->pm, nmax);
@@ -788,6 +1059,7 @@ extern "C" int bb_loglike_launch(bb_ctx* c, const double* theta, int B, void* st
             for (int j = 0; j < n && ok; j++) {
                 const double v = th[c->off[k] + j];
                 if (!(v == v)) ok = false;
+                else if (k == BB_K_RADMAX && v < 0) ok = false;
                 else if (positive_only(k) && (float)v < 0.f) ok = false;
                 else if (k == kft && (float)v > 1.f) ok = false;
             }
@@ -883,6 +1155,9 @@ extern "C" int bb_loglike_collect(bb_ctx* c, double* loglike, float* dens) {
     }
     c->pending = false;
     if (rc) return rc;
+    if (c->ann_free && c->nv > 0 && flag_h[2] != 0)
+        return bb_fail(BB_ERR_ARG, "bb_loglike_collect: a vector's exterior annulus has more than %d sub-rings",
+                       c->annd.cap);
     const bb_like_desc& d = c->d;
     const int nr = d.nr;
     for (int i = 0; i < c->B; i++) loglike[i] = -INFINITY;
@@ -932,8 +1207,10 @@ extern "C" int bb_ctx_tables(bb_ctx* c, const double* theta, int B, double* sr, 
     if (B == 0) return 0;
     int rc = bb_hip_check(hipMemcpy(c->stage_d, theta, (size_t)B * c->d.ndim * sizeof(double), hipMemcpyHostToDevice),
                           "bb_ctx_tables");
-    if (!rc) rc = launch_tables(c, c->plan, (const double*)c->stage_d, B, c->sr, nullptr);
-    if (!rc && c->prof && sr_ext) rc = launch_tables(c, c->ann, (const double*)c->stage_d, B, c->sr_e, nullptr);
+    if (!rc) rc = launch_tables(c, c->plan, true, (const double*)c->stage_d, B, c->sr, nullptr);
+    if (!rc && c->prof && sr_ext)
+        rc = c->ann_free ? launch_ann(c, (const double*)c->stage_d, B, nullptr)
+                         : launch_tables(c, c->ann, false, (const double*)c->stage_d, B, c->sr_e, nullptr);
     if (!rc) rc = bb_hip_check(hipDeviceSynchronize(), "bb_ctx_tables");
     if (!rc)
         rc = bb_hip_check(hipMemcpy(sr, c->sr, (size_t)B * 16 * c->nsub_t * sizeof(double), hipMemcpyDeviceToHost),

@@ -493,10 +493,19 @@ __device__ void nn_build(double* __restrict__ M, const double* __restrict__ G, i
 
 __global__ void __launch_bounds__(64)
 k_nnls(ProfConst pc, const double* __restrict__ gram, const double* __restrict__ lcoef, const int* __restrict__ lcol,
-       const int* __restrict__ near, double* __restrict__ Mall, double* __restrict__ xout) {
+       const int* __restrict__ near, const double* __restrict__ normv, double* __restrict__ Mall,
+       double* __restrict__ xout) {
     extern __shared__ double nn_lds[];
     const int n = pc.n, m = n + 1, lane = threadIdx.x, b = blockIdx.x;
     (void)Mall;
+    if (normv) {
+        // the prior's rows of this vector (its exterior annulus follows its edge)
+        const size_t nrow = n > 2 ? (size_t)(n - 2) : 0;
+        lcoef += (size_t)b * 3 * nrow;
+        lcol += (size_t)b * 3 * nrow;
+        near += (size_t)b * nrow;
+        pc.norm = normv[b];
+    }
     NnLds L;
     L.M = nn_lds;
     L.L = L.M + n * n;
@@ -559,9 +568,10 @@ k_nnls(ProfConst pc, const double* __restrict__ gram, const double* __restrict__
     for (int i = lane; i < n; i += 64) xout[(size_t)b * n + i] = ok ? L.x[i] : __builtin_nan("");
 }
 
-// dens (B, nr) = float32(x[:, :nr]); sr[b, 5, s] = sum_m wval[s][m] dens[b][widx[s][m]].
+// dens (B, nr) = float32(x[:, :nr]); sr[b, 5, s] = sum_m wval[s][m] dens[b][widx[s][m]] (times ew[b][s] if given).
 __global__ void k_set_dens(const double* __restrict__ x, int n, int nr, const int2* __restrict__ widx,
-                           const double2* __restrict__ wval, int nsub, double* __restrict__ sr, float* __restrict__ dens) {
+                           const double2* __restrict__ wval, const double* __restrict__ ew, int nsub,
+                           double* __restrict__ sr, float* __restrict__ dens) {
     const int b = blockIdx.y;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const double* xb = x + (size_t)b * n;
@@ -572,6 +582,7 @@ __global__ void k_set_dens(const double* __restrict__ x, int n, int nr, const in
     double d = 0.0;
     if (wi.x >= 0 && wi.x < nr) d += wv.x * (double)(float)xb[wi.x];
     if (wi.y >= 0 && wi.y < nr) d += wv.y * (double)(float)xb[wi.y];
+    if (ew) d *= ew[(size_t)b * nsub + s];      // (the vector's edge weight of the density)
     sr[((size_t)b * 16 + 5) * nsub + s] = d;
 }
 
@@ -620,8 +631,9 @@ extern "C" int bb_nnls_max() { return BB_NN_MAX; }
 
 extern "C" int bb_launch_profile_solve(int n, int nr, int nok, double smooth, double corr, double floor_, double norm,
                                        double dmed, const double* maps, const double* wpix, const double* ypix,
-                                       int npix, const double* lcoef, const int* lcol, const int* near, int B,
-                                       double* partial, double* gram, double* Mall, double* x, hipStream_t st) {
+                                       int npix, const double* lcoef, const int* lcol, const int* near,
+                                       const double* normv, int B, double* partial, double* gram, double* Mall,
+                                       double* x, hipStream_t st) {
     const int nblk = bb_gram_blocks(npix);
     const int ppb = 2048;
     const int m = n + 1;
@@ -647,14 +659,14 @@ extern "C" int bb_launch_profile_solve(int n, int nr, int nok, double smooth, do
         hipError_t e = hipFuncSetAttribute((const void*)k_nnls, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL(k_nnls, dim3(B), dim3(64), nl, st, pc, gram, lcoef, lcol, near, Mall, x);
+    hipLaunchKernelGGL(k_nnls, dim3(B), dim3(64), nl, st, pc, gram, lcoef, lcol, near, normv, Mall, x);
     return (int)hipGetLastError();
 }
 
-extern "C" int bb_launch_set_dens(const double* x, int B, int n, int nr, const int* widx, const double* wval, int nsub,
-                                  double* sr, float* dens, hipStream_t st) {
+extern "C" int bb_launch_set_dens(const double* x, int B, int n, int nr, const int* widx, const double* wval,
+                                  const double* ew, int nsub, double* sr, float* dens, hipStream_t st) {
     const int nt = nsub > nr ? nsub : nr;
     hipLaunchKernelGGL(k_set_dens, dim3((nt + 255) / 256, B), dim3(256), 0, st, x, n, nr, (const int2*)widx,
-                       (const double2*)wval, nsub, sr, dens);
+                       (const double2*)wval, ew, nsub, sr, dens);
     return (int)hipGetLastError();
 }

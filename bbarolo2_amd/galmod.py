@@ -599,11 +599,12 @@ class SubringPlan:
 
 # the ring keys of the likelihood context, in the order of its ring table (bbgpu.h BB_K_*)
 LIKE_KEYS = ("vrot", "vrad", "vvert", "vdisp", "dens", "z0", "dvdz", "zcyl", "inc", "phi", "xpos", "ypos", "vsys",
-             "fthick", "z0thick", "vdispthick", "sigma", "fracerr")
+             "fthick", "z0thick", "vdispthick", "sigma", "fracerr", "radmax", "radmaxa", "radmaxr")
 
 
 def ring_table(rings):
-    """(len(LIKE_KEYS), nr) float64: the ring values of the keys in the reference's units (sigma and fracerr: 0)."""
+    """(len(LIKE_KEYS), nr) float64: the ring values of the keys in the reference's units (the single values sigma,
+    fracerr and the edges: 0)."""
     out = np.zeros((len(LIKE_KEYS), rings.nr))
     for k, key in enumerate(LIKE_KEYS[:16]):
         out[k] = np.asarray(getattr(rings, key), float)

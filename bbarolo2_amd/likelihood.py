@@ -9,7 +9,8 @@ Mirrors LikelihoodMixin._log_likelihood (bbarolo/bayesian/likelihood.py:567-619)
 with S_box the masked residual sum inside the vector's model box, ss_out the squared data of the likelihood voxels of
 the region outside that box (a summed-area table, likelihood.py:437-440) and nvox the number of likelihood voxels of
 the region. theta -> rings follows _update_rings (bayesian/fitter.py:1109-1136): the free values are stored as
-float32 in the ring table (bayesian/rings.py:54-61), then used as float64.
+float32 in the ring table (bayesian/rings.py:54-61), then used as float64; the free edges of the disk (RADMAX,
+RADMAXA, RADMAXR) are used as float64 and, with the profile, move each vector's exterior annulus.
 
 This module does the per-fit set-up that the reference does in Python (the sub-ring plan of the fixed radii, the
 constants of the deconvolved profile, the model box) and hands it to the context once; every batch then runs in the
@@ -36,17 +37,22 @@ PROFILE_KEYS = ('inc', 'phi', 'xpos', 'ypos', 'radmax', 'z0', 'fthick')
 # ring parameters that make the likelihood zero when negative (likelihood.py:532-535)
 POSITIVE = ('vrot', 'vdisp', 'inc', 'xpos', 'ypos', 'z0', 'dens', 'zcyl', 'z0thick', 'fthick', 'vdispthick')
 
+# the most sub-rings of one layer of a vector's exterior annulus (bb_like.hip BB_ANN_CAP)
+ANN_CAP = 2048
+
 # free parameters that move the model box (DiskModel.box: the centre of the last ring and the layer height)
 BOX_KEYS = ('xpos', 'ypos', 'z0', 'z0thick')
 
-# free parameters that are not ring values
-GLOBAL_KEYS = ('sigma', 'fracerr')
+# free parameters that are not ring values: SIGMA, FRACERR and the edges of the disk (bayesian/fitter.py:1112-1120)
+GLOBAL_KEYS = ('sigma', 'fracerr', 'radmax', 'radmaxa', 'radmaxr')
+EDGE_KEYS = ('radmax', 'radmaxa', 'radmaxr')
 
 
 class FreePars:
     """The free parameters of a fit: an ordered list of (key, 'ring' | 'single'); 'ring' takes nr values of theta
     (one per ring), 'single' one value for all the rings (the FREE keys VROT and INC_SINGLE of the reference,
-    bayesian/fitter.py:434-473). SIGMA and FRACERR are single values that are not ring values."""
+    bayesian/fitter.py:434-473). SIGMA, FRACERR, RADMAX, RADMAXA and RADMAXR are single values that are not ring
+    values."""
 
     def __init__(self, spec, nr):
         self.spec = [(k.lower(), m) for k, m in spec]
@@ -77,7 +83,7 @@ class FreePars:
 
     def negative(self, theta):
         """True where the likelihood is zero (likelihood.py:537-559): a free positive-only value below 0, a free FTHICK
-        above 1, SIGMA not above 0."""
+        above 1, SIGMA not above 0, RADMAX below 0."""
         theta = np.atleast_2d(np.asarray(theta, float))
         bad = np.zeros(theta.shape[0], bool)
         for k, _ in self.spec:
@@ -88,6 +94,8 @@ class FreePars:
                 bad |= np.any(v.astype(np.float32) > 1, axis=1)
             if k == "sigma":
                 bad |= ~(v[:, 0] > 0)
+            if k == "radmax":
+                bad |= v[:, 0] < 0
         return bad
 
     def offsets(self):
@@ -141,11 +149,13 @@ class GpuLikelihood:
     dm: engine.DiskModel of the cube (ANALYTIC or MC). data (nz, ny, nx) float32. likemask (nz, ny, nx) bool (maskmode
     2: only those voxels; None: all). region (ny, nx) bool (None: the likelihood box). likebox ((blo), (bhi)): the
     likelihood region's box. sigma: noise per voxel. likefactor: 1 / correlation volume. fracerr: fixed fractional
-    model error (None: 0). chunk: the most vectors of one launch. halves: the two halves of a launch on two streams."""
+    model error (None: 0). chunk: the most vectors of one launch. halves: the two halves of a launch on two streams.
+    radmax, radmax_width: what a free RADMAX is (like_kwargs radmax, bayesian/fitter.py:556-569): 'edge' the outer edge
+    of the disk, sharp; 'taper' a Gaussian edge of width radmax_width arcsec (default a quarter of the beam FWHM)."""
 
     def __init__(self, dm, data, rings0, free, likemask=None, region=None, likebox=None, sigma=1.0,
                  likefactor=1.0, fracerr=None, maskmode=2, chunk=256, densprof_smooth=SMOOTH_DEFAULT,
-                 likefactor2d=None, beam_arcsec=None, graphs=False, halves=True):
+                 likefactor2d=None, beam_arcsec=None, graphs=False, halves=True, radmax="edge", radmax_width=None):
         self.torch = _torch()
         self.dm = dm
         self.free = free
@@ -154,6 +164,20 @@ class GpuLikelihood:
             if k not in LIKE_KEYS:
                 raise NotImplementedError(f"free {k.upper()} is not supported by this engine")
         r0 = rings0.copy()
+        self.edgewidth = 0.
+        self.ann_cap = None
+        if "radmax" in free.index:
+            mode = str(radmax or "edge")
+            if mode == "ring":
+                raise NotImplementedError("RADMAX as the radius of the last ring (like_kwargs radmax 'ring') moves the "
+                                          "radii of the sub-ring plan: use 'edge' or 'taper'")
+            if mode not in ("edge", "taper"):
+                raise ValueError(f"radmax {mode!r}: 'edge', 'taper' or 'ring'")
+            if mode == "taper":
+                bm = dm.header.Bmaj()
+                w = radmax_width if radmax_width is not None else 0.25 * abs(bm) * 3600.
+                self.edgewidth = float(w)
+            r0.edge, r0.edgewidth = None, 0.       # (the vector's edge replaces a fixed one)
         if "fthick" in free.index:
             r0.twolayer = True           # (a free FTHICK needs the thick layer, also where it starts at 0)
         self.rings0 = float32_rings(r0)
@@ -292,6 +316,10 @@ class GpuLikelihood:
                 else:
                     setattr(prof, name, a.ctypes.data)
             prof.qunit, prof.unit = P.qunit, P.unit
+            R = np.ascontiguousarray(P.radii[:-1], np.float64)
+            keep.append(R)
+            prof.radii = R.ctypes.data
+            prof.dmin = 0.25 * float(np.diff(R).min()) if R.size > 1 else 0.
         d = _lib.bb_like_desc()
         d.nr, d.ndim, d.chunk = nr, max(free.ndim, 1), int(chunk)
         d.mc = 1 if st.galmodtype == "MC" else 0
@@ -306,6 +334,21 @@ class GpuLikelihood:
         r = self.rings0
         ext = r.radii[-1] + r.radsep / 2.
         d.ext = max(ext, outer_edge(r, dm.ringmode))
+        # the edges of the disk (the free ones come from theta; the rest are fixed here and in the plans)
+        d.edgewidth = self.edgewidth
+        d.oedge = outer_edge(r, dm.ringmode)
+        d.edge0 = float(r.edge) if r.edge is not None else math.nan
+        ea, er = st.side_edges(r)
+        d.edgea0 = -1. if ea is None else float(ea)
+        d.edger0 = -1. if er is None else float(er)
+        d.empty = 1 if st.empty else 0
+        if with_profile and any(k in free.index for k in EDGE_KEYS):
+            d.annw = float(self.profile.ext.radsep)
+            span = d.annw if st.empty else d.oedge + d.annw
+            d.ann_cap = int(span / 60. / st.radsep) + 3
+            if d.ann_cap > ANN_CAP:
+                raise NotImplementedError(f"an exterior annulus of {d.ann_cap} sub-rings (at most {ANN_CAP})")
+            self.ann_cap = d.ann_cap
         hd = dm.header
         conv = arcsconv(hd.Cunit(0))
         d.pixas0, d.pixas1 = abs(hd.Cdelt(0)) * conv, abs(hd.Cdelt(1)) * conv
@@ -390,7 +433,8 @@ class GpuLikelihood:
         ext = None
         if self.update_prof:
             pe = self.profile.plan_ext
-            ext = np.empty((B, 16, pe.nsub * (2 if pe.thick else 1)))
+            ns = self.ann_cap or pe.nsub
+            ext = np.empty((B, 16, ns * (2 if pe.thick else 1)))
         _lib.check(_lib.lib().bb_ctx_tables(self._ctx, theta.ctypes.data, B, sr.ctypes.data,
                                             None if ext is None else ext.ctypes.data))
         return sr, ext

@@ -351,12 +351,15 @@ int bb_lag_z(const float* data, const uint8_t* w, const uint8_t* pix_use, const 
  * free (bayesian/density.py:13-88), the model in its own box (likelihood.py:429-435), the beam convolution fused with
  * the masked residual, the squared data of the region outside the box (likelihood.py:437-440), and
  *     log L = -likefactor [(S + S_out) / (2 sigma^2) + nvox ln(2 pi sigma^2) / 2]        (+ the fracerr terms).
- * Ring keys (the rows of the ring table, in the reference's units: km/s, arcsec, degrees; SIGMA and FRACERR are
- * single values of theta): */
-#define BB_NKEY 18
+ * Ring keys (the rows of the ring table, in the reference's units: km/s, arcsec, degrees). SIGMA, FRACERR and the
+ * edges are single values of theta, not ring values: RADMAX the outer edge of the disk (arcsec, used as float64 as
+ * _update_rings sets rings.edge, bayesian/fitter.py:1112-1116; zero likelihood when negative), RADMAXA / RADMAXR the
+ * edges of the approaching / receding halves (fitter.py:1117-1120; negative: no edge, galmod.py:853-863). */
+#define BB_NKEY 21
 enum {
     BB_K_VROT = 0, BB_K_VRAD, BB_K_VVERT, BB_K_VDISP, BB_K_DENS, BB_K_Z0, BB_K_DVDZ, BB_K_ZCYL, BB_K_INC, BB_K_PHI,
-    BB_K_XPOS, BB_K_YPOS, BB_K_VSYS, BB_K_FTHICK, BB_K_Z0THICK, BB_K_VDISPTHICK, BB_K_SIGMA, BB_K_FRACERR
+    BB_K_XPOS, BB_K_YPOS, BB_K_VSYS, BB_K_FTHICK, BB_K_Z0THICK, BB_K_VDISPTHICK, BB_K_SIGMA, BB_K_FRACERR,
+    BB_K_RADMAX, BB_K_RADMAXA, BB_K_RADMAXR
 };
 
 /* The sub-ring plan of a set of fixed radii (the part of GalmodSetup.subrings that does not depend on the ring values,
@@ -397,6 +400,15 @@ typedef struct bb_like_desc {
     double pixas0, pixas1;   /* |CDELT1|, |CDELT2| in arcsec */
     int32_t nx, ny;      /* the cube */
     int64_t mc_cap;      /* MC: first bound of the clouds of one vector in its box (grown when exceeded) */
+    /* The edges of the disk when RADMAX, RADMAXA or RADMAXR is free (the model's density weights, galmod.py:833-851,
+     * and the profile's exterior annulus per vector, profile.py:45-64); the fixed ones are in the plans. */
+    double edgewidth;    /* RADMAX: width of a Gaussian edge (arcsec; 0: a sharp edge, like_kwargs radmax 'edge') */
+    double oedge;        /* arcsec: the outer edge of the disk of the rings (outer_edge, with the ring convention) */
+    double edge0;        /* arcsec: the fixed edge of the disk when RADMAX is not free (NaN: none) */
+    double edgea0, edger0;   /* arcsec: the fixed edges of the approaching / receding halves (negative: none) */
+    double annw;         /* arcsec: the width of the exterior annulus */
+    int32_t ann_cap;     /* sub-rings of one layer of a vector's exterior annulus (capacity) */
+    int32_t empty;       /* EMPTY: the disk starts at its first radius */
 } bb_like_desc;
 
 /* The deconvolved profile of every vector (NULL: DENS fitted, or the profile of the initial rings in the ring table):
@@ -414,13 +426,16 @@ typedef struct bb_like_profile {
     const int32_t* widx_e;
     const double* wval_e;
     double qunit, unit;
+    const double* radii; /* (nr) arcsec: the radii of the rings (the prior's rows per vector when an edge is free) */
+    double dmin;         /* a quarter of the smallest spacing of the radii (profile.py:141-150; 0 with one ring) */
 } bb_like_profile;
 
 /* The context of a fit on the current device (Bayesian3DFIT's set-up, likelihood.py:83-440): m the model (its grid,
  * beam and vertical-sample capacity), d, data (nz, ny, nx) float32, likemask (nz, ny, nx) uint8 (NULL: all), region
  * (ny, nx) uint8 (NULL: all); rings (BB_NKEY, nr) the float32 ring table; free_off[k] the offset of key k in theta
  * (-1: not free), free_ring[k] 1 for one value per ring; plan the model's sub-ring plan, annulus that of the exterior
- * annulus (needed with prof). All HOST arrays, copied. A voxel enters the likelihood if it is in the likemask and the
+ * annulus (needed with prof; with an edge free only its values and ring weights are used: its radii follow each
+ * vector's edge). All HOST arrays, copied. A voxel enters the likelihood if it is in the likemask and the
  * region and its data are finite and non-zero. */
 typedef struct bb_ctx bb_ctx;
 int bb_ctx_create(const bb_model* m, const bb_like_desc* d, const float* data, const uint8_t* likemask,
@@ -436,7 +451,8 @@ int bb_loglike_collect(bb_ctx* c, double* loglike, float* dens);
 /* launch + collect over chunks of any B */
 int bb_loglike_batch(bb_ctx* c, const double* theta, int B, double* loglike, float* dens, void* stream);
 /* The sub-ring tables of B (<= chunk) vectors as the context builds them on the device: sr (B, 16, nsub_total) and
- * sr_ext (B, 16, nsub of the annulus; may be NULL), HOST arrays; synchronous. */
+ * sr_ext (B, 16, nsub of the annulus, or ann_cap per layer with an edge free: the sub-rings past a vector's last
+ * have zero side weights; may be NULL), HOST arrays; synchronous. */
 int bb_ctx_tables(bb_ctx* c, const double* theta, int B, double* sr, double* sr_ext);
 /* nvox and the squared data of the region, the deposit box [blo0, blo1, bhi0, bhi1), the vectors whose profile did not
  * converge and the MC launches repeated with larger buffers (any may be NULL). */

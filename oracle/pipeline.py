@@ -3,8 +3,9 @@ path). The FP64 log-likelihood of one parameter vector, restated from Bayesian3D
 (bbarolo/bayesian/likelihood.py:567-619) with the default Gaussian residuals:
 
   * theta -> rings: the free values, per ring or one for all the rings, stored as float32 in the ring table
-    (bayesian/fitter.py:1109-1136, bayesian/rings.py:54-76);
-  * zero likelihood where a free positive-only parameter is negative or a free FTHICK is above 1
+    (bayesian/fitter.py:1109-1136, bayesian/rings.py:54-76); a free RADMAX, RADMAXA or RADMAXR is the edge of the disk
+    or of a half, as a float (fitter.py:1112-1120);
+  * zero likelihood where a free positive-only parameter or RADMAX is negative or a free FTHICK is above 1
     (likelihood.py:526-559);
   * DENS not fitted: the deconvolved density profile of the vector (bayesian/density.py:13-88 through
     oracle.profile), the densities float32 in the ring table;
@@ -36,8 +37,8 @@ from bbarolo2_amd.smooth import beam_kernel
 
 # ring parameters that make the likelihood zero when negative (likelihood.py:526-535)
 POSITIVE = ('vrot', 'vdisp', 'inc', 'xpos', 'ypos', 'z0', 'dens', 'zcyl', 'z0thick', 'fthick', 'vdispthick')
-# free parameters that are not ring values
-GLOBAL_KEYS = ('sigma', 'fracerr')
+# free parameters that are not ring values (SIGMA, FRACERR, the edges of the disk: bayesian/fitter.py:1112-1120)
+GLOBAL_KEYS = ('sigma', 'fracerr', 'radmax', 'radmaxa', 'radmaxr')
 # free parameters that change the deconvolved profile: with none of them free it is that of the initial rings
 # (bayesian/fitter.py:606-615)
 PROFILE_KEYS = ('inc', 'phi', 'xpos', 'ypos', 'radmax', 'z0', 'fthick')
@@ -69,11 +70,12 @@ class Likelihood:
     h: header; par: GalmodPar; data (nz, ny, nx) float32; likemask (nz, ny, nx) bool or None (all); region
     (ny, nx) bool; likebox ((blo), (bhi)); sigma, likefactor; fracerr: fixed fractional error (0: none);
     smooth: densprof_smooth; likefactor2d: the profile's noise correlation (default likefactor); fft: beam
-    convolutions by FFT (else direct sums)."""
+    convolutions by FFT (else direct sums); radmax_width: the width of the edge of a free RADMAX (0: sharp, arcsec)."""
 
     def __init__(self, h, par, rings0, spec, data, likemask, region, likebox, sigma, likefactor, fracerr=0.,
-                 maskmode=2, smooth=0.3, likefactor2d=None, ringmode="CENTRED", fft=False):
+                 maskmode=2, smooth=0.3, likefactor2d=None, ringmode="CENTRED", fft=False, radmax_width=0.):
         self.h = h
+        self.radmax_width = float(radmax_width)
         self.st = GalmodSetup(h, par)
         self.seed = seed32(par.ISEED)
         self.ringmode = ringmode
@@ -127,6 +129,13 @@ class Likelihood:
                 continue
             v = np.asarray(v if m == "ring" else np.repeat(v, r.nr), np.float32).astype(np.float64)
             setattr(r, k, v)
+        # the edges, as floats (not float32): RADMAX the edge of the disk, RADMAXA / RADMAXR those of the halves
+        if "radmax" in g:
+            r.edge, r.edgewidth = g["radmax"], self.radmax_width
+        if "radmaxa" in g:
+            r.edgeapp = g["radmaxa"]
+        if "radmaxr" in g:
+            r.edgerec = g["radmaxr"]
         return r, g
 
     def zero(self, rings, g):
@@ -134,7 +143,7 @@ class Likelihood:
         for k, _ in self.spec:
             if k in POSITIVE and np.any(np.asarray(getattr(rings, k)) < 0):
                 return True
-            if k in GLOBAL_KEYS and g[k] < 0:
+            if k in ("sigma", "fracerr", "radmax") and g[k] < 0:
                 return True
         if any(k == "fthick" for k, _ in self.spec) and np.any(np.asarray(rings.fthick) > 1):
             return True
