@@ -151,8 +151,8 @@ def likelihood_mask(header, mask, likemask="dilated", dilation=(1., 3)):
 
 class LikelihoodSetup:
     """The set-up of one fit: likemask (uint8 device tensor), likebox, region, the noise (noise.setup_noise's dict:
-    noiserms, beam_area, noise_correlation, likefactor, likefactor2d, note), sigma (LIKEOPTS sigma, or the noise rms)
-    and fracerr."""
+    noiserms, beam_area, noise_correlation, likefactor, likefactor2d, note), sigma (LIKEOPTS sigma, or the noise rms),
+    fracerr, and what a free RADMAX is (LIKEOPTS radmax, radmax_width)."""
 
     def __init__(self, dm, data, mask, rings, free, lo, hi, options=None, side=None):
         from .noise import setup_noise
@@ -171,6 +171,7 @@ class LikelihoodSetup:
         self.sigma = float(opts["sigma"]) if opts.get("sigma") else self.noise["noiserms"]
         self.fracerr = opts.get("fracerr")
         self.densprof_smooth = opts["densprof_smooth"]
+        self.radmax, self.radmax_width = opts.get("radmax") or "edge", opts.get("radmax_width")
 
     def likelihood(self, dm, data, rings, free, **kw):
         """The GpuLikelihood of the fit (kw: chunk, graphs, halves)."""
@@ -178,7 +179,8 @@ class LikelihoodSetup:
         return GpuLikelihood(dm, data, rings, free, likemask=lm, region=self.region, likebox=self.likebox,
                              sigma=self.sigma, likefactor=self.noise["likefactor"],
                              likefactor2d=self.noise["likefactor2d"], fracerr=self.fracerr,
-                             densprof_smooth=self.densprof_smooth, **kw)
+                             densprof_smooth=self.densprof_smooth, radmax=self.radmax,
+                             radmax_width=self.radmax_width, **kw)
 
 
 def setup_likelihood(dm, data, mask, rings, free, lo, hi, options=None, side=None, **kw):

@@ -48,3 +48,36 @@ def test_setup_likelihood_vs_oracle():
                              rng.uniform(30, 33, (6, 1))])
     ll = _compare(like, orc, theta, "set-up")
     assert np.all(np.isfinite(ll))
+
+
+def test_reference_default_free_set_vs_oracle():
+    """The reference's default FREE of METHOD GLOBAL / BAYESIAN (keywords.py:700: VROT VDISP XPOS_SINGLE YPOS_SINGLE
+    VSYS_SINGLE INC_SINGLE PA_SINGLE Z0_SINGLE RADMAX FRACERR), read from a parameter file without FREE: the set-up
+    and the log-likelihoods against the oracle at prior draws (RADMAX from inside the disk to beyond it)."""
+    import torch
+    from test_gpu_ctx import _disk, _data, _compare
+    from bbarolo2_amd.engine import DiskModel
+    from bbarolo2_amd.fitsetup import setup_likelihood
+    from bbarolo2_amd.likelihood import FreePars
+    from bbarolo2_amd.params import ParFile
+    torch.cuda.set_device(0)
+    h, r = _disk()
+    par = GalmodPar(GALMODTYPE="ANALYTIC")
+    data, sig = _data(h, r, par)
+    mask = data > 4 * sig
+    spec = ParFile({"METHOD": "GLOBAL"}).free_spec(r.nr)
+    assert [k for k, _ in spec] == ["vrot", "vdisp", "xpos", "ypos", "vsys", "inc", "phi", "z0", "radmax", "fracerr"]
+    free = FreePars(spec, r.nr)
+    nr = r.nr
+    lo = np.r_[np.zeros(nr), np.zeros(nr), 29., 30., -20., 50., 35., 0., 100., 0.]
+    hi = np.r_[np.full(nr, 300.), np.full(nr, 50.), 34., 35., 30., 70., 55., 20., 200., 0.2]
+    like, s = setup_likelihood(DiskModel(h, par=par), data, mask, r, free, lo, hi, chunk=8)
+    assert like.update_prof and like.box_free and like.ann_cap
+    orc = pipeline.Likelihood(h, par, r, spec, data, s.likemask.cpu().numpy().astype(bool), s.region, s.likebox,
+                              s.sigma, s.noise["likefactor"], likefactor2d=s.noise["likefactor2d"])
+    rng = np.random.default_rng(8)
+    theta = lo + rng.random((5, len(lo))) * (hi - lo)
+    theta[:, :nr] = rng.uniform(120., 180., (5, nr))
+    theta[:, nr:2 * nr] = rng.uniform(5., 15., (5, nr))
+    ll = _compare(like, orc, theta, "default FREE")
+    assert np.all(np.isfinite(ll))
