@@ -71,8 +71,7 @@ _P = C.c_void_p
 _SIGS = {
     "bb_last_error": (C.c_char_p, []),
     "bb_device_count": (C.c_int, [C.POINTER(C.c_int)]),
-    "bb_model_create": (C.c_int, [C.POINTER(bb_grid), _P, _P, _P, C.c_int, _P, C.c_int, C.c_int, C.c_double,
-                                  C.POINTER(_P)]),
+    "bb_model_create": (C.c_int, [C.POINTER(bb_grid), _P, _P, _P, C.c_int, C.c_int, C.c_double, C.POINTER(_P)]),
     "bb_model_destroy": (C.c_int, [_P]),
     "bb_model_set_tile": (C.c_int, [_P, C.c_int, C.c_int]),
     "bb_model_get_tile": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),

@@ -16,14 +16,14 @@
 extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const double* sr, int nsub, int B,
                                          const int* naz, const int4* chunks, int nchunks, const double2* az,
                                          int TX, int TY, double* sub, double* wtab, short4* bbox, short4* sbox,
-                                         const double* cd2i, const double* vedges, const double* ptab, int ntab,
-                                         int lin_axis, float* out, int accumulate, int* bin, int bin_cap,
+                                         const double* cd2i, const double* vedges, int lin_axis, float* out,
+                                         int accumulate, int* bin, int bin_cap,
                                          const int4* vbox, int variant, hipStream_t st);
 extern "C" void bb_set_tile_events(void* ev_begin, void* ev_end);
 extern "C" size_t bb_analytic_sub_bytes();
 extern "C" size_t bb_analytic_wt_bytes();
 extern "C" int bb_analytic_chunk();
-extern "C" size_t bb_analytic_lds_bytes(int nz, int nlines, int TX, int TY, int ntab);
+extern "C" size_t bb_analytic_lds_bytes(int nz, int nlines, int TX, int TY);
 extern "C" size_t bb_analytic_lds_static();
 extern "C" int bb_launch_conv2d_same(const float* in, float* out, int B, int by, int bx, int nz, const float* kern,
                                      int ky, int kx, float scale, hipStream_t st);
@@ -120,7 +120,6 @@ extern "C" int bb_model_destroy(bb_model* m) {
     (void)hipSetDevice(m->device);
     (void)hipFree(m->d_cd2i);
     (void)hipFree(m->d_vedges);
-    (void)hipFree(m->d_ptab);
     (void)hipFree(m->d_kern);
     (void)hipFree(m->d_kern64);
     (void)hipSetDevice(dev);
@@ -128,26 +127,24 @@ extern "C" int bb_model_destroy(bb_model* m) {
     return 0;
 }
 
-extern "C" int bb_model_create(const bb_grid* g, const double* cd2i, const double* vedges, const double* ptab,
-                               int ntab, const double* kern, int ky, int kx, double scale, bb_model** out) {
+extern "C" int bb_model_create(const bb_grid* g, const double* cd2i, const double* vedges, const double* kern, int ky,
+                               int kx, double scale, bb_model** out) {
     if (!out) return fail(BB_ERR_ARG, "bb_model_create: out is NULL");
     *out = nullptr;
     int rc = check_grid(g);
     if (rc) return rc;
-    if (!cd2i || !vedges || !ptab || ntab < 2) return fail(BB_ERR_ARG, "bb_model_create: missing setup arrays");
+    if (!cd2i || !vedges) return fail(BB_ERR_ARG, "bb_model_create: missing setup arrays");
     if (kern && (ky < 1 || kx < 1 || ky % 2 == 0 || kx % 2 == 0 || ky > 511 || kx > bb_conv_max_kx()))
         return fail(BB_ERR_ARG, "bb_model_create: kernel %dx%d must be odd-sized with kx <= %d", ky, kx,
                     bb_conv_max_kx());
     bb_model* m = new bb_model();
     memset(m, 0, sizeof *m);
     m->g = *g;
-    m->ntab = ntab;
     m->scale = scale;
     m->bin_per = BB_BIN_PER_CHUNK;
     (void)hipGetDevice(&m->device);
     rc = upload(&m->d_cd2i, cd2i, (size_t)g->nz, "upload cd2i");
     if (!rc) rc = upload(&m->d_vedges, vedges, (size_t)g->nlines * (g->nz + 1), "upload vedges");
-    if (!rc) rc = upload(&m->d_ptab, ptab, (size_t)ntab, "upload Phi table");
     if (!rc && g->axtyp == 4) {
         bool lin = true;
         for (int k = 1; k < g->nz && lin; k++) lin = cd2i[k] == cd2i[0] && (float)cd2i[k] == (float)cd2i[0];
@@ -168,14 +165,22 @@ extern "C" int bb_model_create(const bb_grid* g, const double* cd2i, const doubl
         if (!rc) rc = upload(&m->d_kern64, kern, kf.size(), "upload kernel");
     }
     if (!rc) {
-        // the largest pixel tile whose spectra (float32), with the channel and Phi tables, fit the LDS of a CU
-        static const int cand[][2] = {{16, 16}, {16, 8}, {8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
+        // the pixel tile: the most pixels (at most 256, sides at most 3:2) whose spectra, with the channel tables,
+        // fit the LDS of a CU, the squarest of those, taller on a tie (fewer sample columns cross a square tile per
+        // pixel: 12x13 instead of 16x8 for 96 channels, 53.3 against 57.3 ms for the 256-vector batch of the
+        // 160x160x96 cube)
         m->TX = 0;
-        for (auto& c : cand)
-            if (bb_analytic_lds_bytes(g->nz, g->nlines, c[0], c[1], ntab) + bb_analytic_lds_static() <= BB_LDS_MAX) {
-                m->TX = c[0];
-                m->TY = c[1];
-                break;
+        int best = 0, bd = 1 << 30;
+        for (int ty = 1; ty <= 256; ty++)
+            for (int tx = (2 * ty + 2) / 3; 2 * tx <= 3 * ty && tx * ty <= 256; tx++) {
+                if (bb_analytic_lds_bytes(g->nz, g->nlines, tx, ty) + bb_analytic_lds_static() > BB_LDS_MAX) break;
+                const int a = tx * ty, d = tx > ty ? tx - ty : ty - tx;
+                if (a > best || (a == best && (d < bd || (d == bd && ty > tx)))) {
+                    best = a;
+                    bd = d;
+                    m->TX = tx;
+                    m->TY = ty;
+                }
             }
         if (!m->TX) rc = fail(BB_ERR_ARG, "bb_model_create: nz = %d channels do not fit the LDS tile", g->nz);
     }
@@ -189,7 +194,7 @@ extern "C" int bb_model_create(const bb_grid* g, const double* cd2i, const doubl
 
 extern "C" int bb_model_set_tile(bb_model* m, int TX, int TY) {
     if (!m || TX < 1 || TY < 1 || TX > 64 || TY > 64) return fail(BB_ERR_ARG, "bb_model_set_tile: bad tile %dx%d", TX, TY);
-    if (bb_analytic_lds_bytes(m->g.nz, m->g.nlines, TX, TY, m->ntab) + bb_analytic_lds_static() > BB_LDS_MAX)
+    if (bb_analytic_lds_bytes(m->g.nz, m->g.nlines, TX, TY) + bb_analytic_lds_static() > BB_LDS_MAX)
         return fail(BB_ERR_ARG, "bb_model_set_tile: a %dx%d tile of %d channels exceeds the LDS", TX, TY, m->g.nz);
     m->TX = TX;
     m->TY = TY;
@@ -346,7 +351,7 @@ int bb_galmod_analytic_vb(const bb_model* m, const bb_azplan* p, bb_box box, con
     w += (size_t)B * p->nchunks * sizeof(short4);
     short4* sbox = (short4*)w;
     rc = bb_launch_galmod_analytic(&m->g, box, sr, nsub, B, p->d_naz, p->d_chunks, p->nchunks, p->d_az, m->TX, m->TY,
-                                   sub, wtab, bbox, sbox, m->d_cd2i, m->d_vedges, m->d_ptab, m->ntab, m->lin, out,
+                                   sub, wtab, bbox, sbox, m->d_cd2i, m->d_vedges, m->lin, out,
                                    accumulate, work_bin(m, p, B, sbox, ntiles), (int)(p->nchunks * (int64_t)m->bin_per), vbox,
                                    m->dep_variant, (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_galmod_analytic launch: %s", hipGetErrorString((hipError_t)rc)) : 0;

@@ -455,50 +455,36 @@ struct Acc {
     }
 };
 
-// Phi(t) from the float32 copy of the table in LDS (same interpolation as galmod.py:183-191).
-__device__ __forceinline__ float lds_phi(double t, const float* ph, int ntab) {
-    const double x = (t + 6.0) * 512.0;
-    if (!(x > 0.0)) return 0.f;
-    if (!(x < (double)(ntab - 1))) return 1.f;
-    const int k = (int)x;
-    const float fr = (float)(x - (double)k);
-    const float a = ph[k];
-    return fmaf(fr, ph[k + 1] - a, a);
-}
-
-// Phi at table coordinate x = 512 (t + 6) from the LDS copy of the table whose first entry is 0 and last 1: x is
-// clamped to the table, so the ends need no test (x <= 0: entry 0; x >= ntab - 1: the last entry).
-// The copy has one more entry, ph[ntab] = 1, so that x = ntab - 1 reads it with a zero weight (the value there, 1, is
-// that of the last interval's end, as ph[ntab - 2] and ph[ntab - 1] are both 1 in float32).
-__device__ __forceinline__ float lds_phi_x(float x, const float* ph, int ntab) {
-    const float xc = __builtin_amdgcn_fmed3f(x, 0.0f, (float)(ntab - 1));     // (x clamped to [0, ntab - 1])
-    const int k = (int)xc;
-    const float fr = __builtin_amdgcn_fractf(xc);       // (xc - k, exactly)
-    const float a = ph[k];
-    return fmaf(fr, ph[k + 1] - a, a);
-}
-
-// The same with the argument in float32: t is a difference of velocities over sigma (the subtraction is FP64),
-// and x = 512 (t + 6) keeps 2.4e-4 of a table step, far below the table's own interpolation error.
-__device__ __forceinline__ float lds_phi_f(float t, const float* ph, int ntab) {
-    return lds_phi_x(fmaf(t, 512.0f, 3072.0f), ph, ntab);
+// Phi(t) = erfc(-t / sqrt 2) / 2, the cumulative normal of the line profile (galmod.py:183-214 tabulates it at 1/512
+// on [-6, 6] and interpolates linearly, an error of up to 1.2e-7), here in registers by Abramowitz & Stegun 7.1.26:
+// erfc(x) = tau (a1 + tau (a2 + tau (a3 + tau (a4 + tau a5)))) exp(-x^2), tau = 1 / (1 + p x), x >= 0, |error| <=
+// 1.5e-7. One reciprocal, one exponential and five fmas instead of two LDS reads at a random address: the deposit's
+// LDS serves its atomics only (61.2 -> 56.6 ms for the 256-vector batch of the 160x160x96 cube).
+__device__ __forceinline__ float phi_as(float t) {
+    const float x = fabsf(t) * 0.70710678f;
+    const float tau = __builtin_amdgcn_rcpf(fmaf(0.3275911f, x, 1.0f));
+    float p = fmaf(tau, 1.061405429f, -1.453152027f);
+    p = fmaf(tau, p, 1.421413741f);
+    p = fmaf(tau, p, -0.284496736f);
+    p = fmaf(tau, p, 0.254829592f);
+    const float q = 0.5f * tau * p * __expf(-x * x);
+    return t < 0.0f ? q : 1.0f - q;
 }
 
 // aw x the channel-integrated line of centre v (galmod.py:194-214) into pixel slot of the tile: the rare
 // paths (a height-dependent velocity, more runs of a sample in the tile than BB_RMAX). Scalar arguments only:
 // an argument passed by reference to a call that is not inlined would have to live in scratch memory.
 __device__ __noinline__ void acc_line(unsigned long long* tile, float inv_q, int nzp, int slot, double aw, double v,
-                                      double sig, const double* ve, const double* cd, const float* sph, int ntab,
-                                      int64_t kc, int64_t K, int nz) {
+                                      double sig, const double* ve, const double* cd, int64_t kc, int64_t K, int nz) {
     int64_t k0 = kc - K;
     if (k0 < 0) k0 = 0;
     int64_t k1 = kc + K + 1;
     if (k1 > nz) k1 = nz;
     if (k1 <= k0) return;
     const double inv = 1.0 / sig;
-    float pp = lds_phi((ve[k0] - v) * inv, sph, ntab);
+    float pp = phi_as((float)((ve[k0] - v) * inv));
     for (int k = (int)k0; k < (int)k1; k++) {
-        const float pn = lds_phi((ve[k + 1] - v) * inv, sph, ntab);
+        const float pn = phi_as((float)((ve[k + 1] - v) * inv));
         const float f = pn - pp;
         pp = pn;
         const unsigned int uq = (unsigned int)fmaf((float)aw * (fabsf(f) * (float)cd[k]), inv_q, 0.5f);
@@ -544,7 +530,7 @@ struct TileGeo {
 
 // A height-dependent velocity (galmod.py:432-465): every vertical pair has its own line.
 __device__ __forceinline__ void an_sample_lag(const bb_grid& g, const bb_box& box, const TileGeo& T, const Acc& A,
-                                           const double* sve, const double* scd, const float* sph, int ntab,
+                                           const double* sve, const double* scd,
                                            const Sub& u, const double* __restrict__ wt, double caz,
                                            double saz) {
     const int nz = g.nz;
@@ -575,11 +561,11 @@ __device__ __forceinline__ void an_sample_lag(const bb_grid& g, const bb_box& bo
             const double a = g.relint[nl] * afl;
             const int64_t kc = bb_lround(bb_velgrid(v, g.axtyp, g.velsys, g.lfreq0[nl], g.crval3, g.cdelt3, g.crpix3));
             if (in1 && in2 && ga0 == gb0 && ga1 == gb1) {
-                acc_line(A.tile, A.inv_q, A.nzp, s1, a * (w1 + w2), v, u.sig, ve, scd, sph, ntab, kc, u.K, nz);
+                acc_line(A.tile, A.inv_q, A.nzp, s1, a * (w1 + w2), v, u.sig, ve, scd, kc, u.K, nz);
                 continue;
             }
-            if (s1 >= 0) acc_line(A.tile, A.inv_q, A.nzp, s1, a * w1, v, u.sig, ve, scd, sph, ntab, kc, u.K, nz);
-            if (s2 >= 0) acc_line(A.tile, A.inv_q, A.nzp, s2, a * w2, v, u.sig, ve, scd, sph, ntab, kc, u.K, nz);
+            if (s1 >= 0) acc_line(A.tile, A.inv_q, A.nzp, s1, a * w1, v, u.sig, ve, scd, kc, u.K, nz);
+            if (s2 >= 0) acc_line(A.tile, A.inv_q, A.nzp, s2, a * w2, v, u.sig, ve, scd, kc, u.K, nz);
         }
     }
 }
@@ -616,7 +602,7 @@ __device__ __forceinline__ bool tile_zrange(const Sub& u, const Track& tr, const
 // same operations, the same values), and the channel factor is a constant, instead of being read from LDS.
 template <bool LIN>
 __device__ __forceinline__ void an_lines(const bb_grid& g, const Acc& A, const double* sve, const float* scf,
-                                         const float* sph, int ntab, const Sub& u, double vplane, double afl,
+                                         const Sub& u, double vplane, double afl,
                                          const int* rs, const float* rw, int nr) {
     const int nz = g.nz;
     for (int nl = 0; nl < g.nlines; nl++) {
@@ -636,10 +622,10 @@ __device__ __forceinline__ void an_lines(const bb_grid& g, const Acc& A, const d
         // the FP64 edge as above; the fma's error, ~1e-6 in t over a window of 64 channels, is that of the FP32
         // cast of the FP64 difference); the edge index is clamped to the window, so the channels past it take
         // two equal values and add 0
-        const float x0 = k1 > k0 ? fmaf((float)(edge((int)k0) - vplane) * inv, 512.0f, 3072.0f) : 0.f;
-        const float dx = (float)(g.cdelt3 / u.sig * 512.0);
+        const float x0 = k1 > k0 ? (float)(edge((int)k0) - vplane) * inv : 0.f;
+        const float dx = (float)(g.cdelt3 / u.sig);
         const int ne = (int)(k1 - k0);
-        float pp = k1 > k0 ? lds_phi_x(x0, sph, ntab) : 0.f;
+        float pp = k1 > k0 ? phi_as(x0) : 0.f;
         // blocks of BB_CB channels: the block's profile values first (independent table reads, one wait), then
         // every run adds the block (one address per run, the channels as immediate offsets)
         for (int kb = (int)k0; kb < (int)k1; kb += BB_CB) {
@@ -649,10 +635,10 @@ __device__ __forceinline__ void an_lines(const bb_grid& g, const Acc& A, const d
             for (int j = 0; j < BB_CB; j++) {            // (independent lookups: the loads issue together)
                 if (LIN) {
                     const int e = min(kb - (int)k0 + j + 1, ne);
-                    pe[j] = lds_phi_x(fmaf((float)e, dx, x0), sph, ntab);
+                    pe[j] = phi_as(fmaf((float)e, dx, x0));
                 } else {
                     const int k = kb + j < nz ? kb + j : nz - 1;      // (past k1: computed, not added)
-                    pe[j] = lds_phi_f((float)(edge(k + 1) - vplane) * inv, sph, ntab);
+                    pe[j] = phi_as((float)(edge(k + 1) - vplane) * inv);
                 }
             }
 #pragma unroll
@@ -687,12 +673,11 @@ __device__ __forceinline__ void an_lines(const bb_grid& g, const Acc& A, const d
 // Deposit the runs of one azimuth sample that fall in the tile (galmod.py:389-431 for one sample).
 template <bool LIN>
 __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, const TileGeo& T, const Acc& A,
-                                          const double* sve, const double* scd, const float* scf, const float* sph,
-                                          int ntab,
+                                          const double* sve, const double* scd, const float* scf,
                                           const Sub& u, const double* __restrict__ wt, double caz,
                                           double saz) {
     if (u.zdep) {
-        an_sample_lag(g, box, T, A, sve, scd, sph, ntab, u, wt, caz, saz);
+        an_sample_lag(g, box, T, A, sve, scd, u, wt, caz, saz);
         return;
     }
     const int nz = g.nz;
@@ -742,7 +727,7 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
                 }
             }
         }
-        if (nr > 0) an_lines<LIN>(g, A, sve, scf, sph, ntab, u, vplane, afl, rs, rw, nr);
+        if (nr > 0) an_lines<LIN>(g, A, sve, scf, u, vplane, afl, rs, rw, nr);
         return;
     }
     // a long column: the runs in the tile are compacted into rs[0..nr); beyond BB_RMAX they are deposited one by one
@@ -778,7 +763,7 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
                     int64_t kc = bb_lround(bb_velgrid(vplane, g.axtyp, g.velsys, g.lfreq0[nl], g.crval3, g.cdelt3,
                                                       g.crpix3));
                     acc_line(A.tile, A.inv_q, A.nzp, sl, g.relint[nl] * afl * ws, vplane, u.sig,
-                             sve + (size_t)nl * (nz + 1), scd, sph, ntab, kc, u.K, nz);
+                             sve + (size_t)nl * (nz + 1), scd, kc, u.K, nz);
                 }
             }
         }
@@ -788,7 +773,7 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
             ws = uw_(u, wt, kz, g.ltype);
         }
     }
-    if (nr > 0) an_lines<LIN>(g, A, sve, scf, sph, ntab, u, vplane, afl, rs, rw, nr);
+    if (nr > 0) an_lines<LIN>(g, A, sve, scf, u, vplane, afl, rs, rw, nr);
 }
 
 // The samples of a batch vector that meet the tile [tx0, tx1] x [ty0, ty1] (box-relative): the chunks whose
@@ -919,8 +904,8 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 // is computed in Phase B from the sample's velocity.
 template <bool LIN, bool NL1>
 __device__ __forceinline__ void an_wave(const bb_grid& g, const bb_box& box, const TileGeo& T, const Acc& A,
-                                        const double* sve, const double* scd, const float* scf, const float* sph,
-                                        int ntab, const int4* __restrict__ chunks, const double2* __restrict__ az,
+                                        const double* sve, const double* scd, const float* scf,
+                                        const int4* __restrict__ chunks, const double2* __restrict__ az,
                                         const double* __restrict__ sub, const double* __restrict__ wtab, uint2* rbuf,
                                         int e) {
     const int lane = (int)(threadIdx.x & 63);
@@ -943,7 +928,7 @@ __device__ __forceinline__ void an_wave(const bb_grid& g, const bb_box& box, con
         const double2 cs = az[ch.w + (e & 31)];
         wt = wtab + (size_t)s * BB_ZCAP;
         if (u.zdep) {
-            an_sample_lag(g, box, T, A, sve, scd, sph, ntab, u, wt, cs.x, cs.y);
+            an_sample_lag(g, box, T, A, sve, scd, u, wt, cs.x, cs.y);
             ok = false;
         } else {
             // (the operations of an_sample)
@@ -960,7 +945,7 @@ __device__ __forceinline__ void an_wave(const bb_grid& g, const bb_box& box, con
     // the window of line 0 (an_lines): first channel k0, ne channels, the table coordinate x0 of its first edge
     int hk0 = 0, hne = 0;
     float hx0 = 0.f, ha = 0.f;
-    const float hinv = (float)(1.0 / u.sig), hdx = (float)(g.cdelt3 / u.sig * 512.0);
+    const float hinv = (float)(1.0 / u.sig), hdx = (float)(g.cdelt3 / u.sig);
     if (NL1 && ok) {
         const int64_t kc = bb_lround(bb_velgrid(vplane, g.axtyp, g.velsys, g.lfreq0[0], g.crval3, g.cdelt3, g.crpix3));
         int64_t k0 = kc - u.K;
@@ -973,7 +958,7 @@ __device__ __forceinline__ void an_wave(const bb_grid& g, const bb_box& box, con
         }
         ha = (float)(g.relint[0] * afl);
         if (LIN && hne > 0)
-            hx0 = fmaf((float)((((double)hk0 + 0.5 - g.crpix3) * g.cdelt3 + g.crval3) - vplane) * hinv, 512.0f, 3072.0f);
+            hx0 = (float)((((double)hk0 + 0.5 - g.crpix3) * g.cdelt3 + g.crval3) - vplane) * hinv;
         if (hne == 0 || ha == 0.f) ok = false;        // (the lane-per-sample deposit adds zeros)
     }
     // the closing positions of the sample's runs: the whole column, or the part of a long one that can meet the tile
@@ -1075,8 +1060,7 @@ __device__ __forceinline__ void an_wave(const bb_grid& g, const bb_box& box, con
                         ne = (int)(q1 - q0);
                         a = (float)(g.relint[nl] * aflL);
                         if (LIN)
-                            x0 = fmaf((float)((((double)k0 + 0.5 - g.crpix3) * g.cdelt3 + g.crval3) - vpl) * inv, 512.0f,
-                                      3072.0f);
+                            x0 = (float)((((double)k0 + 0.5 - g.crpix3) * g.cdelt3 + g.crval3) - vpl) * inv;
                     }
                     float carry = 0.f;
                     for (int E0 = 0; E0 <= ne; E0 += 64) {
@@ -1085,8 +1069,7 @@ __device__ __forceinline__ void an_wave(const bb_grid& g, const bb_box& box, con
                         // the first block takes its own value as the edge below (channel -1 adds 0): no lane mask,
                         // the zeros go to distinct words of the padded tile.
                         const float Ef = (float)min(E0 + lane, ne);
-                        const float q = LIN ? lds_phi_x(fmaf(Ef, dx, x0), sph, ntab)
-                                            : lds_phi_f((float)(ve[k0 + (int)Ef] - vpl) * inv, sph, ntab);
+                        const float q = LIN ? phi_as(fmaf(Ef, dx, x0)) : phi_as((float)(ve[k0 + (int)Ef] - vpl) * inv);
                         if (E0 == 0) carry = readlane_f(q, 0);
                         const float qlo = from_lane_below(q, carry);
                         carry = readlane_f(q, 63);
@@ -1119,10 +1102,10 @@ __global__ void __launch_bounds__(BB_TILE_THREADS)
 k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks, int nchunks,
           const double2* __restrict__ az, const double* __restrict__ suball, const double* __restrict__ wtaball,
           int nsub, const short4* __restrict__ bball, const short4* __restrict__ sball,
-          const double* __restrict__ cd2i, const double* __restrict__ vedges, const double* __restrict__ ptab,
-          int ntab, float* __restrict__ outall, int accumulate, const int* __restrict__ bin, int bin_cap,
+          const double* __restrict__ cd2i, const double* __restrict__ vedges, float* __restrict__ outall,
+          int accumulate, const int* __restrict__ bin, int bin_cap,
           const int4* __restrict__ vbox) {
-    // dynamic LDS: [vedges (nlines, nz+1) f64][cd2i (nz) f64][cd2i (nz) f32][Phi (ntab + 1) f32][pad, tile u64, pad]
+    // dynamic LDS: [vedges (nlines, nz+1) f64][cd2i (nz) f64][cd2i (nz) f32][pad, tile u64, pad]
     extern __shared__ double dyn[];
     // the wavefront deposit holds a smaller sample list and, in the bytes this frees, each wavefront's run buffer
     constexpr int SCAP = MODE == 0 ? BB_AN_SAMP_CAP : BB_AN_WSAMP_CAP;
@@ -1143,9 +1126,8 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     double* sve = dyn;
     double* scd = dyn + (size_t)nl_ * (nz + 1);
     float* scf = (float*)(scd + nz);
-    float* sph = scf + ((nz + 1) & ~1);
     // (one word of padding before the tile and BB_TILE_PAD after it)
-    unsigned long long* tile = (unsigned long long*)(sph + ((ntab + 2) & ~1)) + 1;
+    unsigned long long* tile = (unsigned long long*)(scf + ((nz + 1) & ~1)) + 1;
     const int ntile = TX * TY * nzp;
     const double* sub = suball + (size_t)b * nsub * BB_NSF;
     const double* wtab = wtaball + (size_t)b * nsub * BB_ZCAP;
@@ -1155,7 +1137,6 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
         scd[i] = cd2i[i];
         scf[i] = (float)cd2i[i];
     }
-    for (int i = threadIdx.x; i <= ntab; i += blockDim.x) sph[i] = i == 0 ? 0.0f : (i >= ntab - 1 ? 1.0f : (float)ptab[i]);
     // the fixed-point scale of this vector: amax bounds any single contribution
     double am = 0.;
     for (int s = threadIdx.x; s < nsub; s += blockDim.x) {
@@ -1213,9 +1194,9 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
             Sub u;
             load_sub(u, sub + (size_t)s * BB_NSF, ch.x & 1);
             const double2 cs = az[ch.w + (e & 31)];
-            an_sample<LIN>(g, box, T, A, sve, scd, scf, sph, ntab, u, wtab + (size_t)s * BB_ZCAP, cs.x, cs.y);
+            an_sample<LIN>(g, box, T, A, sve, scd, scf, u, wtab + (size_t)s * BB_ZCAP, cs.x, cs.y);
         } else {
-            an_wave<LIN, MODE == 1>(g, box, T, A, sve, scd, scf, sph, ntab, chunks, az, sub, wtab,
+            an_wave<LIN, MODE == 1>(g, box, T, A, sve, scd, scf, chunks, az, sub, wtab,
                                     rbuf + (threadIdx.x >> 6) * BB_WRUNS, e);
         }
     });
@@ -1232,10 +1213,9 @@ k_an_tile(bb_grid g, bb_box box, int TX, int TY, const int4* __restrict__ chunks
     }
 }
 
-// LDS bytes of the dynamic part of k_an_tile: channel edges and factors (f64), Phi table (f32), the tile (u64) with
-// its padding.
-extern "C" size_t bb_analytic_lds_bytes(int nz, int nlines, int TX, int TY, int ntab) {
-    return ((size_t)nlines * (nz + 1) + nz) * sizeof(double) + (size_t)(((nz + 1) & ~1) + ((ntab + 2) & ~1)) * sizeof(float) +
+// LDS bytes of the dynamic part of k_an_tile: channel edges and factors (f64 and f32), the tile (u64) with its padding.
+extern "C" size_t bb_analytic_lds_bytes(int nz, int nlines, int TX, int TY) {
+    return ((size_t)nlines * (nz + 1) + nz) * sizeof(double) + (size_t)((nz + 1) & ~1) * sizeof(float) +
            ((size_t)TX * TY * bb_tile_stride(nz) + 1 + BB_TILE_PAD) * sizeof(unsigned long long);
 }
 
@@ -1255,8 +1235,8 @@ extern "C" void bb_set_tile_events(void* ev_begin, void* ev_end) {
 extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const double* sr, int nsub, int B,
                                          const int* naz, const int4* chunks, int nchunks, const double2* az,
                                          int TX, int TY, double* sub, double* wtab, short4* bbox, short4* sbox,
-                                         const double* cd2i, const double* vedges, const double* ptab, int ntab,
-                                         int lin_axis, float* out, int accumulate, int* bin, int bin_cap,
+                                         const double* cd2i, const double* vedges, int lin_axis, float* out,
+                                         int accumulate, int* bin, int bin_cap,
                                          const int4* vbox, int variant, hipStream_t st) {
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
     hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, naz, sub, wtab, 0,
@@ -1268,12 +1248,12 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
         const int e = bin_chunks(box, B, bbox, nchunks, TX, TY, bin, bin_cap, st);
         if (e) return e;
     }
-    const size_t lds = bb_analytic_lds_bytes(g->nz, g->nlines, TX, TY, ntab);
+    const size_t lds = bb_analytic_lds_bytes(g->nz, g->nlines, TX, TY);
     // a linear velocity axis with one channel factor: the channel edges and factors computed, not read. variant 0:
     // one lane per sample, 1: the wavefront deposit (one line, or several)
     using Kern = void (*)(bb_grid, bb_box, int, int, const int4*, int, const double2*, const double*, const double*,
-                          int, const short4*, const short4*, const double*, const double*, const double*, int, float*,
-                          int, const int*, int, const int4*);
+                          int, const short4*, const short4*, const double*, const double*, float*, int, const int*, int,
+                          const int4*);
     static const Kern kern[2][3] = {{k_an_tile<false, 0>, k_an_tile<false, 1>, k_an_tile<false, 2>},
                                     {k_an_tile<true, 0>, k_an_tile<true, 1>, k_an_tile<true, 2>}};
     static bool attr = false;
@@ -1292,8 +1272,8 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
     }
     const int mode = variant == 0 ? 0 : (g->nlines == 1 ? 1 : 2);
     hipLaunchKernelGGL(kern[lin_axis != 0][mode], dim3(ntiles, B), dim3(BB_TILE_THREADS), lds, st, *g, box, TX, TY,
-                       chunks, nchunks, az, sub, wtab, nsub, bbox, sbox, cd2i, vedges, ptab, ntab, out, accumulate, bin,
-                       bin_cap, vbox);
+                       chunks, nchunks, az, sub, wtab, nsub, bbox, sbox, cd2i, vedges, out, accumulate, bin, bin_cap,
+                       vbox);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && t_tile_ev[0] && t_tile_ev[1]) e = hipEventRecord(t_tile_ev[1], st);
     return (int)e;

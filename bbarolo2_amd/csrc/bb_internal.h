@@ -30,10 +30,8 @@ static inline int bb_upload(T** dst, const T* src, size_t n, const char* what) {
 struct bb_model {
     bb_grid g;
     int device;
-    int ntab;
     double* d_cd2i;
     double* d_vedges;
-    double* d_ptab;
     int ky, kx;
     float* d_kern;
     double* d_kern64;

@@ -18,7 +18,7 @@ import numpy as np
 
 from . import _lib
 from .galmod import (GalmodSetup, GalmodPar, model_rings, model_box, ringmode_name, zstep_for, nzq_for,
-                     PHI_TAB, ZSAMPLES_MAX, seed32)
+                     ZSAMPLES_MAX, seed32)
 from .smooth import beam_kernel
 
 AN_THREADS = 256
@@ -113,15 +113,14 @@ class DiskModel:
             g = self.grid(nzq)
             cd = np.ascontiguousarray(st.cd2i, np.float64)
             ve = np.ascontiguousarray(st.vedges_lines, np.float64)
-            pt = np.ascontiguousarray(PHI_TAB, np.float64)
             if self.cfield is not None:
                 k = np.ascontiguousarray(self.cfield, np.float64)
                 kp, ky, kx = k.ctypes.data, k.shape[0], k.shape[1]
             else:
                 k, kp, ky, kx = None, None, 0, 0
             out = C.c_void_p()
-            _lib.check(L.bb_model_create(C.byref(g), cd.ctypes.data, ve.ctypes.data, pt.ctypes.data, len(pt), kp,
-                                         ky, kx, float(self.scalefactor), C.byref(out)))
+            _lib.check(L.bb_model_create(C.byref(g), cd.ctypes.data, ve.ctypes.data, kp, ky, kx,
+                                         float(self.scalefactor), C.byref(out)))
             h = out
             if self._bin is not None:
                 _lib.check(L.bb_model_set_bin(h, int(self._bin)))

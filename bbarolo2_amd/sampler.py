@@ -390,6 +390,15 @@ class Sampler:
         return float(np.logaddexp(lz_dead, lz_live))
 
     @property
+    def information(self):
+        """H = the posterior mean of ln(L / Z), in nats: the log Z of a run scatters by about sqrt(H / n_live)
+        (Skilling 2006)."""
+        _, logw, logl = self.posterior()
+        w = np.exp(logw)
+        ok = w > 0
+        return float(np.sum(w[ok] * logl[ok]) - self.log_z)
+
+    @property
     def n_eff(self):
         _, logw, _ = self.posterior()
         w = np.exp(logw)

@@ -3,9 +3,9 @@
 The deposit (bb_galmod_analytic) gives each workgroup a rectangle of TX x TY pixels whose spectra it accumulates
 in LDS. The shape trades the LDS per workgroup (TX TY (nz | 1) 8 bytes: at most one workgroup per CU when it is
 large) against the samples that two tiles both gather (smaller tiles: more of them), and the best one depends on
-the cube (channels, box, ring count, thickness) and on the batch size. bb_model_create takes the largest square
-or 2:1 tile that fits; tune_tiles times the likelihood batch of a fit at every shape the deposit accepts and
-keeps the fastest. The result of the deposit does not depend on the shape (its sums are exact integers in any
+the cube (channels, box, ring count, thickness) and on the batch size. bb_model_create takes the tile of the most
+pixels (at most 256, sides at most 3:2) that fits, the squarest of those; tune_tiles times the likelihood batch of a
+fit at every shape the deposit accepts and keeps the fastest. The result of the deposit does not depend on the shape (its sums are exact integers in any
 order), only its time does. There is no counterpart in the reference (its GALMOD runs per cloud on the CPU).
 """
 

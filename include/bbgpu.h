@@ -50,12 +50,13 @@ int bb_device_count(int* n);
 
 /* The per-fit model object: the header-derived setup of GalmodSetup (bbarolo/galmod.py:609-758) and the beam
  * kernel of DiskModel (galmod.py:1199-1214, FFTConvolver(kernel) of smooth.py:107-143), copied to the current
- * device. cd2i (nz), vedges (nlines, nz+1), ptab (ntab) and kern (ky, kx) are HOST float64 arrays; kern may be
- * NULL (no smoothing); its sizes are odd with kx <= 25. scale: the flux scaling after the convolution (1 for
- * DENSFLUX models). */
+ * device. cd2i (nz), vedges (nlines, nz+1) and kern (ky, kx) are HOST float64 arrays; kern may be NULL (no
+ * smoothing); its sizes are odd with kx <= 25. scale: the flux scaling after the convolution (1 for DENSFLUX
+ * models). The line profile's cumulative normal, which the reference tabulates (galmod.py:183-191), the deposit
+ * evaluates in registers (within 1.5e-7, Abramowitz & Stegun 7.1.26): no table is passed. */
 typedef struct bb_model bb_model;
-int bb_model_create(const bb_grid* g, const double* cd2i, const double* vedges, const double* ptab, int ntab,
-                    const double* kern, int ky, int kx, double scale, bb_model** out);
+int bb_model_create(const bb_grid* g, const double* cd2i, const double* vedges, const double* kern, int ky, int kx,
+                    double scale, bb_model** out);
 int bb_model_destroy(bb_model* m);
 
 /* The pixel tile (TX x TY) of the ANALYTIC deposit: by default the largest whose spectra fit the LDS of a CU;

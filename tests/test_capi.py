@@ -19,12 +19,12 @@ def test_exports_every_declared_symbol():
 def test_argument_errors_before_device_calls():
     L = _lib.lib()
     # NULL output handle
-    rc = L.bb_model_create(None, None, None, None, 0, None, 0, 0, 1.0, None)
+    rc = L.bb_model_create(None, None, None, None, 0, 0, 1.0, None)
     assert rc != 0 and b"NULL" in L.bb_last_error()
     # invalid grid
     g = _lib.bb_grid()
     out = C.c_void_p()
-    rc = L.bb_model_create(C.byref(g), None, None, None, 0, None, 0, 0, 1.0, C.byref(out))
+    rc = L.bb_model_create(C.byref(g), None, None, None, 0, 0, 1.0, C.byref(out))
     assert rc != 0 and b"nz" in L.bb_last_error() and not out.value
     # null model handles
     assert L.bb_galmod_analytic(None, None, _lib.bb_box(0, 0, 4, 4), None, 1, 1, None, 0, None, 0, None) != 0

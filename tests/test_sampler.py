@@ -33,8 +33,9 @@ def test_ellipsoid_contains_points_and_draws_inside():
 
 def test_gaussian_evidence_and_posterior():
     s = Sampler(identity, gauss_logl, D, n_live=400, n_batch=200, seed=1).run(f_live=0.01)
-    # log Z = 0 with the sampler's own error sqrt(H / n_live), H ~ 6.3 nats here
+    # log Z = 0 with the sampler's own error sqrt(H / n_live); H = -D/2 ln(2 pi e SIG^2) = 6.31 nats here
     assert abs(s.log_z) < 0.4, s.log_z
+    assert abs(s.information - (-0.5 * D * math.log(2 * math.pi * math.e * SIG ** 2))) < 0.25, s.information
     pts, logw, _ = s.posterior()
     w = np.exp(logw)
     mean = (w[:, None] * pts).sum(axis=0)
