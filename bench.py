@@ -27,11 +27,16 @@ PEAK_HBM = 8.0e12          # MI355X HBM3E, bytes/s
 # HBM bytes per launch of k_an_tile at batch 256 on the 160x160x96 box, from two rocprofv3 counter passes
 # (FETCH_SIZE, WRITE_SIZE) scaled by the same counters of a plain 1 GiB copy (bench.py --copy-calibration)
 TRAFFIC_SHAPE = (256, 160, 160, 96)
-TRAFFIC_TILE_BYTES = (2780729.8 * 2.0 + 6629300.9 * 1.0) * 1024   # KiB x scale -> bytes
+TRAFFIC_TILE_BYTES = (2642666.8 * 2.0 + 7016962.3 * 1.0) * 1024   # KiB x scale -> bytes
 # LDS instructions (wave level) of one such launch (rocprofv3 --pmc SQ_INSTS_LDS) and the rate of one CU for
-# 64-bit LDS atomics measured by a probe of ds_add_u64 (6.4-6.8 cycles per wave instruction): the deposit's limiter
-LDS_INSTS_TILE = 2.92e9
+# 64-bit LDS atomics measured by a probe of ds_add_u64 (6.4-6.8 cycles per wave instruction): one of the deposit's
+# two limiters
+LDS_INSTS_TILE = 2.485e9
 LDS_ATOMIC_PEAK = 256 * 2.4e9 / 6.6          # wave instructions per second over the 256 CUs at 2.4 GHz
+# VALU instructions (wave level) of that launch (SQ_INSTS_VALU) and the issue rate: one wave64 instruction per 4
+# cycles per SIMD (16 lanes a cycle), 1024 SIMDs at 2.4 GHz: the other limiter
+VALU_INSTS_TILE = 1.798e10
+VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4
 
 
 def workload(n_theta, B, seed=1):
@@ -242,11 +247,14 @@ def main():
     nmap = (like.rings0.nr + 1) * by * bx
     ach = 4 * nvox * B / (tile_ms * 1e-3)
     traffic = TRAFFIC_TILE_BYTES if (B, by, bx, nz) == TRAFFIC_SHAPE else None
-    lds = None
+    lds = valu = None
     if (B, by, bx, nz) == TRAFFIC_SHAPE:
         rate = LDS_INSTS_TILE / (tile_ms * 1e-3)
         lds = {"insts_per_launch": LDS_INSTS_TILE, "achieved": rate, "peak": LDS_ATOMIC_PEAK,
                "unit": "LDS wave instructions/s", "frac": rate / LDS_ATOMIC_PEAK}
+        vrate = VALU_INSTS_TILE / (tile_ms * 1e-3)
+        valu = {"insts_per_launch": VALU_INSTS_TILE, "achieved": vrate, "peak": VALU_ISSUE_PEAK,
+                "unit": "VALU wave instructions/s", "frac": vrate / VALU_ISSUE_PEAK}
     out = {
         "metric": "likelihood evals/s (160\u00b2\u00d796 cube); Bayes fit wall-time, surrogate vs plain", "value": value,
         "unit": "evals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms,
@@ -262,7 +270,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_an_tile", "kernel_ms": round(tile_ms, 4), "achieved": ach / 1e9,
                      "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": ach / PEAK_HBM, "traffic": traffic,
                      "traffic_unit": "bytes per launch", "algorithmic_bytes": 4 * nvox * B,
-                     "limiter": "LDS atomics and memory latency, not HBM (lds gives the LDS fraction)", "lds": lds},
+                     "limiter": "LDS atomics and VALU issue, not HBM (lds and valu give their fractions)",
+                     "lds": lds, "valu": valu},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample",
                                                                       "calibration", "cpu_model", "cpu_count")},
         "loglike_sample": [float(x) for x in ll[:4]],
