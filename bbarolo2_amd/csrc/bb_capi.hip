@@ -18,7 +18,7 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
                                          int TX, int TY, double* sub, double* wtab, short4* bbox, short4* sbox,
                                          const double* cd2i, const double* vedges, int lin_axis, float* out,
                                          int accumulate, int* bin, int bin_cap,
-                                         const int4* vbox, int variant, hipStream_t st);
+                                         const int4* vbox, int variant, int boxes_ready, hipStream_t st);
 extern "C" void bb_set_tile_events(void* ev_begin, void* ev_end);
 extern "C" size_t bb_analytic_sub_bytes();
 extern "C" size_t bb_analytic_wt_bytes();
@@ -326,7 +326,8 @@ extern "C" int bb_galmod_analytic_boxes(const bb_model* m, const bb_azplan* p, b
 }
 
 int bb_galmod_analytic_vb(const bb_model* m, const bb_azplan* p, bb_box box, const int4* vbox, const double* sr,
-                          int nsub, int B, void* work, int64_t work_bytes, float* out, int accumulate, void* stream) {
+                          int nsub, int B, void* work, int64_t work_bytes, float* out, int accumulate, void* stream,
+                          int boxes_ready) {
     if (!m || !p) return fail(BB_ERR_ARG, "bb_galmod_analytic: model or plan is NULL");
     int rc = check_box(m, box);
     if (rc) return rc;
@@ -353,7 +354,7 @@ int bb_galmod_analytic_vb(const bb_model* m, const bb_azplan* p, bb_box box, con
     rc = bb_launch_galmod_analytic(&m->g, box, sr, nsub, B, p->d_naz, p->d_chunks, p->nchunks, p->d_az, m->TX, m->TY,
                                    sub, wtab, bbox, sbox, m->d_cd2i, m->d_vedges, m->lin, out,
                                    accumulate, work_bin(m, p, B, sbox, ntiles), (int)(p->nchunks * (int64_t)m->bin_per), vbox,
-                                   m->dep_variant, (hipStream_t)stream);
+                                   m->dep_variant, boxes_ready && !vbox, (hipStream_t)stream);
     return rc ? fail(BB_ERR_HIP, "bb_galmod_analytic launch: %s", hipGetErrorString((hipError_t)rc)) : 0;
 }
 

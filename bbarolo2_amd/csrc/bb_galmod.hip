@@ -1290,12 +1290,14 @@ extern "C" int bb_launch_galmod_analytic(const bb_grid* g, bb_box box, const dou
                                          int TX, int TY, double* sub, double* wtab, short4* bbox, short4* sbox,
                                          const double* cd2i, const double* vedges, int lin_axis, float* out,
                                          int accumulate, int* bin, int bin_cap,
-                                         const int4* vbox, int variant, hipStream_t st) {
+                                         const int4* vbox, int variant, int boxes_ready, hipStream_t st) {
     const int bx = box.bhi0 - box.blo0, by = box.bhi1 - box.blo1;
     hipLaunchKernelGGL(k_an_prep, dim3((nsub + 63) / 64, B), dim3(64), 0, st, *g, box, sr, nsub, naz, sub, wtab, 0,
                        vbox);
-    hipLaunchKernelGGL(k_an_bbox, dim3((nchunks * BB_CS + 255) / 256, B), dim3(256), 0, st, *g, box, chunks, nchunks,
-                       az, sub, nsub, bbox, sbox, vbox);
+    // (the sample boxes depend on the geometry only: boxes_ready keeps those of the ring maps of the same samples)
+    if (!boxes_ready)
+        hipLaunchKernelGGL(k_an_bbox, dim3((nchunks * BB_CS + 255) / 256, B), dim3(256), 0, st, *g, box, chunks,
+                           nchunks, az, sub, nsub, bbox, sbox, vbox);
     const int ntiles = ((bx + TX - 1) / TX) * ((by + TY - 1) / TY);
     if (bin) {
         const int e = bin_chunks(box, B, bbox, nchunks, TX, TY, bin, bin_cap, st);

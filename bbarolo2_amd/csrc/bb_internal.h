@@ -69,8 +69,11 @@ struct bb_profile {
 
 // The deposits with a model box per vector (vbox: B boxes [x, z) x [y, w) in absolute pixels inside box, the layout
 // of out; NULL: box for every vector), as bb_galmod_analytic / bb_galmod_mc.
+// boxes_ready: the sample boxes in work are those of the same samples in the same box from a previous launch on the
+// same work (the ring maps of the likelihood context): k_an_bbox is not run again.
 int bb_galmod_analytic_vb(const bb_model* m, const bb_azplan* p, bb_box box, const int4* vbox, const double* sr,
-                          int nsub, int B, void* work, int64_t work_bytes, float* out, int accumulate, void* stream);
+                          int nsub, int B, void* work, int64_t work_bytes, float* out, int accumulate, void* stream,
+                          int boxes_ready = 0);
 int bb_galmod_mc_vb(const bb_model* m, bb_box box, const int4* vbox, const double* sr, int nsub, int B, int nvpar,
                     uint32_t seed, int64_t max_clouds, void* work, int64_t work_bytes, float* out, uint64_t* counts,
                     int* overflow, int accumulate, void* stream);

@@ -562,7 +562,17 @@ int run_part(bb_ctx* c, int p, int a, int n, const Stage& D, hipStream_t st, hip
     const int4* va = c->d.box_free ? D.va + a : nullptr;
     const int4* vr = c->d.box_free ? D.vr + a : nullptr;
     if (!c->d.mc) {
-        rc = bb_galmod_analytic_vb(m, c->pm, c->db, va, sr, c->nsub_t, n, c->awork[p], c->awork_b[p], dep, 0, st);
+        // the ring maps' sample boxes serve the deposit (they depend on the geometry only): the same samples (plan
+        // pm, these vectors, this work buffer) in the same box, unless the fixed annulus's maps, on the same work
+        // after them, reached that part of it
+        const bb_box L = c->d.likebox, D0 = c->db;
+        bool reuse = c->prof && !c->d.box_free && L.blo0 == D0.blo0 && L.blo1 == D0.blo1 && L.bhi0 == D0.bhi0 &&
+                     L.bhi1 == D0.bhi1;
+        if (reuse && !c->ann_free)
+            reuse = bb_ring_maps_work_size(c->pe, n) - 256 <=
+                    (int64_t)n * c->nsub_t * (int64_t)(bb_analytic_sub_bytes() + bb_analytic_wt_bytes());
+        rc = bb_galmod_analytic_vb(m, c->pm, c->db, va, sr, c->nsub_t, n, c->awork[p], c->awork_b[p], dep, 0, st,
+                                   reuse ? 1 : 0);
     } else {
         rc = bb_galmod_mc_vb(m, c->db, va, sr, c->nsub_t, n, c->d.nvpar, c->d.seed, c->mc_cap, c->mwork[p],
                              c->mwork_b[p], dep, nullptr, c->mcflag + p, 0, st);
