@@ -9,6 +9,7 @@ reference's default FREE holds RADMAX, keywords.py:700; _update_rings, bayesian/
     inside the disk so that the annulus falls between the rings and the prior's rows change order, near the centre,
     negative: -inf) and with RADMAXA and RADMAXR free (one half without an edge) agree with the FP64 oracle pipeline,
     which sets rings.edge / edgeapp / edgerec from theta and builds the profile, annulus and model of those rings;
+    the same with DENS fitted (no profile: the edge weights go into the model's densities only);
   * a free RADMAX at a value gives the log-likelihood of the same context with that edge fixed (to 1e-7 of the
     residual term: the two differ in the order of the products of the ring weights and in the azimuths of the
     annulus, taken from cos and sin per vector rather than from the plan's recurrence).
@@ -118,3 +119,15 @@ def test_free_edge_equals_fixed_edge():
         resid = np.abs(b)
         print(f"[edge {E}] free {a} fixed {b} |d| {np.abs(a - b)}", flush=True)
         assert np.all(np.abs(a - b) <= 1e-7 * resid)
+
+
+def test_radmax_free_with_dens_fitted_vs_oracle():
+    """DENS fitted (no profile): the edge weights go into the model's densities only."""
+    spec = [("vrot", "single"), ("dens", "ring"), ("radmax", "single")]
+    h, r, par, data, sig, region, box, like = _setup(spec, chunk=8)
+    assert not like.useNorm and like.ann_cap is None
+    orc = pipeline.Likelihood(h, par, r, spec, data, None, region, box, sig, 0.5)
+    d = np.asarray(r.dens, float)
+    theta = np.array([np.r_[150., d, 250.], np.r_[145., 0.8 * d, 131.5], np.r_[150., d, 75.], np.r_[150., d, -1.]])
+    ll = _compare(like, orc, theta, "RADMAX, DENS fitted")
+    assert np.isneginf(ll[-1]) and np.all(np.isfinite(ll[:-1]))
