@@ -558,6 +558,11 @@ __device__ __forceinline__ double uw_(const Sub& u, const double* __restrict__ w
     return uw_cold(j, u.nin, u.step, u.zlim, ltype);
 }
 
+// The lanes of a wavefront take samples BB_SSTRIDE entries apart in a window of 64 BB_SSTRIDE (neighbouring azimuths of a
+// sub-ring, about two pixels apart): far enough apart that their atomics seldom meet the same pixel, near enough that
+// their columns cross the tile alike (runs per sample), 51.6 ms for the 256-vector batch against 52.5 at 4 and 52.9
+// at 12 (the whole window); 64 consecutive samples: 70.4
+#define BB_SSTRIDE 6
 #define BB_LIST_CAP 1024     // chunks meeting the tile, gathered before their samples are tested
 #define BB_AN_SAMP_CAP 8192  // samples meeting the tile, gathered before they are deposited (ANALYTIC deposit; the
                              // LDS it takes is worth more here than float copies of the vertical weights: 66.1 ms
@@ -832,9 +837,10 @@ __device__ __forceinline__ void an_sample(const bb_grid& g, const bb_box& box, c
 // The samples of a batch vector that meet the tile [tx0, tx1] x [ty0, ty1] (box-relative): the chunks whose
 // box meets it are gathered in clist, then their samples whose vertical segment meets it in slist, and dep(e)
 // is called for each, e = (chunk << 5) | azimuth lane, one per thread. The list holds neighbouring azimuths of
-// one sub-ring next to each other; lane l of wavefront w takes entry l * W + w of each window of blockDim
-// entries (W wavefronts), so the lanes of one LDS atomic are W azimuths apart (different pixels: same-address
-// atomics of a wavefront serialise) on neighbouring sub-rings (similar work). The samples of successive batches
+// one sub-ring next to each other; in a full window of blockDim entries (W wavefronts) lane l of wavefront w takes
+// entry 64 S (w / S) + S l + w % S, S = BB_SSTRIDE (a last, partial window: l Wa + w), so the lanes of one LDS
+// atomic are S azimuths apart (different pixels: same-address atomics of a wavefront serialise) on neighbouring
+// sub-rings (similar work). The samples of successive batches
 // of chunks accumulate in slist until it is nearly full or the tile's chunks are done (larger windows: fuller
 // wavefronts). Every thread of the block calls it (it synchronises). SCAP: the entries of slist. WAVE: the
 // wavefronts take 64 consecutive entries at a time instead, and every lane of a wavefront calls dep together
@@ -860,7 +866,12 @@ __device__ __forceinline__ void tile_samples(int nchunks, const short4* __restri
                 // (a last, partial window fills as few wavefronts as it can, its lanes still Wa entries apart)
                 const int m = ns - base2;
                 const int Wa = m >= (int)blockDim.x ? W : (m + 63) >> 6;
-                const int idx = (int)(threadIdx.x >> 6) < Wa ? base2 + (threadIdx.x & 63) * Wa + (threadIdx.x >> 6) : ns;
+                const int wv = (int)(threadIdx.x >> 6), ln = (int)(threadIdx.x & 63);
+                int idx;
+                if (m >= (int)blockDim.x && W % BB_SSTRIDE == 0)      // (a full window: groups of BB_SSTRIDE wavefronts)
+                    idx = base2 + (wv / BB_SSTRIDE) * 64 * BB_SSTRIDE + ln * BB_SSTRIDE + wv % BB_SSTRIDE;
+                else
+                    idx = wv < Wa ? base2 + ln * Wa + wv : ns;
                 if (idx < ns) dep(slist[idx]);
             }
         }
