@@ -27,15 +27,15 @@ PEAK_HBM = 8.0e12          # MI355X HBM3E, bytes/s
 # HBM bytes per launch of k_an_tile at batch 256 on the 160x160x96 box, from two rocprofv3 counter passes
 # (FETCH_SIZE, WRITE_SIZE) scaled by the same counters of a plain 1 GiB copy (bench.py --copy-calibration)
 TRAFFIC_SHAPE = (256, 160, 160, 96)
-TRAFFIC_TILE_BYTES = (2642666.8 * 2.0 + 7016962.3 * 1.0) * 1024   # KiB x scale -> bytes
+TRAFFIC_TILE_BYTES = (2850600.0 * 2.0 + 7313122.0 * 1.0) * 1024   # KiB x scale -> bytes
 # LDS instructions (wave level) of one such launch (rocprofv3 --pmc SQ_INSTS_LDS) and the rate of one CU for
 # 64-bit LDS atomics measured by a probe of ds_add_u64 (6.4-6.8 cycles per wave instruction): one of the deposit's
 # two limiters
-LDS_INSTS_TILE = 2.485e9
+LDS_INSTS_TILE = 2.286e9
 LDS_ATOMIC_PEAK = 256 * 2.4e9 / 6.6          # wave instructions per second over the 256 CUs at 2.4 GHz
 # VALU instructions (wave level) of that launch (SQ_INSTS_VALU) and the issue rate: one wave64 instruction per 4
 # cycles per SIMD (16 lanes a cycle), 1024 SIMDs at 2.4 GHz: the other limiter
-VALU_INSTS_TILE = 1.798e10
+VALU_INSTS_TILE = 1.709e10
 VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4
 
 
